@@ -1,0 +1,82 @@
+"""Loader for libcytnx_amd.so, the C ABI declared in include/cytnx_amd.h.
+
+The library is required: every device operation in this package goes through it, and there is
+no CPU or PyTorch fallback.  A missing or stale build raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcytnx_amd.so")
+ABI_VERSION = 1
+
+OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_NOMEM, ERR_NUMERIC = 0, 1, 2, 3, 4, 5
+
+_lib = None
+
+
+class CxaError(RuntimeError):
+    """A C-ABI call returned a non-zero status; ``code`` is that status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__("[cytnx_amd status %d] %s" % (code, msg))
+        self.code = code
+
+
+class CxaInvalid(CxaError, ValueError):
+    pass
+
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+i64p = ctypes.POINTER(ctypes.c_int64)
+vp = ctypes.c_void_p
+
+
+def _declare(L):
+    sig = {
+        "cxa_last_error": (ctypes.c_char_p, []),
+        "cxa_abi_version": (ctypes.c_int, []),
+        "cxa_permute": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, u64p, u64p, vp, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return sig
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "cytnx_amd: native library %s is missing; build it with "
+                "`python -m cytnx_amd.build` (there is no fallback path)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        v = L.cxa_abi_version()
+        if v != ABI_VERSION:
+            raise RuntimeError("cytnx_amd: library ABI %d, package expects %d; rebuild"
+                               % (v, ABI_VERSION))
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        msg = lib().cxa_last_error().decode(errors="replace")
+        if rc == ERR_INVALID:
+            raise CxaInvalid(rc, msg)
+        raise CxaError(rc, msg)
+
+
+def u64_array(seq):
+    seq = [int(x) for x in seq]
+    return (ctypes.c_uint64 * max(1, len(seq)))(*seq)
+
+
+def i64_array(seq):
+    seq = [int(x) for x in seq]
+    return (ctypes.c_int64 * max(1, len(seq)))(*seq)

@@ -1,0 +1,61 @@
+// Shared plumbing for the cytnx_amd C-ABI: status codes, the thread-local error message,
+// HIP error mapping and dtype sizes.  Every entry point returns a status; none exits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/cytnx_amd.h"
+
+namespace cxa {
+
+void set_error(const std::string &msg);
+const std::string &get_error();
+
+// element size in bytes for a reference dtype id (include/Type.hpp:177-179); 0 if unknown
+inline int elem_size(int dtype) {
+  switch (dtype) {
+    case CXA_C128: return 16;
+    case CXA_C64: return 8;
+    case CXA_F64: return 8;
+    case CXA_F32: return 4;
+    case CXA_I64: return 8;
+    case CXA_U64: return 8;
+    case CXA_I32: return 4;
+    case CXA_U32: return 4;
+    case CXA_I16: return 2;
+    case CXA_U16: return 2;
+    case CXA_BOOL: return 1;
+    default: return 0;
+  }
+}
+
+inline int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+inline int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  set_error(buf);
+  return code;
+}
+
+}  // namespace cxa
+
+#define CXA_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return cxa::fail(CXA_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                             \
+  } while (0)
+
+#define CXA_CHECK(cond, code, ...)                 \
+  do {                                                \
+    if (!(cond)) return cxa::fail((code), __VA_ARGS__); \
+  } while (0)

@@ -1,0 +1,363 @@
+// Grouped, persistent MFMA GEMM for gfx950.
+//
+// One launch runs a list of wave tiles drawn from any number of independent output blocks.
+// Each wave (64 lanes) owns one 16*MI x 16*NI tile and walks the K dimension of every segment
+// of its block in chunks of 16, issuing MI*NI*4 MFMA 16x16x4 instructions per chunk.
+//
+// Operand fragments are loaded straight from global memory (L2/MALL-resident at these sizes)
+// with the k index assigned so that each lane reads 4 consecutive k values: MFMA number j of a
+// chunk takes k = k0 + 4*(lane/16) + j for both A (row lane%16) and B (column lane%16).  The
+// MFMA sums over its 4 k slots, so any assignment that is identical for A and B is exact; this
+// one makes row-major A loads 32-byte contiguous per lane and B loads 128-byte contiguous per
+// 16 lanes.  Accumulator layout of v_mfma_*_16x16x4: lane holds rows 4*(lane/16)+v, column
+// lane%16, v = 0..3.
+//
+// Precision modes: FP64 (v_mfma_f64_16x16x4_f64), FP32 (v_mfma_f32_16x16x4_f32, operands
+// rounded to FP32), FP32X3 (FP64 operands split x = hi + lo in FP32; hi*hi + hi*lo + lo*hi on
+// FP32 MFMA, the two cross terms in their own accumulator, combined in FP64 at the end).
+#include <algorithm>
+
+#include "cxa_common.hpp"
+#include "gemm.hpp"
+
+namespace cxa {
+namespace {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct alignas(16) cd {
+  double re, im;
+};
+struct alignas(8) cf {
+  float re, im;
+};
+
+__device__ __forceinline__ f64x4 mma64(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mma32(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <typename T>
+__device__ __forceinline__ double ld_real(const T *p) {
+  return (double)(*p);
+}
+
+// ---------------------------------------------------------------------------------------
+// real wave tile
+// ---------------------------------------------------------------------------------------
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI>
+__device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
+                                               const char *__restrict__ Bbase,
+                                               char *__restrict__ Cbase, const GBlk &blk,
+                                               const GSeg *__restrict__ segs, int m0, int n0,
+                                               int lane) {
+  constexpr bool F64 = (PREC == CXA_PREC_FP64);
+  constexpr bool X3 = (PREC == CXA_PREC_FP32X3);
+  using acc_t = typename std::conditional<F64, f64x4, f32x4>::type;
+  using op_t = typename std::conditional<F64, double, float>::type;
+  const int r = lane & 15, g = lane >> 4;
+  const int mt = min(16 * MI, blk.M - m0);
+  const int nt = min(16 * NI, blk.N - n0);
+  acc_t acc[MI][NI];
+  acc_t acl[MI][NI];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+      acc[i][j] = acc_t{0, 0, 0, 0};
+      acl[i][j] = acc_t{0, 0, 0, 0};
+    }
+
+  for (int s = 0; s < blk.nseg; ++s) {
+    const GSeg sg = segs[blk.seg0 + s];
+    const TA *Ap = reinterpret_cast<const TA *>(Abase + sg.a_off) + (int64_t)m0 * sg.sam;
+    const TB *Bp = reinterpret_cast<const TB *>(Bbase + sg.b_off) + (int64_t)n0 * sg.sbn;
+    const int K = sg.K;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      op_t a[MI][4], b[NI][4], alo[MI][4], blo[NI][4];
+#pragma unroll
+      for (int i = 0; i < MI; ++i) {
+        const int row = 16 * i + r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int k = k0 + 4 * g + j;
+          const double v =
+              (row < mt && k < K) ? ld_real(Ap + (int64_t)row * sg.sam + (int64_t)k * sg.sak)
+                                  : 0.0;
+          a[i][j] = (op_t)v;
+          if constexpr (X3) alo[i][j] = (float)(v - (double)(float)v);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        const int col = 16 * i + r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int k = k0 + 4 * g + j;
+          const double v =
+              (col < nt && k < K) ? ld_real(Bp + (int64_t)k * sg.sbk + (int64_t)col * sg.sbn)
+                                  : 0.0;
+          b[i][j] = (op_t)v;
+          if constexpr (X3) blo[i][j] = (float)(v - (double)(float)v);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+#pragma unroll
+          for (int jj = 0; jj < NI; ++jj) {
+            if constexpr (F64) {
+              acc[i][jj] = mma64(a[i][j], b[jj][j], acc[i][jj]);
+            } else {
+              acc[i][jj] = mma32(a[i][j], b[jj][j], acc[i][jj]);
+              if constexpr (X3) {
+                acl[i][jj] = mma32(a[i][j], blo[jj][j], acl[i][jj]);
+                acl[i][jj] = mma32(alo[i][j], b[jj][j], acl[i][jj]);
+              }
+            }
+          }
+    }
+  }
+
+  TC *Cp = reinterpret_cast<TC *>(Cbase + blk.c_off);
+  const double alpha = blk.alpha, beta = blk.beta;
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int jj = 0; jj < NI; ++jj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * i + 4 * g + v;
+        const int col = 16 * jj + r;
+        if (row < mt && col < nt) {
+          double val = (double)acc[i][jj][v];
+          if constexpr (X3) val += (double)acl[i][jj][v];
+          val *= alpha;
+          TC *dst = Cp + (int64_t)(m0 + row) * blk.ldc + (n0 + col);
+          if (beta != 0.0) val += beta * (double)(*dst);
+          *dst = (TC)val;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------------------
+// complex wave tile (C128 on FP64 MFMA, C64 on FP32 MFMA): four real products per step
+// ---------------------------------------------------------------------------------------
+template <typename CT, int MI, int NI>
+__device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
+                                               const char *__restrict__ Bbase,
+                                               char *__restrict__ Cbase, const GBlk &blk,
+                                               const GSeg *__restrict__ segs, int m0, int n0,
+                                               int lane) {
+  constexpr bool F64 = std::is_same<CT, cd>::value;
+  using acc_t = typename std::conditional<F64, f64x4, f32x4>::type;
+  using op_t = typename std::conditional<F64, double, float>::type;
+  const int r = lane & 15, g = lane >> 4;
+  const int mt = min(16 * MI, blk.M - m0);
+  const int nt = min(16 * NI, blk.N - n0);
+  acc_t accr[MI][NI], acci[MI][NI];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < NI; ++j) {
+      accr[i][j] = acc_t{0, 0, 0, 0};
+      acci[i][j] = acc_t{0, 0, 0, 0};
+    }
+  for (int s = 0; s < blk.nseg; ++s) {
+    const GSeg sg = segs[blk.seg0 + s];
+    const CT *Ap = reinterpret_cast<const CT *>(Abase + sg.a_off) + (int64_t)m0 * sg.sam;
+    const CT *Bp = reinterpret_cast<const CT *>(Bbase + sg.b_off) + (int64_t)n0 * sg.sbn;
+    const int K = sg.K;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      op_t ar[MI][4], ai[MI][4], br[NI][4], bi[NI][4];
+#pragma unroll
+      for (int i = 0; i < MI; ++i) {
+        const int row = 16 * i + r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int k = k0 + 4 * g + j;
+          CT v{0, 0};
+          if (row < mt && k < K) v = Ap[(int64_t)row * sg.sam + (int64_t)k * sg.sak];
+          ar[i][j] = v.re;
+          ai[i][j] = v.im;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        const int col = 16 * i + r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int k = k0 + 4 * g + j;
+          CT v{0, 0};
+          if (col < nt && k < K) v = Bp[(int64_t)k * sg.sbk + (int64_t)col * sg.sbn];
+          br[i][j] = v.re;
+          bi[i][j] = v.im;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+#pragma unroll
+          for (int jj = 0; jj < NI; ++jj) {
+            if constexpr (F64) {
+              accr[i][jj] = mma64(ar[i][j], br[jj][j], accr[i][jj]);
+              accr[i][jj] = mma64(-ai[i][j], bi[jj][j], accr[i][jj]);
+              acci[i][jj] = mma64(ar[i][j], bi[jj][j], acci[i][jj]);
+              acci[i][jj] = mma64(ai[i][j], br[jj][j], acci[i][jj]);
+            } else {
+              accr[i][jj] = mma32(ar[i][j], br[jj][j], accr[i][jj]);
+              accr[i][jj] = mma32(-ai[i][j], bi[jj][j], accr[i][jj]);
+              acci[i][jj] = mma32(ar[i][j], bi[jj][j], acci[i][jj]);
+              acci[i][jj] = mma32(ai[i][j], br[jj][j], acci[i][jj]);
+            }
+          }
+    }
+  }
+  CT *Cp = reinterpret_cast<CT *>(Cbase + blk.c_off);
+  const double alpha = blk.alpha, beta = blk.beta;
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int jj = 0; jj < NI; ++jj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * i + 4 * g + v;
+        const int col = 16 * jj + r;
+        if (row < mt && col < nt) {
+          double re = alpha * (double)accr[i][jj][v], im = alpha * (double)acci[i][jj][v];
+          CT *dst = Cp + (int64_t)(m0 + row) * blk.ldc + (n0 + col);
+          if (beta != 0.0) {
+            const CT o = *dst;
+            re += beta * (double)o.re;
+            im += beta * (double)o.im;
+          }
+          CT w;
+          w.re = (decltype(w.re))re;
+          w.im = (decltype(w.im))im;
+          *dst = w;
+        }
+      }
+}
+
+constexpr int kWaves = 4;
+
+template <int PREC, typename TA, typename TB, typename TC, bool CPLX>
+__global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict__ tiles,
+                                                        int ntiles, const GBlk *__restrict__ blks,
+                                                        const GSeg *__restrict__ segs,
+                                                        const char *A, const char *B, char *C) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int t = blockIdx.x * kWaves + wave; t < ntiles; t += gridDim.x * kWaves) {
+    const GTile tl = tiles[t];
+    const GBlk blk = blks[tl.blk];
+    if constexpr (CPLX) {
+      switch (tl.kind) {
+        case TK_1x1: wave_tile_cplx<TA, 1, 1>(A, B, C, blk, segs, tl.m0, tl.n0, lane); break;
+        case TK_2x2: wave_tile_cplx<TA, 2, 2>(A, B, C, blk, segs, tl.m0, tl.n0, lane); break;
+        case TK_2x1: wave_tile_cplx<TA, 2, 1>(A, B, C, blk, segs, tl.m0, tl.n0, lane); break;
+        case TK_1x2: wave_tile_cplx<TA, 1, 2>(A, B, C, blk, segs, tl.m0, tl.n0, lane); break;
+        default: wave_tile_cplx<TA, 2, 2>(A, B, C, blk, segs, tl.m0, tl.n0, lane); break;
+      }
+    } else {
+      switch (tl.kind) {
+        case TK_1x1:
+          wave_tile_real<PREC, TA, TB, TC, 1, 1>(A, B, C, blk, segs, tl.m0, tl.n0, lane);
+          break;
+        case TK_2x2:
+          wave_tile_real<PREC, TA, TB, TC, 2, 2>(A, B, C, blk, segs, tl.m0, tl.n0, lane);
+          break;
+        case TK_2x1:
+          wave_tile_real<PREC, TA, TB, TC, 2, 1>(A, B, C, blk, segs, tl.m0, tl.n0, lane);
+          break;
+        case TK_1x2:
+          wave_tile_real<PREC, TA, TB, TC, 1, 2>(A, B, C, blk, segs, tl.m0, tl.n0, lane);
+          break;
+        default:
+          wave_tile_real<PREC, TA, TB, TC, 4, 2>(A, B, C, blk, segs, tl.m0, tl.n0, lane);
+          break;
+      }
+    }
+  }
+}
+
+template <int PREC, typename TA, typename TB, typename TC, bool CPLX>
+int launch_t(const GemmLaunch &g, hipStream_t s) {
+  if (g.ntiles <= 0) return CXA_OK;
+  int grid = g.grid;
+  if (grid <= 0) grid = std::min((g.ntiles + kWaves - 1) / kWaves, 4096);
+  hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, CPLX>), dim3(grid), dim3(64 * kWaves), 0, s,
+                     g.tiles, g.ntiles, g.blks, g.segs, (const char *)g.A, (const char *)g.B,
+                     (char *)g.C);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+}  // namespace
+
+int launch_grouped_gemm(const GemmLaunch &g, hipStream_t s) {
+  const int p = g.prec, a = g.a_dt, b = g.b_dt, c = g.c_dt;
+  if (p == CXA_PREC_FP64) {
+    if (a == CXA_F64 && b == CXA_F64 && c == CXA_F64)
+      return launch_t<CXA_PREC_FP64, double, double, double, false>(g, s);
+    if (a == CXA_C128 && b == CXA_C128 && c == CXA_C128)
+      return launch_t<CXA_PREC_FP64, cd, cd, cd, true>(g, s);
+  } else if (p == CXA_PREC_FP32) {
+#define CXA_F32_CASE(TA_, TB_, TC_, IA, IB, IC) \
+  if (a == IA && b == IB && c == IC) return launch_t<CXA_PREC_FP32, TA_, TB_, TC_, false>(g, s);
+    CXA_F32_CASE(float, float, float, CXA_F32, CXA_F32, CXA_F32)
+    CXA_F32_CASE(float, float, double, CXA_F32, CXA_F32, CXA_F64)
+    CXA_F32_CASE(float, double, float, CXA_F32, CXA_F64, CXA_F32)
+    CXA_F32_CASE(float, double, double, CXA_F32, CXA_F64, CXA_F64)
+    CXA_F32_CASE(double, float, float, CXA_F64, CXA_F32, CXA_F32)
+    CXA_F32_CASE(double, float, double, CXA_F64, CXA_F32, CXA_F64)
+    CXA_F32_CASE(double, double, float, CXA_F64, CXA_F64, CXA_F32)
+    CXA_F32_CASE(double, double, double, CXA_F64, CXA_F64, CXA_F64)
+#undef CXA_F32_CASE
+    if (a == CXA_C64 && b == CXA_C64 && c == CXA_C64)
+      return launch_t<CXA_PREC_FP32, cf, cf, cf, true>(g, s);
+  } else if (p == CXA_PREC_FP32X3) {
+    if (a == CXA_F64 && b == CXA_F64 && c == CXA_F64)
+      return launch_t<CXA_PREC_FP32X3, double, double, double, false>(g, s);
+  }
+  return fail(CXA_ERR_UNSUPPORTED, "grouped GEMM: unsupported precision %d with dtypes (%d,%d,%d)",
+              p, a, b, c);
+}
+
+int default_kind(int M, int N) {
+  if (M <= 16 && N <= 16) return TK_1x1;
+  if (M <= 16) return TK_1x2;
+  if (N <= 16) return TK_2x1;
+  return TK_2x2;
+}
+
+void make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
+                std::vector<GTile> &tiles, int kind_override) {
+  struct Item {
+    GTile t;
+    int64_t cost;
+  };
+  std::vector<Item> items;
+  for (size_t bi = 0; bi < blks.size(); ++bi) {
+    const GBlk &b = blks[bi];
+    if (b.M <= 0 || b.N <= 0) continue;
+    const int kind = kind_override >= 0 ? kind_override : default_kind(b.M, b.N);
+    const int tr = tile_rows(kind), tc = tile_cols(kind);
+    int64_t ksum = 0;
+    for (int s = 0; s < b.nseg; ++s) ksum += segs[b.seg0 + s].K;
+    for (int m0 = 0; m0 < b.M; m0 += tr)
+      for (int n0 = 0; n0 < b.N; n0 += tc)
+        items.push_back({GTile{(int32_t)bi, m0, n0, kind}, (ksum + 16) * tr * tc});
+  }
+  std::stable_sort(items.begin(), items.end(),
+                   [](const Item &x, const Item &y) { return x.cost > y.cost; });
+  tiles.resize(items.size());
+  for (size_t i = 0; i < items.size(); ++i) tiles[i] = items[i].t;
+}
+
+}  // namespace cxa

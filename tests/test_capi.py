@@ -1,0 +1,63 @@
+"""CPU-side checks of the C ABI: the library loads, exports every declared symbol, and
+rejects invalid arguments on the host before touching a device."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+from cytnx_amd import _lib
+
+
+def _declared_functions():
+    names = []
+    for fn in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b(cxa_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_loads_and_abi_version():
+    L = _lib.lib()
+    assert L.cxa_abi_version() == _lib.ABI_VERSION
+
+
+def test_every_declared_symbol_is_exported():
+    declared = _declared_functions()
+    assert "cxa_permute" in declared
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared if not hasattr(L, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True).stdout
+    exported = set(re.findall(r"\bT (cxa_[a-z0-9_]+)", out))
+    assert set(declared) <= exported
+    # nothing exported under the cxa_ prefix that the header does not declare
+    assert exported <= set(declared), sorted(exported - set(declared))
+
+
+def test_permute_rejects_bad_mapper_on_host():
+    L = _lib.lib()
+    shape = _lib.u64_array([2, 3])
+    rc = L.cxa_permute(3, 2, shape, _lib.u64_array([0, 0]), None, None, None)
+    assert rc == _lib.ERR_INVALID
+    assert b"duplicated" in L.cxa_last_error()
+    rc = L.cxa_permute(3, 2, shape, _lib.u64_array([0, 2]), None, None, None)
+    assert rc == _lib.ERR_INVALID
+    rc = L.cxa_permute(99, 2, shape, _lib.u64_array([1, 0]), None, None, None)
+    assert rc == _lib.ERR_UNSUPPORTED
+    # zero elements: nothing to move, succeeds without touching memory
+    rc = L.cxa_permute(3, 2, _lib.u64_array([0, 3]), _lib.u64_array([1, 0]), None, None, None)
+    assert rc == _lib.OK
+
+
+def test_check_maps_status_to_exceptions():
+    with pytest.raises(_lib.CxaInvalid):
+        L = _lib.lib()
+        _lib.check(L.cxa_permute(3, 1, _lib.u64_array([4]), _lib.u64_array([3]), None, None,
+                                 None))
