@@ -34,11 +34,31 @@ i64p = ctypes.POINTER(ctypes.c_int64)
 vp = ctypes.c_void_p
 
 
+ci = ctypes.c_int
+i64 = ctypes.c_int64
+dblp = ctypes.POINTER(ctypes.c_double)
+intp = ctypes.POINTER(ctypes.c_int)
+
+
 def _declare(L):
     sig = {
         "cxa_last_error": (ctypes.c_char_p, []),
-        "cxa_abi_version": (ctypes.c_int, []),
-        "cxa_permute": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, u64p, u64p, vp, vp, vp]),
+        "cxa_abi_version": (ci, []),
+        "cxa_permute": (ci, [ci, ci, u64p, u64p, vp, vp, vp]),
+        "cxa_matmul": (ci, [ci, vp, vp, vp, i64, i64, i64, vp]),
+        "cxa_gemm_batch": (ci, [ci, ctypes.c_char_p, ctypes.c_char_p, i64p, i64p, i64p, dblp,
+                                ctypes.POINTER(vp), i64p, ctypes.POINTER(vp), i64p, dblp,
+                                ctypes.POINTER(vp), i64p, i64, i64p, vp]),
+        "cxa_tensordot_workspace": (ci, [ci, ci, u64p, ci, u64p, ci, u64p, u64p, i64p]),
+        "cxa_tensordot": (ci, [ci, vp, vp, ci, u64p, vp, ci, u64p, ci, u64p, u64p, vp, vp]),
+        "cxa_u1_enumerate_blocks": (ci, [ci, intp, i64p, i64p, i64, u64p, i64p]),
+        "cxa_block_contract_plan": (ci, [ci, ci, ci, u64p, u64p, i64p, ci, ci, u64p, u64p, i64p,
+                                         ci, u64p, u64p, ci, u64p, u64p, i64p,
+                                         ctypes.POINTER(vp)]),
+        "cxa_block_contract_info": (ci, [vp, dblp, i64p, i64p]),
+        "cxa_block_contract_set_tile": (ci, [vp, ci]),
+        "cxa_block_contract_exec": (ci, [vp, vp, vp, vp, vp, ci, vp]),
+        "cxa_block_contract_destroy": (ci, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -80,3 +100,8 @@ def u64_array(seq):
 def i64_array(seq):
     seq = [int(x) for x in seq]
     return (ctypes.c_int64 * max(1, len(seq)))(*seq)
+
+
+def int_array(seq):
+    seq = [int(x) for x in seq]
+    return (ctypes.c_int * max(1, len(seq)))(*seq)

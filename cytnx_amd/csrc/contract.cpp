@@ -1,0 +1,608 @@
+// Contraction entry points: dense matmul, batched GEMM, tensordot, and the block-sparse
+// contraction planner that turns every charge-matched block pair of a BlockUniTensor
+// contraction into segments of one grouped GEMM launch.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "cxa_common.hpp"
+#include "gemm.hpp"
+
+namespace cxa {
+namespace {
+
+bool is_complex(int dt) { return dt == CXA_C128 || dt == CXA_C64; }
+
+// Can the axes `axes` (in this order) of a row-major array with extents `ext` be addressed
+// as one index with a single stride?  Unit axes are ignored.  On success *stride is the
+// element stride of the combined index (0 when every axis is a unit axis).
+bool merged_stride(int rank, const uint64_t *ext, const int *axes, int n, int64_t *stride) {
+  int64_t st[16];
+  int64_t acc = 1;
+  for (int i = rank - 1; i >= 0; --i) {
+    st[i] = acc;
+    acc *= (int64_t)ext[i];
+  }
+  int prev = -1;
+  for (int i = 0; i < n; ++i) {
+    const int ax = axes[i];
+    if (ext[ax] == 1) continue;
+    if (prev >= 0 && st[prev] != st[ax] * (int64_t)ext[ax]) return false;
+    prev = ax;
+  }
+  *stride = prev >= 0 ? st[prev] : 0;
+  return true;
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+template <typename T>
+int upload(DevBuf &d, const std::vector<T> &v) {
+  if (v.empty()) return CXA_OK;
+  CXA_HIP(hipMalloc(&d.p, v.size() * sizeof(T)));
+  CXA_HIP(hipMemcpy(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return CXA_OK;
+}
+
+struct CopyJob {
+  int64_t src_off;  // element offset of the source block
+  int64_t dst_off;  // element offset in the workspace
+  int rank;
+  uint64_t shape[16];
+  uint64_t mapper[16];
+};
+
+}  // namespace
+}  // namespace cxa
+
+struct cxa_bc_plan {
+  int dtype = 0;
+  int es = 0;
+  int device = 0;
+  std::vector<cxa::GBlk> blks;
+  std::vector<cxa::GSeg> segs;
+  std::vector<cxa::GTile> tiles;
+  cxa::DevBuf d_blks, d_segs, d_tiles;
+  bool copyA = false, copyB = false;
+  std::vector<cxa::CopyJob> jobsA, jobsB;
+  int64_t wsA = 0, wsB = 0;  // workspace elements for A and B copies
+  double flops = 0;
+  int64_t npairs = 0;
+  int tile_kind = -1;
+};
+
+namespace cxa {
+namespace {
+
+int upload_plan(cxa_bc_plan *p) {
+  make_tiles(p->blks, p->segs, p->tiles, p->tile_kind);
+  for (auto *d : {&p->d_blks, &p->d_segs, &p->d_tiles}) {
+    if (d->p) {
+      CXA_HIP(hipFree(d->p));
+      d->p = nullptr;
+    }
+  }
+  int rc;
+  if ((rc = upload(p->d_blks, p->blks)) || (rc = upload(p->d_segs, p->segs)) ||
+      (rc = upload(p->d_tiles, p->tiles)))
+    return rc;
+  return CXA_OK;
+}
+
+}  // namespace
+}  // namespace cxa
+
+using namespace cxa;
+
+// ------------------------------------------------------------------------------------------
+// Block-sparse contraction planner.  Pairing follows BlockUniTensor::contract
+// (src/BlockUniTensor.cpp:948-1228): left blocks in order, right blocks whose common-qindex
+// tuple matches (in order), target = the output block with itoi (L free ++ R free).  Each pair
+// becomes one GEMM segment of its target block: M = prod(L free dims), K = prod(common dims),
+// N = prod(R free dims).  Operands are read through strides when the permuted block is still a
+// strided matrix; otherwise the plan copies the operand blocks to the workspace first.
+// ------------------------------------------------------------------------------------------
+extern "C" int cxa_block_contract_plan(int dtype, int rankL, int nbL, const uint64_t *itoiL,
+                                       const uint64_t *shapeL, const int64_t *offL, int rankR,
+                                       int nbR, const uint64_t *itoiR, const uint64_t *shapeR,
+                                       const int64_t *offR, int ncomm, const uint64_t *comm1,
+                                       const uint64_t *comm2, int nbC, const uint64_t *itoiC,
+                                       const uint64_t *shapeC, const int64_t *offC,
+                                       cxa_bc_plan **out) {
+  CXA_CHECK(out, CXA_ERR_INVALID, "cxa_block_contract_plan: null output handle");
+  *out = nullptr;
+  const int es = elem_size(dtype);
+  CXA_CHECK(dtype == CXA_F64 || dtype == CXA_F32 || dtype == CXA_C128 || dtype == CXA_C64,
+            CXA_ERR_UNSUPPORTED, "block contract: unsupported dtype %d", dtype);
+  CXA_CHECK(rankL >= 0 && rankL <= 16 && rankR >= 0 && rankR <= 16, CXA_ERR_UNSUPPORTED,
+            "block contract: rank above 16");
+  CXA_CHECK(ncomm >= 0 && ncomm <= rankL && ncomm <= rankR, CXA_ERR_INVALID,
+            "block contract: %d common legs for ranks %d and %d", ncomm, rankL, rankR);
+  bool isc1[16] = {false}, isc2[16] = {false};
+  for (int c = 0; c < ncomm; ++c) {
+    CXA_CHECK(comm1[c] < (uint64_t)rankL && comm2[c] < (uint64_t)rankR && !isc1[comm1[c]] &&
+                  !isc2[comm2[c]],
+              CXA_ERR_INVALID, "block contract: invalid common index list");
+    isc1[comm1[c]] = true;
+    isc2[comm2[c]] = true;
+  }
+  int nc1[16], nc2[16], n1 = 0, n2 = 0;
+  for (int i = 0; i < rankL; ++i)
+    if (!isc1[i]) nc1[n1++] = i;
+  for (int i = 0; i < rankR; ++i)
+    if (!isc2[i]) nc2[n2++] = i;
+  const int rankC = n1 + n2;
+
+  std::unique_ptr<cxa_bc_plan> p(new cxa_bc_plan());
+  p->dtype = dtype;
+  p->es = es;
+  CXA_HIP(hipGetDevice(&p->device));
+
+  // right blocks by common-qindex tuple (ascending block order), output blocks by itoi
+  std::map<std::vector<uint64_t>, std::vector<int>> mp;
+  for (int b = 0; b < nbR; ++b) {
+    std::vector<uint64_t> key(ncomm);
+    for (int c = 0; c < ncomm; ++c) key[c] = itoiR[(int64_t)b * rankR + comm2[c]];
+    mp[key].push_back(b);
+  }
+  std::map<std::vector<uint64_t>, int> mpC;
+  for (int t = 0; t < nbC; ++t)
+    mpC[std::vector<uint64_t>(itoiC + (int64_t)t * rankC, itoiC + (int64_t)t * rankC + rankC)] =
+        t;
+
+  struct Pair {
+    int a, b, t;
+    int64_t M, K, N;
+  };
+  std::vector<Pair> pairs;
+  for (int a = 0; a < nbL; ++a) {
+    std::vector<uint64_t> key(ncomm);
+    for (int c = 0; c < ncomm; ++c) key[c] = itoiL[(int64_t)a * rankL + comm1[c]];
+    auto it = mp.find(key);
+    if (it == mp.end()) continue;
+    const uint64_t *sa = shapeL + (int64_t)a * rankL;
+    int64_t M = 1, K = 1;
+    for (int i = 0; i < n1; ++i) M *= (int64_t)sa[nc1[i]];
+    for (int c = 0; c < ncomm; ++c) K *= (int64_t)sa[comm1[c]];
+    for (int b : it->second) {
+      const uint64_t *sb = shapeR + (int64_t)b * rankR;
+      int64_t N = 1, Kb = 1;
+      for (int i = 0; i < n2; ++i) N *= (int64_t)sb[nc2[i]];
+      for (int c = 0; c < ncomm; ++c) {
+        CXA_CHECK(sb[comm2[c]] == sa[comm1[c]], CXA_ERR_INVALID,
+                  "block contract: contracted dimension mismatch (%llu vs %llu)",
+                  (unsigned long long)sa[comm1[c]], (unsigned long long)sb[comm2[c]]);
+        Kb *= (int64_t)sb[comm2[c]];
+      }
+      std::vector<uint64_t> tk(rankC);
+      for (int i = 0; i < n1; ++i) tk[i] = itoiL[(int64_t)a * rankL + nc1[i]];
+      for (int i = 0; i < n2; ++i) tk[n1 + i] = itoiR[(int64_t)b * rankR + nc2[i]];
+      auto ct = mpC.find(tk);
+      CXA_CHECK(ct != mpC.end(), CXA_ERR_INVALID,
+                "block contract: pair (%d,%d) has no output block (flux rule violated)", a, b);
+      pairs.push_back({a, b, ct->second, M, K, N});
+      (void)Kb;
+    }
+  }
+  p->npairs = (int64_t)pairs.size();
+
+  // can every operand be read in place?
+  int axA[32], axB[32];
+  for (int i = 0; i < n1; ++i) axA[i] = nc1[i];
+  for (int c = 0; c < ncomm; ++c) axA[n1 + c] = (int)comm1[c];
+  for (int c = 0; c < ncomm; ++c) axB[c] = (int)comm2[c];
+  for (int i = 0; i < n2; ++i) axB[ncomm + i] = nc2[i];
+  std::vector<int64_t> sam(nbL), sak(nbL), sbk(nbR), sbn(nbR);
+  std::vector<char> usedA(nbL, 0), usedB(nbR, 0);
+  for (const Pair &pr : pairs) {
+    usedA[pr.a] = 1;
+    usedB[pr.b] = 1;
+  }
+  for (int a = 0; a < nbL; ++a) {
+    if (!usedA[a]) continue;
+    const uint64_t *sa = shapeL + (int64_t)a * rankL;
+    if (!merged_stride(rankL, sa, axA, n1, &sam[a]) ||
+        !merged_stride(rankL, sa, axA + n1, ncomm, &sak[a]))
+      p->copyA = true;
+  }
+  for (int b = 0; b < nbR; ++b) {
+    if (!usedB[b]) continue;
+    const uint64_t *sb = shapeR + (int64_t)b * rankR;
+    if (!merged_stride(rankR, sb, axB, ncomm, &sbk[b]) ||
+        !merged_stride(rankR, sb, axB + ncomm, n2, &sbn[b]))
+      p->copyB = true;
+  }
+  std::vector<int64_t> wsoffA(nbL, 0), wsoffB(nbR, 0);
+  if (p->copyA) {
+    for (int a = 0; a < nbL; ++a) {
+      if (!usedA[a]) continue;
+      const uint64_t *sa = shapeL + (int64_t)a * rankL;
+      CopyJob j{};
+      j.src_off = offL[a];
+      j.dst_off = p->wsA;
+      j.rank = rankL;
+      int64_t M = 1, K = 1, n = 1;
+      for (int i = 0; i < rankL; ++i) {
+        j.shape[i] = sa[i];
+        j.mapper[i] = (uint64_t)axA[i];
+        n *= (int64_t)sa[i];
+      }
+      for (int i = 0; i < n1; ++i) M *= (int64_t)sa[nc1[i]];
+      for (int c = 0; c < ncomm; ++c) K *= (int64_t)sa[comm1[c]];
+      wsoffA[a] = p->wsA;
+      sam[a] = K;
+      sak[a] = 1;
+      p->wsA += (n + 15) / 16 * 16;
+      p->jobsA.push_back(j);
+      (void)M;
+    }
+  }
+  if (p->copyB) {
+    for (int b = 0; b < nbR; ++b) {
+      if (!usedB[b]) continue;
+      const uint64_t *sb = shapeR + (int64_t)b * rankR;
+      CopyJob j{};
+      j.src_off = offR[b];
+      j.dst_off = p->wsB;
+      j.rank = rankR;
+      int64_t N = 1, n = 1;
+      for (int i = 0; i < rankR; ++i) {
+        j.shape[i] = sb[i];
+        j.mapper[i] = (uint64_t)axB[i];
+        n *= (int64_t)sb[i];
+      }
+      for (int i = 0; i < n2; ++i) N *= (int64_t)sb[nc2[i]];
+      wsoffB[b] = p->wsB;
+      sbk[b] = N;
+      sbn[b] = 1;
+      p->wsB += (n + 15) / 16 * 16;
+      p->jobsB.push_back(j);
+    }
+  }
+
+  // one GBlk per output block, segments grouped by target in pair order
+  std::vector<std::vector<int>> by_target(nbC);
+  for (size_t i = 0; i < pairs.size(); ++i) by_target[pairs[i].t].push_back((int)i);
+  p->blks.resize(nbC);
+  const double cplx = is_complex(dtype) ? 4.0 : 1.0;
+  for (int t = 0; t < nbC; ++t) {
+    GBlk &g = p->blks[t];
+    const uint64_t *sc = shapeC + (int64_t)t * rankC;
+    int64_t M = 1, N = 1;
+    for (int i = 0; i < n1; ++i) M *= (int64_t)sc[i];
+    for (int i = 0; i < n2; ++i) N *= (int64_t)sc[n1 + i];
+    CXA_CHECK(M <= INT32_MAX && N <= INT32_MAX, CXA_ERR_UNSUPPORTED,
+              "block contract: output block too large");
+    g.c_off = offC[t];
+    g.ldc = N;
+    g.alpha = 1.0;
+    g.beta = 0.0;
+    g.M = (int32_t)M;
+    g.N = (int32_t)N;
+    g.seg0 = (int32_t)p->segs.size();
+    g.nseg = (int32_t)by_target[t].size();
+    for (int pi : by_target[t]) {
+      const Pair &pr = pairs[pi];
+      CXA_CHECK(pr.M == M && pr.N == N, CXA_ERR_INVALID,
+                "block contract: output block %d shape does not match its pair (%lld x %lld)", t,
+                (long long)pr.M, (long long)pr.N);
+      CXA_CHECK(pr.K <= INT32_MAX, CXA_ERR_UNSUPPORTED, "block contract: K too large");
+      GSeg s{};
+      s.a_off = p->copyA ? wsoffA[pr.a] : offL[pr.a];
+      s.b_off = p->copyB ? wsoffB[pr.b] : offR[pr.b];
+      s.sam = sam[pr.a];
+      s.sak = sak[pr.a];
+      s.sbk = sbk[pr.b];
+      s.sbn = sbn[pr.b];
+      s.K = (int32_t)pr.K;
+      p->segs.push_back(s);
+      p->flops += 2.0 * cplx * (double)pr.M * (double)pr.N * (double)pr.K;
+    }
+  }
+  int rc = upload_plan(p.get());
+  if (rc) return rc;
+  *out = p.release();
+  return CXA_OK;
+}
+
+extern "C" int cxa_block_contract_info(const cxa_bc_plan *p, double *flops,
+                                       int64_t *workspace_bytes, int64_t *npairs) {
+  CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_info: null plan");
+  if (flops) *flops = p->flops;
+  if (workspace_bytes) *workspace_bytes = (p->wsA + p->wsB) * p->es;
+  if (npairs) *npairs = p->npairs;
+  return CXA_OK;
+}
+
+extern "C" int cxa_block_contract_set_tile(cxa_bc_plan *p, int kind) {
+  CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_set_tile: null plan");
+  CXA_CHECK(kind >= -1 && kind < TK_COUNT, CXA_ERR_INVALID, "unknown tile kind %d", kind);
+  p->tile_kind = kind;
+  return upload_plan(p);
+}
+
+extern "C" int cxa_block_contract_exec(const cxa_bc_plan *p, const void *L, const void *R,
+                                       void *C, void *workspace, int prec, cxa_stream_t stream) {
+  CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_exec: null plan");
+  hipStream_t s = (hipStream_t)stream;
+  const int es = p->es;
+  CXA_CHECK(!(p->copyA || p->copyB) || workspace, CXA_ERR_INVALID,
+            "cxa_block_contract_exec: this plan needs a workspace of %lld bytes",
+            (long long)((p->wsA + p->wsB) * es));
+  char *ws = (char *)workspace;
+  for (const CopyJob &j : p->jobsA) {
+    int rc = cxa_permute(p->dtype, j.rank, j.shape, j.mapper, (const char *)L + j.src_off * es,
+                         ws + j.dst_off * es, stream);
+    if (rc) return rc;
+  }
+  char *wsB = ws ? ws + p->wsA * es : nullptr;
+  for (const CopyJob &j : p->jobsB) {
+    int rc = cxa_permute(p->dtype, j.rank, j.shape, j.mapper, (const char *)R + j.src_off * es,
+                         wsB + j.dst_off * es, stream);
+    if (rc) return rc;
+  }
+  int prec_eff = prec;
+  if (is_complex(p->dtype)) prec_eff = p->dtype == CXA_C128 ? CXA_PREC_FP64 : CXA_PREC_FP32;
+  if (p->dtype == CXA_F32 && prec == CXA_PREC_FP64) prec_eff = CXA_PREC_FP32;
+  GemmLaunch g;
+  g.prec = prec_eff;
+  g.a_dt = g.b_dt = g.c_dt = p->dtype;
+  g.A = p->copyA ? (const void *)ws : L;
+  g.B = p->copyB ? (const void *)wsB : R;
+  g.C = C;
+  g.tiles = (const GTile *)p->d_tiles.p;
+  g.blks = (const GBlk *)p->d_blks.p;
+  g.segs = (const GSeg *)p->d_segs.p;
+  g.ntiles = (int)p->tiles.size();
+  return launch_grouped_gemm(g, s);
+}
+
+extern "C" int cxa_block_contract_destroy(cxa_bc_plan *p) {
+  delete p;
+  return CXA_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Dense row-major matmul: out(Ml x Nr) = inl(Ml x Comm) . inr(Comm x Nr).
+// ------------------------------------------------------------------------------------------
+extern "C" int cxa_matmul(int dtype, void *out, const void *inl, const void *inr, int64_t Ml,
+                          int64_t Comm, int64_t Nr, cxa_stream_t stream) {
+  CXA_CHECK(dtype == CXA_F64 || dtype == CXA_F32 || dtype == CXA_C128 || dtype == CXA_C64,
+            CXA_ERR_UNSUPPORTED, "cxa_matmul: unsupported dtype %d", dtype);
+  CXA_CHECK(Ml >= 0 && Comm >= 0 && Nr >= 0, CXA_ERR_INVALID, "cxa_matmul: negative size");
+  CXA_CHECK(Ml <= INT32_MAX && Nr <= INT32_MAX && Comm <= INT32_MAX, CXA_ERR_UNSUPPORTED,
+            "cxa_matmul: dimension above 2^31");
+  if (Ml == 0 || Nr == 0) return CXA_OK;
+  GemmLaunch g;
+  g.single = true;
+  g.prec = (dtype == CXA_F64 || dtype == CXA_C128) ? CXA_PREC_FP64 : CXA_PREC_FP32;
+  g.a_dt = g.b_dt = g.c_dt = dtype;
+  g.A = inl;
+  g.B = inr;
+  g.C = out;
+  g.blk0 = GBlk{0, Nr, 1.0, 0.0, (int32_t)Ml, (int32_t)Nr, 0, Comm > 0 ? 1 : 0};
+  g.seg0 = GSeg{0, 0, Comm, 1, Nr, 1, (int32_t)Comm, 0};
+  return launch_grouped_gemm(g, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------------------------------
+// Batched GEMM, BLAS column-major convention (the reference's Gemm_Batchfunc_oii):
+// for group g and each of its group_size[g] matrices,
+//   C = alpha[g] * op(A) * op(B) + beta[g] * C   (column-major, leading dims lda/ldb/ldc)
+// Mapped to row-major by computing C^T = op(B)^T op(A)^T.  One grouped launch; the task list is
+// uploaded and the call returns after the launch has been enqueued and the upload completed.
+// ------------------------------------------------------------------------------------------
+extern "C" int cxa_gemm_batch(int dtype, const char *transa, const char *transb,
+                              const int64_t *m, const int64_t *n, const int64_t *k,
+                              const double *alpha, const void *const *a, const int64_t *lda,
+                              const void *const *b, const int64_t *ldb, const double *beta,
+                              void *const *c, const int64_t *ldc, int64_t group_count,
+                              const int64_t *group_size, cxa_stream_t stream) {
+  CXA_CHECK(dtype == CXA_F64 || dtype == CXA_F32 || dtype == CXA_C128 || dtype == CXA_C64,
+            CXA_ERR_UNSUPPORTED, "cxa_gemm_batch: unsupported dtype %d", dtype);
+  CXA_CHECK(group_count >= 0, CXA_ERR_INVALID, "cxa_gemm_batch: negative group count");
+  std::vector<GBlk> blks;
+  std::vector<GSeg> segs;
+  const int es = elem_size(dtype);
+  int64_t idx = 0;
+  for (int64_t gi = 0; gi < group_count; ++gi) {
+    const char ta = transa[gi], tb = transb[gi];
+    CXA_CHECK((ta == 'N' || ta == 'T') && (tb == 'N' || tb == 'T'), CXA_ERR_UNSUPPORTED,
+              "cxa_gemm_batch: transpose flags must be 'N' or 'T' (got %c,%c)", ta, tb);
+    CXA_CHECK(m[gi] >= 0 && n[gi] >= 0 && k[gi] >= 0 && group_size[gi] >= 0, CXA_ERR_INVALID,
+              "cxa_gemm_batch: negative size in group %lld", (long long)gi);
+    CXA_CHECK(m[gi] <= INT32_MAX && n[gi] <= INT32_MAX && k[gi] <= INT32_MAX,
+              CXA_ERR_UNSUPPORTED, "cxa_gemm_batch: dimension above 2^31");
+    for (int64_t j = 0; j < group_size[gi]; ++j, ++idx) {
+      // row-major C'(n x m) = A'(n x k) . B'(k x m) with A' = op(B)^T, B' = op(A)^T
+      CXA_CHECK((uintptr_t)c[idx] % es == 0 && (uintptr_t)a[idx] % es == 0 &&
+                    (uintptr_t)b[idx] % es == 0,
+                CXA_ERR_INVALID, "cxa_gemm_batch: matrix %lld is not element-aligned",
+                (long long)idx);
+      GBlk g{};
+      g.c_off = (int64_t)((uintptr_t)c[idx] / es);
+      g.ldc = ldc[gi];
+      g.alpha = alpha[gi];
+      g.beta = beta[gi];
+      g.M = (int32_t)n[gi];
+      g.N = (int32_t)m[gi];
+      g.seg0 = (int32_t)segs.size();
+      g.nseg = k[gi] > 0 ? 1 : 0;
+      blks.push_back(g);
+      if (g.nseg) {
+        GSeg s{};
+        s.a_off = (int64_t)((uintptr_t)b[idx] / es);
+        s.b_off = (int64_t)((uintptr_t)a[idx] / es);
+        if (tb == 'N') {  // op(B)[kk][nn] = B[kk + nn*ldb]
+          s.sam = ldb[gi];
+          s.sak = 1;
+        } else {  // op(B)[kk][nn] = B[nn + kk*ldb]
+          s.sam = 1;
+          s.sak = ldb[gi];
+        }
+        if (ta == 'N') {  // op(A)[mm][kk] = A[mm + kk*lda]
+          s.sbk = lda[gi];
+          s.sbn = 1;
+        } else {  // op(A)[mm][kk] = A[kk + mm*lda]
+          s.sbk = 1;
+          s.sbn = lda[gi];
+        }
+        s.K = (int32_t)k[gi];
+        segs.push_back(s);
+      }
+    }
+  }
+  std::vector<GTile> tiles;
+  make_tiles(blks, segs, tiles);
+  if (tiles.empty()) return CXA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  DevBuf d_blks, d_segs, d_tiles;
+  int rc;
+  if ((rc = upload(d_blks, blks)) || (rc = upload(d_segs, segs)) || (rc = upload(d_tiles, tiles)))
+    return rc;
+  GemmLaunch g;
+  g.prec = (dtype == CXA_F64 || dtype == CXA_C128) ? CXA_PREC_FP64 : CXA_PREC_FP32;
+  g.a_dt = g.b_dt = g.c_dt = dtype;
+  g.A = nullptr;  // absolute addresses (in elements) are carried in the offsets
+  g.B = nullptr;
+  g.C = nullptr;
+  g.tiles = (const GTile *)d_tiles.p;
+  g.blks = (const GBlk *)d_blks.p;
+  g.segs = (const GSeg *)d_segs.p;
+  g.ntiles = (int)tiles.size();
+  rc = launch_grouped_gemm(g, s);
+  if (rc) return rc;
+  // the task arrays are freed on return: wait for the launch that reads them
+  CXA_HIP(hipStreamSynchronize(s));
+  return CXA_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Dense tensordot: out = sum over idxl/idxr of L * R, output axes = L free then R free
+// (the decomposition of _Tensordot_generic, src/linalg/Tensordot.cpp:29-110).  Operands that
+// stay strided matrices under the permutation are read in place (the permute is fused into
+// the GEMM loads); others are permuted into the caller's workspace first.
+// ------------------------------------------------------------------------------------------
+extern "C" int cxa_tensordot_workspace(int dtype, int rankL, const uint64_t *shapeL, int rankR,
+                                       const uint64_t *shapeR, int ncomm, const uint64_t *idxl,
+                                       const uint64_t *idxr, int64_t *bytes) {
+  const int es = elem_size(dtype);
+  CXA_CHECK(es > 0 && bytes, CXA_ERR_INVALID, "cxa_tensordot_workspace: bad arguments");
+  CXA_CHECK(rankL <= 16 && rankR <= 16 && ncomm <= rankL && ncomm <= rankR, CXA_ERR_INVALID,
+            "cxa_tensordot_workspace: bad ranks");
+  bool isc1[16] = {false}, isc2[16] = {false};
+  for (int c = 0; c < ncomm; ++c) {
+    CXA_CHECK(idxl[c] < (uint64_t)rankL && idxr[c] < (uint64_t)rankR, CXA_ERR_INVALID,
+              "cxa_tensordot_workspace: axis out of range");
+    isc1[idxl[c]] = isc2[idxr[c]] = true;
+  }
+  int axA[32], axB[32], n1 = 0, n2 = 0;
+  for (int i = 0; i < rankL; ++i)
+    if (!isc1[i]) axA[n1++] = i;
+  for (int c = 0; c < ncomm; ++c) axA[n1 + c] = (int)idxl[c];
+  for (int c = 0; c < ncomm; ++c) axB[c] = (int)idxr[c];
+  for (int i = 0; i < rankR; ++i)
+    if (!isc2[i]) axB[ncomm + n2++] = i;
+  int64_t s0, s1, need = 0, nL = 1, nR = 1;
+  for (int i = 0; i < rankL; ++i) nL *= (int64_t)shapeL[i];
+  for (int i = 0; i < rankR; ++i) nR *= (int64_t)shapeR[i];
+  if (!merged_stride(rankL, shapeL, axA, n1, &s0) ||
+      !merged_stride(rankL, shapeL, axA + n1, ncomm, &s1))
+    need += (nL + 15) / 16 * 16;
+  if (!merged_stride(rankR, shapeR, axB, ncomm, &s0) ||
+      !merged_stride(rankR, shapeR, axB + ncomm, n2, &s1))
+    need += (nR + 15) / 16 * 16;
+  *bytes = need * es;
+  return CXA_OK;
+}
+
+extern "C" int cxa_tensordot(int dtype, void *out, const void *L, int rankL,
+                             const uint64_t *shapeL, const void *R, int rankR,
+                             const uint64_t *shapeR, int ncomm, const uint64_t *idxl,
+                             const uint64_t *idxr, void *workspace, cxa_stream_t stream) {
+  CXA_CHECK(dtype == CXA_F64 || dtype == CXA_F32 || dtype == CXA_C128 || dtype == CXA_C64,
+            CXA_ERR_UNSUPPORTED, "cxa_tensordot: unsupported dtype %d", dtype);
+  CXA_CHECK(rankL <= 16 && rankR <= 16, CXA_ERR_UNSUPPORTED, "cxa_tensordot: rank above 16");
+  CXA_CHECK(ncomm >= 1 && ncomm <= rankL && ncomm <= rankR, CXA_ERR_INVALID,
+            "cxa_tensordot: index lists must be non-empty and fit the ranks");
+  const int es = elem_size(dtype);
+  bool isc1[16] = {false}, isc2[16] = {false};
+  for (int c = 0; c < ncomm; ++c) {
+    CXA_CHECK(idxl[c] < (uint64_t)rankL && idxr[c] < (uint64_t)rankR, CXA_ERR_INVALID,
+              "cxa_tensordot: axis out of range");
+    CXA_CHECK(!isc1[idxl[c]] && !isc2[idxr[c]], CXA_ERR_INVALID,
+              "cxa_tensordot: duplicated axis");
+    CXA_CHECK(shapeL[idxl[c]] == shapeR[idxr[c]], CXA_ERR_INVALID,
+              "cxa_tensordot: contracted dimensions differ (%llu vs %llu)",
+              (unsigned long long)shapeL[idxl[c]], (unsigned long long)shapeR[idxr[c]]);
+    isc1[idxl[c]] = isc2[idxr[c]] = true;
+  }
+  int axA[32], axB[32], n1 = 0, n2 = 0;
+  int64_t M = 1, K = 1, N = 1, nL = 1, nR = 1;
+  for (int i = 0; i < rankL; ++i) {
+    nL *= (int64_t)shapeL[i];
+    if (!isc1[i]) {
+      axA[n1++] = i;
+      M *= (int64_t)shapeL[i];
+    }
+  }
+  for (int c = 0; c < ncomm; ++c) {
+    axA[n1 + c] = (int)idxl[c];
+    K *= (int64_t)shapeL[idxl[c]];
+  }
+  for (int c = 0; c < ncomm; ++c) axB[c] = (int)idxr[c];
+  for (int i = 0; i < rankR; ++i) {
+    nR *= (int64_t)shapeR[i];
+    if (!isc2[i]) {
+      axB[ncomm + n2++] = i;
+      N *= (int64_t)shapeR[i];
+    }
+  }
+  CXA_CHECK(M <= INT32_MAX && N <= INT32_MAX && K <= INT32_MAX, CXA_ERR_UNSUPPORTED,
+            "cxa_tensordot: matricized dimension above 2^31");
+  if (M == 0 || N == 0) return CXA_OK;
+  int64_t sam = 0, sak = 0, sbk = 0, sbn = 0;
+  const void *A = L, *B = R;
+  char *ws = (char *)workspace;
+  int64_t used = 0;
+  if (!merged_stride(rankL, shapeL, axA, n1, &sam) ||
+      !merged_stride(rankL, shapeL, axA + n1, ncomm, &sak)) {
+    CXA_CHECK(ws, CXA_ERR_INVALID, "cxa_tensordot: operand L needs a workspace");
+    uint64_t mp[16];
+    for (int i = 0; i < rankL; ++i) mp[i] = (uint64_t)axA[i];
+    int rc = cxa_permute(dtype, rankL, shapeL, mp, L, ws, stream);
+    if (rc) return rc;
+    A = ws;
+    sam = K;
+    sak = 1;
+    used = (nL + 15) / 16 * 16;
+  }
+  if (!merged_stride(rankR, shapeR, axB, ncomm, &sbk) ||
+      !merged_stride(rankR, shapeR, axB + ncomm, n2, &sbn)) {
+    CXA_CHECK(ws, CXA_ERR_INVALID, "cxa_tensordot: operand R needs a workspace");
+    uint64_t mp[16];
+    for (int i = 0; i < rankR; ++i) mp[i] = (uint64_t)axB[i];
+    int rc = cxa_permute(dtype, rankR, shapeR, mp, R, ws + used * es, stream);
+    if (rc) return rc;
+    B = ws + used * es;
+    sbk = N;
+    sbn = 1;
+  }
+  GemmLaunch g;
+  g.single = true;
+  g.prec = (dtype == CXA_F64 || dtype == CXA_C128) ? CXA_PREC_FP64 : CXA_PREC_FP32;
+  g.a_dt = g.b_dt = g.c_dt = dtype;
+  g.A = A;
+  g.B = B;
+  g.C = out;
+  g.blk0 = GBlk{0, N, 1.0, 0.0, (int32_t)M, (int32_t)N, 0, K > 0 ? 1 : 0};
+  g.seg0 = GSeg{0, 0, sam, sak, sbk, sbn, (int32_t)K, 0};
+  return launch_grouped_gemm(g, (hipStream_t)stream);
+}

@@ -9,8 +9,8 @@
 // chunk takes k = k0 + 4*(lane/16) + j for both A (row lane%16) and B (column lane%16).  The
 // MFMA sums over its 4 k slots, so any assignment that is identical for A and B is exact; this
 // one makes row-major A loads 32-byte contiguous per lane and B loads 128-byte contiguous per
-// 16 lanes.  Accumulator layout of v_mfma_*_16x16x4: lane holds rows 4*(lane/16)+v, column
-// lane%16, v = 0..3.
+// 16 lanes.  Accumulators: lane holds column lane%16 of four rows, whose order differs
+// between the FP64 and FP32 instructions (see acc_row).
 //
 // Precision modes: FP64 (v_mfma_f64_16x16x4_f64), FP32 (v_mfma_f32_16x16x4_f32, operands
 // rounded to FP32), FP32X3 (FP64 operands split x = hi + lo in FP32; hi*hi + hi*lo + lo*hi on
@@ -40,20 +40,44 @@ __device__ __forceinline__ f32x4 mma32(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// Accumulator row held in register v by lane group g = lane/16 (column = lane%16), measured on
+// gfx950: v_mfma_f64_16x16x4_f64 interleaves rows (g + 4v), v_mfma_f32_16x16x4_f32 blocks
+// them (4g + v).
+__device__ __forceinline__ int acc_row(bool f64, int g, int v) {
+  return f64 ? g + 4 * v : 4 * g + v;
+}
+
 template <typename T>
 __device__ __forceinline__ double ld_real(const T *p) {
   return (double)(*p);
 }
 
+// base + element offset computed on integers: a batched call passes absolute addresses (in
+// elements) as offsets from a null base, which pointer arithmetic may not express.  Offsets in
+// elements let one task list serve FP64 and FP32 copies of the same block layout.
+template <typename T, typename B>
+__device__ __forceinline__ T *at(B *base, int64_t elem_off) {
+  return reinterpret_cast<T *>(reinterpret_cast<uintptr_t>(base) +
+                               (uintptr_t)elem_off * (uintptr_t)sizeof(T));
+}
+
+struct SegArray {
+  const GSeg *p;
+  __device__ __forceinline__ GSeg operator()(int i) const { return p[i]; }
+};
+struct SegOne {
+  GSeg s;
+  __device__ __forceinline__ GSeg operator()(int) const { return s; }
+};
+
 // ---------------------------------------------------------------------------------------
 // real wave tile
 // ---------------------------------------------------------------------------------------
-template <int PREC, typename TA, typename TB, typename TC, int MI, int NI>
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, typename SEGS>
 __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
                                                const char *__restrict__ Bbase,
                                                char *__restrict__ Cbase, const GBlk &blk,
-                                               const GSeg *__restrict__ segs, int m0, int n0,
-                                               int lane) {
+                                               SEGS segs, int m0, int n0, int lane) {
   constexpr bool F64 = (PREC == CXA_PREC_FP64);
   constexpr bool X3 = (PREC == CXA_PREC_FP32X3);
   using acc_t = typename std::conditional<F64, f64x4, f32x4>::type;
@@ -72,9 +96,9 @@ __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
     }
 
   for (int s = 0; s < blk.nseg; ++s) {
-    const GSeg sg = segs[blk.seg0 + s];
-    const TA *Ap = reinterpret_cast<const TA *>(Abase + sg.a_off) + (int64_t)m0 * sg.sam;
-    const TB *Bp = reinterpret_cast<const TB *>(Bbase + sg.b_off) + (int64_t)n0 * sg.sbn;
+    const GSeg sg = segs(blk.seg0 + s);
+    const TA *Ap = at<const TA>(Abase, sg.a_off) + (int64_t)m0 * sg.sam;
+    const TB *Bp = at<const TB>(Bbase, sg.b_off) + (int64_t)n0 * sg.sbn;
     const int K = sg.K;
     for (int k0 = 0; k0 < K; k0 += 16) {
       op_t a[MI][4], b[NI][4], alo[MI][4], blo[NI][4];
@@ -123,7 +147,7 @@ __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
     }
   }
 
-  TC *Cp = reinterpret_cast<TC *>(Cbase + blk.c_off);
+  TC *Cp = at<TC>(Cbase, blk.c_off);
   const double alpha = blk.alpha, beta = blk.beta;
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -131,7 +155,7 @@ __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
     for (int jj = 0; jj < NI; ++jj)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = 16 * i + 4 * g + v;
+        const int row = 16 * i + acc_row(F64, g, v);
         const int col = 16 * jj + r;
         if (row < mt && col < nt) {
           double val = (double)acc[i][jj][v];
@@ -147,12 +171,11 @@ __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
 // ---------------------------------------------------------------------------------------
 // complex wave tile (C128 on FP64 MFMA, C64 on FP32 MFMA): four real products per step
 // ---------------------------------------------------------------------------------------
-template <typename CT, int MI, int NI>
+template <typename CT, int MI, int NI, typename SEGS>
 __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
                                                const char *__restrict__ Bbase,
                                                char *__restrict__ Cbase, const GBlk &blk,
-                                               const GSeg *__restrict__ segs, int m0, int n0,
-                                               int lane) {
+                                               SEGS segs, int m0, int n0, int lane) {
   constexpr bool F64 = std::is_same<CT, cd>::value;
   using acc_t = typename std::conditional<F64, f64x4, f32x4>::type;
   using op_t = typename std::conditional<F64, double, float>::type;
@@ -168,9 +191,9 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
       acci[i][j] = acc_t{0, 0, 0, 0};
     }
   for (int s = 0; s < blk.nseg; ++s) {
-    const GSeg sg = segs[blk.seg0 + s];
-    const CT *Ap = reinterpret_cast<const CT *>(Abase + sg.a_off) + (int64_t)m0 * sg.sam;
-    const CT *Bp = reinterpret_cast<const CT *>(Bbase + sg.b_off) + (int64_t)n0 * sg.sbn;
+    const GSeg sg = segs(blk.seg0 + s);
+    const CT *Ap = at<const CT>(Abase, sg.a_off) + (int64_t)m0 * sg.sam;
+    const CT *Bp = at<const CT>(Bbase, sg.b_off) + (int64_t)n0 * sg.sbn;
     const int K = sg.K;
     for (int k0 = 0; k0 < K; k0 += 16) {
       op_t ar[MI][4], ai[MI][4], br[NI][4], bi[NI][4];
@@ -218,7 +241,7 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
           }
     }
   }
-  CT *Cp = reinterpret_cast<CT *>(Cbase + blk.c_off);
+  CT *Cp = at<CT>(Cbase, blk.c_off);
   const double alpha = blk.alpha, beta = blk.beta;
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -226,7 +249,7 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
     for (int jj = 0; jj < NI; ++jj)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = 16 * i + 4 * g + v;
+        const int row = 16 * i + acc_row(F64, g, v);
         const int col = 16 * jj + r;
         if (row < mt && col < nt) {
           double re = alpha * (double)accr[i][jj][v], im = alpha * (double)acci[i][jj][v];
@@ -246,6 +269,29 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
 
 constexpr int kWaves = 4;
 
+template <int PREC, typename TA, typename TB, typename TC, bool CPLX, typename SEGS>
+__device__ __forceinline__ void run_tile(int kind, const char *A, const char *B, char *C,
+                                         const GBlk &blk, SEGS segs, int m0, int n0, int lane) {
+  if constexpr (CPLX) {
+    switch (kind) {
+      case TK_1x1: wave_tile_cplx<TA, 1, 1>(A, B, C, blk, segs, m0, n0, lane); break;
+      case TK_2x1: wave_tile_cplx<TA, 2, 1>(A, B, C, blk, segs, m0, n0, lane); break;
+      case TK_1x2: wave_tile_cplx<TA, 1, 2>(A, B, C, blk, segs, m0, n0, lane); break;
+      case TK_2x2: wave_tile_cplx<TA, 2, 2>(A, B, C, blk, segs, m0, n0, lane); break;
+      default: wave_tile_cplx<TA, 4, 2>(A, B, C, blk, segs, m0, n0, lane); break;
+    }
+  } else {
+    switch (kind) {
+      case TK_1x1: wave_tile_real<PREC, TA, TB, TC, 1, 1>(A, B, C, blk, segs, m0, n0, lane); break;
+      case TK_2x2: wave_tile_real<PREC, TA, TB, TC, 2, 2>(A, B, C, blk, segs, m0, n0, lane); break;
+      case TK_2x1: wave_tile_real<PREC, TA, TB, TC, 2, 1>(A, B, C, blk, segs, m0, n0, lane); break;
+      case TK_1x2: wave_tile_real<PREC, TA, TB, TC, 1, 2>(A, B, C, blk, segs, m0, n0, lane); break;
+      default: wave_tile_real<PREC, TA, TB, TC, 4, 2>(A, B, C, blk, segs, m0, n0, lane); break;
+    }
+  }
+}
+
+// grouped launch: tiles, blocks and segments come from device arrays
 template <int PREC, typename TA, typename TB, typename TC, bool CPLX>
 __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict__ tiles,
                                                         int ntiles, const GBlk *__restrict__ blks,
@@ -256,38 +302,45 @@ __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict
   for (int t = blockIdx.x * kWaves + wave; t < ntiles; t += gridDim.x * kWaves) {
     const GTile tl = tiles[t];
     const GBlk blk = blks[tl.blk];
-    if constexpr (CPLX) {
-      switch (tl.kind) {
-        case TK_1x1: wave_tile_cplx<TA, 1, 1>(A, B, C, blk, segs, tl.m0, tl.n0, lane); break;
-        case TK_2x2: wave_tile_cplx<TA, 2, 2>(A, B, C, blk, segs, tl.m0, tl.n0, lane); break;
-        case TK_2x1: wave_tile_cplx<TA, 2, 1>(A, B, C, blk, segs, tl.m0, tl.n0, lane); break;
-        case TK_1x2: wave_tile_cplx<TA, 1, 2>(A, B, C, blk, segs, tl.m0, tl.n0, lane); break;
-        default: wave_tile_cplx<TA, 2, 2>(A, B, C, blk, segs, tl.m0, tl.n0, lane); break;
-      }
-    } else {
-      switch (tl.kind) {
-        case TK_1x1:
-          wave_tile_real<PREC, TA, TB, TC, 1, 1>(A, B, C, blk, segs, tl.m0, tl.n0, lane);
-          break;
-        case TK_2x2:
-          wave_tile_real<PREC, TA, TB, TC, 2, 2>(A, B, C, blk, segs, tl.m0, tl.n0, lane);
-          break;
-        case TK_2x1:
-          wave_tile_real<PREC, TA, TB, TC, 2, 1>(A, B, C, blk, segs, tl.m0, tl.n0, lane);
-          break;
-        case TK_1x2:
-          wave_tile_real<PREC, TA, TB, TC, 1, 2>(A, B, C, blk, segs, tl.m0, tl.n0, lane);
-          break;
-        default:
-          wave_tile_real<PREC, TA, TB, TC, 4, 2>(A, B, C, blk, segs, tl.m0, tl.n0, lane);
-          break;
-      }
-    }
+    run_tile<PREC, TA, TB, TC, CPLX>(tl.kind, A, B, C, blk, SegArray{segs}, tl.m0, tl.n0, lane);
+  }
+}
+
+// single-block launch: one block with one segment, tiles derived from the launch index, so a
+// plain GEMM needs no device-side task list
+template <int PREC, typename TA, typename TB, typename TC, bool CPLX>
+__global__ __launch_bounds__(64 * kWaves) void k_single(GBlk blk, GSeg seg, int kind,
+                                                       int tiles_n, int ntiles, const char *A,
+                                                       const char *B, char *C) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tr = kind == TK_1x1 || kind == TK_1x2 ? 16 : (kind == TK_4x2 ? 64 : 32);
+  const int tc = kind == TK_1x1 || kind == TK_2x1 ? 16 : 32;
+  for (int t = blockIdx.x * kWaves + wave; t < ntiles; t += gridDim.x * kWaves) {
+    const int m0 = (t / tiles_n) * tr, n0 = (t % tiles_n) * tc;
+    run_tile<PREC, TA, TB, TC, CPLX>(kind, A, B, C, blk, SegOne{seg}, m0, n0, lane);
   }
 }
 
 template <int PREC, typename TA, typename TB, typename TC, bool CPLX>
 int launch_t(const GemmLaunch &g, hipStream_t s) {
+  if (g.single) {
+    const GBlk &b = g.blk0;
+    if (b.M <= 0 || b.N <= 0) return CXA_OK;
+    const int kind = g.kind0 >= 0 ? g.kind0 : default_kind(b.M, b.N);
+    const int tiles_m = (b.M + tile_rows(kind) - 1) / tile_rows(kind);
+    const int tiles_n = (b.N + tile_cols(kind) - 1) / tile_cols(kind);
+    const int64_t nt64 = (int64_t)tiles_m * tiles_n;
+    if (nt64 > INT32_MAX) return fail(CXA_ERR_UNSUPPORTED, "GEMM too large: %lld tiles",
+                                      (long long)nt64);
+    const int ntiles = (int)nt64;
+    const int grid = std::min((ntiles + kWaves - 1) / kWaves, 8192);
+    hipLaunchKernelGGL((k_single<PREC, TA, TB, TC, CPLX>), dim3(grid), dim3(64 * kWaves), 0, s,
+                       b, g.seg0, kind, tiles_n, ntiles, (const char *)g.A, (const char *)g.B,
+                       (char *)g.C);
+    CXA_HIP(hipGetLastError());
+    return CXA_OK;
+  }
   if (g.ntiles <= 0) return CXA_OK;
   int grid = g.grid;
   if (grid <= 0) grid = std::min((g.ntiles + kWaves - 1) / kWaves, 4096);

@@ -18,7 +18,7 @@
 namespace cxa {
 
 struct GSeg {
-  int64_t a_off, b_off;  // byte offsets of A[0,0] / B[0,0] from the A / B base pointers
+  int64_t a_off, b_off;  // element offsets of A[0,0] / B[0,0] from the A / B base pointers
   int64_t sam, sak;      // element strides of A along m and k
   int64_t sbk, sbn;      // element strides of B along k and n
   int32_t K;
@@ -26,7 +26,7 @@ struct GSeg {
 };
 
 struct GBlk {
-  int64_t c_off;  // byte offset of C[0,0] from the C base pointer
+  int64_t c_off;  // element offset of C[0,0] from the C base pointer
   int64_t ldc;    // element stride of C along m (C is unit-stride along n)
   double alpha, beta;
   int32_t M, N;
@@ -58,6 +58,11 @@ struct GemmLaunch {
   const GSeg *segs = nullptr;
   int ntiles = 0;
   int grid = 0;  // workgroups; 0 = pick from ntiles
+  // single-block mode: one block with one segment passed by value, no device task arrays
+  bool single = false;
+  GBlk blk0{};
+  GSeg seg0{};
+  int kind0 = -1;
 };
 
 // Enqueue one grouped launch.  Returns a CXA status.

@@ -69,6 +69,98 @@ int cxa_abi_version(void);
 int cxa_permute(int dtype, int rank, const uint64_t *old_shape, const uint64_t *mapper,
                 const void *src, void *dst, cxa_stream_t stream);
 
+/*
+ * Dense row-major matrix product out(Ml x Nr) = inl(Ml x Comm) . inr(Comm x Nr), beta = 0.
+ * Replaces cuMatmul_internal_{d,f,cd,cf} (src/backend/linalg_internal_gpu/cuMatmul_internal.cu
+ * :25-103; type Matmulfunc_oii, src/backend/linalg_internal_interface.hpp:128-131), called
+ * from linalg::Matmul (src/linalg/Matmul.cpp:54-59).  dtype: F64, F32, C128, C64.  Comm may be
+ * 0 (out is zero-filled).  FP64 / C128 run on FP64 MFMA, F32 / C64 on FP32 MFMA.
+ */
+int cxa_matmul(int dtype, void *out, const void *inl, const void *inr, int64_t Ml, int64_t Comm,
+               int64_t Nr, cxa_stream_t stream);
+
+/*
+ * Batched GEMM in the BLAS column-major convention, one grouped launch for all matrices.
+ * Replaces cuGemm_Batch_internal_* (src/backend/linalg_internal_gpu/cuGemm_Batch_internal.cu
+ * :10-65; type Gemm_Batchfunc_oii, src/backend/linalg_internal_interface.hpp:100-105), called
+ * from linalg::Gemm_Batch (src/linalg/Gemm_Batch.cpp:225-230).  For group g and each of its
+ * group_size[g] matrices i (numbered consecutively across groups):
+ *   C_i = alpha[g] * op(A_i) * op(B_i) + beta[g] * C_i,  op = 'N' or 'T' (transa/transb[g]),
+ *   op(A_i) is m[g] x k[g], op(B_i) is k[g] x n[g], leading dimensions lda/ldb/ldc[g].
+ * alpha/beta are real.  a/b/c are host arrays of device pointers.  Synchronous with respect to
+ * the stream: returns after the launch that reads the uploaded task list has finished.
+ */
+int cxa_gemm_batch(int dtype, const char *transa, const char *transb, const int64_t *m,
+                   const int64_t *n, const int64_t *k, const double *alpha,
+                   const void *const *a, const int64_t *lda, const void *const *b,
+                   const int64_t *ldb, const double *beta, void *const *c, const int64_t *ldc,
+                   int64_t group_count, const int64_t *group_size, cxa_stream_t stream);
+
+/*
+ * Dense tensordot: out = sum over the paired axes idxl[i] <-> idxr[i] of L * R, with output
+ * axes = free axes of L (in order) then free axes of R (in order).
+ * Replaces cuTensordot_internal_* (src/backend/linalg_internal_gpu/cuTensordot_internal.cu
+ * :31-188; type Tensordotfunc_oii, src/backend/linalg_internal_interface.hpp:173-175), called
+ * from linalg::Tensordot (src/linalg/Tensordot.cpp:158-161).  Operands that remain strided
+ * matrices under the implied permutation are read in place; the others are permuted into
+ * `workspace` first (size from cxa_tensordot_workspace; may be NULL when that size is 0).
+ */
+int cxa_tensordot_workspace(int dtype, int rankL, const uint64_t *shapeL, int rankR,
+                            const uint64_t *shapeR, int ncomm, const uint64_t *idxl,
+                            const uint64_t *idxr, int64_t *bytes);
+int cxa_tensordot(int dtype, void *out, const void *L, int rankL, const uint64_t *shapeL,
+                  const void *R, int rankR, const uint64_t *shapeR, int ncomm,
+                  const uint64_t *idxl, const uint64_t *idxr, void *workspace,
+                  cxa_stream_t stream);
+
+/*
+ * Zero-flux block list of a U(1) BlockUniTensor, in the reference's odometer order (last bond
+ * fastest).  Replaces the block loop of BlockUniTensor::Init (src/BlockUniTensor.cpp:190-230)
+ * with the flux rule of include/UniTensor.hpp:1139-1166 (KET bonds add +q, BRA bonds -q).
+ * btype[rank]: -1 KET / +1 BRA; nq[rank]: number of qnums per bond; qnums: all bonds' qnums
+ * concatenated.  Writes at most max_blocks tuples (row-major nblocks x rank) to itoi, and the
+ * full count to *nblocks (call again with a larger buffer if it exceeds max_blocks).  Host only.
+ */
+int cxa_u1_enumerate_blocks(int rank, const int *btype, const int64_t *nq, const int64_t *qnums,
+                            int64_t max_blocks, uint64_t *itoi, int64_t *nblocks);
+
+/*
+ * Block-sparse (U(1) BlockUniTensor) contraction: a plan built once from block metadata, then
+ * executed as one grouped MFMA launch over every charge-matched block pair.
+ * This is the GPU branch of BlockUniTensor::contract (src/BlockUniTensor.cpp:948-1228), which
+ * in the reference has no backend hook and issues one Matmul (+ permute copies and an add pass)
+ * per block pair (:1114-1214).  Pairing, target selection and output zero-fill follow it:
+ *   for each left block a, each right block b whose qindices on the common legs equal a's
+ *   (comm1[i] of L pairs with comm2[i] of R, in the caller's order -- the reference sorts the
+ *   common labels, src/utils/vec_intersect.cpp:35-51), out[target] += Lmat_a . Rmat_b, where
+ *   target has itoi = (a's free qindices, b's free qindices), Lmat_a = block a with axes
+ *   (free ++ common), Rmat_b = block b with axes (common ++ free); output blocks without any
+ *   pair are zero.
+ * Metadata: itoi[nb*rank] qindex tuples, shape[nb*rank] block extents, off[nb] element offsets
+ * of each row-major block within its tensor's buffer.  The output's rank is
+ * rankL + rankR - 2*ncomm with axes (L free ++ R free).
+ * exec: L, R, C are the buffers the offsets refer to; workspace must hold the bytes reported
+ * by cxa_block_contract_info (NULL allowed when 0).  prec: CXA_PREC_* for real dtypes
+ * (complex dtypes always use their native precision).  Plans are bound to the device that was
+ * current at creation and may be executed concurrently on different streams with different
+ * workspaces.
+ */
+typedef struct cxa_bc_plan cxa_bc_plan;
+int cxa_block_contract_plan(int dtype, int rankL, int nbL, const uint64_t *itoiL,
+                            const uint64_t *shapeL, const int64_t *offL, int rankR, int nbR,
+                            const uint64_t *itoiR, const uint64_t *shapeR, const int64_t *offR,
+                            int ncomm, const uint64_t *comm1, const uint64_t *comm2, int nbC,
+                            const uint64_t *itoiC, const uint64_t *shapeC, const int64_t *offC,
+                            cxa_bc_plan **plan);
+/* flops: sum over block pairs of 2*M*N*K (x4 for complex); npairs: number of block pairs. */
+int cxa_block_contract_info(const cxa_bc_plan *plan, double *flops, int64_t *workspace_bytes,
+                            int64_t *npairs);
+/* Override the wave-tile shape of every tile (-1 = per-block default); used by the tuner. */
+int cxa_block_contract_set_tile(cxa_bc_plan *plan, int kind);
+int cxa_block_contract_exec(const cxa_bc_plan *plan, const void *L, const void *R, void *C,
+                            void *workspace, int prec, cxa_stream_t stream);
+int cxa_block_contract_destroy(cxa_bc_plan *plan);
+
 #ifdef __cplusplus
 }
 #endif
