@@ -59,6 +59,20 @@ def _declare(L):
         "cxa_block_contract_set_tile": (ci, [vp, ci]),
         "cxa_block_contract_exec": (ci, [vp, vp, vp, vp, vp, ci, vp]),
         "cxa_block_contract_destroy": (ci, [vp]),
+        "cxa_vec_workspace_bytes": (ci, []),
+        "cxa_dot": (ci, [i64, vp, vp, vp, vp, vp]),
+        "cxa_nrm2": (ci, [i64, vp, vp, vp, vp]),
+        "cxa_tridiag_ground": (ci, [ci, dblp, dblp, dblp, dblp]),
+        "cxa_chain_create": (ci, [ci, ctypes.POINTER(vp), ctypes.POINTER(vp), intp, i64p, i64p,
+                                  ctypes.POINTER(vp)]),
+        "cxa_chain_prepare": (ci, [vp, ci, vp]),
+        "cxa_chain_matvec": (ci, [vp, vp, vp, vp]),
+        "cxa_chain_flops": (ci, [vp, dblp]),
+        "cxa_chain_destroy": (ci, [vp]),
+        "cxa_lanczos_create": (ci, [i64, ci, ctypes.POINTER(vp)]),
+        "cxa_lanczos_run": (ci, [vp, vp, vp, vp, ctypes.c_double, ci, ci, ci, dblp, vp, intp,
+                                 dblp, dblp, vp]),
+        "cxa_lanczos_destroy": (ci, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

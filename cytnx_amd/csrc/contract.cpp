@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "cxa_common.hpp"
+#include "contract.hpp"
 #include "gemm.hpp"
 
 namespace cxa {
@@ -328,32 +329,38 @@ extern "C" int cxa_block_contract_set_tile(cxa_bc_plan *p, int kind) {
   return upload_plan(p);
 }
 
-extern "C" int cxa_block_contract_exec(const cxa_bc_plan *p, const void *L, const void *R,
-                                       void *C, void *workspace, int prec, cxa_stream_t stream) {
-  CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_exec: null plan");
-  hipStream_t s = (hipStream_t)stream;
-  const int es = p->es;
+namespace cxa {
+
+// Execute a plan with explicit buffer dtypes (a_dt / b_dt / c_dt may differ from the plan's
+// dtype: FP32 copies of the same layout, see the chain).  Offsets are in elements, so the same
+// task list addresses either copy.  Workspace offsets are in elements of the copied operand.
+int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, int a_dt,
+                  int b_dt, int c_dt, int prec, void *workspace, hipStream_t s) {
+  CXA_CHECK(p, CXA_ERR_INVALID, "block contract exec: null plan");
   CXA_CHECK(!(p->copyA || p->copyB) || workspace, CXA_ERR_INVALID,
-            "cxa_block_contract_exec: this plan needs a workspace of %lld bytes",
-            (long long)((p->wsA + p->wsB) * es));
+            "block contract exec: this plan needs a workspace of %lld bytes",
+            (long long)((p->wsA + p->wsB) * 16));
+  const int esA = elem_size(a_dt), esB = elem_size(b_dt);
   char *ws = (char *)workspace;
   for (const CopyJob &j : p->jobsA) {
-    int rc = cxa_permute(p->dtype, j.rank, j.shape, j.mapper, (const char *)L + j.src_off * es,
-                         ws + j.dst_off * es, stream);
+    int rc = cxa_permute(a_dt, j.rank, j.shape, j.mapper, (const char *)L + j.src_off * esA,
+                         ws + j.dst_off * esA, s);
     if (rc) return rc;
   }
-  char *wsB = ws ? ws + p->wsA * es : nullptr;
+  char *wsB = ws ? ws + p->wsA * esA : nullptr;
   for (const CopyJob &j : p->jobsB) {
-    int rc = cxa_permute(p->dtype, j.rank, j.shape, j.mapper, (const char *)R + j.src_off * es,
-                         wsB + j.dst_off * es, stream);
+    int rc = cxa_permute(b_dt, j.rank, j.shape, j.mapper, (const char *)R + j.src_off * esB,
+                         wsB + j.dst_off * esB, s);
     if (rc) return rc;
   }
   int prec_eff = prec;
   if (is_complex(p->dtype)) prec_eff = p->dtype == CXA_C128 ? CXA_PREC_FP64 : CXA_PREC_FP32;
-  if (p->dtype == CXA_F32 && prec == CXA_PREC_FP64) prec_eff = CXA_PREC_FP32;
+  if ((a_dt == CXA_F32 || b_dt == CXA_F32) && prec == CXA_PREC_FP64) prec_eff = CXA_PREC_FP32;
   GemmLaunch g;
   g.prec = prec_eff;
-  g.a_dt = g.b_dt = g.c_dt = p->dtype;
+  g.a_dt = a_dt;
+  g.b_dt = b_dt;
+  g.c_dt = c_dt;
   g.A = p->copyA ? (const void *)ws : L;
   g.B = p->copyB ? (const void *)wsB : R;
   g.C = C;
@@ -362,6 +369,19 @@ extern "C" int cxa_block_contract_exec(const cxa_bc_plan *p, const void *L, cons
   g.segs = (const GSeg *)p->d_segs.p;
   g.ntiles = (int)p->tiles.size();
   return launch_grouped_gemm(g, s);
+}
+
+int64_t bc_workspace_elems(const cxa_bc_plan *p) { return p->wsA + p->wsB; }
+double bc_flops(const cxa_bc_plan *p) { return p->flops; }
+int bc_dtype(const cxa_bc_plan *p) { return p->dtype; }
+
+}  // namespace cxa
+
+extern "C" int cxa_block_contract_exec(const cxa_bc_plan *p, const void *L, const void *R,
+                                       void *C, void *workspace, int prec, cxa_stream_t stream) {
+  CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_exec: null plan");
+  return bc_exec_typed(p, L, R, C, p->dtype, p->dtype, p->dtype, prec, workspace,
+                       (hipStream_t)stream);
 }
 
 extern "C" int cxa_block_contract_destroy(cxa_bc_plan *p) {

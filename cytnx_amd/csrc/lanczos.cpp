@@ -1,0 +1,450 @@
+// Matvec chains (the effective-Hamiltonian program) and the ground-state Lanczos driver.
+//
+// A chain is a fixed sequence of block-contraction plans: step 1 contracts the input vector
+// with a fixed operand (an environment), each later step contracts the running intermediate
+// with the next fixed operand (MPO tensors, the other environment).  Intermediates and FP32
+// copies of the fixed operands are allocated once, so applying a chain allocates nothing and can
+// be captured into a graph.
+//
+// The Lanczos driver restates Lanczos_Gnd_Ut (src/linalg/Lanczos_Gnd_Ut.cpp:29-162): plain
+// three-term recurrence (optionally with FP64 re-orthogonalisation), tridiagonal eigensolve,
+// stop at the first iteration with |E - E_prev| < CvgCrit or beta == 0, Ritz vector from the
+// stored Krylov basis.  Device scalars and a device iteration counter let a batch of iterations
+// run without host round trips; the host reads the alpha/beta history once per batch and
+// applies the reference's stopping rule iteration by iteration, so the result is the one a
+// per-iteration check would give.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "contract.hpp"
+#include "cxa_common.hpp"
+#include "vecops.hpp"
+
+using namespace cxa;
+
+// ------------------------------------------------------------------------------------------
+// Ground state of a symmetric tridiagonal matrix (diag a[0..n), off-diagonal b[0..n-1)):
+// smallest eigenvalue by Sturm-sequence bisection, eigenvector by inverse iteration with a
+// partially pivoted tridiagonal LU (the LINPACK/LAPACK gtsv elimination).  Only the lowest
+// pair is needed by the ground-state Lanczos, where the reference computes all pairs with
+// LAPACKE_dstev (src/backend/linalg_internal_cpu/Tridiag_internal.cpp:9-42) and uses [0].
+// ------------------------------------------------------------------------------------------
+namespace {
+
+int sturm_count(int n, const double *a, const double *b2, double x, double pivmin) {
+  int cnt = 0;
+  double q = a[0] - x;
+  if (std::fabs(q) < pivmin) q = -pivmin;
+  cnt += q < 0;
+  for (int i = 1; i < n; ++i) {
+    q = (a[i] - x) - b2[i - 1] / q;
+    if (std::fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0;
+  }
+  return cnt;
+}
+
+// solve (T - lam I) x = r in place (r -> x) with partial pivoting
+void gt_solve(int n, const double *a, const double *b, double lam, double tiny,
+              std::vector<double> &r) {
+  std::vector<double> dl(n > 1 ? n - 1 : 1), d(n), du(n > 1 ? n - 1 : 1), du2(n > 2 ? n - 2 : 1, 0.0);
+  std::vector<char> swp(n > 1 ? n - 1 : 1, 0);
+  for (int i = 0; i < n; ++i) d[i] = a[i] - lam;
+  for (int i = 0; i + 1 < n; ++i) dl[i] = du[i] = b[i];
+  for (int i = 0; i + 1 < n; ++i) {
+    if (std::fabs(d[i]) >= std::fabs(dl[i])) {
+      if (d[i] == 0.0) d[i] = tiny;
+      const double f = dl[i] / d[i];
+      dl[i] = f;
+      d[i + 1] -= f * du[i];
+      if (i + 2 < n) du2[i] = 0.0;
+    } else {
+      const double f = d[i] / dl[i];
+      d[i] = dl[i];
+      const double t = d[i + 1];
+      d[i + 1] = du[i] - f * t;
+      if (i + 2 < n) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -f * du2[i];
+      }
+      du[i] = t;
+      dl[i] = f;
+      swp[i] = 1;
+    }
+  }
+  if (d[n - 1] == 0.0) d[n - 1] = tiny;
+  for (int i = 0; i + 1 < n; ++i) {
+    if (swp[i]) std::swap(r[i], r[i + 1]);
+    r[i + 1] -= dl[i] * r[i];
+  }
+  r[n - 1] /= d[n - 1];
+  if (n > 1) r[n - 2] = (r[n - 2] - du[n - 2] * r[n - 1]) / d[n - 2];
+  for (int i = n - 3; i >= 0; --i) r[i] = (r[i] - du[i] * r[i + 1] - du2[i] * r[i + 2]) / d[i];
+}
+
+}  // namespace
+
+extern "C" int cxa_tridiag_ground(int n, const double *a, const double *b, double *E,
+                                  double *z) {
+  CXA_CHECK(n >= 1 && a && E, CXA_ERR_INVALID, "cxa_tridiag_ground: bad arguments");
+  CXA_CHECK(n == 1 || b, CXA_ERR_INVALID, "cxa_tridiag_ground: null off-diagonal");
+  if (n == 1) {
+    *E = a[0];
+    if (z) z[0] = 1.0;
+    return CXA_OK;
+  }
+  std::vector<double> b2(n - 1);
+  double lo = a[0], hi = a[0], nrm = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(b[i]) : 0.0);
+    lo = std::fmin(lo, a[i] - r);
+    hi = std::fmax(hi, a[i] + r);
+    nrm = std::fmax(nrm, std::fabs(a[i]) + r);
+  }
+  for (int i = 0; i + 1 < n; ++i) {
+    CXA_CHECK(std::isfinite(b[i]) && std::isfinite(a[i]), CXA_ERR_NUMERIC,
+              "cxa_tridiag_ground: non-finite entries");
+    b2[i] = b[i] * b[i];
+  }
+  const double eps = 2.220446049250313e-16;
+  // smallest pivot magnitude allowed in the Sturm recurrence (keeps b^2/q finite)
+  const double pivmin = 1e-290 * std::fmax(1.0, nrm * nrm);
+  lo -= eps * nrm + pivmin;
+  hi += eps * nrm + pivmin;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (hi - lo <= 2.0 * eps * std::fmax(std::fabs(lo), std::fabs(hi))) break;
+    if (sturm_count(n, a, b2.data(), mid, pivmin) >= 1)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  const double lam = 0.5 * (lo + hi);
+  *E = lam;
+  if (!z) return CXA_OK;
+  std::vector<double> r(n, 1.0 / std::sqrt((double)n));
+  const double tiny = eps * std::fmax(nrm, 1e-300);
+  for (int k = 0; k < 3; ++k) {
+    gt_solve(n, a, b, lam, tiny, r);
+    double s = 0.0;
+    for (double v : r) s += v * v;
+    s = std::sqrt(s);
+    CXA_CHECK(s > 0 && std::isfinite(s), CXA_ERR_NUMERIC,
+              "cxa_tridiag_ground: inverse iteration failed");
+    for (double &v : r) v /= s;
+  }
+  std::memcpy(z, r.data(), sizeof(double) * n);
+  return CXA_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// chains
+// ------------------------------------------------------------------------------------------
+struct cxa_chain {
+  int device = 0;
+  int nsteps = 0;
+  std::vector<const cxa_bc_plan *> plans;
+  std::vector<const void *> fixed;
+  std::vector<int> fixed_left;
+  std::vector<int64_t> fixed_elems, out_elems;
+  std::vector<void *> mid64, mid32, fixed32;
+  void *ws = nullptr;  // permute workspace (max over steps), 16 bytes per element
+  int prec = CXA_PREC_FP64;
+  double flops = 0;
+  ~cxa_chain() {
+    for (auto *v : {&mid64, &mid32, &fixed32})
+      for (void *p : *v)
+        if (p) (void)hipFree(p);
+    if (ws) (void)hipFree(ws);
+  }
+};
+
+extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const void *const *fixed,
+                                const int *fixed_is_left, const int64_t *fixed_elems,
+                                const int64_t *out_elems, cxa_chain **out) {
+  CXA_CHECK(out && nsteps >= 1 && plans && fixed && fixed_is_left && fixed_elems && out_elems,
+            CXA_ERR_INVALID, "cxa_chain_create: bad arguments");
+  *out = nullptr;
+  std::unique_ptr<cxa_chain> c(new cxa_chain());
+  CXA_HIP(hipGetDevice(&c->device));
+  c->nsteps = nsteps;
+  int64_t wsmax = 0;
+  for (int k = 0; k < nsteps; ++k) {
+    CXA_CHECK(plans[k], CXA_ERR_INVALID, "cxa_chain_create: null plan at step %d", k);
+    CXA_CHECK(bc_dtype(plans[k]) == CXA_F64, CXA_ERR_UNSUPPORTED,
+              "cxa_chain_create: chains run real FP64 plans (step %d)", k);
+    c->plans.push_back(plans[k]);
+    c->fixed.push_back(fixed[k]);
+    c->fixed_left.push_back(fixed_is_left[k] ? 1 : 0);
+    c->fixed_elems.push_back(fixed_elems[k]);
+    c->out_elems.push_back(out_elems[k]);
+    c->flops += bc_flops(plans[k]);
+    wsmax = std::max(wsmax, bc_workspace_elems(plans[k]));
+  }
+  c->mid64.assign(nsteps > 1 ? nsteps - 1 : 0, nullptr);
+  for (int k = 0; k + 1 < nsteps; ++k)
+    CXA_HIP(hipMalloc(&c->mid64[k], std::max<int64_t>(1, out_elems[k]) * sizeof(double)));
+  if (wsmax > 0) CXA_HIP(hipMalloc(&c->ws, wsmax * 16));
+  *out = c.release();
+  return CXA_OK;
+}
+
+// Allocate and fill the FP32 copies a precision mode needs (call before graph capture).
+extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
+  CXA_CHECK(c, CXA_ERR_INVALID, "cxa_chain_prepare: null chain");
+  CXA_CHECK(prec >= CXA_PREC_FP64 && prec <= CXA_PREC_FP32X3, CXA_ERR_INVALID,
+            "cxa_chain_prepare: unknown precision %d", prec);
+  c->prec = prec;
+  if (prec != CXA_PREC_FP32 || !c->fixed32.empty()) return CXA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  c->fixed32.assign(c->nsteps, nullptr);
+  c->mid32.assign(c->mid64.size(), nullptr);
+  for (int k = 0; k < c->nsteps; ++k) {
+    CXA_HIP(hipMalloc(&c->fixed32[k], std::max<int64_t>(1, c->fixed_elems[k]) * sizeof(float)));
+    int rc = vec_cast_f64_f32((const double *)c->fixed[k], (float *)c->fixed32[k],
+                              c->fixed_elems[k], s);
+    if (rc) return rc;
+  }
+  for (size_t k = 0; k < c->mid32.size(); ++k)
+    CXA_HIP(hipMalloc(&c->mid32[k], std::max<int64_t>(1, c->out_elems[k]) * sizeof(float)));
+  return CXA_OK;
+}
+
+namespace {
+
+int chain_apply(cxa_chain *c, const void *x, void *y, hipStream_t s) {
+  const bool f32 = c->prec == CXA_PREC_FP32;
+  for (int k = 0; k < c->nsteps; ++k) {
+    const bool last = k == c->nsteps - 1;
+    const void *in = k == 0 ? x : (f32 ? c->mid32[k - 1] : c->mid64[k - 1]);
+    const int in_dt = (k == 0 || !f32) ? CXA_F64 : CXA_F32;
+    void *outp = last ? y : (f32 ? c->mid32[k] : c->mid64[k]);
+    const int out_dt = (last || !f32) ? CXA_F64 : CXA_F32;
+    const void *fx = f32 ? c->fixed32[k] : c->fixed[k];
+    const int fx_dt = f32 ? CXA_F32 : CXA_F64;
+    int rc;
+    if (c->fixed_left[k])
+      rc = bc_exec_typed(c->plans[k], fx, in, outp, fx_dt, in_dt, out_dt, c->prec, c->ws, s);
+    else
+      rc = bc_exec_typed(c->plans[k], in, fx, outp, in_dt, fx_dt, out_dt, c->prec, c->ws, s);
+    if (rc) return rc;
+  }
+  return CXA_OK;
+}
+
+}  // namespace
+
+// The chain as a matvec callback: ctx is the chain, x/y are FP64 device vectors.
+extern "C" int cxa_chain_matvec(void *ctx, const void *x, void *y, cxa_stream_t stream) {
+  cxa_chain *c = (cxa_chain *)ctx;
+  CXA_CHECK(c && x && y, CXA_ERR_INVALID, "cxa_chain_matvec: bad arguments");
+  CXA_CHECK(c->prec != CXA_PREC_FP32 || !c->fixed32.empty(), CXA_ERR_INVALID,
+            "cxa_chain_matvec: FP32 copies missing; call cxa_chain_prepare first");
+  return chain_apply(c, x, y, (hipStream_t)stream);
+}
+
+extern "C" int cxa_chain_flops(const cxa_chain *c, double *flops) {
+  CXA_CHECK(c && flops, CXA_ERR_INVALID, "cxa_chain_flops: bad arguments");
+  *flops = c->flops;
+  return CXA_OK;
+}
+
+extern "C" int cxa_chain_destroy(cxa_chain *c) {
+  delete c;
+  return CXA_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Lanczos
+// ------------------------------------------------------------------------------------------
+struct cxa_lanczos {
+  int device = 0;
+  int64_t n = 0;
+  int maxiter = 0;
+  double *V = nullptr, *w = nullptr, *cur = nullptr, *part = nullptr, *sc = nullptr,
+         *coef = nullptr;
+  int *counter = nullptr;
+  double *hbuf = nullptr;  // pinned: alpha, beta history and Ritz coefficients
+  hipStream_t own = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  cxa_matvec_fn g_fn = nullptr;
+  void *g_ctx = nullptr;
+  int g_flags = -1;
+  ~cxa_lanczos() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (double *p : {V, w, cur, part, sc, coef})
+      if (p) (void)hipFree(p);
+    if (counter) (void)hipFree(counter);
+    if (hbuf) (void)hipHostFree(hbuf);
+    if (own) (void)hipStreamDestroy(own);
+  }
+  double *As() { return sc; }
+  double *Bs() { return sc + (maxiter + 1); }
+  double *h() { return sc + 2 * (maxiter + 1); }
+  double *nrm() { return sc + 3 * (maxiter + 1); }
+};
+
+extern "C" int cxa_lanczos_create(int64_t n, int maxiter, cxa_lanczos **out) {
+  CXA_CHECK(out && n > 0 && maxiter >= 2, CXA_ERR_INVALID,
+            "cxa_lanczos_create: need n > 0 and Maxiter > 1");
+  *out = nullptr;
+  std::unique_ptr<cxa_lanczos> L(new cxa_lanczos());
+  CXA_HIP(hipGetDevice(&L->device));
+  L->n = n;
+  L->maxiter = maxiter;
+  const int64_t m1 = maxiter + 1;
+  CXA_HIP(hipMalloc(&L->V, sizeof(double) * n * m1));
+  CXA_HIP(hipMalloc(&L->w, sizeof(double) * n));
+  CXA_HIP(hipMalloc(&L->cur, sizeof(double) * n));
+  CXA_HIP(hipMalloc(&L->part, sizeof(double) * kPartials * m1));
+  CXA_HIP(hipMalloc(&L->sc, sizeof(double) * (3 * m1 + 8)));
+  CXA_HIP(hipMalloc(&L->coef, sizeof(double) * m1));
+  CXA_HIP(hipMalloc(&L->counter, sizeof(int)));
+  CXA_HIP(hipHostMalloc((void **)&L->hbuf, sizeof(double) * 3 * m1, hipHostMallocDefault));
+  *out = L.release();
+  return CXA_OK;
+}
+
+extern "C" int cxa_lanczos_destroy(cxa_lanczos *L) {
+  delete L;
+  return CXA_OK;
+}
+
+namespace {
+
+int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, hipStream_t s) {
+  int rc = fn(ctx, L->cur, L->w, (cxa_stream_t)s);
+  if (rc) return rc;
+  if ((rc = vec_dot(L->w, L->cur, L->n, L->part, L->As(), L->counter, 0, 0, s))) return rc;
+  if ((rc = vec_three_term(L->w, L->cur, L->V, L->n, L->As(), L->Bs(), L->counter, L->part, s)))
+    return rc;
+  if (reorth &&
+      (rc = vec_reorth(L->w, L->V, L->n, L->maxiter + 1, L->counter, L->part, L->h(), s)))
+    return rc;
+  if ((rc = vec_finalize(L->part, L->Bs(), L->counter, 0, 1, s))) return rc;
+  if ((rc = vec_next(L->cur, L->V, L->w, L->n, L->Bs(), L->counter, s))) return rc;
+  return vec_counter_add(L->counter, 1, s);
+}
+
+}  // namespace
+
+// flags: bit 0 = FP64 full re-orthogonalisation, bit 1 = replay one captured iteration graph
+extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, const double *Tin,
+                               double CvgCrit, int maxiter, int batch, int flags, double *E_out,
+                               double *evec, int *niter, double *alphas, double *betas,
+                               cxa_stream_t stream) {
+  CXA_CHECK(L && matvec && Tin && E_out, CXA_ERR_INVALID, "cxa_lanczos_run: bad arguments");
+  CXA_CHECK(CvgCrit > 0, CXA_ERR_INVALID, "[ERROR][Lanczos] converge criteria must >0");
+  CXA_CHECK(maxiter >= 2, CXA_ERR_INVALID, "[ERROR][Lanczos] Maxiter must >1");
+  CXA_CHECK(maxiter <= L->maxiter, CXA_ERR_INVALID,
+            "cxa_lanczos_run: Maxiter %d above the context's %d", maxiter, L->maxiter);
+  if (batch < 1) batch = 1;
+  const int reorth = flags & 1, use_graph = (flags >> 1) & 1;
+  hipStream_t s = (hipStream_t)stream;
+  hipStream_t ws = s;
+  hipEvent_t ev = nullptr;
+  if (use_graph && s == nullptr) {  // the null stream cannot be captured: use our own
+    if (!L->own) CXA_HIP(hipStreamCreateWithFlags(&L->own, hipStreamNonBlocking));
+    ws = L->own;
+    CXA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    CXA_HIP(hipEventRecord(ev, s));
+    CXA_HIP(hipStreamWaitEvent(ws, ev, 0));
+  }
+  const int64_t n = L->n;
+  int rc;
+  // psi_1 = Tin / |Tin|
+  if ((rc = vec_counter_set(L->counter, 0, ws))) return rc;
+  if ((rc = vec_dot(Tin, Tin, n, L->part, L->nrm(), nullptr, 0, 1, ws))) return rc;
+  if ((rc = vec_normalize_init(L->cur, L->V, Tin, n, L->nrm(), ws))) return rc;
+
+  // The iteration is captured afresh on every run: the matvec behind (fn, ctx) may have changed
+  // (a chain's precision, its buffers), and one capture per run is amortised over its iterations.
+  if (use_graph) {
+    if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
+    if (L->graph) (void)hipGraphDestroy(L->graph);
+    L->gexec = nullptr;
+    L->graph = nullptr;
+    CXA_HIP(hipStreamBeginCapture(ws, hipStreamCaptureModeThreadLocal));
+    rc = lanczos_body(L, matvec, ctx, reorth, ws);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ws, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    CXA_HIP(e);
+    L->graph = g;
+    CXA_HIP(hipGraphInstantiate(&L->gexec, g, nullptr, nullptr, 0));
+    L->g_fn = matvec;
+    L->g_ctx = ctx;
+    L->g_flags = flags;
+  }
+
+  const int m1 = L->maxiter + 1;
+  double *hA = L->hbuf, *hB = L->hbuf + m1, *hz = L->hbuf + 2 * m1;
+  std::vector<double> z(m1);
+  double E = 0.0, eig = 0.0;
+  int done = -1, it = 0;
+  while (done < 0 && it < maxiter) {
+    const int nb = std::min(batch, maxiter - it);
+    for (int b = 0; b < nb; ++b) {
+      if (use_graph) {
+        CXA_HIP(hipGraphLaunch(L->gexec, ws));
+      } else if ((rc = lanczos_body(L, matvec, ctx, reorth, ws))) {
+        return rc;
+      }
+    }
+    const int upto = it + nb;  // alphas/betas [0, upto) are valid after the batch
+    CXA_HIP(hipMemcpyAsync(hA, L->As(), sizeof(double) * upto, hipMemcpyDeviceToHost, ws));
+    CXA_HIP(hipMemcpyAsync(hB, L->Bs(), sizeof(double) * upto, hipMemcpyDeviceToHost, ws));
+    CXA_HIP(hipStreamSynchronize(ws));
+    for (int j = it; j < upto && done < 0; ++j) {
+      if (j == 0) {
+        E = hA[0];
+        eig = hA[0];
+        if (hB[0] == 0.0) done = 0;  // Tin is an eigenvector
+        continue;
+      }
+      rc = cxa_tridiag_ground(j + 1, hA, hB, &eig, nullptr);
+      if (rc) {  // the reference stops on a failed tridiagonal solve
+        done = j - 1;
+        break;
+      }
+      if (hB[j] == 0.0) {
+        done = j;
+        break;
+      }
+      if (std::fabs(E - eig) < CvgCrit) {
+        done = j;
+        break;
+      }
+      E = eig;
+    }
+    it = upto;
+  }
+  if (done < 0) done = maxiter - 1;
+  const int k = done + 1;  // tridiagonal size at the stop
+  if ((rc = cxa_tridiag_ground(k, hA, hB, &eig, z.data()))) return rc;
+  *E_out = eig;
+  if (niter) *niter = k;
+  if (alphas) std::memcpy(alphas, hA, sizeof(double) * k);
+  if (betas) std::memcpy(betas, hB, sizeof(double) * k);
+  if (evec) {
+    std::memcpy(hz, z.data(), sizeof(double) * k);
+    CXA_HIP(hipMemcpyAsync(L->coef, hz, sizeof(double) * k, hipMemcpyHostToDevice, ws));
+    if ((rc = vec_combine(evec, L->V, L->coef, k, n, ws))) return rc;
+  }
+  if (ws != s) {
+    CXA_HIP(hipEventRecord(ev, ws));
+    CXA_HIP(hipStreamWaitEvent(s, ev, 0));
+    CXA_HIP(hipEventDestroy(ev));
+  } else {
+    CXA_HIP(hipStreamSynchronize(ws));  // the pinned coefficients are reused by the next run
+  }
+  return CXA_OK;
+}

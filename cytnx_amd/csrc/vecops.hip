@@ -1,0 +1,274 @@
+// Vector kernels for the Lanczos driver (FP64): dot products, norms, the three-term update,
+// re-orthogonalisation and Ritz-vector assembly.  Scalars live in device memory and are indexed
+// through a device-side iteration counter, so an iteration enqueues without any host round trip
+// and the same launch sequence can be captured once and replayed.  Reductions are two-stage
+// with a fixed number of partial sums, so results are deterministic run to run.
+#include "cxa_common.hpp"
+#include "vecops.hpp"
+
+namespace cxa {
+namespace {
+
+constexpr int kT = 256;
+
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double red[kT / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kT / 64; ++i) s += red[i];
+  return s;  // valid in thread 0
+}
+
+__global__ __launch_bounds__(kT) void k_dot_partials(const double *__restrict__ x,
+                                                     const double *__restrict__ y, int64_t n,
+                                                     double *__restrict__ part) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    s += x[i] * y[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[counter + offset] = sum(part[0..P)) (mode 0) or its square root (mode 1)
+__global__ __launch_bounds__(kT) void k_finalize(const double *__restrict__ part, int P,
+                                                 double *out, const int *counter, int offset,
+                                                 int mode) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < P; i += kT) s += part[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    const int idx = (counter ? *counter : 0) + offset;
+    out[idx] = mode == 1 ? sqrt(s) : s;
+  }
+}
+
+// w -= (alpha * cur + beta * prev) with alpha = As[it], beta = Bs[it - 1] (0 when it == 0),
+// prev = V[it - 1]; partial sums of the updated w^2 go to part
+__global__ __launch_bounds__(kT) void k_three_term(double *__restrict__ w,
+                                                   const double *__restrict__ cur,
+                                                   const double *__restrict__ V, int64_t n,
+                                                   const double *__restrict__ As,
+                                                   const double *__restrict__ Bs,
+                                                   const int *__restrict__ counter,
+                                                   double *__restrict__ part) {
+  const int it = *counter;
+  const double a = As[it];
+  const double b = it > 0 ? Bs[it - 1] : 0.0;
+  const double *prev = it > 0 ? V + (int64_t)(it - 1) * n : cur;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const double t = a * cur[i] + b * prev[i];
+    const double v = w[i] - t;
+    w[i] = v;
+    s += v * v;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// h_j partial sums: part[j * P + block] = sum over the block's slice of V_j * w, j <= it
+__global__ __launch_bounds__(kT) void k_proj_partials(const double *__restrict__ V,
+                                                      const double *__restrict__ w, int64_t n,
+                                                      const int *__restrict__ counter,
+                                                      double *__restrict__ part) {
+  const int it = *counter;
+  const int j = blockIdx.y;
+  if (j > it) return;
+  const double *v = V + (int64_t)j * n;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    s += v[i] * w[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) part[(int64_t)j * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kT) void k_proj_finalize(const double *__restrict__ part, int P,
+                                                      const int *__restrict__ counter,
+                                                      double *__restrict__ h) {
+  const int j = blockIdx.x;
+  if (j > *counter) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < P; i += kT) s += part[(int64_t)j * P + i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) h[j] = s;
+}
+
+// w -= sum_{j <= it} h_j V_j, then partial sums of w^2
+__global__ __launch_bounds__(kT) void k_proj_apply(double *__restrict__ w,
+                                                   const double *__restrict__ V, int64_t n,
+                                                   const double *__restrict__ h,
+                                                   const int *__restrict__ counter,
+                                                   double *__restrict__ part) {
+  const int it = *counter;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    double v = w[i];
+    for (int j = 0; j <= it; ++j) v -= h[j] * V[(int64_t)j * n + i];
+    w[i] = v;
+    s += v * v;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// cur = V[it + 1] = w / Bs[it]
+__global__ __launch_bounds__(kT) void k_next(double *__restrict__ cur, double *__restrict__ V,
+                                             const double *__restrict__ w, int64_t n,
+                                             const double *__restrict__ Bs,
+                                             const int *__restrict__ counter) {
+  const int it = *counter;
+  const double b = Bs[it];
+  double *dst = V + (int64_t)(it + 1) * n;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const double v = w[i] / b;
+    cur[i] = v;
+    dst[i] = v;
+  }
+}
+
+__global__ void k_counter_add(int *counter, int d) { *counter += d; }
+__global__ void k_counter_set(int *counter, int v) { *counter = v; }
+
+// cur = V[0] = x / nrm[0]
+__global__ __launch_bounds__(kT) void k_normalize_init(double *__restrict__ cur,
+                                                       double *__restrict__ V,
+                                                       const double *__restrict__ x, int64_t n,
+                                                       const double *__restrict__ nrm) {
+  const double s = nrm[0];
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const double v = x[i] / s;
+    cur[i] = v;
+    V[i] = v;
+  }
+}
+
+// out = sum_j coef[j] V_j, accumulated in the order j = 0, 1, ...
+__global__ __launch_bounds__(kT) void k_combine(double *__restrict__ out,
+                                                const double *__restrict__ V,
+                                                const double *__restrict__ coef, int k,
+                                                int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    double v = coef[0] * V[i];
+    for (int j = 1; j < k; ++j) v += coef[j] * V[(int64_t)j * n + i];
+    out[i] = v;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_cast_f64_f32(const double *__restrict__ x,
+                                                     float *__restrict__ y, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    y[i] = (float)x[i];
+}
+
+int grid_of(int64_t n) {
+  int64_t g = (n + kT - 1) / kT;
+  if (g > kPartials) g = kPartials;
+  return (int)(g < 1 ? 1 : g);
+}
+
+}  // namespace
+
+// ---- internal launchers (declared in vecops.hpp) -------------------------------------------
+int vec_dot(const double *x, const double *y, int64_t n, double *part, double *out,
+            const int *counter, int offset, int mode, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_partials, dim3(kPartials), dim3(kT), 0, s, x, y, n, part);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kT), 0, s, part, kPartials, out, counter, offset,
+                     mode);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_three_term(double *w, const double *cur, const double *V, int64_t n, const double *As,
+                   const double *Bs, const int *counter, double *part, hipStream_t s) {
+  hipLaunchKernelGGL(k_three_term, dim3(kPartials), dim3(kT), 0, s, w, cur, V, n, As, Bs,
+                     counter, part);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_reorth(double *w, const double *V, int64_t n, int max_j, const int *counter, double *part,
+               double *h, hipStream_t s) {
+  hipLaunchKernelGGL(k_proj_partials, dim3(kPartials, max_j), dim3(kT), 0, s, V, w, n, counter,
+                     part);
+  hipLaunchKernelGGL(k_proj_finalize, dim3(max_j), dim3(kT), 0, s, part, kPartials, counter, h);
+  hipLaunchKernelGGL(k_proj_apply, dim3(kPartials), dim3(kT), 0, s, w, V, n, h, counter, part);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_finalize(const double *part, double *out, const int *counter, int offset, int mode,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kT), 0, s, part, kPartials, out, counter, offset,
+                     mode);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_next(double *cur, double *V, const double *w, int64_t n, const double *Bs,
+             const int *counter, hipStream_t s) {
+  hipLaunchKernelGGL(k_next, dim3(grid_of(n)), dim3(kT), 0, s, cur, V, w, n, Bs, counter);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_counter_add(int *counter, int d, hipStream_t s) {
+  hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(1), 0, s, counter, d);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_counter_set(int *counter, int v, hipStream_t s) {
+  hipLaunchKernelGGL(k_counter_set, dim3(1), dim3(1), 0, s, counter, v);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_normalize_init(double *cur, double *V, const double *x, int64_t n, const double *nrm,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_normalize_init, dim3(grid_of(n)), dim3(kT), 0, s, cur, V, x, n, nrm);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_combine(double *out, const double *V, const double *coef, int k, int64_t n,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_combine, dim3(grid_of(n)), dim3(kT), 0, s, out, V, coef, k, n);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_cast_f64_f32(const double *x, float *y, int64_t n, hipStream_t s) {
+  if (n <= 0) return CXA_OK;
+  hipLaunchKernelGGL(k_cast_f64_f32, dim3(grid_of(n)), dim3(kT), 0, s, x, y, n);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+}  // namespace cxa
+
+// ---- public entry points ---------------------------------------------------------------------
+// Replaces cuVectordot_internal_d (src/backend/linalg_internal_gpu/cuVectordot_internal.cu
+// :133-180) for real FP64 vectors: result[0] = sum_i x_i y_i, written to device memory.
+extern "C" int cxa_dot(int64_t n, const double *x, const double *y, double *result,
+                       void *workspace, cxa_stream_t stream) {
+  CXA_CHECK(n >= 0 && result && workspace, CXA_ERR_INVALID, "cxa_dot: bad arguments");
+  CXA_CHECK(n == 0 || (x && y), CXA_ERR_INVALID, "cxa_dot: null vector");
+  return cxa::vec_dot(x, y, n, (double *)workspace, result, nullptr, 0, 0, (hipStream_t)stream);
+}
+
+// Replaces cuNorm_internal_d (src/backend/linalg_internal_gpu/cuNorm_internal.cu:14-27):
+// result[0] = sqrt(sum_i x_i^2), written to device memory.
+extern "C" int cxa_nrm2(int64_t n, const double *x, double *result, void *workspace,
+                        cxa_stream_t stream) {
+  CXA_CHECK(n >= 0 && result && workspace, CXA_ERR_INVALID, "cxa_nrm2: bad arguments");
+  CXA_CHECK(n == 0 || x, CXA_ERR_INVALID, "cxa_nrm2: null vector");
+  return cxa::vec_dot(x, x, n, (double *)workspace, result, nullptr, 0, 1, (hipStream_t)stream);
+}
+
+extern "C" int cxa_vec_workspace_bytes(void) { return cxa::kPartials * (int)sizeof(double); }

@@ -454,6 +454,49 @@ def _promote_pair(A: UniTensor, B: UniTensor) -> Tuple[UniTensor, UniTensor, int
     return A.astype(dt), B.astype(dt), dt
 
 
+def contract_meta(A: UniTensor, B: UniTensor, dt: Optional[int] = None,
+                  device: Optional[int] = None):
+    """Output metadata of A.contract(B) without data: (out, c1, c2).
+
+    Common labels sorted (vec_intersect_), output bonds/labels = L free then R free
+    (src/BlockUniTensor.cpp:948-968), rowrank arithmetic of :964-968, output blocks enumerated
+    by the zero-flux rule.  ``out`` is a block UniTensor with an empty storage."""
+    comm, c1, c2 = _common(A.labels(), B.labels())
+    for i, j in zip(c1, c2):
+        if A.is_blockform() and A._bonds[i].type() + B._bonds[j].type() != 0:
+            raise ValueError("A bond with type BRA can only be contracted with type KET. "
+                             "Invalid contraction for label '%s'." % A._labels[i])
+        if A._bonds[i].dim() != B._bonds[j].dim():
+            raise ValueError("contracted bond dimensions differ for label '%s'" % A._labels[i])
+    nc1 = [i for i in range(A.rank()) if i not in c1]
+    nc2 = [i for i in range(B.rank()) if i not in c2]
+    out_bonds = [A._bonds[i] for i in nc1] + [B._bonds[i] for i in nc2]
+    out_labels = [A._labels[i] for i in nc1] + [B._labels[i] for i in nc2]
+    rr = A.rowrank() + B.rowrank() - sum(1 for i in c1 if i < A.rowrank()) \
+        - sum(1 for i in c2 if i < B.rowrank())
+    dt = A.dtype() if dt is None else dt
+    dev = A.device() if device is None else device
+    if out_bonds:
+        out = UniTensor(out_bonds, out_labels, max(0, min(rr, len(out_bonds))), dt, dev,
+                        _alloc=False)
+    else:
+        out = _scalar_block(dt, dev)
+        out._storage = out._storage[:0]
+    return out, c1, c2
+
+
+def meta_elems(T: UniTensor) -> int:
+    """Number of elements of all blocks together."""
+    if T.rank() == 0:
+        return 1
+    return int(np.sum(np.prod(T._shapes.astype(np.int64), axis=1))) if T.Nblocks() else 0
+
+
+def plan_contraction(A: UniTensor, B: UniTensor, C: UniTensor, c1, c2, dtype: int) -> "_Plan":
+    """Cached cxa_block_contract_plan for A x B -> C (metadata only)."""
+    return _plan_for(dtype, A, B, list(c1), list(c2), C)
+
+
 def Contract(A: UniTensor, B: UniTensor) -> UniTensor:
     """UniTensor::contract on the GPU (src/BlockUniTensor.cpp:777-1235,
     src/DenseUniTensor.cpp:1022-1164)."""

@@ -161,6 +161,70 @@ int cxa_block_contract_exec(const cxa_bc_plan *plan, const void *L, const void *
                             void *workspace, int prec, cxa_stream_t stream);
 int cxa_block_contract_destroy(cxa_bc_plan *plan);
 
+/*
+ * FP64 vector reductions into device memory (result[0]); workspace: device buffer of
+ * cxa_vec_workspace_bytes() bytes.  Deterministic (fixed reduction order).
+ * cxa_dot replaces cuVectordot_internal_d (src/backend/linalg_internal_gpu/
+ * cuVectordot_internal.cu:133-180, type Vectordotfunc_oii); cxa_nrm2 replaces cuNorm_internal_d
+ * (src/backend/linalg_internal_gpu/cuNorm_internal.cu:14-27, type Normfunc_oii).
+ */
+int cxa_vec_workspace_bytes(void);
+int cxa_dot(int64_t n, const double *x, const double *y, double *result, void *workspace,
+            cxa_stream_t stream);
+int cxa_nrm2(int64_t n, const double *x, double *result, void *workspace, cxa_stream_t stream);
+
+/*
+ * Lowest eigenpair of the symmetric tridiagonal matrix with diagonal a[n] and off-diagonal
+ * b[n-1] (host).  Replaces the LAPACKE_dstev call that linalg::Tridiag makes
+ * (src/backend/linalg_internal_cpu/Tridiag_internal.cpp:9-42) for the ground-state Lanczos,
+ * which only reads eigenpair 0.  z (may be NULL) receives the unit eigenvector (sign arbitrary).
+ */
+int cxa_tridiag_ground(int n, const double *a, const double *b, double *E, double *z);
+
+/*
+ * Matvec callback used by the Lanczos driver: y = H x for FP64 device vectors, enqueued on
+ * `stream`; returns a CXA status.  The LinOp::matvec plug-in point (include/LinOp.hpp:163-167).
+ */
+typedef int (*cxa_matvec_fn)(void *ctx, const void *x, void *y, cxa_stream_t stream);
+
+/*
+ * Matvec chain: a fixed sequence of block-contraction plans applied to a vector.  Step k
+ * contracts the running operand (the input vector for k = 0, else step k-1's output) with
+ * fixed[k]; fixed_is_left[k] says whether fixed[k] is the plan's L operand.  fixed_elems /
+ * out_elems give buffer sizes in elements.  This replaces the per-call Network::Launch of the
+ * DMRG effective Hamiltonian (src/RegularNetwork.cpp:901-1007 driven by
+ * example/DMRG/dmrg_two_sites_U1.py:12-41) with a planned program whose application allocates
+ * nothing.  Plans must be real FP64 and must outlive the chain.  Precision (cxa_chain_prepare)
+ * selects FP64 (drop-in), FP32 (FP32 copies of the fixed operands and intermediates; input and
+ * output stay FP64) or FP32X3.  cxa_chain_matvec is a cxa_matvec_fn with ctx = the chain.
+ */
+typedef struct cxa_chain cxa_chain;
+int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const void *const *fixed,
+                     const int *fixed_is_left, const int64_t *fixed_elems,
+                     const int64_t *out_elems, cxa_chain **chain);
+int cxa_chain_prepare(cxa_chain *chain, int prec, cxa_stream_t stream);
+int cxa_chain_matvec(void *chain, const void *x, void *y, cxa_stream_t stream);
+int cxa_chain_flops(const cxa_chain *chain, double *flops);
+int cxa_chain_destroy(cxa_chain *chain);
+
+/*
+ * Ground-state Lanczos on FP64 device vectors of length n.  Replaces linalg::Lanczos(...,
+ * "Gnd") -> Lanczos_Gnd_Ut (src/linalg/Lanczos.cpp:158-179, src/linalg/Lanczos_Gnd_Ut.cpp
+ * :29-199): plain three-term recurrence, stop at the first iteration with
+ * |E - E_prev| < CvgCrit or beta == 0, at most Maxiter matvecs, Ritz vector
+ * sum_k z_k psi_k.  A context holds the Krylov basis for up to `maxiter` iterations.
+ * run: batch = iterations enqueued between host checks (results do not depend on it);
+ * flags bit 0 = FP64 full re-orthogonalisation (not in the reference), bit 1 = capture one
+ * iteration as a graph and replay it.  Outputs: E_out (host), evec (device, may be NULL),
+ * niter (tridiagonal size at the stop), alphas/betas (host, may be NULL, niter entries).
+ */
+typedef struct cxa_lanczos cxa_lanczos;
+int cxa_lanczos_create(int64_t n, int maxiter, cxa_lanczos **ctx);
+int cxa_lanczos_run(cxa_lanczos *ctx, cxa_matvec_fn matvec, void *matvec_ctx, const double *Tin,
+                    double CvgCrit, int Maxiter, int batch, int flags, double *E_out, double *evec,
+                    int *niter, double *alphas, double *betas, cxa_stream_t stream);
+int cxa_lanczos_destroy(cxa_lanczos *ctx);
+
 #ifdef __cplusplus
 }
 #endif
