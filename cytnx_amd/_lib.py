@@ -67,6 +67,7 @@ def _declare(L):
                                   ctypes.POINTER(vp)]),
         "cxa_chain_prepare": (ci, [vp, ci, vp]),
         "cxa_chain_matvec": (ci, [vp, vp, vp, vp]),
+        "cxa_chain_step": (ci, [vp, ci, vp, vp, vp]),
         "cxa_chain_flops": (ci, [vp, dblp]),
         "cxa_chain_destroy": (ci, [vp]),
         "cxa_lanczos_create": (ci, [i64, ci, ctypes.POINTER(vp)]),

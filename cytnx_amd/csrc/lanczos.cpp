@@ -215,21 +215,23 @@ extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
 
 namespace {
 
-int chain_apply(cxa_chain *c, const void *x, void *y, hipStream_t s) {
+int chain_step(cxa_chain *c, int k, const void *x, void *y, hipStream_t s) {
   const bool f32 = c->prec == CXA_PREC_FP32;
+  const bool last = k == c->nsteps - 1;
+  const void *in = k == 0 ? x : (f32 ? c->mid32[k - 1] : c->mid64[k - 1]);
+  const int in_dt = (k == 0 || !f32) ? CXA_F64 : CXA_F32;
+  void *outp = last ? y : (f32 ? c->mid32[k] : c->mid64[k]);
+  const int out_dt = (last || !f32) ? CXA_F64 : CXA_F32;
+  const void *fx = f32 ? c->fixed32[k] : c->fixed[k];
+  const int fx_dt = f32 ? CXA_F32 : CXA_F64;
+  if (c->fixed_left[k])
+    return bc_exec_typed(c->plans[k], fx, in, outp, fx_dt, in_dt, out_dt, c->prec, c->ws, s);
+  return bc_exec_typed(c->plans[k], in, fx, outp, in_dt, fx_dt, out_dt, c->prec, c->ws, s);
+}
+
+int chain_apply(cxa_chain *c, const void *x, void *y, hipStream_t s) {
   for (int k = 0; k < c->nsteps; ++k) {
-    const bool last = k == c->nsteps - 1;
-    const void *in = k == 0 ? x : (f32 ? c->mid32[k - 1] : c->mid64[k - 1]);
-    const int in_dt = (k == 0 || !f32) ? CXA_F64 : CXA_F32;
-    void *outp = last ? y : (f32 ? c->mid32[k] : c->mid64[k]);
-    const int out_dt = (last || !f32) ? CXA_F64 : CXA_F32;
-    const void *fx = f32 ? c->fixed32[k] : c->fixed[k];
-    const int fx_dt = f32 ? CXA_F32 : CXA_F64;
-    int rc;
-    if (c->fixed_left[k])
-      rc = bc_exec_typed(c->plans[k], fx, in, outp, fx_dt, in_dt, out_dt, c->prec, c->ws, s);
-    else
-      rc = bc_exec_typed(c->plans[k], in, fx, outp, in_dt, fx_dt, out_dt, c->prec, c->ws, s);
+    int rc = chain_step(c, k, x, y, s);
     if (rc) return rc;
   }
   return CXA_OK;
@@ -244,6 +246,18 @@ extern "C" int cxa_chain_matvec(void *ctx, const void *x, void *y, cxa_stream_t 
   CXA_CHECK(c->prec != CXA_PREC_FP32 || !c->fixed32.empty(), CXA_ERR_INVALID,
             "cxa_chain_matvec: FP32 copies missing; call cxa_chain_prepare first");
   return chain_apply(c, x, y, (hipStream_t)stream);
+}
+
+// One step of the chain (for per-kernel timing): step 0 reads x, the last step writes y, the
+// others use the chain's intermediates left by the previous step.
+extern "C" int cxa_chain_step(void *ctx, int step, const void *x, void *y, cxa_stream_t stream) {
+  cxa_chain *c = (cxa_chain *)ctx;
+  CXA_CHECK(c && step >= 0 && step < c->nsteps, CXA_ERR_INVALID, "cxa_chain_step: bad step");
+  CXA_CHECK((step > 0 || x) && (step < c->nsteps - 1 || y), CXA_ERR_INVALID,
+            "cxa_chain_step: missing input or output vector");
+  CXA_CHECK(c->prec != CXA_PREC_FP32 || !c->fixed32.empty(), CXA_ERR_INVALID,
+            "cxa_chain_step: FP32 copies missing; call cxa_chain_prepare first");
+  return chain_step(c, step, x, y, (hipStream_t)stream);
 }
 
 extern "C" int cxa_chain_flops(const cxa_chain *c, double *flops) {

@@ -204,6 +204,8 @@ int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const void *const *f
                      const int64_t *out_elems, cxa_chain **chain);
 int cxa_chain_prepare(cxa_chain *chain, int prec, cxa_stream_t stream);
 int cxa_chain_matvec(void *chain, const void *x, void *y, cxa_stream_t stream);
+/* one step only (step 0 reads x, the last writes y): per-kernel timing */
+int cxa_chain_step(void *chain, int step, const void *x, void *y, cxa_stream_t stream);
 int cxa_chain_flops(const cxa_chain *chain, double *flops);
 int cxa_chain_destroy(cxa_chain *chain);
 
