@@ -31,16 +31,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_matvec(H, x, y, steps, warmup, torch):
+def time_matvec(H, x, y, steps, warmup, torch, graph=True):
+    """Average matvec time over `steps` (HIP events on the launch stream).  With graph=True each
+    matvec is one replay of the captured chain (one host launch per matvec); otherwise each
+    grouped GEMM is launched from the host."""
+    if graph:
+        H.chain.prepare(H.precision)
+        g = H.chain.graph(x, y, 1)
+        run = g.launch
+    else:
+        def run():
+            H.apply_flat(x, y)
     for _ in range(warmup):
-        H.apply_flat(x, y)
+        run()
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     start.record()
     for _ in range(steps):
-        H.apply_flat(x, y)
+        run()
     end.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -133,6 +143,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the tiling autotuner (per-block default tiles)")
     ap.add_argument("--lanczos-cvg", type=float, default=1e-12)
     ap.add_argument("--lanczos-maxiter", type=int, default=600)
     ap.add_argument("--profile", action="store_true",
@@ -160,22 +172,38 @@ def main() -> int:
     problem = [t.to(0) for t in host]
     psi, L, M1, M2, R = problem
     H = Heff2(psi, L, M1, M2, R)
-    flops = H.flops
-    step_flops = [p.flops for p in H.chain._plans]
+    flops = H.flops  # the reference order's four-step count
+    step_flops = [p.flops for p in H.chain._plans]  # per launched kernel
     x = psi._storage
     y = torch.empty_like(x)
     log("flops/matvec %.4e, block pairs per step %s, psi elements %d" % (flops, H.pairs, x.numel()))
 
     modes = {"fp64": PREC_FP64, "fp32": PREC_FP32, "fp32x3": PREC_FP32X3}
     res = {}
+    tuned = {}
     for name, prec in modes.items():
         H.set_precision(prec)
-        ms, wall = time_matvec(H, x, y, args.steps, args.warmup, torch)
+        if not args.no_tune:
+            tuned[name] = H.tune(reps=20)
+            log(name, "tuned", json.dumps(tuned[name]))
+        ms, wall = time_matvec(H, x, y, args.steps, args.warmup, torch, graph=True)
+        ms_eager, _ = time_matvec(H, x, y, args.steps, args.warmup, torch, graph=False)
         per = time_steps(H, x, y, max(20, args.steps // 4), torch)
         res[name] = {"ms_per_step": ms, "tflops": flops / (ms * 1e-3) / 1e12,
                      "pct_peak": 100.0 * flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS[name],
-                     "step_ms": per, "wall_s": wall}
+                     "ms_per_step_host_launched": ms_eager, "step_ms": per, "wall_s": wall}
         log(name, json.dumps(res[name]))
+    # normwise error of the reduced-precision matvecs against FP64 on the benchmark input
+    errs = {}
+    H.set_precision(PREC_FP64)
+    y64 = torch.empty_like(x)
+    H.apply_flat(x, y64)
+    for name in ("fp32", "fp32x3"):
+        H.set_precision(modes[name])
+        yy = torch.empty_like(x)
+        H.apply_flat(x, yy)
+        errs[name] = float((torch.linalg.vector_norm(yy - y64) /
+                            torch.linalg.vector_norm(y64)).item())
     H.set_precision(PREC_FP64)
 
     lanczos = None
@@ -205,9 +233,8 @@ def main() -> int:
     achieved = step_flops[k_dom] / (dom_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[main_mode],
                 "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_mode], "traffic": None,
-                "kernel": "k_grouped (chain step %d of 4: %s)" % (
-                    k_dom, ["L.psi", "x W1", "x W2", "x R"][k_dom] if len(step_flops) == 4
-                    else str(k_dom)),
+                "kernel": "grouped GEMM, chain step %d of %d (%s)" % (
+                    k_dom, len(step_flops), H.step_names[k_dom]),
                 "kernel_flops": step_flops[k_dom], "kernel_ms": dom_ms}
 
     cpu = None
@@ -233,11 +260,14 @@ def main() -> int:
                                "Heisenberg MPO D=5, order L.psi->W1->W2->R" % args.chi,
                    "chi": args.chi, "sectors": len(heisenberg.gaussian_sectors(args.chi)[0]),
                    "psi_elements": int(x.numel()), "block_pairs_per_step": H.pairs,
-                   "flops_per_matvec": flops, "step_flops": step_flops},
+                   "flops_per_matvec": flops, "kernels": H.step_names,
+                   "kernel_flops": step_flops},
         "fp64": res["fp64"],
         "fp32": res["fp32"],
         "fp32x3": res["fp32x3"],
         "speedup_fp32_vs_fp64": res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"],
+        "tuned_tiles": tuned,
+        "matvec_rel_error_vs_fp64": errs,
         "lanczos": lanczos,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -57,6 +57,9 @@ def _declare(L):
                                          ctypes.POINTER(vp)]),
         "cxa_block_contract_info": (ci, [vp, dblp, i64p, i64p]),
         "cxa_block_contract_set_tile": (ci, [vp, ci]),
+        "cxa_block_contract_set_split": (ci, [vp, ci]),
+        "cxa_block_contract_set_config": (ci, [vp, ci, ci, ci]),
+        "cxa_block_contract_get_config": (ci, [vp, ci, intp, intp, intp]),
         "cxa_block_contract_exec": (ci, [vp, vp, vp, vp, vp, ci, vp]),
         "cxa_block_contract_destroy": (ci, [vp]),
         "cxa_vec_workspace_bytes": (ci, []),
@@ -68,8 +71,12 @@ def _declare(L):
         "cxa_chain_prepare": (ci, [vp, ci, vp]),
         "cxa_chain_matvec": (ci, [vp, vp, vp, vp]),
         "cxa_chain_step": (ci, [vp, ci, vp, vp, vp]),
+        "cxa_chain_tune": (ci, [vp, vp, vp, ci, intp, intp, dblp, vp]),
         "cxa_chain_flops": (ci, [vp, dblp]),
         "cxa_chain_destroy": (ci, [vp]),
+        "cxa_chain_graph_create": (ci, [vp, vp, vp, ci, ctypes.POINTER(vp)]),
+        "cxa_graph_launch": (ci, [vp, vp]),
+        "cxa_graph_destroy": (ci, [vp]),
         "cxa_lanczos_create": (ci, [i64, ci, ctypes.POINTER(vp)]),
         "cxa_lanczos_run": (ci, [vp, vp, vp, vp, ctypes.c_double, ci, ci, ci, dblp, vp, intp,
                                  dblp, dblp, vp]),

@@ -63,37 +63,55 @@ struct CopyJob {
 }  // namespace
 }  // namespace cxa
 
+// One tiling of a plan's blocks: the tile list on the device plus how it is launched.  A plan
+// keeps one per precision mode, so switching precision never re-uploads (launches stay
+// capturable) and the tuner can pick different shapes for FP64 and FP32.
+struct PlanVariant {
+  int kind = -1;        // tile kind override, -1 = per-block default
+  int split_mode = -1;  // -1 automatic, 0 off, 1 on
+  std::vector<cxa::GTile> tiles;
+  cxa::DevBuf d_tiles;
+  bool splitk = false;
+  int tile_class = 0;
+};
+
 struct cxa_bc_plan {
   int dtype = 0;
   int es = 0;
   int device = 0;
   std::vector<cxa::GBlk> blks;
   std::vector<cxa::GSeg> segs;
-  std::vector<cxa::GTile> tiles;
-  cxa::DevBuf d_blks, d_segs, d_tiles;
+  cxa::DevBuf d_blks, d_segs;
+  PlanVariant var[3];  // indexed by CXA_PREC_*
   bool copyA = false, copyB = false;
   std::vector<cxa::CopyJob> jobsA, jobsB;
   int64_t wsA = 0, wsB = 0;  // workspace elements for A and B copies
   double flops = 0;
   int64_t npairs = 0;
-  int tile_kind = -1;
 };
 
 namespace cxa {
 namespace {
 
-int upload_plan(cxa_bc_plan *p) {
-  make_tiles(p->blks, p->segs, p->tiles, p->tile_kind);
-  for (auto *d : {&p->d_blks, &p->d_segs, &p->d_tiles}) {
-    if (d->p) {
-      CXA_HIP(hipFree(d->p));
-      d->p = nullptr;
-    }
+int build_variant(cxa_bc_plan *p, int prec) {
+  const bool real = p->dtype == CXA_F64 || p->dtype == CXA_F32;
+  PlanVariant &v = p->var[prec];
+  v.tile_class = make_tiles(p->blks, p->segs, v.tiles, v.kind, real);
+  v.splitk = real && v.tile_class != 2 &&
+             (v.split_mode == 1 ||
+              (v.split_mode < 0 && prefer_splitk(p->blks, p->segs, v.tiles, v.tile_class)));
+  if (v.d_tiles.p) {
+    CXA_HIP(hipFree(v.d_tiles.p));
+    v.d_tiles.p = nullptr;
   }
+  return upload(v.d_tiles, v.tiles);
+}
+
+int upload_plan(cxa_bc_plan *p) {
   int rc;
-  if ((rc = upload(p->d_blks, p->blks)) || (rc = upload(p->d_segs, p->segs)) ||
-      (rc = upload(p->d_tiles, p->tiles)))
-    return rc;
+  if ((rc = upload(p->d_blks, p->blks)) || (rc = upload(p->d_segs, p->segs))) return rc;
+  for (int prec = 0; prec < 3; ++prec)
+    if ((rc = build_variant(p, prec))) return rc;
   return CXA_OK;
 }
 
@@ -322,11 +340,42 @@ extern "C" int cxa_block_contract_info(const cxa_bc_plan *p, double *flops,
   return CXA_OK;
 }
 
+extern "C" int cxa_block_contract_set_config(cxa_bc_plan *p, int prec, int kind, int split) {
+  CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_set_config: null plan");
+  CXA_CHECK(prec >= -1 && prec <= CXA_PREC_FP32X3, CXA_ERR_INVALID, "unknown precision %d", prec);
+  CXA_CHECK(kind >= -1 && kind < TK_COUNT, CXA_ERR_INVALID, "unknown tile kind %d", kind);
+  CXA_CHECK(split >= -1 && split <= 1, CXA_ERR_INVALID, "split mode must be -1, 0 or 1");
+  for (int q = 0; q < 3; ++q) {
+    if (prec >= 0 && q != prec) continue;
+    if (p->var[q].kind == kind && p->var[q].split_mode == split && p->var[q].d_tiles.p) continue;
+    p->var[q].kind = kind;
+    p->var[q].split_mode = split;
+    int rc = build_variant(p, q);
+    if (rc) return rc;
+  }
+  return CXA_OK;
+}
+
+extern "C" int cxa_block_contract_get_config(const cxa_bc_plan *p, int prec, int *kind,
+                                             int *split, int *ntiles) {
+  CXA_CHECK(p && prec >= 0 && prec <= CXA_PREC_FP32X3, CXA_ERR_INVALID,
+            "cxa_block_contract_get_config: bad arguments");
+  const PlanVariant &v = p->var[prec];
+  if (kind) *kind = v.kind;
+  if (split) *split = v.splitk ? 1 : 0;
+  if (ntiles) *ntiles = (int)v.tiles.size();
+  return CXA_OK;
+}
+
 extern "C" int cxa_block_contract_set_tile(cxa_bc_plan *p, int kind) {
   CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_set_tile: null plan");
   CXA_CHECK(kind >= -1 && kind < TK_COUNT, CXA_ERR_INVALID, "unknown tile kind %d", kind);
-  p->tile_kind = kind;
-  return upload_plan(p);
+  for (int q = 0; q < 3; ++q) {
+    p->var[q].kind = kind;
+    int rc = build_variant(p, q);
+    if (rc) return rc;
+  }
+  return CXA_OK;
 }
 
 namespace cxa {
@@ -364,10 +413,13 @@ int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, i
   g.A = p->copyA ? (const void *)ws : L;
   g.B = p->copyB ? (const void *)wsB : R;
   g.C = C;
-  g.tiles = (const GTile *)p->d_tiles.p;
+  const PlanVariant &v = p->var[prec_eff];
+  g.tiles = (const GTile *)v.d_tiles.p;
   g.blks = (const GBlk *)p->d_blks.p;
   g.segs = (const GSeg *)p->d_segs.p;
-  g.ntiles = (int)p->tiles.size();
+  g.ntiles = (int)v.tiles.size();
+  g.splitk = v.splitk;
+  g.tile_class = v.tile_class;
   return launch_grouped_gemm(g, s);
 }
 
@@ -376,6 +428,17 @@ double bc_flops(const cxa_bc_plan *p) { return p->flops; }
 int bc_dtype(const cxa_bc_plan *p) { return p->dtype; }
 
 }  // namespace cxa
+
+extern "C" int cxa_block_contract_set_split(cxa_bc_plan *p, int mode) {
+  CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_set_split: null plan");
+  CXA_CHECK(mode >= -1 && mode <= 1, CXA_ERR_INVALID, "split mode must be -1, 0 or 1");
+  for (int q = 0; q < 3; ++q) {
+    p->var[q].split_mode = mode;
+    int rc = build_variant(p, q);
+    if (rc) return rc;
+  }
+  return CXA_OK;
+}
 
 extern "C" int cxa_block_contract_exec(const cxa_bc_plan *p, const void *L, const void *R,
                                        void *C, void *workspace, int prec, cxa_stream_t stream) {
@@ -480,7 +543,7 @@ extern "C" int cxa_gemm_batch(int dtype, const char *transa, const char *transb,
     }
   }
   std::vector<GTile> tiles;
-  make_tiles(blks, segs, tiles);
+  const int cls = make_tiles(blks, segs, tiles, -1, dtype == CXA_F64 || dtype == CXA_F32);
   if (tiles.empty()) return CXA_OK;
   hipStream_t s = (hipStream_t)stream;
   DevBuf d_blks, d_segs, d_tiles;
@@ -497,6 +560,7 @@ extern "C" int cxa_gemm_batch(int dtype, const char *transa, const char *transb,
   g.blks = (const GBlk *)d_blks.p;
   g.segs = (const GSeg *)d_segs.p;
   g.ntiles = (int)tiles.size();
+  g.tile_class = cls;
   rc = launch_grouped_gemm(g, s);
   if (rc) return rc;
   // the task arrays are freed on return: wait for the launch that reads them

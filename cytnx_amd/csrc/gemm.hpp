@@ -33,14 +33,17 @@ struct GBlk {
   int32_t seg0, nseg;
 };
 
-// wave-tile shapes (rows x cols in 16-wide MFMA blocks)
-enum TileKind : int32_t { TK_1x1 = 0, TK_2x2 = 1, TK_2x1 = 2, TK_1x2 = 3, TK_4x2 = 4, TK_COUNT };
+// tile shapes: MFMA wave tiles (rows x cols) and TK_ROWS, 64 rows of a block with N <= 8
+// streamed one row per lane
+enum TileKind : int32_t {
+  TK_1x1 = 0, TK_2x2 = 1, TK_2x1 = 2, TK_1x2 = 3, TK_4x2 = 4, TK_ROWS = 5, TK_COUNT
+};
 inline int tile_rows(int kind) {
-  static const int r[TK_COUNT] = {16, 32, 32, 16, 64};
+  static const int r[TK_COUNT] = {16, 32, 32, 16, 64, 64};
   return r[kind];
 }
 inline int tile_cols(int kind) {
-  static const int c[TK_COUNT] = {16, 32, 16, 32, 32};
+  static const int c[TK_COUNT] = {16, 32, 16, 32, 32, 8};
   return c[kind];
 }
 
@@ -58,6 +61,8 @@ struct GemmLaunch {
   const GSeg *segs = nullptr;
   int ntiles = 0;
   int grid = 0;  // workgroups; 0 = pick from ntiles
+  bool splitk = false;  // one workgroup per tile, K chunks interleaved over its waves
+  int tile_class = 0;   // the class make_tiles returned
   // single-block mode: one block with one segment passed by value, no device task arrays
   bool single = false;
   GBlk blk0{};
@@ -69,9 +74,16 @@ struct GemmLaunch {
 int launch_grouped_gemm(const GemmLaunch &g, hipStream_t s);
 
 // Host-side planning helper: cut every block into tiles of the chosen kind and order tiles by
-// decreasing work (longest K first) so the persistent loop balances.
-void make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
-                std::vector<GTile> &tiles, int kind_override = -1);
+// decreasing work (longest K first) so the persistent loop balances.  All tiles of a launch
+// share one class (its kernel instantiation), returned: 0 = MFMA tiles up to 32x32, 1 = row
+// streaming (taken when every block has N <= 8 and allow_rows), 2 = 64x32 MFMA tiles (only on
+// request).  allow_rows is false for complex dtypes.
+int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
+               std::vector<GTile> &tiles, int kind_override = -1, bool allow_rows = true);
+
+// Whether a tile list is better run with workgroup split-K (few tiles, long K).
+bool prefer_splitk(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
+                   const std::vector<GTile> &tiles, int tile_class);
 
 // Tile kind used for a block of M x N when no tuned choice is given.
 int default_kind(int M, int N);

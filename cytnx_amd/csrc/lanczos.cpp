@@ -20,6 +20,7 @@
 
 #include "contract.hpp"
 #include "cxa_common.hpp"
+#include "gemm.hpp"
 #include "vecops.hpp"
 
 using namespace cxa;
@@ -268,6 +269,111 @@ extern "C" int cxa_chain_flops(const cxa_chain *c, double *flops) {
 
 extern "C" int cxa_chain_destroy(cxa_chain *c) {
   delete c;
+  return CXA_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Tile-shape autotuner: for every step of the chain, at the chain's current precision, time
+// each candidate tiling (tile shape x workgroup split-K) on the chain's own buffers and keep the
+// fastest.  The choice is stored in the step's plan for that precision only.
+// ------------------------------------------------------------------------------------------
+extern "C" int cxa_chain_tune(cxa_chain *c, const void *x, void *y, int reps, int *kinds,
+                              int *splits, double *ms, cxa_stream_t stream) {
+  CXA_CHECK(c && x && y && reps >= 1, CXA_ERR_INVALID, "cxa_chain_tune: bad arguments");
+  CXA_CHECK(c->prec != CXA_PREC_FP32 || !c->fixed32.empty(), CXA_ERR_INVALID,
+            "cxa_chain_tune: FP32 copies missing; call cxa_chain_prepare first");
+  hipStream_t s = (hipStream_t)stream;
+  static const int cand[][2] = {{-1, -1}, {-1, 0}, {-1, 1}, {TK_1x1, 0}, {TK_1x1, 1},
+                                {TK_2x1, 0}, {TK_2x1, 1}, {TK_1x2, 0}, {TK_1x2, 1},
+                                {TK_2x2, 0}, {TK_2x2, 1}, {TK_4x2, 0}, {TK_ROWS, 0},
+                                {TK_ROWS, 1}};
+  const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
+  hipEvent_t e0, e1;
+  CXA_HIP(hipEventCreate(&e0));
+  CXA_HIP(hipEventCreate(&e1));
+  int rc = chain_apply(c, x, y, s);  // every intermediate in place
+  for (int k = 0; k < c->nsteps && rc == CXA_OK; ++k) {
+    cxa_bc_plan *plan = const_cast<cxa_bc_plan *>(c->plans[k]);
+    double best = 1e30;
+    int bk = -1, bs = -1;
+    for (int ci = 0; ci < ncand && rc == CXA_OK; ++ci) {
+      if ((rc = cxa_block_contract_set_config(plan, c->prec, cand[ci][0], cand[ci][1]))) break;
+      for (int w = 0; w < 2 && rc == CXA_OK; ++w) rc = chain_step(c, k, x, y, s);
+      if (rc) break;
+      if (hipEventRecord(e0, s) != hipSuccess) {
+        rc = fail(CXA_ERR_HIP, "cxa_chain_tune: event record failed");
+        break;
+      }
+      for (int r = 0; r < reps && rc == CXA_OK; ++r) rc = chain_step(c, k, x, y, s);
+      if (rc) break;
+      float t = 0;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+        rc = fail(CXA_ERR_HIP, "cxa_chain_tune: event timing failed");
+        break;
+      }
+      const double per = (double)t / reps;
+      if (per < best) {
+        best = per;
+        bk = cand[ci][0];
+        bs = cand[ci][1];
+      }
+    }
+    if (rc == CXA_OK) rc = cxa_block_contract_set_config(plan, c->prec, bk, bs);
+    if (kinds) kinds[k] = bk;
+    if (splits) splits[k] = bs;
+    if (ms) ms[k] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+// ------------------------------------------------------------------------------------------
+// A chain application captured as a graph for fixed x / y (repeats applications back to back),
+// replayed with one launch.
+// ------------------------------------------------------------------------------------------
+struct cxa_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+  ~cxa_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+};
+
+extern "C" int cxa_chain_graph_create(cxa_chain *c, const void *x, void *y, int repeats,
+                                      cxa_graph **out) {
+  CXA_CHECK(c && x && y && out && repeats >= 1, CXA_ERR_INVALID,
+            "cxa_chain_graph_create: bad arguments");
+  CXA_CHECK(c->prec != CXA_PREC_FP32 || !c->fixed32.empty(), CXA_ERR_INVALID,
+            "cxa_chain_graph_create: FP32 copies missing; call cxa_chain_prepare first");
+  *out = nullptr;
+  std::unique_ptr<cxa_graph> g(new cxa_graph());
+  CXA_HIP(hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking));
+  CXA_HIP(hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal));
+  int rc = CXA_OK;
+  for (int r = 0; r < repeats && rc == CXA_OK; ++r) rc = chain_apply(c, x, y, g->cap);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(g->cap, &graph);
+  g->graph = graph;
+  if (rc) return rc;
+  CXA_HIP(e);
+  CXA_HIP(hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0));
+  *out = g.release();
+  return CXA_OK;
+}
+
+extern "C" int cxa_graph_launch(cxa_graph *g, cxa_stream_t stream) {
+  CXA_CHECK(g && g->exec, CXA_ERR_INVALID, "cxa_graph_launch: null graph");
+  CXA_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return CXA_OK;
+}
+
+extern "C" int cxa_graph_destroy(cxa_graph *g) {
+  delete g;
   return CXA_OK;
 }
 

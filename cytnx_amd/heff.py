@@ -29,8 +29,8 @@ import torch
 from . import _lib
 from .device import stream_handle, torch_device
 from .lanczos import LinOp, lanczos_gnd_native
-from .unitensor import (BD_BRA, BD_KET, PREC_FP32, PREC_FP64, Bond, UniTensor, contract_meta,
-                        meta_elems, plan_contraction)
+from .unitensor import (BD_BRA, BD_KET, PREC_FP32, PREC_FP64, Bond, Contract, UniTensor,
+                        contract_meta, meta_elems, plan_contraction)
 
 
 class _Chain:
@@ -65,10 +65,36 @@ class _Chain:
         _lib.check(_lib.lib().cxa_chain_matvec(self.handle, x.data_ptr(), y.data_ptr(),
                                                stream_handle(self.device)))
 
+    def graph(self, x: torch.Tensor, y: torch.Tensor, repeats: int = 1) -> "ChainGraph":
+        """The chain applied to fixed x -> y, captured once and replayed with one launch."""
+        return ChainGraph(self, x, y, repeats)
+
     def __del__(self):
         try:
             if self.handle:
                 _lib.lib().cxa_chain_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class ChainGraph:
+    """Captured chain application (cxa_chain_graph_create); keeps its vectors alive."""
+
+    def __init__(self, chain: _Chain, x: torch.Tensor, y: torch.Tensor, repeats: int):
+        self.chain, self.x, self.y, self.repeats = chain, x, y, repeats
+        h = ctypes.c_void_p(0)
+        torch.cuda.synchronize(chain.device)
+        _lib.check(_lib.lib().cxa_chain_graph_create(chain.handle, x.data_ptr(), y.data_ptr(),
+                                                     int(repeats), ctypes.byref(h)))
+        self.handle = h.value
+
+    def launch(self) -> None:
+        _lib.check(_lib.lib().cxa_graph_launch(self.handle, stream_handle(self.chain.device)))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.lib().cxa_graph_destroy(self.handle)
         except Exception:
             pass
 
@@ -103,6 +129,22 @@ class _HeffBase(LinOp):
         self.apply_flat(v._storage, out._storage)
         return out
 
+    def tune(self, reps: int = 20) -> List[dict]:
+        """Autotune every step's tiling at the current precision on this operator's own
+        buffers (cxa_chain_tune); returns the per-step choice and its time."""
+        self.chain.prepare(self.precision)
+        n = len(self.chain.steps)
+        x = self.psi_template._storage if self.psi_template._storage.numel() else None
+        if x is None:
+            x = torch.zeros(self.nx(), dtype=torch.float64, device=torch_device(self.device()))
+        y = torch.empty_like(x)
+        kinds, splits = (ctypes.c_int * n)(), (ctypes.c_int * n)()
+        ms = (ctypes.c_double * n)()
+        _lib.check(_lib.lib().cxa_chain_tune(self.chain.handle, x.data_ptr(), y.data_ptr(),
+                                             int(reps), kinds, splits, ms,
+                                             stream_handle(self.device())))
+        return [{"kind": kinds[i], "split": splits[i], "ms": ms[i]} for i in range(n)]
+
     def native_matvec(self):
         """(function pointer, context) for the C Lanczos driver."""
         self.chain.prepare(self.precision)
@@ -116,10 +158,15 @@ class _HeffBase(LinOp):
 
 
 class Heff2(_HeffBase):
-    """Two-site U(1) effective Hamiltonian (see module docstring)."""
+    """Two-site U(1) effective Hamiltonian (see module docstring).
+
+    With ``fuse_mpo`` (default) the two MPO tensors are contracted once into the two-site MPO
+    W12 = M1.M2 (over the shared MPO bond), and the matvec runs three grouped launches:
+    L.psi, .W12, .R -- the intermediate of the M1 step is never stored.  ``flops`` is always the
+    algorithmic count of the reference's four-step order L.psi -> M1 -> M2 -> R."""
 
     def __init__(self, psi: UniTensor, L: UniTensor, M1: UniTensor, M2: UniTensor,
-                 R: UniTensor):
+                 R: UniTensor, fuse_mpo: bool = True):
         for T in (psi, L, M1, M2, R):
             if not T.is_blockform():
                 raise TypeError("Heff2 takes U(1) block tensors")
@@ -134,23 +181,38 @@ class Heff2(_HeffBase):
         Lp = L.permute([lab_l, lab_w, lab_lp])   # (l, w, l')
         Rp = R.permute([lab_r, lab_w2, lab_rp])  # (r, w'', r')
         psi_m = psi.clone_meta()
-        steps = []
+        ref_steps = []
         X, c1, c2 = contract_meta(Lp, psi_m, 3, dev)
-        steps.append((plan_contraction(Lp, psi_m, X, c1, c2, 3), Lp, True, X))
+        ref_steps.append((plan_contraction(Lp, psi_m, X, c1, c2, 3), Lp, True, X))
         Y, c1, c2 = contract_meta(X, M1, 3, dev)
-        steps.append((plan_contraction(X, M1, Y, c1, c2, 3), M1, False, Y))
+        ref_steps.append((plan_contraction(X, M1, Y, c1, c2, 3), M1, False, Y))
         Z, c1, c2 = contract_meta(Y, M2, 3, dev)
-        steps.append((plan_contraction(Y, M2, Z, c1, c2, 3), M2, False, Z))
+        ref_steps.append((plan_contraction(Y, M2, Z, c1, c2, 3), M2, False, Z))
         O, c1, c2 = contract_meta(Z, Rp, 3, dev)
-        steps.append((plan_contraction(Z, Rp, O, c1, c2, 3), Rp, False, O))
+        ref_steps.append((plan_contraction(Z, Rp, O, c1, c2, 3), Rp, False, O))
         if O.itoi() != psi.itoi() or [b.dim() for b in O.bonds()] != psi.shape():
             raise ValueError("Heff2: output block layout differs from psi's")
+        self.ref_flops = float(sum(st[0].flops for st in ref_steps))
+        self.pairs = [st[0].npairs for st in ref_steps]
+        steps = ref_steps
+        if fuse_mpo:
+            W12 = Contract(M1, M2)
+            Zf, c1, c2 = contract_meta(X, W12, 3, dev)
+            if Zf.labels() != Z.labels() or Zf.itoi() != Z.itoi():
+                raise ValueError("Heff2: fused MPO step changed the intermediate layout")
+            steps = [ref_steps[0], (plan_contraction(X, W12, Zf, c1, c2, 3), W12, False, Zf),
+                     ref_steps[3]]
+        self.fused = bool(fuse_mpo)
         self.out_labels = O.labels()
-        self.steps_meta = [X, Y, Z, O]
-        self.pairs = [s[0].npairs for s in steps]
-        self.copies = [s[0].ws_bytes for s in steps]
+        self.copies = [st[0].ws_bytes for st in steps]
+        self.step_names = (["L.psi", ".W12", ".R"] if fuse_mpo
+                           else ["L.psi", ".M1", ".M2", ".R"])
         self.chain = _Chain(steps, dev)
         super().__init__(meta_elems(psi), dev)
+
+    @property
+    def flops(self) -> float:
+        return self.ref_flops
 
 
 def _trivial_bond(dim: int, btype: int) -> Bond:

@@ -155,8 +155,16 @@ int cxa_block_contract_plan(int dtype, int rankL, int nbL, const uint64_t *itoiL
 /* flops: sum over block pairs of 2*M*N*K (x4 for complex); npairs: number of block pairs. */
 int cxa_block_contract_info(const cxa_bc_plan *plan, double *flops, int64_t *workspace_bytes,
                             int64_t *npairs);
-/* Override the wave-tile shape of every tile (-1 = per-block default); used by the tuner. */
+/* Override the tile shape of every tile (-1 = per-block default: MFMA 16x16..32x32 tiles,
+ * row streaming for blocks with N <= 8) and the workgroup split-K mode (-1 automatic, 0 off,
+ * 1 on); used by the tuner. */
 int cxa_block_contract_set_tile(cxa_bc_plan *plan, int kind);
+int cxa_block_contract_set_split(cxa_bc_plan *plan, int mode);
+/* The same for one precision mode only (prec = -1: all); get returns the chosen shape, whether
+ * split-K is on and the tile count for a precision. */
+int cxa_block_contract_set_config(cxa_bc_plan *plan, int prec, int kind, int split);
+int cxa_block_contract_get_config(const cxa_bc_plan *plan, int prec, int *kind, int *split,
+                                  int *ntiles);
 int cxa_block_contract_exec(const cxa_bc_plan *plan, const void *L, const void *R, void *C,
                             void *workspace, int prec, cxa_stream_t stream);
 int cxa_block_contract_destroy(cxa_bc_plan *plan);
@@ -206,8 +214,26 @@ int cxa_chain_prepare(cxa_chain *chain, int prec, cxa_stream_t stream);
 int cxa_chain_matvec(void *chain, const void *x, void *y, cxa_stream_t stream);
 /* one step only (step 0 reads x, the last writes y): per-kernel timing */
 int cxa_chain_step(void *chain, int step, const void *x, void *y, cxa_stream_t stream);
+/* Autotune: for each step at the chain's current precision, time every candidate tiling
+ * (tile shape x workgroup split-K) `reps` times on x -> y and keep the fastest; per-step
+ * choices and times (ms) are returned in kinds/splits/ms (arrays of nsteps, may be NULL).
+ * Synchronizes the stream. */
+int cxa_chain_tune(cxa_chain *chain, const void *x, void *y, int reps, int *kinds, int *splits,
+                   double *ms, cxa_stream_t stream);
 int cxa_chain_flops(const cxa_chain *chain, double *flops);
 int cxa_chain_destroy(cxa_chain *chain);
+
+/*
+ * The chain applied `repeats` times to fixed device vectors x -> y, captured once as a graph
+ * (current precision) and replayed by cxa_graph_launch on any stream: one host launch per
+ * replay instead of one per grouped GEMM.  No counterpart in the reference (its GPU path is
+ * synchronous per call).
+ */
+typedef struct cxa_graph cxa_graph;
+int cxa_chain_graph_create(cxa_chain *chain, const void *x, void *y, int repeats,
+                           cxa_graph **graph);
+int cxa_graph_launch(cxa_graph *graph, cxa_stream_t stream);
+int cxa_graph_destroy(cxa_graph *graph);
 
 /*
  * Ground-state Lanczos on FP64 device vectors of length n.  Replaces linalg::Lanczos(...,

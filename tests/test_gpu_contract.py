@@ -167,3 +167,38 @@ def test_contract_rejects_bra_bra():
     B = UniTensor([b1, b1.redirect()], ["c", "b"], device=0)  # BRA meets BRA on "b"
     with pytest.raises(ValueError):
         Contract(A, B)
+
+
+@pytest.mark.parametrize("kind,split", [(-1, 0), (-1, 1), (5, -1), (0, 1), (1, 1), (2, 1),
+                                        (3, 1), (4, 0), (5, 1)])
+def test_block_contract_forced_kernel_variants(kind, split):
+    """Every tile shape, the row-streaming path and workgroup split-K give the oracle's answer
+    (FP64 1e-12), including on blocks whose N exceeds the row path's limit."""
+    from cytnx_amd import _lib
+    from cytnx_amd.device import stream_handle
+    from cytnx_amd.unitensor import contract_meta, plan_contraction
+    rng = np.random.default_rng(200 + kind + 10 * split)
+    bl = Bond(BD_KET, [[-1], [0], [2], [1]], [2, 3, 1, 37])
+    bw = Bond(BD_KET, [[0], [2], [-2]], [3, 1, 1])
+    br = Bond(BD_BRA, [[-2], [-1], [0], [1], [3], [2]], [1, 20, 3, 2, 1, 33])
+    A = _rand_ut([bl, bw.redirect(), br], ["l", "w", "r"], 2, rng)
+    B = _rand_ut([br.redirect(), Bond(BD_BRA, [[-2], [0], [2], [1], [3]], [2, 70, 1, 3, 5])],
+                 ["r", "x"], 1, rng)
+    C, c1, c2 = contract_meta(A, B)
+    plan = plan_contraction(A, B, C, c1, c2, 3)
+    L = _lib.lib()
+    try:
+        _lib.check(L.cxa_block_contract_set_tile(plan.handle, kind))
+        _lib.check(L.cxa_block_contract_set_split(plan.handle, split))
+        out = torch.full((int(np.sum(np.prod(C._shapes.astype(np.int64), axis=1))),), np.nan,
+                         dtype=torch.float64, device="cuda:0")
+        _lib.check(L.cxa_block_contract_exec(plan.handle, A._storage.data_ptr(),
+                                             B._storage.data_ptr(), out.data_ptr(),
+                                             plan.workspace(), 0, stream_handle(0)))
+        C._storage = out
+        ref = po.block_contract(_oracle_of(A), _oracle_of(B))
+        assert relerr(C.to_dense(), ref.to_dense()) <= 1e-12
+        assert not torch.isnan(out).any()
+    finally:
+        _lib.check(L.cxa_block_contract_set_tile(plan.handle, -1))
+        _lib.check(L.cxa_block_contract_set_split(plan.handle, -1))

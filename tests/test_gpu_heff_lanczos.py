@@ -49,10 +49,11 @@ def problem(chi, seed=1, hermitian=True):
     return [t.to(0) for t in (psi, L, M1, M2, R)]
 
 
+@pytest.mark.parametrize("fuse", [True, False])
 @pytest.mark.parametrize("chi", [16, 64, 160])
-def test_heff2_fp64_matches_oracle_chain(chi):
+def test_heff2_fp64_matches_oracle_chain(chi, fuse):
     psi, L, M1, M2, R = problem(chi, seed=chi)
-    H = Heff2(psi, L, M1, M2, R)
+    H = Heff2(psi, L, M1, M2, R, fuse_mpo=fuse)
     print("heff2 chi", chi, "pairs", H.pairs, "copy bytes", H.copies, flush=True)
     assert all(c == 0 for c in H.copies)  # every operand read in place
     out = H.matvec(psi)
@@ -95,9 +96,22 @@ def test_heff2_dense_closed_form_and_symmetry():
     assert np.abs(Hm - Hm.T).max() <= 1e-12 * np.abs(Hm).max()  # hermitised environments
 
 
+def test_heff2_tuned_tilings_match_oracle_at_chi512():
+    """After the autotuner picks tilings (MFMA shapes, split-K, row streaming) for FP64, the
+    matvec still equals the oracle's chain at 1e-12."""
+    psi, L, M1, M2, R = problem(512, seed=21)
+    H = Heff2(psi, L, M1, M2, R)
+    choice = H.tune(reps=3)
+    print("tuned", choice, flush=True)
+    out = H.matvec(psi)
+    ref = oracle_heff2(psi, L, M1, M2, R)
+    assert relerr(out._storage.cpu().numpy(), ref.flat()) <= 1e-12
+
+
 @pytest.mark.parametrize("prec,tol", [(1, 5e-6), (2, 5e-6)])
 def test_heff2_reduced_precision_error_levels(prec, tol):
-    psi, L, M1, M2, R = problem(64, seed=5)
+    # chi=384: sectors up to 31 wide, so the MFMA tiles (not only row streaming) run
+    psi, L, M1, M2, R = problem(384, seed=5)
     H = Heff2(psi, L, M1, M2, R)
     y64 = H.matvec(psi)._storage.cpu().numpy()
     H.set_precision(prec)
