@@ -4,7 +4,11 @@
 Workload: the U(1) two-site effective-Hamiltonian matvec of DMRG at bond dimension chi=1024
 (31 charge sectors q in [-15, 15], sizes proportional to exp(-q^2/50); spin-1/2 Heisenberg MPO,
 D=5 grouped as charges {0 x3, +2, -2}; iid N(0,1) blocks, environments symmetrised so the
-operator is Hermitian), contracted L.psi -> W1 -> W2 -> R as four grouped MFMA launches.
+operator is Hermitian).  The two MPO tensors are contracted once into W12, and each matvec is
+two launches (--chain mix, default): the mix launch L.psi.W12 -> Y, whose L.psi slices stay on
+chip, then the grouped GEMM Y.R.  --chain w12 runs three launches (L.psi, .W12, .R), --chain ref
+the reference's four.  The flop count is always the reference's four-step
+L.psi -> W1 -> W2 -> R count.
 
 A "step" is one matvec.  ``value`` is the algorithmic TFLOP/s of the FP32 matvec (the bulk of
 the FP32 + refinement scheme), with inputs resident in HBM; FP64 and FP32x3 rates, the energy
@@ -24,6 +28,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp32x3": 157.3}
+PEAK_HBM_GBS = 8000.0
 METRIC = "U(1) Heff matvec TFLOP/s & %peak, FP32+refine vs FP64 at ΔE≤10^-10"
 
 
@@ -78,6 +83,23 @@ def time_steps(H, x, y, reps, torch):
         b.record()
         torch.cuda.synchronize()
         out.append(a.elapsed_time(b) / reps)
+    return out
+
+
+def step_bytes(H, n_x, prec):
+    """Algorithmic bytes of each chain step: its running operand read once, its fixed operand
+    read once, its output written once.  In FP32 mode the fixed operands and intermediates are
+    FP32 copies, the input vector and the final output FP64."""
+    from cytnx_amd.unitensor import PREC_FP32, meta_elems
+    mid = 4 if prec == PREC_FP32 else 8
+    out = []
+    n_in, b_in = n_x, 8
+    nsteps = len(H.chain.steps)
+    for k, (plan, fixed, left, meta) in enumerate(H.chain.steps):
+        n_out = meta_elems(meta)
+        b_out = 8 if (k == nsteps - 1 or prec != PREC_FP32) else mid
+        out.append(n_in * b_in + fixed._storage.numel() * mid + n_out * b_out)
+        n_in, b_in = n_out, b_out
     return out
 
 
@@ -148,7 +170,16 @@ def main() -> int:
     ap.add_argument("--lanczos-cvg", type=float, default=1e-12)
     ap.add_argument("--lanczos-maxiter", type=int, default=600)
     ap.add_argument("--profile", action="store_true",
-                    help="matvec loops only (for rocprofv3 runs)")
+                    help="matvec loops only (for rocprofv3 runs), plus one 256 MiB device "
+                         "copy whose counters calibrate FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--modes", default="fp64,fp32,fp32x3",
+                    help="comma-separated precisions to time (fp32 is the reported value)")
+    ap.add_argument("--save-tiles", default=None, help="write the tuned tilings (JSON)")
+    ap.add_argument("--load-tiles", default=None,
+                    help="apply tilings from a --save-tiles file instead of tuning")
+    ap.add_argument("--chain", default="mix", choices=["mix", "w12", "ref"],
+                    help="matvec program: mix (two launches), w12 (three), ref (four, the "
+                         "reference's contraction order)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -171,19 +202,34 @@ def main() -> int:
     host = heisenberg.two_site_problem(args.chi, args.seed, hermitian=True)
     problem = [t.to(0) for t in host]
     psi, L, M1, M2, R = problem
-    H = Heff2(psi, L, M1, M2, R)
+    H = Heff2(psi, L, M1, M2, R, fuse_mpo=args.chain != "ref", mix=args.chain == "mix")
+    if H.mode != args.chain:
+        print("bench.py: chain %s unavailable for this operator" % args.chain, file=sys.stderr)
+        return 2
     flops = H.flops  # the reference order's four-step count
     step_flops = [p.flops for p in H.chain._plans]  # per launched kernel
     x = psi._storage
     y = torch.empty_like(x)
     log("flops/matvec %.4e, block pairs per step %s, psi elements %d" % (flops, H.pairs, x.numel()))
 
-    modes = {"fp64": PREC_FP64, "fp32": PREC_FP32, "fp32x3": PREC_FP32X3}
+    all_modes = {"fp64": PREC_FP64, "fp32": PREC_FP32, "fp32x3": PREC_FP32X3}
+    wanted = [m.strip() for m in args.modes.split(",") if m.strip()]
+    if "fp32" not in wanted or any(m not in all_modes for m in wanted):
+        print("--modes must list fp32 and only fp64 / fp32 / fp32x3", file=sys.stderr)
+        return 2
+    modes = {m: all_modes[m] for m in wanted}
     res = {}
     tuned = {}
+    loaded = None
+    if args.load_tiles:
+        with open(args.load_tiles) as f:
+            loaded = json.load(f)
     for name, prec in modes.items():
         H.set_precision(prec)
-        if not args.no_tune:
+        if loaded is not None and name in loaded:
+            H.set_tiles(loaded[name])
+            tuned[name] = loaded[name]
+        elif not args.no_tune:
             tuned[name] = H.tune(reps=20)
             log(name, "tuned", json.dumps(tuned[name]))
         ms, wall = time_matvec(H, x, y, args.steps, args.warmup, torch, graph=True)
@@ -193,12 +239,22 @@ def main() -> int:
                      "pct_peak": 100.0 * flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS[name],
                      "ms_per_step_host_launched": ms_eager, "step_ms": per, "wall_s": wall}
         log(name, json.dumps(res[name]))
+    if args.save_tiles:
+        with open(args.save_tiles, "w") as f:
+            json.dump(tuned, f)
+    if args.profile:
+        # calibration for HBM counters: one copy of exactly 256 MiB read + 256 MiB written
+        a = torch.ones(32 << 20, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     # normwise error of the reduced-precision matvecs against FP64 on the benchmark input
     errs = {}
     H.set_precision(PREC_FP64)
     y64 = torch.empty_like(x)
     H.apply_flat(x, y64)
-    for name in ("fp32", "fp32x3"):
+    for name in [m for m in ("fp32", "fp32x3") if m in modes]:
         H.set_precision(modes[name])
         yy = torch.empty_like(x)
         H.apply_flat(x, yy)
@@ -230,12 +286,23 @@ def main() -> int:
     main_mode = "fp32"
     k_dom = max(range(len(step_flops)), key=lambda k: res[main_mode]["step_ms"][k])
     dom_ms = res[main_mode]["step_ms"][k_dom]
-    achieved = step_flops[k_dom] / (dom_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[main_mode],
-                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_mode], "traffic": None,
-                "kernel": "grouped GEMM, chain step %d of %d (%s)" % (
-                    k_dom, len(step_flops), H.step_names[k_dom]),
-                "kernel_flops": step_flops[k_dom], "kernel_ms": dom_ms}
+    sbytes = step_bytes(H, x.numel(), modes[main_mode])
+    # the kernel's own bound: the larger of its MFMA time and its HBM time at peak
+    t_mfma = step_flops[k_dom] / (PEAK_TFLOPS[main_mode] * 1e12)
+    t_hbm = sbytes[k_dom] / (PEAK_HBM_GBS * 1e9)
+    if t_hbm > t_mfma:
+        achieved = sbytes[k_dom] / (dom_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": achieved / PEAK_HBM_GBS}
+    else:
+        achieved = step_flops[k_dom] / (dom_ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[main_mode],
+                    "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_mode]}
+    roofline.update({"traffic": None,
+                     "kernel": "chain step %d of %d (%s)" % (k_dom, len(step_flops),
+                                                             H.step_names[k_dom]),
+                     "kernel_flops": step_flops[k_dom], "kernel_bytes": sbytes[k_dom],
+                     "kernel_ms": dom_ms})
 
     cpu = None
     if not args.no_cpu and not args.profile:
@@ -260,12 +327,13 @@ def main() -> int:
                                "Heisenberg MPO D=5, order L.psi->W1->W2->R" % args.chi,
                    "chi": args.chi, "sectors": len(heisenberg.gaussian_sectors(args.chi)[0]),
                    "psi_elements": int(x.numel()), "block_pairs_per_step": H.pairs,
-                   "flops_per_matvec": flops, "kernels": H.step_names,
+                   "flops_per_matvec": flops, "chain": H.mode, "kernels": H.step_names,
                    "kernel_flops": step_flops},
-        "fp64": res["fp64"],
+        "fp64": res.get("fp64"),
         "fp32": res["fp32"],
-        "fp32x3": res["fp32x3"],
-        "speedup_fp32_vs_fp64": res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"],
+        "fp32x3": res.get("fp32x3"),
+        "speedup_fp32_vs_fp64": (res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"]
+                                 if "fp64" in res else None),
         "tuned_tiles": tuned,
         "matvec_rel_error_vs_fp64": errs,
         "lanczos": lanczos,

@@ -62,6 +62,8 @@ def _declare(L):
         "cxa_block_contract_get_config": (ci, [vp, ci, intp, intp, intp]),
         "cxa_block_contract_exec": (ci, [vp, vp, vp, vp, vp, ci, vp]),
         "cxa_block_contract_destroy": (ci, [vp]),
+        "cxa_mix_contract_plan": (ci, [ci, ci, i64p, i64, i64p, i64, i64p, dblp,
+                                       ctypes.POINTER(vp)]),
         "cxa_vec_workspace_bytes": (ci, []),
         "cxa_dot": (ci, [i64, vp, vp, vp, vp, vp]),
         "cxa_nrm2": (ci, [i64, vp, vp, vp, vp]),

@@ -69,10 +69,12 @@ struct CopyJob {
 struct PlanVariant {
   int kind = -1;        // tile kind override, -1 = per-block default
   int split_mode = -1;  // -1 automatic, 0 off, 1 on
-  std::vector<cxa::GTile> tiles;
+  std::vector<cxa::GTile> tiles;  // kXcds ranges of per_xcd tiles
   cxa::DevBuf d_tiles;
+  int per_xcd = 0;
   bool splitk = false;
   int tile_class = 0;
+  int launch_kind = -1;  // mix plans: the one tile kind of the launch
 };
 
 struct cxa_bc_plan {
@@ -88,15 +90,34 @@ struct cxa_bc_plan {
   int64_t wsA = 0, wsB = 0;  // workspace elements for A and B copies
   double flops = 0;
   int64_t npairs = 0;
+  // mix plans (cxa_mix_contract_plan): blks are items, segs their slices, outs the outputs
+  bool mix = false;
+  std::vector<cxa::MixOut> outs;
+  cxa::DevBuf d_outs;
 };
 
 namespace cxa {
 namespace {
 
+// tile kinds a mix launch can run (its slices stay in registers / LDS per tile)
+bool mix_kind_ok(int kind) {
+  return kind == TK_1x1 || kind == TK_2x1 || kind == TK_1x2 || kind == TK_2x2;
+}
+
 int build_variant(cxa_bc_plan *p, int prec) {
   const bool real = p->dtype == CXA_F64 || p->dtype == CXA_F32;
   PlanVariant &v = p->var[prec];
-  v.tile_class = make_tiles(p->blks, p->segs, v.tiles, v.kind, real);
+  if (p->mix) {
+    v.launch_kind = mix_kind_ok(v.kind) ? v.kind : TK_1x2;
+    v.tile_class = make_tiles(p->blks, p->segs, v.tiles, &v.per_xcd, v.launch_kind, false);
+    v.splitk = false;
+    if (v.d_tiles.p) {
+      CXA_HIP(hipFree(v.d_tiles.p));
+      v.d_tiles.p = nullptr;
+    }
+    return upload(v.d_tiles, v.tiles);
+  }
+  v.tile_class = make_tiles(p->blks, p->segs, v.tiles, &v.per_xcd, v.kind, real);
   v.splitk = real && v.tile_class != 2 &&
              (v.split_mode == 1 ||
               (v.split_mode < 0 && prefer_splitk(p->blks, p->segs, v.tiles, v.tile_class)));
@@ -110,6 +131,7 @@ int build_variant(cxa_bc_plan *p, int prec) {
 int upload_plan(cxa_bc_plan *p) {
   int rc;
   if ((rc = upload(p->d_blks, p->blks)) || (rc = upload(p->d_segs, p->segs))) return rc;
+  if (p->mix && (rc = upload(p->d_outs, p->outs))) return rc;
   for (int prec = 0; prec < 3; ++prec)
     if ((rc = build_variant(p, prec))) return rc;
   return CXA_OK;
@@ -331,6 +353,83 @@ extern "C" int cxa_block_contract_plan(int dtype, int rankL, int nbL, const uint
   return CXA_OK;
 }
 
+// ------------------------------------------------------------------------------------------
+// Mix plan: items of one M x N shape, each the weighted sums of up to CXA_MIX_SLICES products
+// (see cytnx_amd.h).  Runs through the block-contract exec / config / destroy entry points.
+// ------------------------------------------------------------------------------------------
+extern "C" int cxa_mix_contract_plan(int dtype, int nitems, const int64_t *items,
+                                     int64_t nslices, const int64_t *slices, int64_t nouts,
+                                     const int64_t *outs, const double *coef, cxa_bc_plan **out) {
+  CXA_CHECK(out, CXA_ERR_INVALID, "cxa_mix_contract_plan: null output handle");
+  *out = nullptr;
+  CXA_CHECK(dtype == CXA_F64, CXA_ERR_UNSUPPORTED,
+            "cxa_mix_contract_plan: mix plans run real FP64 operands (dtype %d)", dtype);
+  CXA_CHECK(nitems >= 0 && nslices >= 0 && nouts >= 0 && (nitems == 0 || items) &&
+                (nslices == 0 || slices) && (nouts == 0 || (outs && coef)),
+            CXA_ERR_INVALID, "cxa_mix_contract_plan: bad arguments");
+  CXA_CHECK(CXA_MIX_SLICES == kMixSlices, CXA_ERR_INVALID, "mix slice bound mismatch");
+  std::unique_ptr<cxa_bc_plan> p(new cxa_bc_plan());
+  p->dtype = dtype;
+  p->es = elem_size(dtype);
+  p->mix = true;
+  CXA_HIP(hipGetDevice(&p->device));
+  int64_t s_at = 0, o_at = 0;
+  for (int i = 0; i < nitems; ++i) {
+    const int64_t M = items[4 * i], N = items[4 * i + 1];
+    const int64_t ns = items[4 * i + 2], no = items[4 * i + 3];
+    CXA_CHECK(M >= 0 && N >= 0 && M <= INT32_MAX && N <= INT32_MAX, CXA_ERR_INVALID,
+              "mix plan: item %d has shape %lld x %lld", i, (long long)M, (long long)N);
+    CXA_CHECK(ns >= 0 && ns <= kMixSlices, CXA_ERR_UNSUPPORTED,
+              "mix plan: item %d has %lld slices (at most %d)", i, (long long)ns, kMixSlices);
+    CXA_CHECK(no >= 0 && s_at + ns <= nslices && o_at + no <= nouts, CXA_ERR_INVALID,
+              "mix plan: item %d runs past the slice or output list", i);
+    GBlk b{};
+    b.M = (int32_t)M;
+    b.N = (int32_t)N;
+    b.alpha = 1.0;
+    b.seg0 = (int32_t)s_at;
+    b.nseg = (int32_t)ns;
+    b.out0 = (int32_t)o_at;
+    b.nout = (int32_t)no;
+    p->blks.push_back(b);
+    for (int64_t s = s_at; s < s_at + ns; ++s) {
+      const int64_t *d = slices + 7 * s;
+      CXA_CHECK(d[0] >= 0 && d[1] >= 0 && d[6] >= 0 && d[6] <= INT32_MAX, CXA_ERR_INVALID,
+                "mix plan: slice %lld has a negative offset or bad K", (long long)s);
+      GSeg sg{};
+      sg.a_off = d[0];
+      sg.b_off = d[1];
+      sg.sam = d[2];
+      sg.sak = d[3];
+      sg.sbk = d[4];
+      sg.sbn = d[5];
+      sg.K = (int32_t)d[6];
+      p->segs.push_back(sg);
+      p->flops += 2.0 * (double)M * (double)N * (double)sg.K;
+    }
+    for (int64_t o = o_at; o < o_at + no; ++o) {
+      CXA_CHECK(outs[2 * o] >= 0 && outs[2 * o + 1] >= N, CXA_ERR_INVALID,
+                "mix plan: output %lld has a negative offset or a row stride below N",
+                (long long)o);
+      MixOut mo{};
+      mo.c_off = outs[2 * o];
+      mo.ldc = outs[2 * o + 1];
+      for (int s = 0; s < kMixSlices; ++s) mo.coef[s] = s < ns ? coef[kMixSlices * o + s] : 0.0;
+      p->outs.push_back(mo);
+    }
+    s_at += ns;
+    o_at += no;
+  }
+  CXA_CHECK(s_at == nslices && o_at == nouts, CXA_ERR_INVALID,
+            "mix plan: items cover %lld slices / %lld outputs, lists hold %lld / %lld",
+            (long long)s_at, (long long)o_at, (long long)nslices, (long long)nouts);
+  p->npairs = nslices;
+  int rc = upload_plan(p.get());
+  if (rc) return rc;
+  *out = p.release();
+  return CXA_OK;
+}
+
 extern "C" int cxa_block_contract_info(const cxa_bc_plan *p, double *flops,
                                        int64_t *workspace_bytes, int64_t *npairs) {
   CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_info: null plan");
@@ -345,6 +444,9 @@ extern "C" int cxa_block_contract_set_config(cxa_bc_plan *p, int prec, int kind,
   CXA_CHECK(prec >= -1 && prec <= CXA_PREC_FP32X3, CXA_ERR_INVALID, "unknown precision %d", prec);
   CXA_CHECK(kind >= -1 && kind < TK_COUNT, CXA_ERR_INVALID, "unknown tile kind %d", kind);
   CXA_CHECK(split >= -1 && split <= 1, CXA_ERR_INVALID, "split mode must be -1, 0 or 1");
+  CXA_CHECK(!p->mix || ((kind == -1 || mix_kind_ok(kind)) && split != 1), CXA_ERR_UNSUPPORTED,
+            "mix plans run MFMA tiles up to 32x32 without split-K (kind %d, split %d)", kind,
+            split);
   for (int q = 0; q < 3; ++q) {
     if (prec >= 0 && q != prec) continue;
     if (p->var[q].kind == kind && p->var[q].split_mode == split && p->var[q].d_tiles.p) continue;
@@ -363,7 +465,11 @@ extern "C" int cxa_block_contract_get_config(const cxa_bc_plan *p, int prec, int
   const PlanVariant &v = p->var[prec];
   if (kind) *kind = v.kind;
   if (split) *split = v.splitk ? 1 : 0;
-  if (ntiles) *ntiles = (int)v.tiles.size();
+  if (ntiles) {
+    int n = 0;
+    for (const GTile &t : v.tiles) n += t.blk >= 0;
+    *ntiles = n;
+  }
   return CXA_OK;
 }
 
@@ -418,11 +524,17 @@ int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, i
   g.blks = (const GBlk *)p->d_blks.p;
   g.segs = (const GSeg *)p->d_segs.p;
   g.ntiles = (int)v.tiles.size();
+  g.per_xcd = v.per_xcd;
   g.splitk = v.splitk;
   g.tile_class = v.tile_class;
+  if (p->mix) {
+    g.outs = (const MixOut *)p->d_outs.p;
+    g.kind0 = v.launch_kind;
+  }
   return launch_grouped_gemm(g, s);
 }
 
+bool bc_is_mix(const cxa_bc_plan *p) { return p->mix; }
 int64_t bc_workspace_elems(const cxa_bc_plan *p) { return p->wsA + p->wsB; }
 double bc_flops(const cxa_bc_plan *p) { return p->flops; }
 int bc_dtype(const cxa_bc_plan *p) { return p->dtype; }
@@ -543,7 +655,8 @@ extern "C" int cxa_gemm_batch(int dtype, const char *transa, const char *transb,
     }
   }
   std::vector<GTile> tiles;
-  const int cls = make_tiles(blks, segs, tiles, -1, dtype == CXA_F64 || dtype == CXA_F32);
+  int per_xcd = 0;
+  const int cls = make_tiles(blks, segs, tiles, &per_xcd, -1, dtype == CXA_F64 || dtype == CXA_F32);
   if (tiles.empty()) return CXA_OK;
   hipStream_t s = (hipStream_t)stream;
   DevBuf d_blks, d_segs, d_tiles;
@@ -560,6 +673,7 @@ extern "C" int cxa_gemm_batch(int dtype, const char *transa, const char *transb,
   g.blks = (const GBlk *)d_blks.p;
   g.segs = (const GSeg *)d_segs.p;
   g.ntiles = (int)tiles.size();
+  g.per_xcd = per_xcd;
   g.tile_class = cls;
   rc = launch_grouped_gemm(g, s);
   if (rc) return rc;

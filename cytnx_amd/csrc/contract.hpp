@@ -11,6 +11,7 @@ namespace cxa {
 
 int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, int a_dt,
                   int b_dt, int c_dt, int prec, void *workspace, hipStream_t s);
+bool bc_is_mix(const cxa_bc_plan *p);
 int64_t bc_workspace_elems(const cxa_bc_plan *p);
 double bc_flops(const cxa_bc_plan *p);
 int bc_dtype(const cxa_bc_plan *p);

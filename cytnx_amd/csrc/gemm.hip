@@ -160,36 +160,92 @@ __device__ __forceinline__ void mma_frag(RealAcc<PREC, MI, NI> &acc,
       }
 }
 
+// Chunks in flight per wave: enough loads outstanding to cover an L2/MALL round trip with a
+// tile's whole K range at the usual block sizes (K <= 64 is one round trip), within a register
+// budget that keeps several waves per SIMD.
+template <int PREC, int MI, int NI>
+constexpr int pipe_depth() {
+  return (MI * NI >= 8 || (PREC != CXA_PREC_FP32 && MI * NI >= 4)) ? 2 : 4;
+}
+
+// Walks the 16-deep K chunks c of a block's concatenated segments with c % nslices == slice,
+// in order; all fields are wave-uniform.
+template <typename TA, typename TB, typename SEGS>
+struct ChunkCursor {
+  SEGS segs;
+  int seg0, nseg, nslices, m0, n0;
+  int s, k0;  // current segment and chunk start within it
+  GSeg sg;
+  const TA *Ap;
+  const TB *Bp;
+  __device__ __forceinline__ void enter(const char *Abase, const char *Bbase) {
+    sg = segs(seg0 + s);
+    Ap = at<const TA>(Abase, sg.a_off) + (int64_t)m0 * sg.sam;
+    Bp = at<const TB>(Bbase, sg.b_off) + (int64_t)n0 * sg.sbn;
+  }
+  // skip `nchunk` chunks forward (crossing segment ends)
+  __device__ __forceinline__ void skip(int nchunk, const char *Abase, const char *Bbase) {
+    k0 += 16 * nchunk;
+    bool moved = false;
+    while (s < nseg) {
+      const int span = 16 * ((sg.K + 15) >> 4);
+      if (k0 < span) break;
+      k0 -= span;
+      ++s;
+      if (s < nseg) {
+        sg = segs(seg0 + s);
+        moved = true;
+      }
+    }
+    if (moved && s < nseg) enter(Abase, Bbase);
+  }
+};
+
 // Accumulate the chunks c with c % nslices == slice of the block's concatenated K range.
-// Within a segment the next chunk's fragments are loaded before the current chunk's MFMAs
-// issue, so global-memory latency overlaps the matrix work.
+// A ring of pipe_depth() fragment sets keeps that many chunks' loads in flight across the
+// whole range (segment boundaries included): the MFMAs of chunk c issue while chunks
+// c+1 .. c+depth-1 are still arriving, and chunk c+depth is loaded into the set just consumed.
 template <int PREC, typename TA, typename TB, int MI, int NI, typename SEGS>
 __device__ __forceinline__ void mfma_accumulate(RealAcc<PREC, MI, NI> &acc,
                                                 const char *__restrict__ Abase,
                                                 const char *__restrict__ Bbase, const GBlk &blk,
                                                 SEGS segs, int m0, int n0, int lane, int slice,
                                                 int nslices) {
+  constexpr int D = pipe_depth<PREC, MI, NI>();
   const int r = lane & 15, g = lane >> 4;
   const int mt = min(16 * MI, blk.M - m0);
   const int nt = min(16 * NI, blk.N - n0);
-  const int step = 16 * nslices;
-  int chunk_base = 0;
+  // chunks of this slice over all segments
+  int total = 0, cb = 0;
   for (int s = 0; s < blk.nseg; ++s) {
-    const GSeg sg = segs(blk.seg0 + s);
-    const TA *Ap = at<const TA>(Abase, sg.a_off) + (int64_t)m0 * sg.sam;
-    const TB *Bp = at<const TB>(Bbase, sg.b_off) + (int64_t)n0 * sg.sbn;
-    const int nchunks = (sg.K + 15) >> 4;
-    const int first = ((slice - chunk_base) % nslices + nslices) % nslices;
-    chunk_base += nchunks;
-    int k0 = 16 * first;
-    if (k0 >= sg.K) continue;
-    Frag<PREC, MI, NI> cur, nxt;
-    load_frag<PREC, TA, TB, MI, NI>(cur, Ap, Bp, sg, k0, mt, nt, r, g);
-    for (; k0 < sg.K; k0 += step) {
-      const bool more = k0 + step < sg.K;
-      if (more) load_frag<PREC, TA, TB, MI, NI>(nxt, Ap, Bp, sg, k0 + step, mt, nt, r, g);
-      mma_frag<PREC, MI, NI>(acc, cur);
-      if (more) cur = nxt;
+    const int nc = (segs(blk.seg0 + s).K + 15) >> 4;
+    const int first = ((slice - cb) % nslices + nslices) % nslices;
+    if (first < nc) total += (nc - first + nslices - 1) / nslices;
+    cb += nc;
+  }
+  if (total == 0) return;
+  ChunkCursor<TA, TB, SEGS> cur{segs, blk.seg0, blk.nseg, nslices, m0, n0, 0, 0};
+  cur.enter(Abase, Bbase);
+  cur.skip(slice, Abase, Bbase);  // also steps over empty leading segments
+  Frag<PREC, MI, NI> f[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    if (d < total) {
+      load_frag<PREC, TA, TB, MI, NI>(f[d], cur.Ap, cur.Bp, cur.sg, cur.k0, mt, nt, r, g);
+      cur.skip(nslices, Abase, Bbase);
+    }
+  }
+  for (int base = 0; base < total; base += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int c = base + d;
+      if (c < total) {
+        mma_frag<PREC, MI, NI>(acc, f[d]);
+        if (c + D < total) {
+          load_frag<PREC, TA, TB, MI, NI>(f[d], cur.Ap, cur.Bp, cur.sg, cur.k0, mt, nt, r, g);
+          cur.skip(nslices, Abase, Bbase);
+        }
+      }
     }
   }
 }
@@ -265,8 +321,29 @@ __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
   __syncthreads();
 }
 
-// Row streaming for N <= kRowsN: lane = output row, FP64 accumulation.  The K loop runs the
-// k values with k % nslices == slice, four at a time with the loads issued ahead of the FMAs.
+// Walks the k values kk of a block's concatenated segments with kk % nslices == slice, in
+// order (wave-uniform).
+template <typename SEGS>
+struct RowCursor {
+  SEGS segs;
+  int seg0, nseg;
+  int s, k;
+  GSeg sg;
+  __device__ __forceinline__ void skip(int n) {
+    k += n;
+    while (s < nseg && k >= sg.K) {
+      k -= sg.K;
+      ++s;
+      if (s < nseg) sg = segs(seg0 + s);
+    }
+  }
+};
+
+// Row streaming for N <= kRowsN: lane = output row, FP64 accumulation.  The k values of all
+// segments form one sequence; kRowsDepth loads of A run ahead of the FMAs that consume them
+// (B rows are wave-uniform reads).
+constexpr int kRowsDepth = 8;
+
 template <typename TA, typename TB, typename SEGS>
 __device__ __forceinline__ void rows_accumulate(double (&acc)[kRowsN], const char *Abase,
                                                 const char *Bbase, const GBlk &blk, SEGS segs,
@@ -274,28 +351,41 @@ __device__ __forceinline__ void rows_accumulate(double (&acc)[kRowsN], const cha
   const int N = blk.N;
 #pragma unroll
   for (int n = 0; n < kRowsN; ++n) acc[n] = 0.0;
-  int kbase = 0;
+  int total = 0, kb = 0;
   for (int s = 0; s < blk.nseg; ++s) {
-    const GSeg sg = segs(blk.seg0 + s);
-    const TA *Ap = at<const TA>(Abase, sg.a_off) + (valid ? (int64_t)row * sg.sam : 0);
-    const TB *Bp = at<const TB>(Bbase, sg.b_off);
-    int k = ((slice - kbase) % nslices + nslices) % nslices;
-    kbase += sg.K;
-    for (; k < sg.K; k += 4 * nslices) {
-      double a[4];
+    const int K = segs(blk.seg0 + s).K;
+    const int first = ((slice - kb) % nslices + nslices) % nslices;
+    if (first < K) total += (K - first + nslices - 1) / nslices;
+    kb += K;
+  }
+  if (total == 0) return;
+  RowCursor<SEGS> lc{segs, blk.seg0, blk.nseg, 0, 0};
+  lc.sg = segs(blk.seg0);
+  lc.skip(slice);  // also steps over empty leading segments
+  RowCursor<SEGS> cc = lc;
+  double a[kRowsDepth];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int kk = k + u * nslices;
-        a[u] = (valid && kk < sg.K) ? (double)Ap[(int64_t)kk * sg.sak] : 0.0;
-      }
+  for (int d = 0; d < kRowsDepth; ++d) {
+    if (d < total) {
+      const TA *Ap = at<const TA>(Abase, lc.sg.a_off);
+      a[d] = valid ? (double)Ap[(int64_t)row * lc.sg.sam + (int64_t)lc.k * lc.sg.sak] : 0.0;
+      lc.skip(nslices);
+    }
+  }
+  for (int base = 0; base < total; base += kRowsDepth) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int kk = k + u * nslices;
-        if (kk < sg.K) {
-          const TB *bk = Bp + (int64_t)kk * sg.sbk;
+    for (int d = 0; d < kRowsDepth; ++d) {
+      const int c = base + d;
+      if (c < total) {
+        const TB *bk = at<const TB>(Bbase, cc.sg.b_off) + (int64_t)cc.k * cc.sg.sbk;
 #pragma unroll
-          for (int n = 0; n < kRowsN; ++n)
-            if (n < N) acc[n] += a[u] * (double)bk[(int64_t)n * sg.sbn];
+        for (int n = 0; n < kRowsN; ++n)
+          if (n < N) acc[n] += a[d] * (double)bk[(int64_t)n * cc.sg.sbn];
+        cc.skip(nslices);
+        if (c + kRowsDepth < total) {
+          const TA *Ap = at<const TA>(Abase, lc.sg.a_off);
+          a[d] = valid ? (double)Ap[(int64_t)row * lc.sg.sam + (int64_t)lc.k * lc.sg.sak] : 0.0;
+          lc.skip(nslices);
         }
       }
     }
@@ -469,16 +559,20 @@ __device__ __forceinline__ void run_tile(int kind, const char *A, const char *B,
   }
 }
 
-// grouped launch: tiles, blocks and segments come from device arrays; one wave per tile
+// grouped launch: tiles, blocks and segments come from device arrays; one wave per tile.
+// Workgroup w runs on XCD w % kXcds and takes tiles from that XCD's range of the list.
 template <int PREC, typename TA, typename TB, typename TC, bool CPLX, int CLS>
 __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict__ tiles,
-                                                        int ntiles, const GBlk *__restrict__ blks,
+                                                        int per_xcd, const GBlk *__restrict__ blks,
                                                         const GSeg *__restrict__ segs,
                                                         const char *A, const char *B, char *C) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int t = blockIdx.x * kWaves + wave; t < ntiles; t += gridDim.x * kWaves) {
-    const GTile tl = tiles[t];
+  const GTile *range = tiles + (int64_t)(blockIdx.x % kXcds) * per_xcd;
+  const int stride = (gridDim.x / kXcds) * kWaves;
+  for (int t = (blockIdx.x / kXcds) * kWaves + wave; t < per_xcd; t += stride) {
+    const GTile tl = range[t];
+    if (tl.blk < 0) continue;  // padding
     const GBlk blk = blks[tl.blk];
     run_tile<PREC, TA, TB, TC, CPLX, CLS>(tl.kind, A, B, C, blk, SegArray{segs}, tl.m0, tl.n0,
                                           lane);
@@ -489,13 +583,16 @@ __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict
 // class or the row class)
 template <int PREC, typename TA, typename TB, typename TC, int CLS>
 __global__ __launch_bounds__(64 * kWaves) void k_grouped_split(
-    const GTile *__restrict__ tiles, int ntiles, const GBlk *__restrict__ blks,
+    const GTile *__restrict__ tiles, int per_xcd, const GBlk *__restrict__ blks,
     const GSeg *__restrict__ segs, const char *A, const char *B, char *C) {
   __shared__ double red[kWaves][16][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const GTile tl = tiles[t];
+  const GTile *range = tiles + (int64_t)(blockIdx.x % kXcds) * per_xcd;
+  const int stride = gridDim.x / kXcds;
+  for (int t = blockIdx.x / kXcds; t < per_xcd; t += stride) {
+    const GTile tl = range[t];
+    if (tl.blk < 0) continue;  // padding (uniform over the workgroup)
     const GBlk blk = blks[tl.blk];
     const SegArray sa{segs};
     if constexpr (CLS == TC_ROWS) {
@@ -517,6 +614,202 @@ __global__ __launch_bounds__(64 * kWaves) void k_grouped_split(
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------------------
+// mix launches: one workgroup per tile of an item; wave w computes slices w, w + kWaves, ...
+// (each into its own accumulator), the slice tiles meet in LDS, and the waves then write the
+// item's outputs, each a fixed-order weighted sum of the slice tiles.
+// ---------------------------------------------------------------------------------------
+struct SegStride {
+  const GSeg *p;
+  int base, stride;
+  __device__ __forceinline__ GSeg operator()(int i) const { return p[base + i * stride]; }
+};
+
+// Segment i of `segs` (nseg <= NACC) accumulates into acc[i]; one load pipeline runs across
+// all of them (as in mfma_accumulate).
+template <int PREC, typename TA, typename TB, int MI, int NI, int NACC, typename SEGS>
+__device__ __forceinline__ void mfma_accumulate_multi(RealAcc<PREC, MI, NI> (&acc)[NACC],
+                                                      const char *__restrict__ Abase,
+                                                      const char *__restrict__ Bbase, SEGS segs,
+                                                      int nseg, int M, int N, int m0, int n0,
+                                                      int lane) {
+  constexpr int D = pipe_depth<PREC, MI, NI>();
+  const int r = lane & 15, g = lane >> 4;
+  const int mt = min(16 * MI, M - m0);
+  const int nt = min(16 * NI, N - n0);
+  int nc[NACC];
+  int total = 0;
+#pragma unroll
+  for (int q = 0; q < NACC; ++q) {
+    nc[q] = q < nseg ? (segs(q).K + 15) >> 4 : 0;
+    total += nc[q];
+  }
+  if (total == 0) return;
+  ChunkCursor<TA, TB, SEGS> cur{segs, 0, nseg, 1, m0, n0, 0, 0};
+  cur.enter(Abase, Bbase);
+  cur.skip(0, Abase, Bbase);
+  int cs = 0, left = nc[0];
+#pragma unroll
+  for (int q = 1; q < NACC; ++q)
+    if (left == 0 && cs == q - 1) {
+      cs = q;
+      left = nc[q];
+    }
+  Frag<PREC, MI, NI> f[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    if (d < total) {
+      load_frag<PREC, TA, TB, MI, NI>(f[d], cur.Ap, cur.Bp, cur.sg, cur.k0, mt, nt, r, g);
+      cur.skip(1, Abase, Bbase);
+    }
+  }
+  for (int base = 0; base < total; base += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int c = base + d;
+      if (c < total) {
+#pragma unroll
+        for (int q = 0; q < NACC; ++q)
+          if (q == cs) mma_frag<PREC, MI, NI>(acc[q], f[d]);
+        if (--left == 0) {
+#pragma unroll
+          for (int q = 1; q < NACC; ++q)
+            if (left == 0 && cs == q - 1) {
+              cs = q;
+              left = nc[q];
+            }
+        }
+        if (c + D < total) {
+          load_frag<PREC, TA, TB, MI, NI>(f[d], cur.Ap, cur.Bp, cur.sg, cur.k0, mt, nt, r, g);
+          cur.skip(1, Abase, Bbase);
+        }
+      }
+    }
+  }
+}
+
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI>
+__global__ __launch_bounds__(64 * kWaves) void k_mix(const GTile *__restrict__ tiles,
+                                                    int per_xcd, const GBlk *__restrict__ items,
+                                                    const GSeg *__restrict__ slices,
+                                                    const MixOut *__restrict__ outs,
+                                                    const char *A, const char *B, char *C) {
+  using acc_t = typename std::conditional<PREC == CXA_PREC_FP32, float, double>::type;
+  constexpr int E = MI * NI * 4;                   // accumulator values per lane and slice
+  constexpr int NACC = kMixSlices / kWaves;        // slices per wave
+  constexpr bool F64 = RealAcc<PREC, MI, NI>::F64;
+  __shared__ acc_t xs[kMixSlices][E][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = lane & 15, g = lane >> 4;
+  const GTile *range = tiles + (int64_t)(blockIdx.x % kXcds) * per_xcd;
+  const int stride = gridDim.x / kXcds;
+  for (int t = blockIdx.x / kXcds; t < per_xcd; t += stride) {
+    const GTile tl = range[t];
+    if (tl.blk < 0) continue;  // padding (uniform over the workgroup)
+    const GBlk it = items[tl.blk];
+    RealAcc<PREC, MI, NI> acc[NACC];
+#pragma unroll
+    for (int q = 0; q < NACC; ++q) acc[q].zero();
+    const int mine = it.nseg > wave ? (it.nseg - wave + kWaves - 1) / kWaves : 0;
+    if (mine > 0)
+      mfma_accumulate_multi<PREC, TA, TB, MI, NI, NACC>(
+          acc, A, B, SegStride{slices, it.seg0 + wave, kWaves}, mine, it.M, it.N, tl.m0, tl.n0,
+          lane);
+#pragma unroll
+    for (int q = 0; q < NACC; ++q) {
+      const int s = wave + q * kWaves;
+      if (s < it.nseg) {
+#pragma unroll
+        for (int i = 0; i < MI; ++i)
+#pragma unroll
+          for (int jj = 0; jj < NI; ++jj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) xs[s][(i * NI + jj) * 4 + v][lane] = (acc_t)acc[q].get(i, jj, v);
+      }
+    }
+    __syncthreads();
+    const int mt = min(16 * MI, it.M - tl.m0), nt = min(16 * NI, it.N - tl.n0);
+    for (int j = wave; j < it.nout; j += kWaves) {
+      const MixOut &o = outs[it.out0 + j];
+      TC *Cp = at<TC>(C, o.c_off);
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int jj = 0; jj < NI; ++jj)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int e = (i * NI + jj) * 4 + v;
+            double val = 0.0;
+#pragma unroll
+            for (int s = 0; s < kMixSlices; ++s)
+              if (s < it.nseg) val += o.coef[s] * (double)xs[s][e][lane];
+            const int row = 16 * i + acc_row(F64, g, v);
+            const int col = 16 * jj + r;
+            if (row < mt && col < nt)
+              Cp[(int64_t)(tl.m0 + row) * o.ldc + (tl.n0 + col)] = (TC)val;
+          }
+    }
+    __syncthreads();  // the next tile reuses xs
+  }
+}
+
+template <int PREC, typename TA, typename TB, typename TC>
+int launch_mix_t(const GemmLaunch &g, hipStream_t s) {
+  if (g.per_xcd <= 0) return CXA_OK;
+  if (g.ntiles != kXcds * g.per_xcd)
+    return fail(CXA_ERR_INVALID, "mix GEMM: tile list of %d is not %d ranges of %d", g.ntiles,
+                kXcds, g.per_xcd);
+  const int P = g.per_xcd;
+  const int grid = kXcds * std::min(P, 2048);
+  const dim3 blk(64 * kWaves);
+  const char *A = (const char *)g.A, *B = (const char *)g.B;
+  char *C = (char *)g.C;
+  switch (g.kind0) {
+    case TK_1x1:
+      hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, 1, 1>), dim3(grid), blk, 0, s, g.tiles, P,
+                         g.blks, g.segs, g.outs, A, B, C);
+      break;
+    case TK_2x1:
+      hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, 2, 1>), dim3(grid), blk, 0, s, g.tiles, P,
+                         g.blks, g.segs, g.outs, A, B, C);
+      break;
+    case TK_1x2:
+      hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, 1, 2>), dim3(grid), blk, 0, s, g.tiles, P,
+                         g.blks, g.segs, g.outs, A, B, C);
+      break;
+    case TK_2x2:
+      hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, 2, 2>), dim3(grid), blk, 0, s, g.tiles, P,
+                         g.blks, g.segs, g.outs, A, B, C);
+      break;
+    default:
+      return fail(CXA_ERR_INVALID, "mix GEMM: tile kind %d is not an MFMA tile up to 32x32",
+                  g.kind0);
+  }
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int launch_mix(const GemmLaunch &g, hipStream_t s) {
+  const int p = g.prec, a = g.a_dt, b = g.b_dt, c = g.c_dt;
+  if (p == CXA_PREC_FP64 && a == CXA_F64 && b == CXA_F64 && c == CXA_F64)
+    return launch_mix_t<CXA_PREC_FP64, double, double, double>(g, s);
+  if (p == CXA_PREC_FP32X3 && a == CXA_F64 && b == CXA_F64 && c == CXA_F64)
+    return launch_mix_t<CXA_PREC_FP32X3, double, double, double>(g, s);
+  if (p == CXA_PREC_FP32) {
+    if (a == CXA_F32 && b == CXA_F64 && c == CXA_F32)
+      return launch_mix_t<CXA_PREC_FP32, float, double, float>(g, s);
+    if (a == CXA_F32 && b == CXA_F32 && c == CXA_F32)
+      return launch_mix_t<CXA_PREC_FP32, float, float, float>(g, s);
+    if (a == CXA_F32 && b == CXA_F64 && c == CXA_F64)
+      return launch_mix_t<CXA_PREC_FP32, float, double, double>(g, s);
+    if (a == CXA_F64 && b == CXA_F64 && c == CXA_F64)
+      return launch_mix_t<CXA_PREC_FP32, double, double, double>(g, s);
+  }
+  return fail(CXA_ERR_UNSUPPORTED, "mix GEMM: unsupported precision %d with dtypes (%d,%d,%d)",
+              p, a, b, c);
 }
 
 // single-block launch: one block with one segment, tiles derived from the launch index, so a
@@ -555,36 +848,42 @@ int launch_t(const GemmLaunch &g, hipStream_t s) {
     CXA_HIP(hipGetLastError());
     return CXA_OK;
   }
-  if (g.ntiles <= 0) return CXA_OK;
+  if (g.per_xcd <= 0) return CXA_OK;
+  if (g.ntiles != kXcds * g.per_xcd)
+    return fail(CXA_ERR_INVALID, "grouped GEMM: tile list of %d is not %d ranges of %d",
+                g.ntiles, kXcds, g.per_xcd);
+  if (g.grid % kXcds)
+    return fail(CXA_ERR_INVALID, "grouped GEMM: grid %d is not a multiple of %d", g.grid, kXcds);
   const int cls = g.tile_class;
   if (CPLX && cls != TC_MIXED)
     return fail(CXA_ERR_UNSUPPORTED, "complex GEMM tiles must be MFMA tiles up to 32x32");
-  const int wg_grid = g.grid > 0 ? g.grid : std::min(g.ntiles, 8192);
-  const int wave_grid = g.grid > 0 ? g.grid : std::min((g.ntiles + kWaves - 1) / kWaves, 4096);
+  const int P = g.per_xcd;
+  const int wg_grid = g.grid > 0 ? g.grid : kXcds * std::min(P, 2048);
+  const int wave_grid = g.grid > 0 ? g.grid : kXcds * std::min((P + kWaves - 1) / kWaves, 1024);
   if constexpr (!CPLX) {
     if (g.splitk && cls == TC_ROWS)
       hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_ROWS>), dim3(wg_grid),
-                         dim3(64 * kWaves), 0, s, g.tiles, g.ntiles, g.blks, g.segs,
+                         dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
                          (const char *)g.A, (const char *)g.B, (char *)g.C);
     else if (g.splitk && cls == TC_MIXED)
       hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED>), dim3(wg_grid),
-                         dim3(64 * kWaves), 0, s, g.tiles, g.ntiles, g.blks, g.segs,
+                         dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
                          (const char *)g.A, (const char *)g.B, (char *)g.C);
     else if (cls == TC_ROWS)
       hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_ROWS>), dim3(wave_grid),
-                         dim3(64 * kWaves), 0, s, g.tiles, g.ntiles, g.blks, g.segs,
+                         dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
                          (const char *)g.A, (const char *)g.B, (char *)g.C);
     else if (cls == TC_BIG)
       hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_BIG>), dim3(wave_grid),
-                         dim3(64 * kWaves), 0, s, g.tiles, g.ntiles, g.blks, g.segs,
+                         dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
                          (const char *)g.A, (const char *)g.B, (char *)g.C);
     else
       hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_MIXED>), dim3(wave_grid),
-                         dim3(64 * kWaves), 0, s, g.tiles, g.ntiles, g.blks, g.segs,
+                         dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
                          (const char *)g.A, (const char *)g.B, (char *)g.C);
   } else {
     hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, true, TC_MIXED>), dim3(wave_grid),
-                       dim3(64 * kWaves), 0, s, g.tiles, g.ntiles, g.blks, g.segs,
+                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
                        (const char *)g.A, (const char *)g.B, (char *)g.C);
   }
   CXA_HIP(hipGetLastError());
@@ -594,6 +893,7 @@ int launch_t(const GemmLaunch &g, hipStream_t s) {
 }  // namespace
 
 int launch_grouped_gemm(const GemmLaunch &g, hipStream_t s) {
+  if (g.outs) return launch_mix(g, s);
   const int p = g.prec, a = g.a_dt, b = g.b_dt, c = g.c_dt;
   if (p == CXA_PREC_FP64) {
     if (a == CXA_F64 && b == CXA_F64 && c == CXA_F64)
@@ -630,7 +930,7 @@ int default_kind(int M, int N) {
 }
 
 int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
-               std::vector<GTile> &tiles, int kind_override, bool allow_rows) {
+               std::vector<GTile> &tiles, int *per_xcd, int kind_override, bool allow_rows) {
   struct Item {
     GTile t;
     int64_t cost;
@@ -643,10 +943,19 @@ int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
   if (kind_override == TK_4x2 && allow_rows) cls = TC_BIG;  // allow_rows: real dtypes only
   else if (kind_override == TK_ROWS ? all_narrow : (kind_override < 0 && all_narrow))
     cls = TC_ROWS;
-  std::vector<Item> items;
-  for (size_t bi = 0; bi < blks.size(); ++bi) {
+  // blocks in operand order: first segment's A offset, then its B offset
+  std::vector<int> order;
+  for (size_t bi = 0; bi < blks.size(); ++bi)
+    if (blks[bi].M > 0 && blks[bi].N > 0) order.push_back((int)bi);
+  auto key = [&](int bi) {
     const GBlk &b = blks[bi];
-    if (b.M <= 0 || b.N <= 0) continue;
+    return b.nseg > 0 ? std::make_pair(segs[b.seg0].a_off, segs[b.seg0].b_off)
+                      : std::make_pair(INT64_MAX, INT64_MAX);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
+  std::vector<Item> items;
+  for (int bi : order) {
+    const GBlk &b = blks[bi];
     int kind;
     if (cls == TC_ROWS)
       kind = TK_ROWS;
@@ -663,25 +972,48 @@ int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
       for (int n0 = 0; n0 < b.N; n0 += tc)
         items.push_back({GTile{(int32_t)bi, m0, n0, kind}, (ksum + 16) * tr * tc});
   }
-  std::stable_sort(items.begin(), items.end(),
-                   [](const Item &x, const Item &y) { return x.cost > y.cost; });
-  tiles.resize(items.size());
-  for (size_t i = 0; i < items.size(); ++i) tiles[i] = items[i].t;
+  // kXcds contiguous ranges of equal work: a tile goes to the range holding the midpoint of
+  // its cost interval
+  int64_t total = 0;
+  for (const Item &it : items) total += it.cost;
+  std::vector<std::vector<Item>> ranges(kXcds);
+  int64_t cum = 0;
+  for (const Item &it : items) {
+    const double mid = (double)cum + 0.5 * (double)it.cost;
+    int x = total > 0 ? (int)(mid * kXcds / (double)total) : 0;
+    x = std::min(std::max(x, 0), kXcds - 1);
+    ranges[x].push_back(it);
+    cum += it.cost;
+  }
+  size_t P = 0;
+  for (auto &r : ranges) {
+    std::stable_sort(r.begin(), r.end(),
+                     [](const Item &x, const Item &y) { return x.cost > y.cost; });
+    P = std::max(P, r.size());
+  }
+  P = (P + kWaves - 1) / kWaves * kWaves;
+  tiles.assign(kXcds * P, GTile{-1, 0, 0, 0});
+  for (int x = 0; x < kXcds; ++x)
+    for (size_t i = 0; i < ranges[x].size(); ++i) tiles[x * P + i] = ranges[x][i].t;
+  *per_xcd = (int)P;
   return cls;
 }
 
 bool prefer_splitk(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
                    const std::vector<GTile> &tiles, int tile_class) {
   if (tiles.empty() || tile_class == TC_BIG) return false;
-  int64_t work = 0;
+  int64_t work = 0, n = 0;
   for (const GTile &t : tiles) {
+    if (t.blk < 0) continue;
     const GBlk &b = blks[t.blk];
+    ++n;
     for (int s = 0; s < b.nseg; ++s)
       work += tile_class == TC_ROWS ? segs[b.seg0 + s].K : (segs[b.seg0 + s].K + 15) / 16;
   }
+  if (n == 0) return false;
   // few tiles (under two per SIMD of a 256-CU device) that each walk a long K range
-  const double avg = (double)work / (double)tiles.size();
-  return tiles.size() < 2048 && avg >= (tile_class == TC_ROWS ? 64.0 : 8.0);
+  const double avg = (double)work / (double)n;
+  return n < 2048 && avg >= (tile_class == TC_ROWS ? 64.0 : 8.0);
 }
 
 }  // namespace cxa

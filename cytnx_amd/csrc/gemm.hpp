@@ -31,6 +31,17 @@ struct GBlk {
   double alpha, beta;
   int32_t M, N;
   int32_t seg0, nseg;
+  int32_t out0, nout;  // mix launches only: the item's outputs (MixOut)
+};
+
+// Mix launches (a grouped GEMM with a linear-combination epilogue).  An item is a GBlk whose
+// segments are separate products ("slices") X_s = A_s B_s of one M x N shape; the item writes
+// nout output matrices  C_j[m, n] = sum_s coef_j[s] * X_s[m, n]  (C_j at c_off + m*ldc + n).
+// The slices are never stored: one workgroup computes all of an item's slices for its tile.
+constexpr int kMixSlices = 8;
+struct MixOut {
+  int64_t c_off, ldc;
+  double coef[kMixSlices];
 };
 
 // tile shapes: MFMA wave tiles (rows x cols) and TK_ROWS, 64 rows of a block with N <= 8
@@ -59,10 +70,12 @@ struct GemmLaunch {
   const GTile *tiles = nullptr;  // device arrays
   const GBlk *blks = nullptr;
   const GSeg *segs = nullptr;
-  int ntiles = 0;
-  int grid = 0;  // workgroups; 0 = pick from ntiles
+  int ntiles = 0;   // = kXcds * per_xcd
+  int per_xcd = 0;  // tiles per XCD range (make_tiles)
+  int grid = 0;     // workgroups (a multiple of kXcds); 0 = pick from per_xcd
   bool splitk = false;  // one workgroup per tile, K chunks interleaved over its waves
   int tile_class = 0;   // the class make_tiles returned
+  const MixOut *outs = nullptr;  // non-null: mix launch (blks are items, segs are slices)
   // single-block mode: one block with one segment passed by value, no device task arrays
   bool single = false;
   GBlk blk0{};
@@ -73,15 +86,26 @@ struct GemmLaunch {
 // Enqueue one grouped launch.  Returns a CXA status.
 int launch_grouped_gemm(const GemmLaunch &g, hipStream_t s);
 
-// Host-side planning helper: cut every block into tiles of the chosen kind and order tiles by
-// decreasing work (longest K first) so the persistent loop balances.  All tiles of a launch
-// share one class (its kernel instantiation), returned: 0 = MFMA tiles up to 32x32, 1 = row
-// streaming (taken when every block has N <= 8 and allow_rows), 2 = 64x32 MFMA tiles (only on
-// request).  allow_rows is false for complex dtypes.
-int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
-               std::vector<GTile> &tiles, int kind_override = -1, bool allow_rows = true);
+// Workgroups are dispatched to the device's XCDs round-robin (workgroup w runs on XCD
+// w % kXcds), and each XCD has its own L2.  Tile lists are therefore cut into kXcds ranges of
+// equal length; workgroup w takes tiles from range w % kXcds only.
+constexpr int kXcds = 8;
 
-// Whether a tile list is better run with workgroup split-K (few tiles, long K).
+// Host-side planning helper: cut every block into tiles of the chosen kind.  Blocks are taken
+// in operand order (offset of their first A, then B, operand), which keeps blocks that read the
+// same operand blocks together, and the sequence is split into kXcds contiguous ranges of equal
+// work, so each XCD's L2 holds about 1/kXcds of the operands.  Within a range tiles run in
+// decreasing work (longest K first) so the persistent loop balances; ranges are padded to a
+// common length *per_xcd (a multiple of 4) with tiles of blk = -1, which the kernels skip.
+// All tiles of a launch share one class (its kernel instantiation), returned: 0 = MFMA tiles
+// up to 32x32, 1 = row streaming (taken when every block has N <= 8 and allow_rows), 2 = 64x32
+// MFMA tiles (only on request).  allow_rows is false for complex dtypes.
+int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
+               std::vector<GTile> &tiles, int *per_xcd, int kind_override = -1,
+               bool allow_rows = true);
+
+// Whether a tile list is better run with workgroup split-K (few tiles, long K); padding
+// tiles are ignored.
 bool prefer_splitk(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
                    const std::vector<GTile> &tiles, int tile_class);
 

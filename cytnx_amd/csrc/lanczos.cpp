@@ -297,6 +297,10 @@ extern "C" int cxa_chain_tune(cxa_chain *c, const void *x, void *y, int reps, in
     double best = 1e30;
     int bk = -1, bs = -1;
     for (int ci = 0; ci < ncand && rc == CXA_OK; ++ci) {
+      // mix plans: explicit MFMA tiles up to 32x32, no split-K
+      if (bc_is_mix(plan) && (cand[ci][0] < 0 || cand[ci][0] == TK_4x2 ||
+                              cand[ci][0] == TK_ROWS || cand[ci][1] == 1))
+        continue;
       if ((rc = cxa_block_contract_set_config(plan, c->prec, cand[ci][0], cand[ci][1]))) break;
       for (int w = 0; w < 2 && rc == CXA_OK; ++w) rc = chain_step(c, k, x, y, s);
       if (rc) break;

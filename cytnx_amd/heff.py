@@ -30,7 +30,7 @@ from . import _lib
 from .device import stream_handle, torch_device
 from .lanczos import LinOp, lanczos_gnd_native
 from .unitensor import (BD_BRA, BD_KET, PREC_FP32, PREC_FP64, Bond, Contract, UniTensor,
-                        contract_meta, meta_elems, plan_contraction)
+                        _Plan, contract_meta, meta_elems, plan_contraction)
 
 
 class _Chain:
@@ -99,6 +99,161 @@ class ChainGraph:
             pass
 
 
+MIX_SLICES = 8  # CXA_MIX_SLICES
+
+
+class MixUnsupported(ValueError):
+    """The operator does not fit a mix launch (an item with more than MIX_SLICES slices)."""
+
+
+def _row_strides(shape) -> np.ndarray:
+    st = np.ones(len(shape), dtype=np.int64)
+    for d in range(len(shape) - 2, -1, -1):
+        st[d] = st[d + 1] * int(shape[d + 1])
+    return st
+
+
+def _sector_offsets(b: Bond) -> np.ndarray:
+    degs = b.getDegeneracies()
+    return np.concatenate([[0], np.cumsum(degs)[:-1]]).astype(np.int64)
+
+
+def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: np.ndarray,
+                    Y: UniTensor, labY: dict):
+    """Task arrays of the mix launch Y = W12 applied to (Lp . psi), never storing Lp . psi.
+
+    Lp: (l, w, l') environment blocks; psi: the two-site state (l, s1, s2, r); W12: the dense
+    two-site MPO with axes (w, s1, s1', w'', s2, s2'); Y: the output layout, blocks
+    (l', s1', s2', w'', r) with r contiguous.  labL / labP / labY map the roles (l, w, lp /
+    l, s1, s2, r / lp, s1p, s2p, w2, r) to each tensor's labels.  One item per (l', r) sector
+    pair of Y; its slices are the products  X[l', r] = sum_l Lp[l, w, l'] psi[l, s1, s2, r]
+    for every (w, s1, s2) element that reaches one of the item's outputs, and each output
+    Y[l', s1', s2', w'', r] is the sum over slices of W12[w, s1, s1', w'', s2, s2'] * X.
+
+    Returns (items (n,4), slices (m,7), outs (k,2), coef (k,MIX_SLICES)) as int64 / float64
+    arrays, or raises MixUnsupported."""
+    pL = {k: Lp.labels().index(labL[k]) for k in ("l", "w", "lp")}
+    pP = {k: psi.labels().index(labP[k]) for k in ("l", "s1", "s2", "r")}
+    pY = {k: Y.labels().index(labY[k]) for k in ("lp", "s1p", "s2p", "w2", "r")}
+    offw = _sector_offsets(Lp.bonds()[pL["w"]])
+    offs1 = _sector_offsets(psi.bonds()[pP["s1"]])
+    offs2 = _sector_offsets(psi.bonds()[pP["s2"]])
+    offs1p = _sector_offsets(Y.bonds()[pY["s1p"]])
+    offs2p = _sector_offsets(Y.bonds()[pY["s2p"]])
+    offw2 = _sector_offsets(Y.bonds()[pY["w2"]])
+    L_by = {}
+    for bi, t in enumerate(Lp._itoi):
+        L_by.setdefault((int(t[pL["l"]]), int(t[pL["lp"]])), []).append(bi)
+    P_by = {}
+    for bi, t in enumerate(psi._itoi):
+        P_by.setdefault(int(t[pP["r"]]), []).append(bi)
+    Y_by = {}
+    for yi, t in enumerate(Y._itoi):
+        Y_by.setdefault((int(t[pY["lp"]]), int(t[pY["r"]])), []).append(yi)
+    items, slices, outs, coefs = [], [], [], []
+    for (qap, qc), ylist in Y_by.items():
+        cand = []
+        for pi in P_by.get(qc, []):
+            tp, shp = psi._itoi[pi], psi._shapes[pi]
+            stp = _row_strides(shp)
+            qa = int(tp[pP["l"]])
+            for li in L_by.get((qa, qap), []):
+                tl, shl = Lp._itoi[li], Lp._shapes[li]
+                stl = _row_strides(shl)
+                K = int(shl[pL["l"]])
+                if K != int(shp[pP["l"]]):
+                    raise ValueError("Heff2 mix plan: L and psi disagree on a sector of l")
+                for iw in range(int(shl[pL["w"]])):
+                    for i1 in range(int(shp[pP["s1"]])):
+                        for i2 in range(int(shp[pP["s2"]])):
+                            desc = (int(Lp._offsets[li] + iw * stl[pL["w"]]),
+                                    int(psi._offsets[pi] + i1 * stp[pP["s1"]]
+                                        + i2 * stp[pP["s2"]]),
+                                    int(stl[pL["lp"]]), int(stl[pL["l"]]),
+                                    int(stp[pP["l"]]), int(stp[pP["r"]]), K)
+                            g = (int(offw[int(tl[pL["w"]])] + iw),
+                                 int(offs1[int(tp[pP["s1"]])] + i1),
+                                 int(offs2[int(tp[pP["s2"]])] + i2))
+                            cand.append((desc, g, int(shl[pL["lp"]]), int(shp[pP["r"]])))
+        olist = []
+        M = N = None
+        for yi in ylist:
+            ty, shy = Y._itoi[yi], Y._shapes[yi]
+            sty = _row_strides(shy)
+            if sty[pY["r"]] != 1:
+                raise ValueError("Heff2 mix plan: r must be the contiguous axis of Y")
+            M, N = int(shy[pY["lp"]]), int(shy[pY["r"]])
+            for i1p in range(int(shy[pY["s1p"]])):
+                for i2p in range(int(shy[pY["s2p"]])):
+                    for j in range(int(shy[pY["w2"]])):
+                        c_off = int(Y._offsets[yi] + i1p * sty[pY["s1p"]]
+                                    + i2p * sty[pY["s2p"]] + j * sty[pY["w2"]])
+                        olist.append((c_off, int(sty[pY["lp"]]),
+                                      (int(offs1p[int(ty[pY["s1p"]])] + i1p),
+                                       int(offw2[int(ty[pY["w2"]])] + j),
+                                       int(offs2p[int(ty[pY["s2p"]])] + i2p))))
+        C = np.zeros((len(olist), len(cand)))
+        for o, (_, _, (s1p, w2, s2p)) in enumerate(olist):
+            for s, (_, (w, s1, s2), _, _) in enumerate(cand):
+                C[o, s] = W12[w, s1, s1p, w2, s2, s2p]
+        keep = np.any(C != 0.0, axis=0) if len(olist) else np.zeros(len(cand), dtype=bool)
+        cand = [c for c, k in zip(cand, keep) if k]
+        C = C[:, keep]
+        if len(cand) > MIX_SLICES:
+            raise MixUnsupported("an item of the two-site operator has %d slices (at most %d)"
+                                 % (len(cand), MIX_SLICES))
+        for c in cand:
+            if (c[2], c[3]) != (M, N):
+                raise ValueError("Heff2 mix plan: slice shape differs from its outputs'")
+        items.append((M, N, len(cand), len(olist)))
+        slices.extend(c[0] for c in cand)
+        for o, (c_off, ldc, _) in enumerate(olist):
+            outs.append((c_off, ldc))
+            row = np.zeros(MIX_SLICES)
+            row[:len(cand)] = C[o]
+            coefs.append(row)
+    as64 = lambda rows, w: np.asarray(rows, dtype=np.int64).reshape(-1, w)
+    return (as64(items, 4), as64(slices, 7), as64(outs, 2),
+            np.asarray(coefs, dtype=np.float64).reshape(-1, MIX_SLICES))
+
+
+def check_mix_bounds(arrays, na: int, nb: int, nc: int) -> None:
+    """Every element a mix launch reads (A: na elements, B: nb) or writes (C: nc) lies inside
+    its buffer; raises ValueError otherwise (checked on the host before any launch)."""
+    items, slices, outs, _ = arrays
+    ns = items[:, 2].astype(np.int64)
+    no = items[:, 3].astype(np.int64)
+    M = np.repeat(items[:, 0], ns)
+    N = np.repeat(items[:, 1], ns)
+    if len(slices):
+        a_off, b_off, sam, sak, sbk, sbn, K = (slices[:, i] for i in range(7))
+        live = (K > 0) & (M > 0) & (N > 0)
+        amax = a_off + (M - 1) * sam + (K - 1) * sak
+        bmax = b_off + (K - 1) * sbk + (N - 1) * sbn
+        if np.any(live & ((amax >= na) | (bmax >= nb) | (a_off < 0) | (b_off < 0))):
+            raise ValueError("mix plan reads outside its operands")
+    if len(outs):
+        Mo = np.repeat(items[:, 0], no)
+        No = np.repeat(items[:, 1], no)
+        cmax = outs[:, 0] + (Mo - 1) * outs[:, 1] + No - 1
+        if np.any((Mo > 0) & (No > 0) & ((cmax >= nc) | (outs[:, 0] < 0))):
+            raise ValueError("mix plan writes outside its output")
+
+
+class _MixPlan(_Plan):
+    """A cxa_mix_contract_plan handle (exec / info / destroy are the block-contract ones)."""
+
+    def __init__(self, arrays, device: int):
+        items, slices, outs, coef = (np.ascontiguousarray(a) for a in arrays)
+        h = ctypes.c_void_p(0)
+        torch.cuda.set_device(device)
+        _lib.check(_lib.lib().cxa_mix_contract_plan(
+            3, int(items.shape[0]), items.ctypes.data_as(_lib.i64p), int(slices.shape[0]),
+            slices.ctypes.data_as(_lib.i64p), int(outs.shape[0]), outs.ctypes.data_as(_lib.i64p),
+            coef.ctypes.data_as(_lib.dblp), ctypes.byref(h)))
+        super().__init__(h.value, device)
+
+
 class _HeffBase(LinOp):
     """Shared driver of the planned chains: FP64 storage of the vector, selectable precision
     of the contraction bulk."""
@@ -145,6 +300,15 @@ class _HeffBase(LinOp):
                                              stream_handle(self.device())))
         return [{"kind": kinds[i], "split": splits[i], "ms": ms[i]} for i in range(n)]
 
+    def set_tiles(self, choice: Sequence[dict]) -> None:
+        """Apply a per-step tiling choice (as returned by tune) at the current precision."""
+        if len(choice) != len(self.chain.steps):
+            raise ValueError("tiling choice has %d steps, chain has %d"
+                             % (len(choice), len(self.chain.steps)))
+        for plan, c in zip(self.chain._plans, choice):
+            _lib.check(_lib.lib().cxa_block_contract_set_config(
+                plan.handle, int(self.precision), int(c["kind"]), int(c["split"])))
+
     def native_matvec(self):
         """(function pointer, context) for the C Lanczos driver."""
         self.chain.prepare(self.precision)
@@ -161,12 +325,16 @@ class Heff2(_HeffBase):
     """Two-site U(1) effective Hamiltonian (see module docstring).
 
     With ``fuse_mpo`` (default) the two MPO tensors are contracted once into the two-site MPO
-    W12 = M1.M2 (over the shared MPO bond), and the matvec runs three grouped launches:
-    L.psi, .W12, .R -- the intermediate of the M1 step is never stored.  ``flops`` is always the
-    algorithmic count of the reference's four-step order L.psi -> M1 -> M2 -> R."""
+    W12 = M1.M2 (over the shared MPO bond).  With ``mix`` as well (default) the matvec is two
+    launches: a mix launch that forms the (w, s1, s2) slices of L.psi per (l', r) sector pair in
+    registers and writes their W12-weighted sums Y(l', s1', s2', w'', r) directly, then the
+    grouped GEMM Y.R over (w'', r) -- neither L.psi nor the M1 intermediate is stored.  Without
+    ``mix`` (or when an item would exceed the mix launch's slice bound) the matvec is three
+    grouped launches L.psi, .W12, .R.  ``flops`` is always the algorithmic count of the
+    reference's four-step order L.psi -> M1 -> M2 -> R."""
 
     def __init__(self, psi: UniTensor, L: UniTensor, M1: UniTensor, M2: UniTensor,
-                 R: UniTensor, fuse_mpo: bool = True):
+                 R: UniTensor, fuse_mpo: bool = True, mix: bool = True):
         for T in (psi, L, M1, M2, R):
             if not T.is_blockform():
                 raise TypeError("Heff2 takes U(1) block tensors")
@@ -195,6 +363,8 @@ class Heff2(_HeffBase):
         self.ref_flops = float(sum(st[0].flops for st in ref_steps))
         self.pairs = [st[0].npairs for st in ref_steps]
         steps = ref_steps
+        self.step_names = ["L.psi", ".M1", ".M2", ".R"]
+        self.mode = "ref"
         if fuse_mpo:
             W12 = Contract(M1, M2)
             Zf, c1, c2 = contract_meta(X, W12, 3, dev)
@@ -202,13 +372,53 @@ class Heff2(_HeffBase):
                 raise ValueError("Heff2: fused MPO step changed the intermediate layout")
             steps = [ref_steps[0], (plan_contraction(X, W12, Zf, c1, c2, 3), W12, False, Zf),
                      ref_steps[3]]
+            self.step_names = ["L.psi", ".W12", ".R"]
+            self.mode = "w12"
+            if mix:
+                try:
+                    steps = self._mix_steps(Lp, psi_m, M1, M2, W12, Zf, Rp, psi, dev)
+                    self.step_names = ["L.psi.W12 (mix)", ".R"]
+                    self.mode = "mix"
+                except MixUnsupported:
+                    pass
         self.fused = bool(fuse_mpo)
         self.out_labels = O.labels()
         self.copies = [st[0].ws_bytes for st in steps]
-        self.step_names = (["L.psi", ".W12", ".R"] if fuse_mpo
-                           else ["L.psi", ".M1", ".M2", ".R"])
         self.chain = _Chain(steps, dev)
         super().__init__(meta_elems(psi), dev)
+
+    @staticmethod
+    def _mix_steps(Lp, psi_m, M1, M2, W12, Zf, Rp, psi, dev):
+        """The two launches of the mix chain: (mix plan, Lp) -> Y, then (Y . R) -> out."""
+        lab_l, lab_w, lab_lp = Lp.labels()
+        lab_r, lab_w2, lab_rp = Rp.labels()
+        s1 = [x for x in psi.labels() if x in M1.labels()]
+        s2 = [x for x in psi.labels() if x in M2.labels()]
+        s1p = [x for x in M1.labels() if x not in (lab_w, s1[0]) and x not in M2.labels()]
+        s2p = [x for x in M2.labels() if x not in (lab_w2, s2[0]) and x not in M1.labels()]
+        if len(s1) != 1 or len(s2) != 1 or len(s1p) != 1 or len(s2p) != 1:
+            raise MixUnsupported("Heff2 mix plan: unexpected network labels")
+        # Y (l', s1', s2', w'', r); the labels of its two contracted legs sort as (w'', r), so
+        # the contraction over (w'', r) reads Y and R (w'', r, r') as strided matrices
+        kw, kr = "~mix0", "~mix1"
+        zb = {lab: Zf.bonds()[i] for i, lab in enumerate(Zf.labels())}
+        Y = UniTensor([zb[lab_lp], zb[s1p[0]], zb[s2p[0]], zb[lab_w2], zb[lab_r]],
+                      [lab_lp, s1p[0], s2p[0], kw, kr], 3, 3, dev, _alloc=False)
+        Wd = W12.to_dense()
+        wl = W12.labels()
+        Wd = np.ascontiguousarray(np.transpose(
+            Wd, [wl.index(x) for x in (lab_w, s1[0], s1p[0], lab_w2, s2[0], s2p[0])]))
+        arrays = mix_plan_arrays(
+            Lp, {"l": lab_l, "w": lab_w, "lp": lab_lp},
+            psi_m, {"l": lab_l, "s1": s1[0], "s2": s2[0], "r": lab_r}, Wd,
+            Y, {"lp": lab_lp, "s1p": s1p[0], "s2p": s2p[0], "w2": kw, "r": kr})
+        check_mix_bounds(arrays, Lp._storage.numel(), meta_elems(psi_m), meta_elems(Y))
+        plan = _MixPlan(arrays, dev)
+        Rpp = Rp.permute([lab_w2, lab_r, lab_rp]).relabel([kw, kr, lab_rp])
+        O, c1, c2 = contract_meta(Y, Rpp, 3, dev)
+        if O.itoi() != psi.itoi() or [b.dim() for b in O.bonds()] != psi.shape():
+            raise ValueError("Heff2 mix chain: output block layout differs from psi's")
+        return [(plan, Lp, True, Y), (plan_contraction(Y, Rpp, O, c1, c2, 3), Rpp, False, O)]
 
     @property
     def flops(self) -> float:

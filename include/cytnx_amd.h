@@ -170,6 +170,26 @@ int cxa_block_contract_exec(const cxa_bc_plan *plan, const void *L, const void *
 int cxa_block_contract_destroy(cxa_bc_plan *plan);
 
 /*
+ * Mix plan: a block contraction followed by a small linear map, in one launch.  Item i (of
+ * nitems) has shape M x N, ns slices and no outputs (items[4*i..] = M, N, ns, no); its slices
+ * are the next ns rows of `slices` (7 int64 each: A offset, B offset, A strides along m and k,
+ * B strides along k and n, K -- element units, as in the block-contract plans), its outputs the
+ * next no rows of `outs` (c_off, ldc) with weights coef[CXA_MIX_SLICES*o + s].  exec computes
+ *   X_s = A_s . B_s (M x N each, never stored),   C[c_off + m*ldc + n] = sum_s coef[s] X_s[m, n]
+ * for every output.  ns <= CXA_MIX_SLICES.  It replaces, for the DMRG effective Hamiltonian,
+ * the contractions of L, psi and the two MPO tensors in the matvec network
+ * (example/DMRG/dmrg_two_sites_U1.py:18-37, launched per matvec at :20): the slices are the
+ * (MPO index, physical indices) slices of L.psi and the weights are the elements of the
+ * two-site MPO M1.M2.  FP64 operands only
+ * (FP32 copies through a chain); exec / info / set_config / get_config / destroy are the
+ * block-contract entry points (tile kinds up to 32x32, no split-K; flops count the products).
+ */
+#define CXA_MIX_SLICES 8
+int cxa_mix_contract_plan(int dtype, int nitems, const int64_t *items, int64_t nslices,
+                          const int64_t *slices, int64_t nouts, const int64_t *outs,
+                          const double *coef, cxa_bc_plan **plan);
+
+/*
  * FP64 vector reductions into device memory (result[0]); workspace: device buffer of
  * cxa_vec_workspace_bytes() bytes.  Deterministic (fixed reduction order).
  * cxa_dot replaces cuVectordot_internal_d (src/backend/linalg_internal_gpu/

@@ -49,12 +49,13 @@ def problem(chi, seed=1, hermitian=True):
     return [t.to(0) for t in (psi, L, M1, M2, R)]
 
 
-@pytest.mark.parametrize("fuse", [True, False])
+@pytest.mark.parametrize("mode", ["ref", "w12", "mix"])
 @pytest.mark.parametrize("chi", [16, 64, 160])
-def test_heff2_fp64_matches_oracle_chain(chi, fuse):
+def test_heff2_fp64_matches_oracle_chain(chi, mode):
     psi, L, M1, M2, R = problem(chi, seed=chi)
-    H = Heff2(psi, L, M1, M2, R, fuse_mpo=fuse)
-    print("heff2 chi", chi, "pairs", H.pairs, "copy bytes", H.copies, flush=True)
+    H = Heff2(psi, L, M1, M2, R, fuse_mpo=mode != "ref", mix=mode == "mix")
+    print("heff2 chi", chi, "mode", H.mode, "pairs", H.pairs, "copy bytes", H.copies, flush=True)
+    assert H.mode == mode
     assert all(c == 0 for c in H.copies)  # every operand read in place
     out = H.matvec(psi)
     torch.cuda.synchronize()
@@ -96,13 +97,27 @@ def test_heff2_dense_closed_form_and_symmetry():
     assert np.abs(Hm - Hm.T).max() <= 1e-12 * np.abs(Hm).max()  # hermitised environments
 
 
-def test_heff2_tuned_tilings_match_oracle_at_chi512():
+@pytest.mark.parametrize("mix", [True, False])
+def test_heff2_tuned_tilings_match_oracle_at_chi512(mix):
     """After the autotuner picks tilings (MFMA shapes, split-K, row streaming) for FP64, the
     matvec still equals the oracle's chain at 1e-12."""
     psi, L, M1, M2, R = problem(512, seed=21)
-    H = Heff2(psi, L, M1, M2, R)
+    H = Heff2(psi, L, M1, M2, R, mix=mix)
     choice = H.tune(reps=3)
-    print("tuned", choice, flush=True)
+    print("tuned", H.mode, choice, flush=True)
+    out = H.matvec(psi)
+    ref = oracle_heff2(psi, L, M1, M2, R)
+    assert relerr(out._storage.cpu().numpy(), ref.flat()) <= 1e-12
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_heff2_mix_every_tile_kind_matches_oracle(kind):
+    """Each MFMA tile shape of the mix launch (16x16, 32x32, 32x16, 16x32) at chi=256, where
+    sector widths (up to 21) leave partial tiles in both directions."""
+    psi, L, M1, M2, R = problem(256, seed=23)
+    H = Heff2(psi, L, M1, M2, R)
+    assert H.mode == "mix"
+    H.set_tiles([{"kind": kind, "split": 0}, {"kind": -1, "split": -1}])
     out = H.matvec(psi)
     ref = oracle_heff2(psi, L, M1, M2, R)
     assert relerr(out._storage.cpu().numpy(), ref.flat()) <= 1e-12

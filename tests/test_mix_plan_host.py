@@ -1,0 +1,99 @@
+"""Host checks of the two-site mix plan (cytnx_amd.heff.mix_plan_arrays).
+
+The task arrays are executed by a numpy restatement of the mix launch's semantics (slices are
+strided matrix products, outputs their weighted sums) and compared with the dense closed form
+Y[l', s1', s2', w'', r] = sum W12[w, s1, s1', w'', s2, s2'] L[w, l, l'] psi[l, s1, s2, r],
+restricted to Y's blocks.  A wrong stride, offset, coefficient index or slice/output pairing
+changes Y at 1e-14 relative or more."""
+import numpy as np
+import pytest
+import torch
+
+from cytnx_amd import heisenberg
+from cytnx_amd.heff import MIX_SLICES, mix_plan_arrays
+from cytnx_amd.unitensor import UniTensor
+
+
+def _meta(bonds, labels):
+    return UniTensor(bonds, labels, 1, 3, -1, _alloc=False)
+
+
+def _fill_from_dense(T: UniTensor, D: np.ndarray) -> np.ndarray:
+    flat = np.zeros(int(sum(np.prod(s.astype(np.int64)) for s in T._shapes)))
+    offs = [np.concatenate([[0], np.cumsum(b.getDegeneracies())[:-1]]).astype(int)
+            for b in T.bonds()]
+    for bi, t in enumerate(T._itoi):
+        shp = [int(s) for s in T._shapes[bi]]
+        sl = tuple(slice(offs[a][int(q)], offs[a][int(q)] + shp[a]) for a, q in enumerate(t))
+        o = int(T._offsets[bi])
+        flat[o:o + int(np.prod(shp))] = D[sl].ravel()
+    return flat
+
+
+def _run_mix(arrays, A, B, ny):
+    items, slices, outs, coef = arrays
+    Y = np.full(ny, np.nan)
+    s_at = o_at = 0
+    for M, N, ns, no in items:
+        X = []
+        for s in range(s_at, s_at + ns):
+            a_off, b_off, sam, sak, sbk, sbn, K = (int(v) for v in slices[s])
+            Am = np.array([[A[a_off + m * sam + k * sak] for k in range(K)] for m in range(M)])
+            Bm = np.array([[B[b_off + k * sbk + n * sbn] for n in range(N)] for k in range(K)])
+            X.append(Am @ Bm if K else np.zeros((M, N)))
+        for o in range(o_at, o_at + no):
+            c_off, ldc = (int(v) for v in outs[o])
+            val = np.zeros((M, N))
+            for s in range(ns):
+                val += coef[o, s] * X[s]
+            for m in range(M):
+                Y[c_off + m * ldc:c_off + m * ldc + N] = val[m]
+        s_at += ns
+        o_at += no
+    assert s_at == len(slices) and o_at == len(outs)
+    return Y
+
+
+@pytest.mark.parametrize("chi", [8, 20])
+def test_mix_plan_matches_dense_closed_form(chi):
+    psi, L, M1, M2, R = heisenberg.two_site_problem(chi, seed=chi, hermitian=True)
+    # L permuted to (l, w, l'), laid out afresh
+    Lb = L.bonds()
+    Lp = _meta([Lb[1], Lb[0], Lb[2]], ["-1", "-5", "0"])
+    Ld = L.to_dense()                                   # (w, l, l')
+    A = _fill_from_dense(Lp, np.transpose(Ld, (1, 0, 2)))
+    B = psi.storage().numpy()
+    W = np.einsum("awsx,wvty->asxvty", M1.to_dense(), M2.to_dense())  # (w,s1,s1',w'',s2,s2')
+    Yb = [Lb[2], M1.bonds()[3], M2.bonds()[3], M2.bonds()[1], psi.bonds()[3]]
+    Y = _meta(Yb, ["0", "1", "2", "k0", "k1"])
+    arrays = mix_plan_arrays(Lp, {"l": "-1", "w": "-5", "lp": "0"},
+                             psi, {"l": "-1", "s1": "-2", "s2": "-3", "r": "-4"}, W,
+                             Y, {"lp": "0", "s1p": "1", "s2p": "2", "w2": "k0", "r": "k1"})
+    items, slices, outs, coef = arrays
+    assert items[:, 2].max() <= MIX_SLICES
+    ny = int(sum(np.prod(s.astype(np.int64)) for s in Y._shapes))
+    got = _run_mix(arrays, A, B, ny)
+    assert not np.isnan(got).any()  # every element of Y written
+    Pd = psi.to_dense()
+    dense = np.einsum("asxvty,alp,lstr->pxyvr", W, Ld, Pd, optimize=True)
+    want = _fill_from_dense(Y, dense)
+    assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+
+
+def test_mix_plan_xxz_item_bounds():
+    """For the XXZ MPO the items hold at most 8 slices (8 = 6 from the three degenerate MPO
+    states of charge 0 times two spin pairs, plus one each from charges +2 and -2)."""
+    psi, L, M1, M2, R = heisenberg.two_site_problem(48, seed=2, hermitian=True)
+    Lb = L.bonds()
+    Lp = _meta([Lb[1], Lb[0], Lb[2]], ["-1", "-5", "0"])
+    W = np.einsum("awsx,wvty->asxvty", M1.to_dense(), M2.to_dense())
+    Y = _meta([Lb[2], M1.bonds()[3], M2.bonds()[3], M2.bonds()[1], psi.bonds()[3]],
+              ["0", "1", "2", "k0", "k1"])
+    items, slices, outs, coef = mix_plan_arrays(
+        Lp, {"l": "-1", "w": "-5", "lp": "0"},
+        psi, {"l": "-1", "s1": "-2", "s2": "-3", "r": "-4"}, W,
+        Y, {"lp": "0", "s1p": "1", "s2p": "2", "w2": "k0", "r": "k1"})
+    assert items[:, 2].max() == 8
+    # the weights are W12's 30 nonzero elements, each used by every item it reaches
+    assert np.count_nonzero(W) == 30
+    assert set(np.unique(coef[coef != 0]).round(12)) <= set(np.unique(W[W != 0]).round(12))
