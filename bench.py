@@ -103,6 +103,108 @@ def step_bytes(H, n_x, prec):
     return out
 
 
+def bench_permute(torch, nperm=32):
+    """Rank-6 permute sweep: shapes (16,8,16,8,16,8) and (32,4,32,4,32,4), FP64 and complex128,
+    `nperm` seeded permutations.  Each permutation is timed as one replay of a captured graph
+    of back-to-back permutes over rotating buffers (>= 1 GiB in + out, so the Infinity Cache
+    cannot hold them); GB/s counts one read and one write of the tensor."""
+    import numpy as np
+    from cytnx_amd import _lib
+    from cytnx_amd.device import stream_handle
+    L = _lib.lib()
+    rng = np.random.default_rng(20240)
+    perms = [rng.permutation(6) for _ in range(nperm)]
+    res = {}
+    for shape in [(16, 8, 16, 8, 16, 8), (32, 4, 32, 4, 32, 4)]:
+        for name, dt_id, tdt, es in [("f64", 3, torch.float64, 8),
+                                     ("c128", 1, torch.complex128, 16)]:
+            n = int(np.prod(shape))
+            nb = max(2, -(-(1 << 30) // (2 * n * es)))
+            src = torch.rand(nb * n, dtype=tdt, device="cuda:0")
+            dst = torch.empty_like(src)
+            shp = _lib.u64_array(shape)
+            gbs = []
+            for p in perms:
+                mp = _lib.u64_array(p)
+                g = torch.cuda.CUDAGraph()
+                cs = torch.cuda.Stream()
+                with torch.cuda.graph(g, stream=cs):
+                    hs = stream_handle(0)
+                    for i in range(nb):
+                        _lib.check(L.cxa_permute(dt_id, 6, shp, mp, src.data_ptr() + i * n * es,
+                                                 dst.data_ptr() + i * n * es, hs))
+                g.replay()
+                torch.cuda.synchronize()
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record()
+                for _ in range(3):
+                    g.replay()
+                b.record()
+                torch.cuda.synchronize()
+                per = a.elapsed_time(b) * 1e-3 / (3 * nb)
+                gbs.append(2.0 * n * es / per / 1e9)
+                del g
+            key = "%s_%s" % ("x".join(map(str, shape)), name)
+            res[key] = {"mean_GBs": float(np.mean(gbs)), "min_GBs": float(np.min(gbs)),
+                        "max_GBs": float(np.max(gbs)),
+                        "frac_of_8TBs": float(np.mean(gbs)) / PEAK_HBM_GBS,
+                        "buffers_rotated": nb}
+            del src, dst
+    allg = [v["mean_GBs"] for v in res.values()]
+    res["mean_GBs"] = float(np.mean(allg))
+    res["permutations"] = nperm
+    return res
+
+
+def bench_c1(torch, cpu=True, iters=100):
+    """Dense one-site Heff at chi=128 (L, A, R iid N(0,1), exact XXZ MPO D=5, d=2), order
+    ((L.A).W).R, `iters` FP64 Lanczos iterations (CvgCrit below any reachable |dE|)."""
+    import numpy as np
+    from cytnx_amd import heisenberg
+    from cytnx_amd.heff import Heff1Dense
+    from cytnx_amd.lanczos import lanczos_gnd_native
+    chi, D, d = 128, 5, 2
+    rng = np.random.default_rng(1)
+    Lh = rng.standard_normal((chi, D, chi))
+    Ah = rng.standard_normal((chi, d, chi))
+    Rh = rng.standard_normal((chi, D, chi))
+    W = heisenberg.xxz_mpo_dense()
+    H = Heff1Dense(torch.from_numpy(Lh).cuda(), torch.from_numpy(W).cuda(),
+                   torch.from_numpy(Rh).cuda(), Ah.shape)
+    x0 = torch.from_numpy(Ah.ravel().copy()).cuda()
+    cvg = 1e-300
+    lanczos_gnd_native(H, x0, cvg, iters, batch=iters)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E, v, k, _, _ = lanczos_gnd_native(H, x0, cvg, iters, batch=iters)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    flops = 4 * D * d * chi ** 3 + 2 * D * D * d * d * chi * chi
+    out = {"iterations": int(k), "seconds": t, "flops_per_matvec": flops,
+           "tflops": k * flops / t / 1e12, "ms_per_iteration": t / k * 1e3, "E": E}
+    if cpu:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import pyoracle as po
+        blas = po.use_openblas(nthreads=min(16, os.cpu_count() or 1))
+
+        def mv(x):
+            A = x.reshape(chi, d, chi)
+            X = po.tensordot(Lh, A, [0], [0])         # (w, l', s, r)
+            Y = po.tensordot(X, W, [2, 0], [2, 0])    # (l', r, w', s')
+            return po.tensordot(Y, Rh, [1, 2], [0, 1]).ravel()  # (l', s', r')
+
+        t0 = time.perf_counter()
+        Eo, _, ko = po.lanczos_gnd(mv, Ah.ravel().copy(), cvg, k)
+        tc = time.perf_counter() - t0
+        po.use_reference_loops()
+        out["cpu"] = {"seconds": tc, "iterations": int(ko), "tflops": ko * flops / tc / 1e12,
+                      "cores": min(16, os.cpu_count() or 1), "kind": "port",
+                      "blas": blas, "E": Eo}
+        out["speedup_vs_cpu"] = tc / t * (k / max(ko, 1))
+    return out
+
+
 def cpu_baseline(problem, flops, seconds):
     import numpy as np
     sys.path.insert(0, os.path.join(HERE, "oracle"))
@@ -177,6 +279,8 @@ def main() -> int:
     ap.add_argument("--save-tiles", default=None, help="write the tuned tilings (JSON)")
     ap.add_argument("--load-tiles", default=None,
                     help="apply tilings from a --save-tiles file instead of tuning")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the permute sweep and the dense one-site Lanczos lines")
     ap.add_argument("--chain", default="mix", choices=["mix", "w12", "ref"],
                     help="matvec program: mix (two launches), w12 (three), ref (four, the "
                          "reference's contraction order)")
@@ -249,6 +353,10 @@ def main() -> int:
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+        # a short FP64 Lanczos run (20 iterations) for the vector kernels' trace
+        H.set_precision(PREC_FP64)
+        lanczos_gnd_native(H, x.clone(), 1e-300, 20, batch=20)
+        torch.cuda.synchronize()
     # normwise error of the reduced-precision matvecs against FP64 on the benchmark input
     errs = {}
     H.set_precision(PREC_FP64)
@@ -265,22 +373,29 @@ def main() -> int:
     lanczos = None
     if not args.no_lanczos and not args.profile:
         x0 = x.clone()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        E64, v64, n64, _, _ = lanczos_gnd_native(H, x0, args.lanczos_cvg, args.lanczos_maxiter,
-                                                 batch=16)
-        torch.cuda.synchronize()
-        t64 = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        Emx, vmx, n1, n2 = ground_state_mixed(H, x0, args.lanczos_cvg, args.lanczos_maxiter,
-                                              batch=16)
-        torch.cuda.synchronize()
-        tmx = time.perf_counter() - t0
-        dE = abs(Emx - E64)
-        lanczos = {"E_fp64": E64, "iters_fp64": n64, "time_fp64_s": t64,
-                   "E_fp32_refine": Emx, "iters_fp32": n1, "iters_fp64_refine": n2,
-                   "time_fp32_refine_s": tmx, "dE": dE, "dE_ok": bool(dE <= 1e-10),
-                   "CvgCrit": args.lanczos_cvg}
+        lanczos = {"CvgCrit": args.lanczos_cvg}
+        for graph in (False, True):
+            tag = "_graph" if graph else ""
+            lanczos_gnd_native(H, x0, args.lanczos_cvg, 20, batch=16, graph=graph)  # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            E64, v64, n64, _, _ = lanczos_gnd_native(H, x0, args.lanczos_cvg,
+                                                     args.lanczos_maxiter, batch=16, graph=graph)
+            torch.cuda.synchronize()
+            t64 = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            Emx, vmx, n1, n2 = ground_state_mixed(H, x0, args.lanczos_cvg, args.lanczos_maxiter,
+                                                  batch=16, graph=graph)
+            torch.cuda.synchronize()
+            tmx = time.perf_counter() - t0
+            dE = abs(Emx - E64)
+            lanczos.update({"E_fp64": E64, "iters_fp64": n64, "time_fp64%s_s" % tag: t64,
+                            "E_fp32_refine": Emx, "iters_fp32": n1, "iters_fp64_refine": n2,
+                            "time_fp32_refine%s_s" % tag: tmx, "dE": dE,
+                            "dE_ok": bool(dE <= 1e-10)})
+        lanczos["speedup_fp32_refine_vs_fp64"] = (
+            min(lanczos["time_fp64_s"], lanczos["time_fp64_graph_s"]) /
+            min(lanczos["time_fp32_refine_s"], lanczos["time_fp32_refine_graph_s"]))
         log("lanczos", json.dumps(lanczos))
 
     main_mode = "fp32"
@@ -309,6 +424,15 @@ def main() -> int:
         log("cpu baseline (~%.0f s)" % args.cpu_seconds)
         cpu = cpu_baseline(host, flops, args.cpu_seconds)
         log("cpu", json.dumps(cpu))
+
+    extra = {}
+    if not args.no_extra and not args.profile:
+        log("permute sweep")
+        extra["permute_sweep"] = bench_permute(torch)
+        log("permute", json.dumps(extra["permute_sweep"]))
+        log("dense one-site Heff, 100 Lanczos iterations")
+        extra["one_site_dense_lanczos"] = bench_c1(torch, cpu=not args.no_cpu)
+        log("one-site", json.dumps(extra["one_site_dense_lanczos"]))
 
     out = {
         "metric": METRIC,
@@ -339,6 +463,7 @@ def main() -> int:
         "lanczos": lanczos,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        **extra,
     }
     if cpu:
         out["speedup_fp32_vs_cpu"] = res[main_mode]["tflops"] / cpu["value"]

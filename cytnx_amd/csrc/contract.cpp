@@ -99,18 +99,16 @@ struct cxa_bc_plan {
 namespace cxa {
 namespace {
 
-// tile kinds a mix launch can run (its slices stay in registers / LDS per tile)
-bool mix_kind_ok(int kind) {
-  return kind == TK_1x1 || kind == TK_2x1 || kind == TK_1x2 || kind == TK_2x2;
-}
+// tile kinds a mix launch can run (every slice of an item stays in the wave's registers)
+bool mix_kind_ok(int kind) { return kind == TK_1x1 || kind == TK_2x1 || kind == TK_1x2; }
 
 int build_variant(cxa_bc_plan *p, int prec) {
   const bool real = p->dtype == CXA_F64 || p->dtype == CXA_F32;
   PlanVariant &v = p->var[prec];
   if (p->mix) {
-    v.launch_kind = mix_kind_ok(v.kind) ? v.kind : TK_1x2;
+    v.launch_kind = mix_kind_ok(v.kind) ? v.kind : TK_1x1;
     v.tile_class = make_tiles(p->blks, p->segs, v.tiles, &v.per_xcd, v.launch_kind, false);
-    v.splitk = false;
+    v.splitk = v.split_mode == 1;  // mix plans: four-wave workgroups (two slices per wave)
     if (v.d_tiles.p) {
       CXA_HIP(hipFree(v.d_tiles.p));
       v.d_tiles.p = nullptr;
@@ -444,9 +442,8 @@ extern "C" int cxa_block_contract_set_config(cxa_bc_plan *p, int prec, int kind,
   CXA_CHECK(prec >= -1 && prec <= CXA_PREC_FP32X3, CXA_ERR_INVALID, "unknown precision %d", prec);
   CXA_CHECK(kind >= -1 && kind < TK_COUNT, CXA_ERR_INVALID, "unknown tile kind %d", kind);
   CXA_CHECK(split >= -1 && split <= 1, CXA_ERR_INVALID, "split mode must be -1, 0 or 1");
-  CXA_CHECK(!p->mix || ((kind == -1 || mix_kind_ok(kind)) && split != 1), CXA_ERR_UNSUPPORTED,
-            "mix plans run MFMA tiles up to 32x32 without split-K (kind %d, split %d)", kind,
-            split);
+  CXA_CHECK(!p->mix || kind == -1 || mix_kind_ok(kind), CXA_ERR_UNSUPPORTED,
+            "mix plans run 16x16, 32x16 or 16x32 tiles (kind %d)", kind);
   for (int q = 0; q < 3; ++q) {
     if (prec >= 0 && q != prec) continue;
     if (p->var[q].kind == kind && p->var[q].split_mode == split && p->var[q].d_tiles.p) continue;

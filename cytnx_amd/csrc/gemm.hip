@@ -617,9 +617,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_grouped_split(
 }
 
 // ---------------------------------------------------------------------------------------
-// mix launches: one workgroup per tile of an item; wave w computes slices w, w + kWaves, ...
-// (each into its own accumulator), the slice tiles meet in LDS, and the waves then write the
-// item's outputs, each a fixed-order weighted sum of the slice tiles.
+// mix launches: one workgroup per tile of an item, of kMixSlices waves (wave s computes slice s:
+// the longest dependent load chain of a tile is one slice's K range) or of 4 waves (wave w
+// computes slices w and w + 4: no idle waves on items with up to 4 slices); the slice tiles meet
+// in LDS, and the waves then write the item's outputs, each a fixed-order weighted sum of the
+// slice tiles.
 // ---------------------------------------------------------------------------------------
 struct SegStride {
   const GSeg *p;
@@ -690,15 +692,15 @@ __device__ __forceinline__ void mfma_accumulate_multi(RealAcc<PREC, MI, NI> (&ac
   }
 }
 
-template <int PREC, typename TA, typename TB, typename TC, int MI, int NI>
-__global__ __launch_bounds__(64 * kWaves) void k_mix(const GTile *__restrict__ tiles,
-                                                    int per_xcd, const GBlk *__restrict__ items,
-                                                    const GSeg *__restrict__ slices,
-                                                    const MixOut *__restrict__ outs,
-                                                    const char *A, const char *B, char *C) {
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_mix(const GTile *__restrict__ tiles,
+                                                   int per_xcd, const GBlk *__restrict__ items,
+                                                   const GSeg *__restrict__ slices,
+                                                   const MixOut *__restrict__ outs,
+                                                   const char *A, const char *B, char *C) {
   using acc_t = typename std::conditional<PREC == CXA_PREC_FP32, float, double>::type;
-  constexpr int E = MI * NI * 4;                   // accumulator values per lane and slice
-  constexpr int NACC = kMixSlices / kWaves;        // slices per wave
+  constexpr int E = MI * NI * 4;              // accumulator values per lane and slice
+  constexpr int NACC = kMixSlices / WAVES;    // slices per wave
   constexpr bool F64 = RealAcc<PREC, MI, NI>::F64;
   __shared__ acc_t xs[kMixSlices][E][64];
   const int lane = threadIdx.x & 63;
@@ -710,29 +712,31 @@ __global__ __launch_bounds__(64 * kWaves) void k_mix(const GTile *__restrict__ t
     const GTile tl = range[t];
     if (tl.blk < 0) continue;  // padding (uniform over the workgroup)
     const GBlk it = items[tl.blk];
-    RealAcc<PREC, MI, NI> acc[NACC];
+    const int mine = it.nseg > wave ? (it.nseg - wave + WAVES - 1) / WAVES : 0;
+    if (mine > 0) {
+      RealAcc<PREC, MI, NI> acc[NACC];
 #pragma unroll
-    for (int q = 0; q < NACC; ++q) acc[q].zero();
-    const int mine = it.nseg > wave ? (it.nseg - wave + kWaves - 1) / kWaves : 0;
-    if (mine > 0)
+      for (int q = 0; q < NACC; ++q) acc[q].zero();
       mfma_accumulate_multi<PREC, TA, TB, MI, NI, NACC>(
-          acc, A, B, SegStride{slices, it.seg0 + wave, kWaves}, mine, it.M, it.N, tl.m0, tl.n0,
+          acc, A, B, SegStride{slices, it.seg0 + wave, WAVES}, mine, it.M, it.N, tl.m0, tl.n0,
           lane);
 #pragma unroll
-    for (int q = 0; q < NACC; ++q) {
-      const int s = wave + q * kWaves;
-      if (s < it.nseg) {
+      for (int q = 0; q < NACC; ++q) {
+        const int sl = wave + q * WAVES;
+        if (sl < it.nseg) {
 #pragma unroll
-        for (int i = 0; i < MI; ++i)
+          for (int i = 0; i < MI; ++i)
 #pragma unroll
-          for (int jj = 0; jj < NI; ++jj)
+            for (int jj = 0; jj < NI; ++jj)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) xs[s][(i * NI + jj) * 4 + v][lane] = (acc_t)acc[q].get(i, jj, v);
+              for (int v = 0; v < 4; ++v)
+                xs[sl][(i * NI + jj) * 4 + v][lane] = (acc_t)acc[q].get(i, jj, v);
+        }
       }
     }
     __syncthreads();
     const int mt = min(16 * MI, it.M - tl.m0), nt = min(16 * NI, it.N - tl.n0);
-    for (int j = wave; j < it.nout; j += kWaves) {
+    for (int j = wave; j < it.nout; j += WAVES) {
       const MixOut &o = outs[it.out0 + j];
       TC *Cp = at<TC>(C, o.c_off);
 #pragma unroll
@@ -744,8 +748,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_mix(const GTile *__restrict__ t
             const int e = (i * NI + jj) * 4 + v;
             double val = 0.0;
 #pragma unroll
-            for (int s = 0; s < kMixSlices; ++s)
-              if (s < it.nseg) val += o.coef[s] * (double)xs[s][e][lane];
+            for (int q = 0; q < kMixSlices; ++q)
+              if (q < it.nseg) val += o.coef[q] * (double)xs[q][e][lane];
             const int row = 16 * i + acc_row(F64, g, v);
             const int col = 16 * jj + r;
             if (row < mt && col < nt)
@@ -756,36 +760,32 @@ __global__ __launch_bounds__(64 * kWaves) void k_mix(const GTile *__restrict__ t
   }
 }
 
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI>
+void launch_mix_kind(const GemmLaunch &g, int grid, hipStream_t s) {
+  const char *A = (const char *)g.A, *B = (const char *)g.B;
+  char *C = (char *)g.C;
+  if (g.splitk)  // four waves, two slices each
+    hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, MI, NI, 4>), dim3(grid), dim3(64 * 4), 0, s,
+                       g.tiles, g.per_xcd, g.blks, g.segs, g.outs, A, B, C);
+  else  // one wave per slice
+    hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, MI, NI, kMixSlices>), dim3(grid),
+                       dim3(64 * kMixSlices), 0, s, g.tiles, g.per_xcd, g.blks, g.segs, g.outs,
+                       A, B, C);
+}
+
 template <int PREC, typename TA, typename TB, typename TC>
 int launch_mix_t(const GemmLaunch &g, hipStream_t s) {
   if (g.per_xcd <= 0) return CXA_OK;
   if (g.ntiles != kXcds * g.per_xcd)
     return fail(CXA_ERR_INVALID, "mix GEMM: tile list of %d is not %d ranges of %d", g.ntiles,
                 kXcds, g.per_xcd);
-  const int P = g.per_xcd;
-  const int grid = kXcds * std::min(P, 2048);
-  const dim3 blk(64 * kWaves);
-  const char *A = (const char *)g.A, *B = (const char *)g.B;
-  char *C = (char *)g.C;
+  const int grid = kXcds * std::min(g.per_xcd, 2048);
   switch (g.kind0) {
-    case TK_1x1:
-      hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, 1, 1>), dim3(grid), blk, 0, s, g.tiles, P,
-                         g.blks, g.segs, g.outs, A, B, C);
-      break;
-    case TK_2x1:
-      hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, 2, 1>), dim3(grid), blk, 0, s, g.tiles, P,
-                         g.blks, g.segs, g.outs, A, B, C);
-      break;
-    case TK_1x2:
-      hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, 1, 2>), dim3(grid), blk, 0, s, g.tiles, P,
-                         g.blks, g.segs, g.outs, A, B, C);
-      break;
-    case TK_2x2:
-      hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, 2, 2>), dim3(grid), blk, 0, s, g.tiles, P,
-                         g.blks, g.segs, g.outs, A, B, C);
-      break;
+    case TK_1x1: launch_mix_kind<PREC, TA, TB, TC, 1, 1>(g, grid, s); break;
+    case TK_2x1: launch_mix_kind<PREC, TA, TB, TC, 2, 1>(g, grid, s); break;
+    case TK_1x2: launch_mix_kind<PREC, TA, TB, TC, 1, 2>(g, grid, s); break;
     default:
-      return fail(CXA_ERR_INVALID, "mix GEMM: tile kind %d is not an MFMA tile up to 32x32",
+      return fail(CXA_ERR_INVALID, "mix GEMM: tile kind %d is not 16x16, 32x16 or 16x32",
                   g.kind0);
   }
   CXA_HIP(hipGetLastError());
@@ -972,19 +972,12 @@ int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
       for (int n0 = 0; n0 < b.N; n0 += tc)
         items.push_back({GTile{(int32_t)bi, m0, n0, kind}, (ksum + 16) * tr * tc});
   }
-  // kXcds contiguous ranges of equal work: a tile goes to the range holding the midpoint of
-  // its cost interval
-  int64_t total = 0;
-  for (const Item &it : items) total += it.cost;
+  // kXcds contiguous ranges of equal tile count (at these sizes a tile's time is mostly its
+  // fixed chain of dependent loads, so equal counts balance better than equal flops, and the
+  // ranges need no padding beyond the last few tiles)
   std::vector<std::vector<Item>> ranges(kXcds);
-  int64_t cum = 0;
-  for (const Item &it : items) {
-    const double mid = (double)cum + 0.5 * (double)it.cost;
-    int x = total > 0 ? (int)(mid * kXcds / (double)total) : 0;
-    x = std::min(std::max(x, 0), kXcds - 1);
-    ranges[x].push_back(it);
-    cum += it.cost;
-  }
+  const size_t per = (items.size() + kXcds - 1) / kXcds;
+  for (size_t i = 0; i < items.size(); ++i) ranges[per ? i / per : 0].push_back(items[i]);
   size_t P = 0;
   for (auto &r : ranges) {
     std::stable_sort(r.begin(), r.end(),

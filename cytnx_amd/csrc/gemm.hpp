@@ -94,7 +94,7 @@ constexpr int kXcds = 8;
 // Host-side planning helper: cut every block into tiles of the chosen kind.  Blocks are taken
 // in operand order (offset of their first A, then B, operand), which keeps blocks that read the
 // same operand blocks together, and the sequence is split into kXcds contiguous ranges of equal
-// work, so each XCD's L2 holds about 1/kXcds of the operands.  Within a range tiles run in
+// tile count, so each XCD's L2 holds about 1/kXcds of the operands.  Within a range tiles run in
 // decreasing work (longest K first) so the persistent loop balances; ranges are padded to a
 // common length *per_xcd (a multiple of 4) with tiles of blk = -1, which the kernels skip.
 // All tiles of a launch share one class (its kernel instantiation), returned: 0 = MFMA tiles

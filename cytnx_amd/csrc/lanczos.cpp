@@ -297,9 +297,9 @@ extern "C" int cxa_chain_tune(cxa_chain *c, const void *x, void *y, int reps, in
     double best = 1e30;
     int bk = -1, bs = -1;
     for (int ci = 0; ci < ncand && rc == CXA_OK; ++ci) {
-      // mix plans: explicit MFMA tiles up to 32x32, no split-K
-      if (bc_is_mix(plan) && (cand[ci][0] < 0 || cand[ci][0] == TK_4x2 ||
-                              cand[ci][0] == TK_ROWS || cand[ci][1] == 1))
+      // mix plans: explicit 16x16, 32x16 or 16x32 tiles; split 0 / 1 = eight / four waves
+      if (bc_is_mix(plan) && (cand[ci][0] < 0 || cand[ci][0] == TK_2x2 ||
+                              cand[ci][0] == TK_4x2 || cand[ci][0] == TK_ROWS))
         continue;
       if ((rc = cxa_block_contract_set_config(plan, c->prec, cand[ci][0], cand[ci][1]))) break;
       for (int w = 0; w < 2 && rc == CXA_OK; ++w) rc = chain_step(c, k, x, y, s);
@@ -393,14 +393,17 @@ struct cxa_lanczos {
   int *counter = nullptr;
   double *hbuf = nullptr;  // pinned: alpha, beta history and Ritz coefficients
   hipStream_t own = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
-  cxa_matvec_fn g_fn = nullptr;
-  void *g_ctx = nullptr;
-  int g_flags = -1;
+  // captured iterations: one (gexec1) and `gbatch` back to back (gexecB)
+  hipGraphExec_t gexec1 = nullptr, gexecB = nullptr;
+  int gbatch = 0;
+  void drop_graphs() {
+    if (gexec1) (void)hipGraphExecDestroy(gexec1);
+    if (gexecB) (void)hipGraphExecDestroy(gexecB);
+    gexec1 = gexecB = nullptr;
+    gbatch = 0;
+  }
   ~cxa_lanczos() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graphs();
     for (double *p : {V, w, cur, part, sc, coef})
       if (p) (void)hipFree(p);
     if (counter) (void)hipFree(counter);
@@ -441,23 +444,47 @@ extern "C" int cxa_lanczos_destroy(cxa_lanczos *L) {
 
 namespace {
 
-int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, hipStream_t s) {
+// One iteration.  it >= 0: the iteration index is passed to the kernels by value; it < 0: they
+// read the device counter, which the iteration then advances (the captured form).
+int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, int it,
+                 hipStream_t s) {
+  const IterIdx iter{it < 0 ? L->counter : nullptr, it < 0 ? 0 : it};
+  double *part_a = L->part, *part_b = L->part + kPartials;
   int rc = fn(ctx, L->cur, L->w, (cxa_stream_t)s);
   if (rc) return rc;
-  if ((rc = vec_dot(L->w, L->cur, L->n, L->part, L->As(), L->counter, 0, 0, s))) return rc;
-  if ((rc = vec_three_term(L->w, L->cur, L->V, L->n, L->As(), L->Bs(), L->counter, L->part, s)))
+  // alpha partials; alpha with the three-term update; (re-orthogonalisation); beta with the next
+  // basis vector
+  if ((rc = vec_dot_partials(L->w, L->cur, L->n, part_a, s))) return rc;
+  if ((rc = vec_three_term(L->w, L->cur, L->V, L->n, part_a, L->As(), L->Bs(), iter, part_b, s)))
     return rc;
-  if (reorth &&
-      (rc = vec_reorth(L->w, L->V, L->n, L->maxiter + 1, L->counter, L->part, L->h(), s)))
+  if (reorth && (rc = vec_reorth(L->w, L->V, L->n, L->maxiter + 1, iter, L->part, L->h(),
+                                 part_b, s)))
     return rc;
-  if ((rc = vec_finalize(L->part, L->Bs(), L->counter, 0, 1, s))) return rc;
-  if ((rc = vec_next(L->cur, L->V, L->w, L->n, L->Bs(), L->counter, s))) return rc;
-  return vec_counter_add(L->counter, 1, s);
+  if ((rc = vec_next(L->cur, L->V, L->w, L->n, part_b, L->Bs(), iter, s))) return rc;
+  return it < 0 ? vec_counter_add(L->counter, 1, s) : CXA_OK;
+}
+
+// `iters` iterations (device-counter form) captured on stream ws as one executable graph
+int capture_iterations(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, int iters,
+                       hipStream_t ws, hipGraphExec_t *out) {
+  CXA_HIP(hipStreamBeginCapture(ws, hipStreamCaptureModeThreadLocal));
+  int rc = CXA_OK;
+  for (int i = 0; i < iters && rc == CXA_OK; ++i) rc = lanczos_body(L, fn, ctx, reorth, -1, ws);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(ws, &g);
+  if (rc == CXA_OK && e == hipSuccess) {
+    if (hipGraphInstantiate(out, g, nullptr, nullptr, 0) != hipSuccess)
+      rc = fail(CXA_ERR_HIP, "Lanczos: graph instantiation failed");
+  } else if (rc == CXA_OK) {
+    rc = fail(CXA_ERR_HIP, "Lanczos: graph capture failed (%s)", hipGetErrorString(e));
+  }
+  if (g) (void)hipGraphDestroy(g);
+  return rc;
 }
 
 }  // namespace
 
-// flags: bit 0 = FP64 full re-orthogonalisation, bit 1 = replay one captured iteration graph
+// flags: bit 0 = FP64 full re-orthogonalisation, bit 1 = replay captured iteration graphs
 extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, const double *Tin,
                                double CvgCrit, int maxiter, int batch, int flags, double *E_out,
                                double *evec, int *niter, double *alphas, double *betas,
@@ -486,27 +513,16 @@ extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
   if ((rc = vec_dot(Tin, Tin, n, L->part, L->nrm(), nullptr, 0, 1, ws))) return rc;
   if ((rc = vec_normalize_init(L->cur, L->V, Tin, n, L->nrm(), ws))) return rc;
 
-  // The iteration is captured afresh on every run: the matvec behind (fn, ctx) may have changed
-  // (a chain's precision, its buffers), and one capture per run is amortised over its iterations.
+  // The iterations are captured afresh on every run (the matvec behind (fn, ctx) may have
+  // changed: a chain's precision, its buffers): one iteration, and a whole batch of them so that
+  // a full batch is one host launch.
   if (use_graph) {
-    if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
-    if (L->graph) (void)hipGraphDestroy(L->graph);
-    L->gexec = nullptr;
-    L->graph = nullptr;
-    CXA_HIP(hipStreamBeginCapture(ws, hipStreamCaptureModeThreadLocal));
-    rc = lanczos_body(L, matvec, ctx, reorth, ws);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(ws, &g);
-    if (rc) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
+    L->drop_graphs();
+    if ((rc = capture_iterations(L, matvec, ctx, reorth, 1, ws, &L->gexec1))) return rc;
+    if (batch > 1 && maxiter >= batch) {
+      if ((rc = capture_iterations(L, matvec, ctx, reorth, batch, ws, &L->gexecB))) return rc;
+      L->gbatch = batch;
     }
-    CXA_HIP(e);
-    L->graph = g;
-    CXA_HIP(hipGraphInstantiate(&L->gexec, g, nullptr, nullptr, 0));
-    L->g_fn = matvec;
-    L->g_ctx = ctx;
-    L->g_flags = flags;
   }
 
   const int m1 = L->maxiter + 1;
@@ -516,11 +532,15 @@ extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
   int done = -1, it = 0;
   while (done < 0 && it < maxiter) {
     const int nb = std::min(batch, maxiter - it);
-    for (int b = 0; b < nb; ++b) {
-      if (use_graph) {
-        CXA_HIP(hipGraphLaunch(L->gexec, ws));
-      } else if ((rc = lanczos_body(L, matvec, ctx, reorth, ws))) {
-        return rc;
+    if (use_graph && nb == L->gbatch) {
+      CXA_HIP(hipGraphLaunch(L->gexecB, ws));
+    } else {
+      for (int b = 0; b < nb; ++b) {
+        if (use_graph) {
+          CXA_HIP(hipGraphLaunch(L->gexec1, ws));
+        } else if ((rc = lanczos_body(L, matvec, ctx, reorth, it + b, ws))) {
+          return rc;
+        }
       }
     }
     const int upto = it + nb;  // alphas/betas [0, upto) are valid after the batch

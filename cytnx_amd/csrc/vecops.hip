@@ -25,6 +25,11 @@ __device__ __forceinline__ double block_sum(double v) {
   return s;  // valid in thread 0
 }
 
+// Reductions finish in a separate one-block launch rather than in the last-arriving block of
+// the partials launch: the L2s of the XCDs are not coherent with each other, so a device-scope
+// fence in every block writes back and invalidates its L2, which costs far more than the
+// kernel boundary that orders the partials for the finishing launch.
+
 __global__ __launch_bounds__(kT) void k_dot_partials(const double *__restrict__ x,
                                                      const double *__restrict__ y, int64_t n,
                                                      double *__restrict__ part) {
@@ -48,17 +53,32 @@ __global__ __launch_bounds__(kT) void k_finalize(const double *__restrict__ part
   }
 }
 
-// w -= (alpha * cur + beta * prev) with alpha = As[it], beta = Bs[it - 1] (0 when it == 0),
-// prev = V[it - 1]; partial sums of the updated w^2 go to part
+// Sum of the P partials a previous launch left in part, in the fixed order of k_finalize, by
+// every block of the consuming launch (so no separate finishing launch is needed: the kernel
+// boundary orders the partials); the value is returned to all threads.
+__device__ __forceinline__ double all_sum_partials(const double *__restrict__ part, int P) {
+  __shared__ double bc;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < P; i += kT) s += part[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) bc = s;
+  __syncthreads();
+  return bc;
+}
+
+// w -= (alpha * cur + beta * prev) with alpha = the dot product whose partials are in part_a
+// (recorded in As[it] by block 0), beta = Bs[it - 1] (0 when it == 0), prev = V[it - 1];
+// partial sums of the updated w^2 go to part_b
 __global__ __launch_bounds__(kT) void k_three_term(double *__restrict__ w,
                                                    const double *__restrict__ cur,
                                                    const double *__restrict__ V, int64_t n,
-                                                   const double *__restrict__ As,
-                                                   const double *__restrict__ Bs,
-                                                   const int *__restrict__ counter,
-                                                   double *__restrict__ part) {
-  const int it = *counter;
-  const double a = As[it];
+                                                   const double *__restrict__ part_a,
+                                                   double *__restrict__ As,
+                                                   const double *__restrict__ Bs, IterIdx iter,
+                                                   double *__restrict__ part_b) {
+  const int it = iter.get();
+  const double a = all_sum_partials(part_a, kPartials);
+  if (blockIdx.x == 0 && threadIdx.x == 0) As[it] = a;
   const double b = it > 0 ? Bs[it - 1] : 0.0;
   const double *prev = it > 0 ? V + (int64_t)(it - 1) * n : cur;
   double s = 0.0;
@@ -69,15 +89,14 @@ __global__ __launch_bounds__(kT) void k_three_term(double *__restrict__ w,
     s += v * v;
   }
   s = block_sum(s);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) part_b[blockIdx.x] = s;
 }
 
 // h_j partial sums: part[j * P + block] = sum over the block's slice of V_j * w, j <= it
 __global__ __launch_bounds__(kT) void k_proj_partials(const double *__restrict__ V,
                                                       const double *__restrict__ w, int64_t n,
-                                                      const int *__restrict__ counter,
-                                                      double *__restrict__ part) {
-  const int it = *counter;
+                                                      IterIdx iter, double *__restrict__ part) {
+  const int it = iter.get();
   const int j = blockIdx.y;
   if (j > it) return;
   const double *v = V + (int64_t)j * n;
@@ -89,10 +108,9 @@ __global__ __launch_bounds__(kT) void k_proj_partials(const double *__restrict__
 }
 
 __global__ __launch_bounds__(kT) void k_proj_finalize(const double *__restrict__ part, int P,
-                                                      const int *__restrict__ counter,
-                                                      double *__restrict__ h) {
+                                                      IterIdx iter, double *__restrict__ h) {
   const int j = blockIdx.x;
-  if (j > *counter) return;
+  if (j > iter.get()) return;
   double s = 0.0;
   for (int i = threadIdx.x; i < P; i += kT) s += part[(int64_t)j * P + i];
   s = block_sum(s);
@@ -102,10 +120,9 @@ __global__ __launch_bounds__(kT) void k_proj_finalize(const double *__restrict__
 // w -= sum_{j <= it} h_j V_j, then partial sums of w^2
 __global__ __launch_bounds__(kT) void k_proj_apply(double *__restrict__ w,
                                                    const double *__restrict__ V, int64_t n,
-                                                   const double *__restrict__ h,
-                                                   const int *__restrict__ counter,
+                                                   const double *__restrict__ h, IterIdx iter,
                                                    double *__restrict__ part) {
-  const int it = *counter;
+  const int it = iter.get();
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
     double v = w[i];
@@ -117,13 +134,15 @@ __global__ __launch_bounds__(kT) void k_proj_apply(double *__restrict__ w,
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// cur = V[it + 1] = w / Bs[it]
+// beta = |w| from the partials in part_b (recorded in Bs[it] by block 0), then
+// cur = V[it + 1] = w / beta
 __global__ __launch_bounds__(kT) void k_next(double *__restrict__ cur, double *__restrict__ V,
                                              const double *__restrict__ w, int64_t n,
-                                             const double *__restrict__ Bs,
-                                             const int *__restrict__ counter) {
-  const int it = *counter;
-  const double b = Bs[it];
+                                             const double *__restrict__ part_b,
+                                             double *__restrict__ Bs, IterIdx iter) {
+  const int it = iter.get();
+  const double b = sqrt(all_sum_partials(part_b, kPartials));
+  if (blockIdx.x == 0 && threadIdx.x == 0) Bs[it] = b;
   double *dst = V + (int64_t)(it + 1) * n;
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
     const double v = w[i] / b;
@@ -132,8 +151,8 @@ __global__ __launch_bounds__(kT) void k_next(double *__restrict__ cur, double *_
   }
 }
 
-__global__ void k_counter_add(int *counter, int d) { *counter += d; }
 __global__ void k_counter_set(int *counter, int v) { *counter = v; }
+__global__ void k_counter_add(int *counter, int d) { *counter += d; }
 
 // cur = V[0] = x / nrm[0]
 __global__ __launch_bounds__(kT) void k_normalize_init(double *__restrict__ cur,
@@ -184,20 +203,28 @@ int vec_dot(const double *x, const double *y, int64_t n, double *part, double *o
   return CXA_OK;
 }
 
-int vec_three_term(double *w, const double *cur, const double *V, int64_t n, const double *As,
-                   const double *Bs, const int *counter, double *part, hipStream_t s) {
-  hipLaunchKernelGGL(k_three_term, dim3(kPartials), dim3(kT), 0, s, w, cur, V, n, As, Bs,
-                     counter, part);
+int vec_dot_partials(const double *x, const double *y, int64_t n, double *part, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_partials, dim3(kPartials), dim3(kT), 0, s, x, y, n, part);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }
 
-int vec_reorth(double *w, const double *V, int64_t n, int max_j, const int *counter, double *part,
-               double *h, hipStream_t s) {
-  hipLaunchKernelGGL(k_proj_partials, dim3(kPartials, max_j), dim3(kT), 0, s, V, w, n, counter,
-                     part);
-  hipLaunchKernelGGL(k_proj_finalize, dim3(max_j), dim3(kT), 0, s, part, kPartials, counter, h);
-  hipLaunchKernelGGL(k_proj_apply, dim3(kPartials), dim3(kT), 0, s, w, V, n, h, counter, part);
+int vec_three_term(double *w, const double *cur, const double *V, int64_t n,
+                   const double *part_a, double *As, const double *Bs, IterIdx iter,
+                   double *part_b, hipStream_t s) {
+  hipLaunchKernelGGL(k_three_term, dim3(kPartials), dim3(kT), 0, s, w, cur, V, n, part_a, As, Bs,
+                     iter, part_b);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_reorth(double *w, const double *V, int64_t n, int max_j, IterIdx iter, double *part,
+               double *h, double *part_out, hipStream_t s) {
+  // with the iteration known on the host only the live projections are launched
+  const int nj = iter.p ? max_j : iter.v + 1;
+  hipLaunchKernelGGL(k_proj_partials, dim3(kPartials, nj), dim3(kT), 0, s, V, w, n, iter, part);
+  hipLaunchKernelGGL(k_proj_finalize, dim3(nj), dim3(kT), 0, s, part, kPartials, iter, h);
+  hipLaunchKernelGGL(k_proj_apply, dim3(kPartials), dim3(kT), 0, s, w, V, n, h, iter, part_out);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }
@@ -210,9 +237,9 @@ int vec_finalize(const double *part, double *out, const int *counter, int offset
   return CXA_OK;
 }
 
-int vec_next(double *cur, double *V, const double *w, int64_t n, const double *Bs,
-             const int *counter, hipStream_t s) {
-  hipLaunchKernelGGL(k_next, dim3(grid_of(n)), dim3(kT), 0, s, cur, V, w, n, Bs, counter);
+int vec_next(double *cur, double *V, const double *w, int64_t n, const double *part_b,
+             double *Bs, IterIdx iter, hipStream_t s) {
+  hipLaunchKernelGGL(k_next, dim3(grid_of(n)), dim3(kT), 0, s, cur, V, w, n, part_b, Bs, iter);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }

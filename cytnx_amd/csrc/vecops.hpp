@@ -12,14 +12,31 @@ constexpr int kPartials = 512;  // partial sums per reduction (fixed: determinis
 // out[(*counter) + offset] = x.y (mode 0) or sqrt(x.y) (mode 1); counter may be null (0)
 int vec_dot(const double *x, const double *y, int64_t n, double *part, double *out,
             const int *counter, int offset, int mode, hipStream_t s);
-int vec_three_term(double *w, const double *cur, const double *V, int64_t n, const double *As,
-                   const double *Bs, const int *counter, double *part, hipStream_t s);
-int vec_reorth(double *w, const double *V, int64_t n, int max_j, const int *counter, double *part,
-               double *h, hipStream_t s);
+// The Lanczos iteration index: read from a device counter (launches captured once and
+// replayed) or passed by value (launches enqueued per iteration).
+struct IterIdx {
+  const int *p;
+  int v;
+  __device__ __forceinline__ int get() const { return p ? *p : v; }
+};
+
+// Lanczos iteration kernels.  Each reduction leaves kPartials partial sums that the next launch
+// sums itself in a fixed order (no separate finishing launch).
+// part = kPartials partials of x.y
+int vec_dot_partials(const double *x, const double *y, int64_t n, double *part, hipStream_t s);
+// As[it] = sum(part_a); w -= As[it] cur + Bs[it-1] V[it-1]; part_b = partials of |w|^2
+int vec_three_term(double *w, const double *cur, const double *V, int64_t n,
+                   const double *part_a, double *As, const double *Bs, IterIdx iter,
+                   double *part_b, hipStream_t s);
+// w -= sum_{j <= it} (V_j . w) V_j (classical Gram-Schmidt against the stored basis);
+// part_out = partials of the new |w|^2; part: kPartials * max_j scratch
+int vec_reorth(double *w, const double *V, int64_t n, int max_j, IterIdx iter, double *part,
+               double *h, double *part_out, hipStream_t s);
 int vec_finalize(const double *part, double *out, const int *counter, int offset, int mode,
                  hipStream_t s);
-int vec_next(double *cur, double *V, const double *w, int64_t n, const double *Bs,
-             const int *counter, hipStream_t s);
+// Bs[it] = sqrt(sum(part_b)); cur = V[it+1] = w / Bs[it]
+int vec_next(double *cur, double *V, const double *w, int64_t n, const double *part_b,
+             double *Bs, IterIdx iter, hipStream_t s);
 int vec_counter_add(int *counter, int d, hipStream_t s);
 int vec_counter_set(int *counter, int v, hipStream_t s);
 int vec_normalize_init(double *cur, double *V, const double *x, int64_t n, const double *nrm,

@@ -119,7 +119,7 @@ def _sector_offsets(b: Bond) -> np.ndarray:
 
 
 def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: np.ndarray,
-                    Y: UniTensor, labY: dict):
+                    Y: UniTensor, labY: dict, live: Optional[np.ndarray] = None):
     """Task arrays of the mix launch Y = W12 applied to (Lp . psi), never storing Lp . psi.
 
     Lp: (l, w, l') environment blocks; psi: the two-site state (l, s1, s2, r); W12: the dense
@@ -131,7 +131,8 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
     Y[l', s1', s2', w'', r] is the sum over slices of W12[w, s1, s1', w'', s2, s2'] * X.
 
     Returns (items (n,4), slices (m,7), outs (k,2), coef (k,MIX_SLICES)) as int64 / float64
-    arrays, or raises MixUnsupported."""
+    arrays, or raises MixUnsupported.  With live (a bool array of Y.Nblocks()), marks the Y
+    blocks that some slice reaches (the others are zero for every input)."""
     pL = {k: Lp.labels().index(labL[k]) for k in ("l", "w", "lp")}
     pP = {k: psi.labels().index(labP[k]) for k in ("l", "s1", "s2", "r")}
     pY = {k: Y.labels().index(labY[k]) for k in ("lp", "s1p", "s2p", "w2", "r")}
@@ -176,6 +177,7 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
                                  int(offs2[int(tp[pP["s2"]])] + i2))
                             cand.append((desc, g, int(shl[pL["lp"]]), int(shp[pP["r"]])))
         olist = []
+        oblock = []
         M = N = None
         for yi in ylist:
             ty, shy = Y._itoi[yi], Y._shapes[yi]
@@ -192,10 +194,15 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
                                       (int(offs1p[int(ty[pY["s1p"]])] + i1p),
                                        int(offw2[int(ty[pY["w2"]])] + j),
                                        int(offs2p[int(ty[pY["s2p"]])] + i2p))))
+                        oblock.append(yi)
         C = np.zeros((len(olist), len(cand)))
         for o, (_, _, (s1p, w2, s2p)) in enumerate(olist):
             for s, (_, (w, s1, s2), _, _) in enumerate(cand):
                 C[o, s] = W12[w, s1, s1p, w2, s2, s2p]
+        if live is not None:
+            for o, yi in enumerate(oblock):
+                if np.any(C[o] != 0.0):
+                    live[yi] = True
         keep = np.any(C != 0.0, axis=0) if len(olist) else np.zeros(len(cand), dtype=bool)
         cand = [c for c, k in zip(cand, keep) if k]
         C = C[:, keep]
@@ -215,6 +222,20 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
     as64 = lambda rows, w: np.asarray(rows, dtype=np.int64).reshape(-1, w)
     return (as64(items, 4), as64(slices, 7), as64(outs, 2),
             np.asarray(coefs, dtype=np.float64).reshape(-1, MIX_SLICES))
+
+
+def subset_blocks(T: UniTensor, keep: np.ndarray) -> UniTensor:
+    """Metadata of T restricted to the blocks with keep[i] (same order, packed layout)."""
+    out = T.clone_meta()
+    out._itoi = np.ascontiguousarray(T._itoi[keep])
+    out._shapes = np.ascontiguousarray(T._shapes[keep])
+    sizes = np.prod(out._shapes.astype(np.int64), axis=1) if len(out._shapes) else \
+        np.zeros(0, dtype=np.int64)
+    out._offsets = np.zeros(len(sizes), dtype=np.int64)
+    if len(sizes):
+        out._offsets[1:] = np.cumsum(sizes)[:-1]
+    out._storage = T._storage[:0]
+    return out
 
 
 def check_mix_bounds(arrays, na: int, nb: int, nc: int) -> None:
@@ -408,10 +429,15 @@ class Heff2(_HeffBase):
         wl = W12.labels()
         Wd = np.ascontiguousarray(np.transpose(
             Wd, [wl.index(x) for x in (lab_w, s1[0], s1p[0], lab_w2, s2[0], s2p[0])]))
-        arrays = mix_plan_arrays(
-            Lp, {"l": lab_l, "w": lab_w, "lp": lab_lp},
-            psi_m, {"l": lab_l, "s1": s1[0], "s2": s2[0], "r": lab_r}, Wd,
-            Y, {"lp": lab_lp, "s1p": s1p[0], "s2p": s2p[0], "w2": kw, "r": kr})
+        args = (Lp, {"l": lab_l, "w": lab_w, "lp": lab_lp},
+                psi_m, {"l": lab_l, "s1": s1[0], "s2": s2[0], "r": lab_r}, Wd)
+        ylab = {"lp": lab_lp, "s1p": s1p[0], "s2p": s2p[0], "w2": kw, "r": kr}
+        # Y blocks that no (w, s1, s2) slice reaches are zero for every psi: leave them out of
+        # Y, so neither the mix launch nor the contraction with R touches them
+        live = np.zeros(Y.Nblocks(), dtype=bool)
+        mix_plan_arrays(*args, Y, ylab, live)
+        Y = subset_blocks(Y, live)
+        arrays = mix_plan_arrays(*args, Y, ylab)
         check_mix_bounds(arrays, Lp._storage.numel(), meta_elems(psi_m), meta_elems(Y))
         plan = _MixPlan(arrays, dev)
         Rpp = Rp.permute([lab_w2, lab_r, lab_rp]).relabel([kw, kr, lab_rp])

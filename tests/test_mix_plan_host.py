@@ -10,7 +10,7 @@ import pytest
 import torch
 
 from cytnx_amd import heisenberg
-from cytnx_amd.heff import MIX_SLICES, mix_plan_arrays
+from cytnx_amd.heff import MIX_SLICES, check_mix_bounds, mix_plan_arrays, subset_blocks
 from cytnx_amd.unitensor import UniTensor
 
 
@@ -78,6 +78,40 @@ def test_mix_plan_matches_dense_closed_form(chi):
     dense = np.einsum("asxvty,alp,lstr->pxyvr", W, Ld, Pd, optimize=True)
     want = _fill_from_dense(Y, dense)
     assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("chi", [8, 20])
+def test_mix_plan_pruned_layout(chi):
+    """Y blocks no slice reaches are zero in the dense closed form; the plan on the pruned
+    layout (those blocks left out, the rest packed) still reproduces every kept block."""
+    psi, L, M1, M2, R = heisenberg.two_site_problem(chi, seed=chi + 1, hermitian=True)
+    Lb = L.bonds()
+    Lp = _meta([Lb[1], Lb[0], Lb[2]], ["-1", "-5", "0"])
+    Ld = L.to_dense()
+    A = _fill_from_dense(Lp, np.transpose(Ld, (1, 0, 2)))
+    B = psi.storage().numpy()
+    W = np.einsum("awsx,wvty->asxvty", M1.to_dense(), M2.to_dense())
+    Y = _meta([Lb[2], M1.bonds()[3], M2.bonds()[3], M2.bonds()[1], psi.bonds()[3]],
+              ["0", "1", "2", "k0", "k1"])
+    labs = ({"l": "-1", "w": "-5", "lp": "0"}, {"l": "-1", "s1": "-2", "s2": "-3", "r": "-4"},
+            {"lp": "0", "s1p": "1", "s2p": "2", "w2": "k0", "r": "k1"})
+    live = np.zeros(Y.Nblocks(), dtype=bool)
+    mix_plan_arrays(Lp, labs[0], psi, labs[1], W, Y, labs[2], live)
+    assert 0 < live.sum() < Y.Nblocks()
+    dense = np.einsum("asxvty,alp,lstr->pxyvr", W, Ld, psi.to_dense(), optimize=True)
+    full = _fill_from_dense(Y, dense)
+    for bi in np.nonzero(~live)[0]:
+        o = int(Y._offsets[bi])
+        assert not np.any(full[o:o + int(np.prod(Y._shapes[bi].astype(np.int64)))])
+    Yl = subset_blocks(Y, live)
+    arrays = mix_plan_arrays(Lp, labs[0], psi, labs[1], W, Yl, labs[2])
+    ny = int(sum(np.prod(s.astype(np.int64)) for s in Yl._shapes))
+    check_mix_bounds(arrays, A.size, B.size, ny)
+    got = _run_mix(arrays, A, B, ny)
+    assert not np.isnan(got).any()
+    want = _fill_from_dense(Yl, dense)
+    assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+    assert arrays[0][:, 2].min() >= 1  # every kept item has a slice
 
 
 def test_mix_plan_xxz_item_bounds():
