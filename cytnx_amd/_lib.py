@@ -61,6 +61,7 @@ def _declare(L):
         "cxa_block_contract_set_config": (ci, [vp, ci, ci, ci]),
         "cxa_block_contract_get_config": (ci, [vp, ci, intp, intp, intp]),
         "cxa_block_contract_exec": (ci, [vp, vp, vp, vp, vp, ci, vp]),
+        "cxa_block_contract_exec_ex": (ci, [vp, vp, vp, vp, vp, ci, ci, vp]),
         "cxa_block_contract_destroy": (ci, [vp]),
         "cxa_mix_contract_plan": (ci, [ci, ci, i64p, i64, i64p, i64, i64p, dblp,
                                        ctypes.POINTER(vp)]),

@@ -487,7 +487,7 @@ namespace cxa {
 // dtype: FP32 copies of the same layout, see the chain).  Offsets are in elements, so the same
 // task list addresses either copy.  Workspace offsets are in elements of the copied operand.
 int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, int a_dt,
-                  int b_dt, int c_dt, int prec, void *workspace, hipStream_t s) {
+                  int b_dt, int c_dt, int prec, void *workspace, hipStream_t s, int conj) {
   CXA_CHECK(p, CXA_ERR_INVALID, "block contract exec: null plan");
   CXA_CHECK(!(p->copyA || p->copyB) || workspace, CXA_ERR_INVALID,
             "block contract exec: this plan needs a workspace of %lld bytes",
@@ -524,6 +524,7 @@ int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, i
   g.per_xcd = v.per_xcd;
   g.splitk = v.splitk;
   g.tile_class = v.tile_class;
+  g.conj = is_complex(p->dtype) ? conj : 0;
   if (p->mix) {
     g.outs = (const MixOut *)p->d_outs.p;
     g.kind0 = v.launch_kind;
@@ -554,6 +555,16 @@ extern "C" int cxa_block_contract_exec(const cxa_bc_plan *p, const void *L, cons
   CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_exec: null plan");
   return bc_exec_typed(p, L, R, C, p->dtype, p->dtype, p->dtype, prec, workspace,
                        (hipStream_t)stream);
+}
+
+extern "C" int cxa_block_contract_exec_ex(const cxa_bc_plan *p, const void *L, const void *R,
+                                          void *C, void *workspace, int prec, int flags,
+                                          cxa_stream_t stream) {
+  CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_exec_ex: null plan");
+  CXA_CHECK(flags >= 0 && flags <= 3, CXA_ERR_INVALID, "cxa_block_contract_exec_ex: flags %d",
+            flags);
+  return bc_exec_typed(p, L, R, C, p->dtype, p->dtype, p->dtype, prec, workspace,
+                       (hipStream_t)stream, flags);
 }
 
 extern "C" int cxa_block_contract_destroy(cxa_bc_plan *p) {

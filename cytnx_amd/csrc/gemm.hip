@@ -435,14 +435,16 @@ __device__ __forceinline__ void wg_rows(const char *__restrict__ Abase,
 // ---------------------------------------------------------------------------------------
 // complex MFMA tiles (C128 on FP64 MFMA, C64 on FP32 MFMA): four real products per step
 // ---------------------------------------------------------------------------------------
+// conj: bit 0 conjugates A, bit 1 conjugates B as they are loaded (the adjoint contractions).
 template <typename CT, int MI, int NI, typename SEGS>
 __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
                                                const char *__restrict__ Bbase,
                                                char *__restrict__ Cbase, const GBlk &blk,
-                                               SEGS segs, int m0, int n0, int lane) {
+                                               SEGS segs, int m0, int n0, int lane, int conj) {
   constexpr bool F64 = std::is_same<CT, cd>::value;
   using acc_t = typename std::conditional<F64, f64x4, f32x4>::type;
   using op_t = typename std::conditional<F64, double, float>::type;
+  const op_t sa = (conj & 1) ? op_t(-1) : op_t(1), sb = (conj & 2) ? op_t(-1) : op_t(1);
   const int r = lane & 15, g = lane >> 4;
   const int mt = min(16 * MI, blk.M - m0);
   const int nt = min(16 * NI, blk.N - n0);
@@ -470,7 +472,7 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
           CT v{0, 0};
           if (row < mt && k < K) v = Ap[(int64_t)row * sg.sam + (int64_t)k * sg.sak];
           ar[i][j] = v.re;
-          ai[i][j] = v.im;
+          ai[i][j] = sa * v.im;
         }
       }
 #pragma unroll
@@ -482,7 +484,7 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
           CT v{0, 0};
           if (col < nt && k < K) v = Bp[(int64_t)k * sg.sbk + (int64_t)col * sg.sbn];
           br[i][j] = v.re;
-          bi[i][j] = v.im;
+          bi[i][j] = sb * v.im;
         }
       }
 #pragma unroll
@@ -537,13 +539,14 @@ enum TileClass { TC_MIXED = 0, TC_ROWS = 1, TC_BIG = 2 };
 
 template <int PREC, typename TA, typename TB, typename TC, bool CPLX, int CLS, typename SEGS>
 __device__ __forceinline__ void run_tile(int kind, const char *A, const char *B, char *C,
-                                         const GBlk &blk, SEGS segs, int m0, int n0, int lane) {
+                                         const GBlk &blk, SEGS segs, int m0, int n0, int lane,
+                                         int conj) {
   if constexpr (CPLX) {
     switch (kind) {
-      case TK_1x1: wave_tile_cplx<TA, 1, 1>(A, B, C, blk, segs, m0, n0, lane); break;
-      case TK_2x1: wave_tile_cplx<TA, 2, 1>(A, B, C, blk, segs, m0, n0, lane); break;
-      case TK_1x2: wave_tile_cplx<TA, 1, 2>(A, B, C, blk, segs, m0, n0, lane); break;
-      default: wave_tile_cplx<TA, 2, 2>(A, B, C, blk, segs, m0, n0, lane); break;
+      case TK_1x1: wave_tile_cplx<TA, 1, 1>(A, B, C, blk, segs, m0, n0, lane, conj); break;
+      case TK_2x1: wave_tile_cplx<TA, 2, 1>(A, B, C, blk, segs, m0, n0, lane, conj); break;
+      case TK_1x2: wave_tile_cplx<TA, 1, 2>(A, B, C, blk, segs, m0, n0, lane, conj); break;
+      default: wave_tile_cplx<TA, 2, 2>(A, B, C, blk, segs, m0, n0, lane, conj); break;
     }
   } else if constexpr (CLS == TC_ROWS) {
     wave_rows<TA, TB, TC>(A, B, C, blk, segs, m0, lane);
@@ -565,7 +568,8 @@ template <int PREC, typename TA, typename TB, typename TC, bool CPLX, int CLS>
 __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict__ tiles,
                                                         int per_xcd, const GBlk *__restrict__ blks,
                                                         const GSeg *__restrict__ segs,
-                                                        const char *A, const char *B, char *C) {
+                                                        const char *A, const char *B, char *C,
+                                                        int conj) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const GTile *range = tiles + (int64_t)(blockIdx.x % kXcds) * per_xcd;
@@ -575,7 +579,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict
     if (tl.blk < 0) continue;  // padding
     const GBlk blk = blks[tl.blk];
     run_tile<PREC, TA, TB, TC, CPLX, CLS>(tl.kind, A, B, C, blk, SegArray{segs}, tl.m0, tl.n0,
-                                          lane);
+                                          lane, conj);
   }
 }
 
@@ -817,14 +821,15 @@ int launch_mix(const GemmLaunch &g, hipStream_t s) {
 template <int PREC, typename TA, typename TB, typename TC, bool CPLX>
 __global__ __launch_bounds__(64 * kWaves) void k_single(GBlk blk, GSeg seg, int kind,
                                                        int tiles_n, int ntiles, const char *A,
-                                                       const char *B, char *C) {
+                                                       const char *B, char *C, int conj) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tr = kind == TK_1x1 || kind == TK_1x2 ? 16 : 32;
   const int tc = kind == TK_1x1 || kind == TK_2x1 ? 16 : 32;
   for (int t = blockIdx.x * kWaves + wave; t < ntiles; t += gridDim.x * kWaves) {
     const int m0 = (t / tiles_n) * tr, n0 = (t % tiles_n) * tc;
-    run_tile<PREC, TA, TB, TC, CPLX, TC_MIXED>(kind, A, B, C, blk, SegOne{seg}, m0, n0, lane);
+    run_tile<PREC, TA, TB, TC, CPLX, TC_MIXED>(kind, A, B, C, blk, SegOne{seg}, m0, n0, lane,
+                                               conj);
   }
 }
 
@@ -844,7 +849,7 @@ int launch_t(const GemmLaunch &g, hipStream_t s) {
     const int grid = std::min((ntiles + kWaves - 1) / kWaves, 8192);
     hipLaunchKernelGGL((k_single<PREC, TA, TB, TC, CPLX>), dim3(grid), dim3(64 * kWaves), 0, s,
                        b, g.seg0, kind, tiles_n, ntiles, (const char *)g.A, (const char *)g.B,
-                       (char *)g.C);
+                       (char *)g.C, g.conj);
     CXA_HIP(hipGetLastError());
     return CXA_OK;
   }
@@ -872,19 +877,19 @@ int launch_t(const GemmLaunch &g, hipStream_t s) {
     else if (cls == TC_ROWS)
       hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_ROWS>), dim3(wave_grid),
                          dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
-                         (const char *)g.A, (const char *)g.B, (char *)g.C);
+                         (const char *)g.A, (const char *)g.B, (char *)g.C, g.conj);
     else if (cls == TC_BIG)
       hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_BIG>), dim3(wave_grid),
                          dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
-                         (const char *)g.A, (const char *)g.B, (char *)g.C);
+                         (const char *)g.A, (const char *)g.B, (char *)g.C, g.conj);
     else
       hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_MIXED>), dim3(wave_grid),
                          dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
-                         (const char *)g.A, (const char *)g.B, (char *)g.C);
+                         (const char *)g.A, (const char *)g.B, (char *)g.C, g.conj);
   } else {
     hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, true, TC_MIXED>), dim3(wave_grid),
                        dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
-                       (const char *)g.A, (const char *)g.B, (char *)g.C);
+                       (const char *)g.A, (const char *)g.B, (char *)g.C, g.conj);
   }
   CXA_HIP(hipGetLastError());
   return CXA_OK;

@@ -76,6 +76,7 @@ struct GemmLaunch {
   bool splitk = false;  // one workgroup per tile, K chunks interleaved over its waves
   int tile_class = 0;   // the class make_tiles returned
   const MixOut *outs = nullptr;  // non-null: mix launch (blks are items, segs are slices)
+  int conj = 0;  // complex dtypes: bit 0 conjugates A, bit 1 conjugates B
   // single-block mode: one block with one segment passed by value, no device task arrays
   bool single = false;
   GBlk blk0{};

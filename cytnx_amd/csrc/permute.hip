@@ -126,6 +126,156 @@ __global__ __launch_bounds__(kThreads) void permute_gather(const T *__restrict__
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// 32-bit forms (tensors below 2^31 elements, the common case): every index decode is a
+// multiply-high, an add and a shift instead of 64-bit division, and each thread stages several
+// elements in registers so that its loads are in flight together.
+// ---------------------------------------------------------------------------------------
+
+// Division of n < 2^31 by a run-time invariant d in [1, 2^31] (Granlund & Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994, section 4):
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, n / d = (mulhi(n, m) + n) >> l.
+struct FastDiv {
+  uint32_t d, m, l;
+  static FastDiv make(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    f.l = 0;
+    while ((uint64_t(1) << f.l) < d) ++f.l;
+    f.m = (uint32_t)((((uint64_t(1) << 32) * ((uint64_t(1) << f.l) - d)) / d) + 1);
+    return f;
+  }
+  __device__ __forceinline__ uint32_t div(uint32_t n) const {
+    return (__umulhi(n, m) + n) >> l;
+  }
+};
+
+struct TileArgs32 {
+  int na, nb, no, w, h;
+  uint32_t prodA, prodB, ntiles;
+  FastDiv dnca, dncb, dw, dh;
+  FastDiv a_div[kMaxGroup];  // A axes (input-contiguous), outermost first
+  uint32_t a_ostr[kMaxGroup];
+  FastDiv b_div[kMaxGroup];  // B axes (output-contiguous), outermost first
+  uint32_t b_istr[kMaxGroup];
+  FastDiv o_div[kMaxOuter];  // remaining axes
+  uint32_t o_istr[kMaxOuter], o_ostr[kMaxOuter];
+};
+
+// decode x = sum_k idx_k * stride_k for a mixed-radix index (outermost digit first)
+__device__ __forceinline__ uint32_t decode32(uint32_t x, int n, const FastDiv *div,
+                                             const uint32_t *stride) {
+  uint32_t off = 0;
+  for (int k = n - 1; k >= 0; --k) {
+    const uint32_t q = div[k].div(x);
+    off += (x - q * div[k].d) * stride[k];
+    x = q;
+  }
+  return off;
+}
+
+template <typename T, int MAXE>
+__global__ __launch_bounds__(kThreads) void permute_tiled32(const T *__restrict__ src,
+                                                            T *__restrict__ dst, TileArgs32 p) {
+  constexpr int PER = MAXE / kThreads;
+  __shared__ T tile[MAXE + 64];
+  __shared__ uint32_t aoff[1024];
+  __shared__ uint32_t boff[1024];
+  const int w = p.w, h = p.h, ws = w + 1;
+  const uint32_t wh = (uint32_t)(w * h);
+  for (uint32_t t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
+    uint32_t rem = t, q = p.dnca.div(rem);
+    const uint32_t ca = rem - q * p.dnca.d;
+    rem = q;
+    q = p.dncb.div(rem);
+    const uint32_t cb = rem - q * p.dncb.d;
+    rem = q;
+    uint32_t oin = 0, oout = 0;
+    for (int i = p.no - 1; i >= 0; --i) {
+      q = p.o_div[i].div(rem);
+      const uint32_t x = rem - q * p.o_div[i].d;
+      rem = q;
+      oin += x * p.o_istr[i];
+      oout += x * p.o_ostr[i];
+    }
+    const uint32_t a0 = ca * w, b0 = cb * h;
+    const uint32_t tw = min((uint32_t)w, p.prodA - a0), th = min((uint32_t)h, p.prodB - b0);
+    __syncthreads();  // the previous tile is fully written out
+    for (int i = threadIdx.x; i < w + h; i += kThreads) {
+      if (i < w)
+        aoff[i] = decode32(a0 + i, p.na, p.a_div, p.a_ostr);
+      else
+        boff[i - w] = decode32(b0 + (i - w), p.nb, p.b_div, p.b_istr);
+    }
+    __syncthreads();
+    // read: consecutive threads walk A, which is contiguous in the input
+    const T *s = src + oin + a0;
+    T r[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t i = threadIdx.x + k * kThreads;
+      const uint32_t b = p.dw.div(i), a = i - b * w;
+      if (i < wh && a < tw && b < th) r[k] = s[boff[b] + a];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t i = threadIdx.x + k * kThreads;
+      const uint32_t b = p.dw.div(i), a = i - b * w;
+      if (i < wh && a < tw && b < th) tile[b * ws + a] = r[k];
+    }
+    __syncthreads();
+    // write: consecutive threads walk B, which is contiguous in the output
+    T *d = dst + oout + b0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t i = threadIdx.x + k * kThreads;
+      const uint32_t a = p.dh.div(i), b = i - a * h;
+      if (i < wh && a < tw && b < th) r[k] = tile[b * ws + a];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t i = threadIdx.x + k * kThreads;
+      const uint32_t a = p.dh.div(i), b = i - a * h;
+      if (i < wh && a < tw && b < th) d[aoff[a] + b] = r[k];
+    }
+  }
+}
+
+struct RowArgs32 {
+  int nd;              // row axes (the output's axes except the kept innermost one)
+  uint32_t nvec;       // total / V
+  FastDiv dvpr;        // vectors per row
+  FastDiv n_div[16];   // row axes in output order, outermost first
+  uint32_t n_istr[16];  // their input strides (in elements)
+};
+
+// Innermost axis kept: output vector v (V elements) is vector v % vpr of output row v / vpr;
+// the row's input offset is decoded once per vector with FastDiv.  U vectors per thread, all
+// loads issued before the stores.
+template <typename T, typename VT, int U>
+__global__ __launch_bounds__(kThreads) void permute_rows32(const T *__restrict__ src,
+                                                           T *__restrict__ dst, RowArgs32 p) {
+  constexpr int V = sizeof(VT) / sizeof(T);
+  const uint32_t G = gridDim.x * kThreads;
+  for (uint32_t v0 = blockIdx.x * kThreads + threadIdx.x; v0 < p.nvec; v0 += U * G) {
+    VT r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t v = v0 + u * G;
+      if (v < p.nvec) {
+        const uint32_t row = p.dvpr.div(v), j = v - row * p.dvpr.d;
+        const uint32_t in = decode32(row, p.nd, p.n_div, p.n_istr) + j * V;
+        r[u] = *reinterpret_cast<const VT *>(src + in);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t v = v0 + u * G;
+      if (v < p.nvec) *reinterpret_cast<VT *>(dst + (size_t)v * V) = r[u];
+    }
+  }
+}
+
 struct Reduced {
   int rank = 0;
   int64_t shape[16];  // old (input) extents, row-major
@@ -256,6 +406,37 @@ int launch_tiled(const Reduced &r, const void *src, void *dst, hipStream_t s) {
   for (int k = 0; k < no; ++k) outer *= p.o_ext[k];
   p.ntiles = outer * p.nca * p.ncb;
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(p.ntiles, 256 * 64));
+  if (r.total < (int64_t(1) << 31)) {  // every offset and index fits in 32 bits
+    TileArgs32 q{};
+    q.na = p.na;
+    q.nb = p.nb;
+    q.no = p.no;
+    q.w = p.w;
+    q.h = p.h;
+    q.prodA = (uint32_t)p.prodA;
+    q.prodB = (uint32_t)p.prodB;
+    q.ntiles = (uint32_t)p.ntiles;
+    q.dnca = FastDiv::make((uint32_t)p.nca);
+    q.dncb = FastDiv::make((uint32_t)p.ncb);
+    q.dw = FastDiv::make((uint32_t)p.w);
+    q.dh = FastDiv::make((uint32_t)p.h);
+    for (int k = 0; k < p.na; ++k) {
+      q.a_div[k] = FastDiv::make((uint32_t)p.a_ext[k]);
+      q.a_ostr[k] = (uint32_t)p.a_ostr[k];
+    }
+    for (int k = 0; k < p.nb; ++k) {
+      q.b_div[k] = FastDiv::make((uint32_t)p.b_ext[k]);
+      q.b_istr[k] = (uint32_t)p.b_istr[k];
+    }
+    for (int k = 0; k < p.no; ++k) {
+      q.o_div[k] = FastDiv::make((uint32_t)p.o_ext[k]);
+      q.o_istr[k] = (uint32_t)p.o_istr[k];
+      q.o_ostr[k] = (uint32_t)p.o_ostr[k];
+    }
+    hipLaunchKernelGGL((permute_tiled32<T, MAXE>), dim3(grid), dim3(kThreads), 0, s,
+                       (const T *)src, (T *)dst, q);
+    return 0;
+  }
   hipLaunchKernelGGL((permute_tiled<T, MAXE>), dim3(grid), dim3(kThreads), 0, s,
                      (const T *)src, (T *)dst, p);
   return 0;
@@ -279,6 +460,21 @@ int launch_gather(const Reduced &r, const void *src, void *dst, hipStream_t s) {
   const bool inner_kept = r.perm[r.rank - 1] == r.rank - 1;
   constexpr int V = sizeof(T) >= 16 ? 1 : 16 / (int)sizeof(T);
   const bool aligned = ((uintptr_t)src % 16 == 0) && ((uintptr_t)dst % 16 == 0);
+  if (inner_kept && inner % V == 0 && aligned && r.total < (int64_t(1) << 31)) {
+    RowArgs32 q{};
+    q.nd = r.rank - 1;
+    q.nvec = (uint32_t)(r.total / V);
+    q.dvpr = FastDiv::make((uint32_t)(inner / V));
+    for (int j = 0; j + 1 < r.rank; ++j) {
+      q.n_div[j] = FastDiv::make((uint32_t)g.n_ext[j]);
+      q.n_istr[j] = (uint32_t)g.n_istr[j];
+    }
+    constexpr int U = 4;
+    const int grid = grid_for((q.nvec + U - 1) / U, kThreads);
+    hipLaunchKernelGGL((permute_rows32<T, U128, U>), dim3(grid), dim3(kThreads), 0, s,
+                       (const T *)src, (T *)dst, q);
+    return 0;
+  }
   if (inner_kept && V > 1 && inner % V == 0 && aligned) {
     hipLaunchKernelGGL((permute_gather<T, V>), dim3(grid_for(r.total / V, kThreads)),
                        dim3(kThreads), 0, s, (const T *)src, (T *)dst, g);

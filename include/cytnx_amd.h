@@ -167,6 +167,12 @@ int cxa_block_contract_get_config(const cxa_bc_plan *plan, int prec, int *kind, 
                                   int *ntiles);
 int cxa_block_contract_exec(const cxa_bc_plan *plan, const void *L, const void *R, void *C,
                             void *workspace, int prec, cxa_stream_t stream);
+/* exec with operand conjugation for complex plans (ignored for real ones): flags bit 0
+ * conjugates L, bit 1 conjugates R as they are read.  Used by the adjoint (reverse-mode)
+ * contractions, dL = dC . conj(R)^T and dR = conj(L)^T . dC, which have no counterpart in the
+ * reference (Cytnx has no automatic differentiation); callers are differentiable front ends. */
+int cxa_block_contract_exec_ex(const cxa_bc_plan *plan, const void *L, const void *R, void *C,
+                               void *workspace, int prec, int flags, cxa_stream_t stream);
 int cxa_block_contract_destroy(cxa_bc_plan *plan);
 
 /*
