@@ -1,0 +1,146 @@
+"""Network.Launch on the GPU.
+
+* The two-site DMRG projector network (example/DMRG/dmrg_two_sites_U1.py:30-36) on U(1) block
+  tensors: the default left-to-right order, an explicit ORDER and the searched order all give
+  the oracle's chain of block contractions (FP64, 1e-12 normwise), permuted to TOUT.
+* Dense networks A(a,b,c) B(c,d) C(d,e) (tests/Network_test.cpp:24-100 shapes of use): default
+  order, ORDER line, setOrder, the searched order, the static Contract builder, reuse after
+  swapping tensors, a TOUT without ';', against numpy.einsum.
+"""
+import numpy as np
+import pytest
+import torch
+
+from cytnx_amd import heisenberg
+from cytnx_amd.network import Network
+from cytnx_amd.tensor import Tensor
+from cytnx_amd.unitensor import Bond, UniTensor
+
+from test_gpu_heff_lanczos import oracle_heff2, problem, relerr
+
+pytestmark = pytest.mark.gpu
+
+DMRG = ["psi: -1,-2,-3,-4", "L: -5,-1,0", "R: -7,-4,3", "M1: -5,-6,-2,1", "M2: -6,-7,-3,2",
+        "TOUT: 0,1;2,3"]
+
+
+def _fresh_labels(T, prefix):
+    return T.relabel(["%s%d" % (prefix, i) for i in range(T.rank())])
+
+
+@pytest.mark.parametrize("order", ["default", "line", "optimal"])
+@pytest.mark.parametrize("chi", [16, 64])
+def test_dmrg_projector_network_matches_oracle(chi, order):
+    psi, L, M1, M2, R = problem(chi, seed=chi + 1)
+    lines = list(DMRG)
+    if order == "line":
+        lines.insert(5, "ORDER: (((L,psi),M1),(M2,R))")
+    net = Network()
+    net.FromString(lines)
+    # the tensors' own labels do not matter: Launch relabels them to the declared ones
+    net.PutUniTensors(["L", "M1", "M2", "R"], [_fresh_labels(L, "l"), _fresh_labels(M1, "m"),
+                                               _fresh_labels(M2, "n"), _fresh_labels(R, "r")])
+    net.PutUniTensor("psi", _fresh_labels(psi, "p"))
+    if order == "optimal":
+        net.setOrder(True, "")
+        print("searched order", net.getOrder(), flush=True)
+    out = net.Launch()
+    torch.cuda.synchronize()
+    assert out.labels() == ["0", "1", "2", "3"] and out.rowrank() == 2
+    ref = oracle_heff2(psi, L, M1, M2, R)
+    assert out.itoi() == ref.itoi
+    assert relerr(out.storage().cpu().numpy(), ref.flat()) <= 1e-12
+
+
+def _dense(a, labels):
+    T = UniTensor([Bond(int(s)) for s in a.shape], labels, 1, 3, -1)
+    T._block = Tensor.from_numpy(np.ascontiguousarray(a), device=0)
+    return T
+
+
+def _dense_np(T):
+    return T._block.numpy()
+
+
+@pytest.fixture(scope="module")
+def abc():
+    rng = np.random.default_rng(17)
+    a = rng.standard_normal((3, 4, 5))
+    b = rng.standard_normal((5, 6))
+    c = rng.standard_normal((6, 7))
+    return a, b, c, np.einsum("abc,cd,de->abe", a, b, c)
+
+
+@pytest.mark.parametrize("how", ["default", "line", "setorder", "optimal"])
+def test_dense_network_orders(abc, how):
+    a, b, c, want = abc
+    lines = ["A: a,b,c", "B: c,d", "C: d,e", "TOUT: a,b;e"]
+    if how == "line":
+        lines.insert(3, "ORDER:(A,(B,C))")
+    net = Network()
+    net.FromString(lines)
+    net.PutUniTensors(["A", "B", "C"], [_dense(a, ["x", "y", "z"]), _dense(b, ["u", "v"]),
+                                        _dense(c, ["s", "t"])])
+    if how == "setorder":
+        net.setOrder(False, "(A,(B,C))")
+    elif how == "optimal":
+        net.setOrder(True, "")
+    out = net.Launch()
+    assert out.labels() == ["a", "b", "e"] and out.rowrank() == 2
+    np.testing.assert_allclose(_dense_np(out), want, rtol=1e-12, atol=1e-12)
+    # reuse: Launch again gives the same result
+    np.testing.assert_array_equal(_dense_np(net.Launch()), _dense_np(out))
+
+
+def test_dense_network_tout_without_semicolon_and_reuse(abc):
+    a, b, c, want = abc
+    net = Network()
+    net.FromString(["A: a,b,c", "B: c,d", "C: d,e", "TOUT: a,b,e"])
+    net.PutUniTensors(["A", "B", "C"], [_dense(a, ["0", "1", "2"]), _dense(b, ["0", "1"]),
+                                        _dense(c, ["0", "1"])])
+    out = net.Launch()
+    assert out.rowrank() == 1
+    np.testing.assert_allclose(_dense_np(out), want, rtol=1e-12, atol=1e-12)
+    # permuted output: TOUT e,a;b
+    net2 = Network()
+    net2.FromString(["A: a,b,c", "B: c,d", "C: d,e", "TOUT: e,a;b"])
+    net2.PutUniTensors(["A", "B", "C"], [_dense(a, ["0", "1", "2"]), _dense(b, ["0", "1"]),
+                                         _dense(c, ["0", "1"])])
+    np.testing.assert_allclose(_dense_np(net2.Launch()), np.transpose(want, (2, 0, 1)),
+                               rtol=1e-12, atol=1e-12)
+
+
+def test_static_contract_builder(abc):
+    a, b, c, want = abc
+    A, B, C = _dense(a, ["a", "b", "c"]), _dense(b, ["c", "d"]), _dense(c, ["d", "e"])
+    res = Network.Contract([A, B, C], "a,b;e").Launch()
+    np.testing.assert_allclose(_dense_np(res), want, rtol=1e-12, atol=1e-12)
+    res2 = Network.Contract([A, B, C], "a,b;e", ["A", "B", "C"], "(A,(B,C))").Launch()
+    np.testing.assert_allclose(_dense_np(res2), want, rtol=1e-12, atol=1e-12)
+
+
+def test_block_network_three_tensors_matches_direct_contract():
+    """tests/Network_test.cpp:102-160: A(a,e) B(a,c_,d_,h) C(e,f_,g_,h), TOUT c_,d_;f_,g_,
+    against Contract(Contract(B, C), A) permuted, for every order."""
+    from cytnx_amd.unitensor import BD_BRA, BD_KET, Contract
+    rng = np.random.default_rng(5)
+    b1 = Bond(BD_KET, [[-1], [0], [1]], [2, 3, 2])
+    b2 = Bond(BD_KET, [[0], [1]], [3, 2])
+    A = heisenberg.random_block([b1.redirect(), b1], ["a", "e"], 1, rng).to(0)
+    B = heisenberg.random_block([b1, b2, b2.redirect(), b1.redirect()],
+                                ["a", "c_", "d_", "h"], 2, rng).to(0)
+    C = heisenberg.random_block([b1.redirect(), b2.redirect(), b2, b1],
+                                ["e", "f_", "g_", "h"], 2, rng).to(0)
+    ref = Contract(Contract(B, C), A).permute(["c_", "d_", "f_", "g_"], 2)
+    for order in [None, "(A,(B,C))", "optimal"]:
+        net = Network()
+        lines = ["A: a,e", "B: a,c_,d_,h", "C: e,f_,g_,h", "TOUT: c_,d_;f_,g_"]
+        if order and order != "optimal":
+            lines.insert(3, "ORDER:" + order)
+        net.FromString(lines)
+        net.PutUniTensors(["A", "B", "C"], [A, B, C])
+        if order == "optimal":
+            net.setOrder(True, "")
+        out = net.Launch()
+        assert out.itoi() == ref.itoi()
+        assert relerr(out.storage().cpu().numpy(), ref.storage().cpu().numpy()) <= 1e-12
