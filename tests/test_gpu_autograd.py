@@ -55,10 +55,11 @@ def test_contract_ad_matches_dense_autograd(complex_):
     b2 = Bond(BD_KET, [[-1], [1]], [2, 3])
     b3 = Bond(BD_KET, [[-1], [0], [1]], [2, 1, 3])
     b4 = Bond(BD_KET, [[0], [2]], [3, 2])
+    bl = Bond(BD_KET, [[-1], [1], [3]], [2, 3, 1])
     # A(i, k, j, m) and B(m, l, k): common legs k and m sit in the middle / at the ends
     A = _rand_block([b1, b2, b3.redirect(), b4], ["i", "k", "j", "m"], rng, complex_)
-    B = _rand_block([b4.redirect(), b1.redirect(), b2.redirect()], ["m", "l", "k"], rng,
-                    complex_)
+    B = _rand_block([b4.redirect(), bl, b2.redirect()], ["m", "l", "k"], rng, complex_)
+    assert A.Nblocks() > 1 and B.Nblocks() > 1
     A._storage.requires_grad_(True)
     B._storage.requires_grad_(True)
     C = contract_ad(A, B)
