@@ -1,0 +1,223 @@
+"""Two-site U(1) DMRG on the GPU: the sweep of the reference's example
+(example/DMRG/dmrg_two_sites_U1.py:44-205) with this package's operators.
+
+Index conventions (those of the example and of ``heff.Heff2``):
+
+    A_k    (a_k KET, s_k BRA, a_{k+1} BRA)       q(a_{k+1}) = q(a_k) - q(s_k), q(a_0) = 0
+    L_k    (w BRA, a BRA, a' KET)                 left environment of site k
+    R_k    (w KET, a KET, a' BRA)                 right environment of bond k-1 | k
+    M      (w KET, w' BRA, s KET, s' BRA)         the XXZ MPO (heisenberg.xxz_mpo_block)
+
+One bond update (p, p+1): psi = A_p . A_{p+1}; ground state of the two-site effective
+Hamiltonian (Heff2: the L.psi.W12 mix launch and the .R grouped GEMM) by the native Lanczos
+driver, FP64 or FP32 + FP64 refinement; Svd_truncate(psi, min(d dim_l, d dim_r, chi)) with the
+singular values normalised; the singular values absorbed towards the sweep direction; and the
+environment of the site left behind updated by three block contractions
+(L' = ((L.A).M).A^dagger, as the example's network "L: -2,-1,-3 / A: -1,-4,1 / M: -2,0,-4,-5 /
+A_Conj: -3,-5,2").  A^dagger with the example's ``Dagger().permute_(labels)`` is the complex
+conjugate with every bond redirected, in the same block layout.
+
+The initial state is a random MPS at the target bond dimension (sectors from
+``mps.bond_sectors``), brought to right-canonical form by Gesvd from the right; the sweeps then
+run left to right and right to left.  Energies are recorded per bond update like the example's
+``Ekeep``.
+"""
+from __future__ import annotations
+
+import time
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from .heff import Heff2
+from .heisenberg import MPO_GROUPS, mpo_bond, phys_bond, xxz_mpo_block
+from .lanczos import ground_state_mixed, lanczos_gnd_native
+from .mps import bond_sectors
+from .svd import Gesvd, Svd_truncate
+from .unitensor import BD_BRA, BD_KET, Bond, Contract, UniTensor
+
+
+def _bra(A: UniTensor) -> UniTensor:
+    """A.Dagger().permute_(A.labels()): conjugate data, every bond redirected, same layout."""
+    out = A.Conj()
+    out._bonds = [b.redirect() for b in out._bonds]
+    return out
+
+
+def left_env_step(Lk: UniTensor, A: UniTensor, M: UniTensor) -> UniTensor:
+    """L_{k+1}(w', a_{k+1}, a'_{k+1}) from L_k(w, a, a'), A_k(a, s, a_{k+1}), M(w, w', s, t)."""
+    L = Lk.relabel(["~w", "~a", "~b"])
+    X = Contract(L, A.relabel(["~a", "~s", "~an"]))
+    Y = Contract(X, M.relabel(["~w", "~wn", "~s", "~t"]))
+    Z = Contract(Y, _bra(A).relabel(["~b", "~t", "~bn"]))
+    return Z.permute(["~wn", "~an", "~bn"], 1)
+
+
+def right_env_step(Rk: UniTensor, B: UniTensor, M: UniTensor) -> UniTensor:
+    """R_k(w, a_k, a'_k) from R_{k+1}(w', a_{k+1}, a'_{k+1}), B_k(a_k, s, a_{k+1}),
+    M(w, w', s, t)."""
+    R = Rk.relabel(["~w", "~a", "~b"])
+    X = Contract(R, B.relabel(["~an", "~s", "~a"]))
+    Y = Contract(X, M.relabel(["~wn", "~w", "~s", "~t"]))
+    Z = Contract(Y, _bra(B).relabel(["~bn", "~t", "~b"]))
+    return Z.permute(["~wn", "~an", "~bn"], 1)
+
+
+def _boundary(left: bool, device: int, q: int = 0) -> UniTensor:
+    """L_0 or R_L on the 1-dimensional virtual bond of charge q: 1 at the MPO's left-boundary
+    state (4, the identity row) or right-boundary state (0)."""
+    vb = Bond(BD_KET, [[q]], [1])
+    bw = mpo_bond()
+    if left:
+        E = UniTensor([bw.redirect(), vb.redirect(), vb], ["w", "a", "b"], 1, 3, -1)
+        state = 4
+    else:
+        E = UniTensor([bw, vb, vb.redirect()], ["w", "a", "b"], 1, 3, -1)
+        state = 0
+    for bi, t in enumerate(E.itoi()):
+        states = MPO_GROUPS[t[0]][1]
+        if state in states:
+            blk = np.zeros([int(s) for s in E._shapes[bi]])
+            blk[states.index(state), 0, 0] = 1.0
+            E.put_block(blk, bi)
+    return E.to(device)
+
+
+def random_mps(L: int, chi: int, seed: int = 1, device: int = 0, Q: int = 0) -> List[UniTensor]:
+    """A_0..A_{L-1} (conventions above) with iid N(0,1) blocks at the bond dimensions of
+    mps.bond_sectors in the sector of total charge Q = sum of 2Sz (bond charges negated to this
+    convention: q(a_k) = -(charge of sites 0..k-1), so q(a_L) = -Q)."""
+    rng = np.random.default_rng(seed)
+    phys = phys_bond().redirect()
+    bonds = []
+    for k in range(L + 1):
+        q, d = bond_sectors(L, k, chi, Q)
+        bonds.append(Bond(BD_KET, [[-x] for x in q], d))
+    As = []
+    for k in range(L):
+        A = UniTensor([bonds[k], phys, bonds[k + 1].redirect()],
+                      ["a%d" % k, "s%d" % k, "a%d" % (k + 1)], 2, 3, -1, _alloc=False)
+        n = int(sum(np.prod(s.astype(np.int64)) for s in A._shapes))
+        A._storage = torch.from_numpy(rng.standard_normal(n)).to("cuda:%d" % device)
+        As.append(A)
+    return As
+
+
+def right_canonicalize(As: List[UniTensor]) -> List[UniTensor]:
+    """Gesvd from the right: A_k <- vT (right-orthonormal), A_{k-1} <- A_{k-1}.U.S; A_0 then
+    normalised."""
+    As = list(As)
+    for k in range(len(As) - 1, 0, -1):
+        A = As[k].clone_meta()
+        A.set_rowrank_(1)
+        S, U, vT = Gesvd(A)
+        As[k] = vT.relabel(As[k].labels()).set_rowrank_(2)
+        US = Contract(U, S)                                   # (a_k, _aux_R)
+        prev = Contract(As[k - 1], US.relabel([As[k].labels()[0], "~aux"]))
+        As[k - 1] = prev.relabel(As[k - 1].labels()).set_rowrank_(2)
+    As[0] = As[0] / As[0].Norm()
+    return As
+
+
+class DMRG:
+    """Two-site DMRG for the open XXZ chain (see module docstring)."""
+
+    def __init__(self, L: int, chi: int, J: float = 1.0, Jz: float = 1.0, seed: int = 1,
+                 device: int = 0, maxit: int = 30, cvg: float = 1e-14, prec: str = "fp64",
+                 mix: bool = True, Q: int = 0):
+        if prec not in ("fp64", "mixed"):
+            raise ValueError("prec is 'fp64' or 'mixed'")
+        if L < 2:
+            raise ValueError("DMRG needs at least 2 sites")
+        self.L, self.chi, self.device = L, chi, device
+        self.maxit, self.cvg, self.prec, self.mix = maxit, cvg, prec, mix
+        self.M = xxz_mpo_block(J, Jz, device=device)
+        self.As = right_canonicalize(random_mps(L, chi, seed, device, Q))
+        self.LR: List[Optional[UniTensor]] = [None] * (L + 1)
+        self.LR[0] = _boundary(True, device)
+        self.LR[L] = _boundary(False, device, -Q)
+        for k in range(L - 1, 0, -1):
+            self.LR[k] = right_env_step(self.LR[k + 1], self.As[k], self.M)
+        self.energies: List[float] = []
+        self.stats: Dict[str, float] = {"heff": 0.0, "lanczos": 0.0, "svd": 0.0, "env": 0.0,
+                                        "matvecs": 0, "flops": 0.0, "updates": 0}
+
+    def _optimize(self, p: int) -> UniTensor:
+        t0 = time.perf_counter()
+        A, B = self.As[p], self.As[p + 1]
+        psi = Contract(A, B)                                  # (a_p, s_p, s_{p+1}, a_{p+2})
+        labels = psi.labels()
+        H = Heff2(psi.relabel(["-1", "-2", "-3", "-4"]),
+                  self.LR[p].relabel(["-5", "-1", "0"]),
+                  self.M.relabel(["-5", "-6", "-2", "1"]),
+                  self.M.relabel(["-6", "-7", "-3", "2"]),
+                  self.LR[p + 2].relabel(["-7", "-4", "3"]), mix=self.mix)
+        torch.cuda.synchronize(self.device)
+        t1 = time.perf_counter()
+        x0 = psi._storage / torch.linalg.vector_norm(psi._storage)
+        if self.prec == "fp64":
+            E, v, n, _, _ = lanczos_gnd_native(H, x0, self.cvg, self.maxit)
+        else:
+            E, v, n1, n2 = ground_state_mixed(H, x0, self.cvg, self.maxit)
+            n = n1 + n2
+        torch.cuda.synchronize(self.device)
+        t2 = time.perf_counter()
+        out = psi.clone_meta()
+        out._storage = v
+        out._labels = labels
+        self.energies.append(float(E))
+        st = self.stats
+        st["heff"] += t1 - t0
+        st["lanczos"] += t2 - t1
+        st["matvecs"] += n
+        st["flops"] += n * H.ref_flops
+        st["updates"] += 1
+        return out
+
+    def _split(self, p: int, psi: UniTensor, absorb_left: bool) -> None:
+        t0 = time.perf_counter()
+        A, B = self.As[p], self.As[p + 1]
+        dim_l, dim_r = A.shape()[0], B.shape()[2]
+        d = A.shape()[1]
+        new_dim = min(dim_l * d, dim_r * d, self.chi)
+        psi.set_rowrank_(2)
+        s, U, vT = Svd_truncate(psi, new_dim)
+        s = s / s.Norm()
+        if absorb_left:
+            self.As[p] = Contract(U, s).relabel(A.labels()).set_rowrank_(2)
+            self.As[p + 1] = vT.relabel(B.labels()).set_rowrank_(2)
+        else:
+            self.As[p] = U.relabel(A.labels()).set_rowrank_(2)
+            self.As[p + 1] = Contract(s, vT).relabel(B.labels()).set_rowrank_(2)
+        torch.cuda.synchronize(self.device)
+        t1 = time.perf_counter()
+        if absorb_left:
+            self.LR[p + 1] = right_env_step(self.LR[p + 2], self.As[p + 1], self.M)
+        else:
+            self.LR[p + 1] = left_env_step(self.LR[p], self.As[p], self.M)
+        torch.cuda.synchronize(self.device)
+        t2 = time.perf_counter()
+        self.stats["svd"] += t1 - t0
+        self.stats["env"] += t2 - t1
+
+    def sweep_left_to_right(self) -> None:
+        for p in range(self.L - 1):
+            self._split(p, self._optimize(p), absorb_left=False)
+
+    def sweep_right_to_left(self) -> None:
+        for p in range(self.L - 2, -1, -1):
+            self._split(p, self._optimize(p), absorb_left=True)
+
+    def run(self, sweeps: int) -> List[float]:
+        """`sweeps` full sweeps (left to right, then right to left); energies per update."""
+        for _ in range(sweeps):
+            self.sweep_left_to_right()
+            self.sweep_right_to_left()
+        return self.energies
+
+    def bond_dims(self) -> List[int]:
+        return [A.shape()[2] for A in self.As[:-1]]
+
+
+__all__ = ["DMRG", "random_mps", "right_canonicalize", "left_env_step", "right_env_step"]

@@ -157,4 +157,23 @@ def Gemm_Batch(alpha: Sequence[float], a: Sequence[Tensor], b: Sequence[Tensor],
         VP(*flat_c), _lib.i64_array(ldc), cnt, _lib.i64_array(gsz), stream_handle(dev)))
 
 
-__all__ = ["Matmul", "Tensordot", "Gemm_Batch"]
+def Gesvd(T, is_U: bool = True, is_vT: bool = True):
+    """linalg::Gesvd on a UniTensor (cytnx_amd.svd)."""
+    from .svd import Gesvd as _gesvd
+    return _gesvd(T, is_U, is_vT)
+
+
+def Svd(T, is_UvT: bool = True):
+    """linalg::Svd on a UniTensor (cytnx_amd.svd)."""
+    from .svd import Svd as _svd
+    return _svd(T, is_UvT)
+
+
+def Svd_truncate(T, keepdim: int, err: float = 0.0, is_UvT: bool = True, return_err: int = 0,
+                 mindim: int = 1):
+    """linalg::Svd_truncate on a UniTensor (cytnx_amd.svd)."""
+    from .svd import Svd_truncate as _svdt
+    return _svdt(T, keepdim, err, is_UvT, return_err, mindim)
+
+
+__all__ = ["Matmul", "Tensordot", "Gemm_Batch", "Gesvd", "Svd", "Svd_truncate"]

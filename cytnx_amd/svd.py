@@ -1,0 +1,298 @@
+"""Singular value decompositions of UniTensors on the GPU: Gesvd / Svd / Svd_truncate.
+
+Block (U(1)) tensors (Gesvd_BlockUT_internal, src/linalg/Gesvd.cpp:191-385):
+
+* the rowrank splits the legs into rows and columns; a block's sector key is the charge its row
+  legs carry, sum(+q for KET legs, -q for BRA legs);
+* per sector (keys ascending) the blocks form a complete grid of row groups x column groups
+  (each group a distinct qindex tuple, sorted lexicographically); the grid is one matrix, and
+  its thin SVD gives that sector's singular values (descending), U rows per row group and vT
+  columns per column group;
+* S is diagonal on the bond aux = KET(sector keys, min(m, n) each), labels (_aux_L, _aux_R);
+  U = (row legs, aux BRA) labelled (row labels, _aux_L) with rowrank = the input rowrank;
+  vT = (aux KET, column legs) labelled (_aux_R, column labels) with rowrank 1.  U's blocks are
+  ordered sector by sector, row groups in order; vT's likewise by column groups.
+
+Svd_truncate (Svd_truncate_Block_UTs_internal, src/linalg/Svd_truncate.cpp:153-334): all
+singular values sorted ascending; the cut value Smin is the keepdim-th largest, raised while it
+is below err and more than max(mindim, 1) values remain; every value >= Smin is kept (so a
+degeneracy at the cut is kept whole), sectors left empty drop out of the aux bond, and
+return_err = 1 appends the largest dropped value, > 1 every dropped value (descending).
+
+Dense UniTensors reshape to one matrix.  Every SVD is torch.linalg.svd on the tensor's GPU, in
+the tensor's dtype (FP64 / complex128 for the DMRG path); singular values are real.
+"""
+from __future__ import annotations
+
+from typing import Dict, List, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from .device import CPU, TORCH_OF, Type_C128, Type_C64, Type_F32, Type_F64, dtype_id
+from .tensor import Tensor
+from .unitensor import BD_BRA, BD_KET, Bond, UniTensor, _block_layout
+
+_REAL_OF = {Type_C128: Type_F64, Type_C64: Type_F32, Type_F64: Type_F64, Type_F32: Type_F32}
+
+
+def _check_input(T: UniTensor, who: str) -> None:
+    if T.rank() <= 1:
+        raise ValueError("[ERROR][%s] Input UniTensor should have rank>1, but rank is %d"
+                         % (who, T.rank()))
+    if T.rowrank() < 1:
+        raise ValueError("[ERROR][%s] Input UniTensor should have rowrank>0, but rowrank is %d"
+                         % (who, T.rowrank()))
+    if T.rowrank() >= T.rank():
+        raise ValueError("[ERROR][%s] Input UniTensor should have rowrank<rank, but rowrank is "
+                         "%d and rank is %d" % (who, T.rowrank(), T.rank()))
+    if T.is_diag():
+        raise ValueError("[ERROR][%s] Input UniTensor is diagonal, so the SVD is trivial and not "
+                         "supported." % who)
+    if T.device() == CPU:
+        raise NotImplementedError("cytnx_amd implements the Device.cuda branch; move the "
+                                  "UniTensor to the GPU")
+
+
+def _from_blocks(bonds: Sequence[Bond], labels: Sequence[str], rowrank: int,
+                 itoi: np.ndarray, blocks: List[torch.Tensor], dtype: int,
+                 device: int) -> UniTensor:
+    """Block UniTensor with the given block order and data."""
+    out = UniTensor(bonds, labels, rowrank, dtype, device, _alloc=False)
+    out._itoi = np.ascontiguousarray(itoi, dtype=np.uint64).reshape(-1, len(bonds))
+    out._shapes, out._offsets, total = _block_layout(out._bonds, out._itoi)
+    if blocks:
+        out._storage = torch.cat([b.reshape(-1) for b in blocks]).contiguous()
+    else:
+        out._storage = torch.zeros(0, dtype=TORCH_OF[dtype], device=out._storage.device)
+    if out._storage.numel() != total:
+        raise RuntimeError("internal error: block sizes do not match the layout")
+    return out
+
+
+def _diag_from_blocks(bond: Bond, blocks: List[torch.Tensor], dtype: int,
+                      device: int) -> UniTensor:
+    S = UniTensor([bond, bond.redirect()], ["_aux_L", "_aux_R"], 1, dtype, device,
+                  is_diag=True, _alloc=False)
+    S._storage = torch.cat([b.reshape(-1) for b in blocks]).contiguous() if blocks else \
+        torch.zeros(0, dtype=TORCH_OF[dtype], device=S._storage.device)
+    return S
+
+
+class _Sector:
+    __slots__ = ("key", "rows", "cols", "rsz", "csz", "s", "u", "vh")
+
+
+def _block_gesvd(T: UniTensor, is_U: bool, is_vT: bool) -> List[_Sector]:
+    r, rank = T.rowrank(), T.rank()
+    bonds = T._bonds
+    itoi = T._itoi.astype(np.int64)
+    degs = [np.asarray(b.getDegeneracies(), dtype=np.int64) for b in bonds]
+    keys = np.zeros(T.Nblocks(), dtype=np.int64)
+    for i in range(r):
+        sgn = 1 if bonds[i].type() == BD_KET else -1
+        keys += sgn * np.asarray(bonds[i].qnums(), dtype=np.int64)[itoi[:, i]]
+    sectors = []
+    for key in sorted(set(keys.tolist())):
+        blist = np.nonzero(keys == key)[0]
+        rows = sorted(set(tuple(itoi[b, :r]) for b in blist))
+        cols = sorted(set(tuple(itoi[b, r:]) for b in blist))
+        if len(rows) * len(cols) != len(blist):
+            raise RuntimeError("[Internal ERROR] the blocks of a sector do not form a grid")
+        rpos = {t: i for i, t in enumerate(rows)}
+        cpos = {t: i for i, t in enumerate(cols)}
+        rsz = [int(np.prod([degs[i][t[i]] for i in range(r)])) for t in rows]
+        csz = [int(np.prod([degs[r + j][t[j]] for j in range(rank - r)])) for t in cols]
+        roff = np.concatenate([[0], np.cumsum(rsz)]).astype(int)
+        coff = np.concatenate([[0], np.cumsum(csz)]).astype(int)
+        M = torch.zeros((int(roff[-1]), int(coff[-1])), dtype=T._storage.dtype,
+                        device=T._storage.device)
+        for b in blist:
+            i = rpos[tuple(itoi[b, :r])]
+            j = cpos[tuple(itoi[b, r:])]
+            M[roff[i]:roff[i + 1], coff[j]:coff[j + 1]] = \
+                T.get_block_(int(b)).reshape(rsz[i], csz[j])
+        sec = _Sector()
+        sec.key, sec.rows, sec.cols, sec.rsz, sec.csz = int(key), rows, cols, rsz, csz
+        if is_U or is_vT:
+            U, s, Vh = torch.linalg.svd(M, full_matrices=False)
+            sec.u = U if is_U else None
+            sec.vh = Vh if is_vT else None
+        else:
+            s = torch.linalg.svdvals(M)
+            sec.u = sec.vh = None
+        sec.s = s
+        sectors.append(sec)
+    return sectors
+
+
+def _assemble_block(T: UniTensor, sectors: List[_Sector], keep: List[int], is_U: bool,
+                    is_vT: bool) -> List[UniTensor]:
+    """S, U, vT from the sectors, keeping the first keep[i] singular values of sector i and
+    dropping sectors with keep[i] == 0."""
+    r, rank = T.rowrank(), T.rank()
+    dt, dev = T.dtype(), T.device()
+    live = [i for i, k in enumerate(keep) if k > 0]
+    aux = Bond(BD_KET, [[sectors[i].key] for i in live], [keep[i] for i in live])
+    S = _diag_from_blocks(aux, [sectors[i].s[:keep[i]] for i in live], _REAL_OF[dt], dev)
+    out = [S]
+    if is_U:
+        itoi, blocks = [], []
+        for new_i, i in enumerate(live):
+            sec = sectors[i]
+            off = 0
+            for t, m in zip(sec.rows, sec.rsz):
+                shp = [int(T._bonds[a].getDegeneracies()[t[a]]) for a in range(r)]
+                blocks.append(sec.u[off:off + m, :keep[i]].reshape(shp + [keep[i]]))
+                itoi.append(list(t) + [new_i])
+                off += m
+        out.append(_from_blocks(list(T._bonds[:r]) + [aux.redirect()],
+                                list(T._labels[:r]) + ["_aux_L"], r,
+                                np.asarray(itoi, dtype=np.uint64).reshape(-1, r + 1), blocks,
+                                dt, dev))
+    if is_vT:
+        itoi, blocks = [], []
+        for new_i, i in enumerate(live):
+            sec = sectors[i]
+            off = 0
+            for t, n in zip(sec.cols, sec.csz):
+                shp = [int(T._bonds[r + a].getDegeneracies()[t[a]]) for a in range(rank - r)]
+                blocks.append(sec.vh[:keep[i], off:off + n].reshape([keep[i]] + shp))
+                itoi.append([new_i] + list(t))
+                off += n
+        out.append(_from_blocks([aux] + list(T._bonds[r:]), ["_aux_R"] + list(T._labels[r:]),
+                                1, np.asarray(itoi, dtype=np.uint64).reshape(-1, rank - r + 1),
+                                blocks, dt, dev))
+    return out
+
+
+def _dense_matrix(T: UniTensor) -> Tuple[torch.Tensor, List[int]]:
+    t = T._block.contiguous()
+    shp = t.shape()
+    m = int(np.prod(shp[:T.rowrank()]))
+    return t.torch().reshape(m, -1), shp
+
+
+def _assemble_dense(T: UniTensor, U, s, Vh, k: int, is_U: bool, is_vT: bool) -> List[UniTensor]:
+    r = T.rowrank()
+    shp = T._block.shape()
+    dev = T.device()
+    rdt = _REAL_OF[T.dtype()]
+    S = UniTensor([Bond(k), Bond(k)], ["_aux_L", "_aux_R"], 1, rdt, dev, is_diag=True)
+    S._block = Tensor.from_storage(s[:k].contiguous(), [k])
+    out = [S]
+    if is_U:
+        Ut = UniTensor([Bond(d) for d in shp[:r]] + [Bond(k)],
+                       list(T._labels[:r]) + ["_aux_L"], r, T.dtype(), dev)
+        Ut._block = Tensor.from_storage(U[:, :k].contiguous(), list(shp[:r]) + [k])
+        out.append(Ut)
+    if is_vT:
+        vT = UniTensor([Bond(k)] + [Bond(d) for d in shp[r:]],
+                       ["_aux_R"] + list(T._labels[r:]), 1, T.dtype(), dev)
+        vT._block = Tensor.from_storage(Vh[:k, :].contiguous(), [k] + list(shp[r:]))
+        out.append(vT)
+    return out
+
+
+def Gesvd(T: UniTensor, is_U: bool = True, is_vT: bool = True) -> List[UniTensor]:
+    """[S, U (if is_U), vT (if is_vT)] (linalg::Gesvd, src/linalg/Gesvd.cpp)."""
+    _check_input(T, "Gesvd")
+    if T.is_blockform():
+        sectors = _block_gesvd(T, is_U, is_vT)
+        return _assemble_block(T, sectors, [int(sec.s.numel()) for sec in sectors], is_U, is_vT)
+    M, _ = _dense_matrix(T)
+    if is_U or is_vT:
+        U, s, Vh = torch.linalg.svd(M, full_matrices=False)
+    else:
+        U, s, Vh = None, torch.linalg.svdvals(M), None
+    return _assemble_dense(T, U, s, Vh, int(s.numel()), is_U, is_vT)
+
+
+def Svd(T: UniTensor, is_UvT: bool = True) -> List[UniTensor]:
+    """[S, U, vT] or [S] (linalg::Svd)."""
+    return Gesvd(T, is_UvT, is_UvT)
+
+
+def _cut(svals: np.ndarray, keepdim: int, err: float, mindim: int
+         ) -> Tuple[float, int, np.ndarray]:
+    """(Smin, number dropped, all values ascending) of the block truncation rule."""
+    Sall = np.sort(svals)  # ascending
+    n = Sall.size
+    keep = keepdim
+    if keep < n:
+        smidx = n - keep
+    else:
+        keep, smidx = n, 0
+    Smin = Sall[smidx]
+    lo = max(1, mindim)
+    while Smin < err and keep > lo:
+        keep -= 1
+        smidx += 1
+        Smin = Sall[smidx]
+    while smidx > 0 and not (Sall[smidx - 1] < Smin):
+        smidx -= 1
+    return float(Smin), smidx, Sall
+
+
+def _cut_dense(s_desc: np.ndarray, keepdim: int, err: float, mindim: int) -> int:
+    """Kept count of the dense rule (memcpyTruncation, src/backend/linalg_internal_cpu/
+    memcpyTruncation.cpp:13-60): the first min(n, keepdim) values, then trailing values below
+    err dropped while more than max(mindim, 1) remain; no degeneracy rule."""
+    mindim = max(1, mindim)
+    k = min(int(s_desc.size), int(keepdim))
+    for i in range(k - 1, -1, -1):
+        if s_desc[i] < err and k - 1 >= mindim:
+            k -= 1
+        else:
+            break
+    return max(k, 1) if s_desc.size else 0
+
+
+def _err_tensor(Sall: np.ndarray, dropped: int, return_err: int, dtype: int,
+                device: int) -> UniTensor:
+    if return_err == 1:
+        T = UniTensor([], [], 0, dtype, device)
+        v = Sall[dropped - 1] if dropped else 0.0
+        T._block = Tensor.from_storage(torch.tensor([v], dtype=TORCH_OF[dtype],
+                                                    device=T._block.storage().device), [])
+        return T
+    vals = Sall[:dropped][::-1].copy()
+    T = UniTensor([Bond(dropped)], ["0"], 0, dtype, device)
+    T._block = Tensor.from_storage(torch.tensor(vals, dtype=TORCH_OF[dtype],
+                                                device=T._block.storage().device), [dropped])
+    return T
+
+
+def Svd_truncate(T: UniTensor, keepdim: int, err: float = 0.0, is_UvT: bool = True,
+                 return_err: int = 0, mindim: int = 1) -> List[UniTensor]:
+    """[S, U, vT] (+ [err]) keeping at most keepdim singular values (see module docstring)."""
+    _check_input(T, "Svd_truncate")
+    if keepdim < 1:
+        raise ValueError("[ERROR][Svd_truncate] keepdim must be >=1")
+    rdt = _REAL_OF[T.dtype()]
+    if T.is_blockform():
+        sectors = _block_gesvd(T, is_UvT, is_UvT)
+        svals = np.concatenate([sec.s.detach().cpu().numpy() for sec in sectors]) \
+            if sectors else np.zeros(0)
+        Smin, dropped, Sall = _cut(svals, keepdim, err, mindim)
+        keep = []
+        for sec in sectors:
+            s = sec.s.detach().cpu().numpy()
+            keep.append(int(np.count_nonzero(s >= Smin)))
+        out = _assemble_block(T, sectors, keep, is_UvT, is_UvT)
+    else:
+        M, _ = _dense_matrix(T)
+        if is_UvT:
+            U, s, Vh = torch.linalg.svd(M, full_matrices=False)
+        else:
+            U, s, Vh = None, torch.linalg.svdvals(M), None
+        sh = s.detach().cpu().numpy()
+        k = _cut_dense(sh, keepdim, err, mindim)
+        dropped, Sall = sh.size - k, np.sort(sh)
+        out = _assemble_dense(T, U, s, Vh, k, is_UvT, is_UvT)
+    if return_err:
+        out.append(_err_tensor(Sall, dropped, int(return_err), rdt, T.device()))
+    return out
+
+
+__all__ = ["Gesvd", "Svd", "Svd_truncate"]

@@ -194,8 +194,10 @@ class UniTensor:
     def __init__(self, bonds: Sequence[Bond] = (), labels: Optional[Sequence[str]] = None,
                  rowrank: int = -1, dtype: int = Type_F64, device: int = CPU,
                  is_diag: bool = False, name: str = "", _alloc: bool = True):
-        if is_diag:
-            raise NotImplementedError("diagonal UniTensors are outside the contraction hot path")
+        self._is_diag = bool(is_diag)
+        if self._is_diag:
+            self._init_diag(bonds, labels, rowrank, dtype, device, name, _alloc)
+            return
         self._bonds = [b.clone() for b in bonds]
         self._labels = [str(x) for x in labels] if labels is not None else \
             [str(i) for i in range(len(bonds))]
@@ -220,6 +222,45 @@ class UniTensor:
             self._block = Tensor([b.dim() for b in self._bonds], dtype, device)
         if self._rowrank < 0 or self._rowrank > len(self._bonds):
             raise ValueError("rowrank must be >=0 and <=rank.")
+
+    def _init_diag(self, bonds, labels, rowrank, dtype, device, name, alloc) -> None:
+        """Diagonal rank-2 UniTensor (the singular values of an SVD): only the diagonal is
+        stored.  Block form: bonds (b, b.redirect()), block i is the diagonal of the (i, i)
+        sector, deg_i elements, in qnum order (UniTensor.hpp is_diag, src/BlockUniTensor.cpp
+        Init with is_diag)."""
+        if len(bonds) != 2:
+            raise ValueError("[ERROR] is_diag=True requires a rank-2 UniTensor.")
+        b0, b1 = bonds[0], bonds[1]
+        if b0.dim() != b1.dim():
+            raise ValueError("[ERROR] is_diag=True requires the two bonds to have the same "
+                             "dimension.")
+        self._bonds = [b0.clone(), b1.clone()]
+        self._labels = [str(x) for x in labels] if labels is not None else ["0", "1"]
+        if len(self._labels) != 2 or self._labels[0] == self._labels[1]:
+            raise ValueError("invalid labels for a diagonal UniTensor")
+        self._name = name
+        self._rowrank = 1 if rowrank == -1 else int(rowrank)
+        self._is_block = b0.has_symmetry()
+        if self._is_block:
+            if b0.qnums() != b1.qnums() or b0.getDegeneracies() != b1.getDegeneracies() or \
+                    b0.type() + b1.type() != 0:
+                raise ValueError("a diagonal block UniTensor needs bonds (b, b.redirect())")
+            nq = len(b0.qnums())
+            self._itoi = np.repeat(np.arange(nq, dtype=np.uint64)[:, None], 2, axis=1)
+            degs = np.asarray(b0.getDegeneracies(), dtype=np.uint64)
+            self._shapes = np.stack([degs, degs], axis=1)
+            self._offsets = np.zeros(nq, dtype=np.int64)
+            if nq:
+                self._offsets[1:] = np.cumsum(degs.astype(np.int64))[:-1]
+            total = int(degs.sum())
+            self._storage = torch.zeros(total if alloc else 0, dtype=TORCH_OF[dtype],
+                                        device=torch_device(device))
+            self._block = None
+        else:
+            self._block = Tensor([b0.dim()], dtype, device)
+
+    def is_diag(self) -> bool:
+        return getattr(self, "_is_diag", False)
 
     # ------------------------------------------------------------------ metadata
     def is_blockform(self) -> bool:
@@ -280,8 +321,10 @@ class UniTensor:
         """View of block i (torch, in place on the flat buffer)."""
         if not self._is_block:
             return self._block.torch()
-        n = int(np.prod(self._shapes[i].astype(np.int64))) if self.rank() else 1
         o = int(self._offsets[i])
+        if self.is_diag():
+            return self._storage[o:o + int(self._shapes[i][0])]
+        n = int(np.prod(self._shapes[i].astype(np.int64))) if self.rank() else 1
         return self._storage[o:o + n].view([int(s) for s in self._shapes[i]])
 
     def get_blocks_(self) -> List[torch.Tensor]:
@@ -341,11 +384,20 @@ class UniTensor:
     def to_dense(self) -> np.ndarray:
         """Dense host array (zeros outside the blocks)."""
         if not self._is_block:
+            if self.is_diag():
+                return np.diag(self._block.numpy())
             return self._block.numpy()
         out = np.zeros(self.shape(), dtype=fileio.NP_DTYPE[self.dtype()])
         offs = [np.concatenate([[0], np.cumsum(b.getDegeneracies())[:-1]]).astype(int)
                 for b in self._bonds]
         host = self._storage.detach().cpu().numpy()
+        if self.is_diag():
+            for bi in range(self.Nblocks()):
+                d = int(self._shapes[bi][0])
+                o = int(self._offsets[bi])
+                s0 = offs[0][int(self._itoi[bi][0])]
+                out[np.arange(s0, s0 + d), np.arange(s0, s0 + d)] = host[o:o + d]
+            return out
         for bi in range(self.Nblocks()):
             shp = [int(s) for s in self._shapes[bi]]
             o = int(self._offsets[bi])
@@ -400,6 +452,11 @@ class UniTensor:
         out._bonds = [self._bonds[m].clone() for m in mp]
         out._labels = [self._labels[m] for m in mp]
         out._rowrank = self._rowrank if rowrank < 0 else int(rowrank)
+        if out._rowrank < 0 or out._rowrank > self.rank():
+            raise ValueError("rowrank must be >=0 and <=rank.")
+        if self.is_diag():
+            # a diagonal stays diagonal: only the bonds and labels trade places
+            return out
         if not self._is_block:
             out._block = self._block.permute(mp)
             return out
@@ -423,6 +480,101 @@ class UniTensor:
     def permute_(self, mapper, rowrank: int = -1) -> "UniTensor":
         p = self.permute(mapper, rowrank)
         self.__dict__.update(p.__dict__)
+        return self
+
+    # ------------------------------------------------------------------ conjugation
+    def Conj(self) -> "UniTensor":
+        """Complex conjugate (real dtypes: the same data)."""
+        out = self.clone_meta()
+        if self._is_block:
+            st = self._storage
+            out._storage = torch.conj_physical(st) if st.is_complex() else st
+        else:
+            t = self._block.contiguous()
+            st = t.storage()
+            out._block = Tensor.from_storage(torch.conj_physical(st) if st.is_complex() else st,
+                                             t.shape())
+        return out
+
+    def Conj_(self) -> "UniTensor":
+        self.__dict__.update(self.Conj().__dict__)
+        return self
+
+    def Transpose(self) -> "UniTensor":
+        """Every bond redirected, the index order reversed, rowrank -> rank - rowrank
+        (BlockUniTensor::Transpose_, src/BlockUniTensor.cpp:1237-1245)."""
+        rank = self.rank()
+        out = self.clone_meta()
+        out._bonds = [b.redirect() if b.type() != BD_REG else b.clone() for b in self._bonds]
+        return out.permute(list(range(rank - 1, -1, -1)), rank - self._rowrank)
+
+    def Transpose_(self) -> "UniTensor":
+        self.__dict__.update(self.Transpose().__dict__)
+        return self
+
+    def Dagger(self) -> "UniTensor":
+        """Conj().Transpose() (include/UniTensor.hpp:1731-1738)."""
+        return self.Conj().Transpose()
+
+    def Dagger_(self) -> "UniTensor":
+        self.__dict__.update(self.Dagger().__dict__)
+        return self
+
+    # ------------------------------------------------------------------ arithmetic
+    def _with_storage(self, st: torch.Tensor) -> "UniTensor":
+        out = self.clone_meta()
+        if self._is_block:
+            out._storage = st
+        else:
+            out._block = Tensor.from_storage(st, self._block.contiguous().shape())
+        return out
+
+    def _flat(self) -> torch.Tensor:
+        return self._storage if self._is_block else self._block.contiguous().storage()
+
+    def _same_layout(self, other: "UniTensor") -> bool:
+        if self._is_block != other._is_block or self.is_diag() != other.is_diag():
+            return False
+        if self._is_block:
+            return (self._itoi.shape == other._itoi.shape and
+                    bool(np.all(self._itoi == other._itoi)) and
+                    bool(np.all(self._shapes == other._shapes)))
+        return self.shape() == other.shape()
+
+    def __mul__(self, other):
+        if isinstance(other, UniTensor):
+            if not self._same_layout(other):
+                raise ValueError("elementwise product needs UniTensors of the same layout")
+            return self._with_storage(self._flat() * other._flat())
+        return self._with_storage(self._flat() * other)
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, other):
+        if isinstance(other, UniTensor):
+            if not self._same_layout(other):
+                raise ValueError("elementwise division needs UniTensors of the same layout")
+            return self._with_storage(self._flat() / other._flat())
+        return self._with_storage(self._flat() / other)
+
+    def __neg__(self):
+        return self._with_storage(-self._flat())
+
+    def __add__(self, other: "UniTensor"):
+        if not isinstance(other, UniTensor) or not self._same_layout(other):
+            raise ValueError("addition needs UniTensors of the same layout")
+        return self._with_storage(self._flat() + other._flat())
+
+    def __sub__(self, other: "UniTensor"):
+        if not isinstance(other, UniTensor) or not self._same_layout(other):
+            raise ValueError("subtraction needs UniTensors of the same layout")
+        return self._with_storage(self._flat() - other._flat())
+
+    def normalize(self) -> "UniTensor":
+        return self / self.Norm()
+
+    def normalize_(self) -> "UniTensor":
+        self.__dict__.update(self.normalize().__dict__)
         return self
 
     # ------------------------------------------------------------------ norms
@@ -516,6 +668,8 @@ def Contract(A: UniTensor, B: UniTensor) -> UniTensor:
     out_labels = [A._labels[i] for i in nc1] + [B._labels[i] for i in nc2]
     rr = A.rowrank() + B.rowrank() - sum(1 for i in c1 if i < A.rowrank()) \
         - sum(1 for i in c2 if i < B.rowrank())
+    if A.is_diag() or B.is_diag():
+        return _contract_diag(A, B, c1, c2, out_labels, max(0, min(rr, len(out_bonds))))
     if not A.is_blockform():
         from . import linalg
         if not comm:
@@ -541,6 +695,61 @@ def Contract(A: UniTensor, B: UniTensor) -> UniTensor:
     _lib.check(_lib.lib().cxa_block_contract_exec(
         plan.handle, A._storage.data_ptr(), B._storage.data_ptr(), out._storage.data_ptr(),
         plan.workspace(), PREC_FP64, stream_handle(dev)))
+    return out
+
+
+def _contract_diag(A: UniTensor, B: UniTensor, c1: List[int], c2: List[int],
+                   out_labels: List[str], rowrank: int) -> UniTensor:
+    """Contraction with a diagonal operand over one common leg: the other operand's blocks
+    scaled along that leg (Tensordot_dg, src/linalg/Tensordot_dg.cpp; the block case of
+    src/BlockUniTensor.cpp:1026-1044), then the legs ordered as Contract orders them."""
+    if len(c1) != 1:
+        raise NotImplementedError("contraction with a diagonal UniTensor over %d legs"
+                                  % len(c1))
+    if A.is_diag() and B.is_diag():
+        out = A.clone_meta()
+        out._bonds = [A._bonds[1 - c1[0]].clone(), B._bonds[1 - c2[0]].clone()]
+        out._labels = list(out_labels)
+        out._rowrank = 1
+        if A._is_block:
+            out._storage = A._storage * B._storage
+        else:
+            out._block = Tensor.from_storage(A._block.storage() * B._block.storage(),
+                                             [A._bonds[0].dim()])
+        return out
+    diag_left = A.is_diag()
+    D, T, ax, dax = (A, B, c2[0], c1[0]) if diag_left else (B, A, c1[0], c2[0])
+    if T.device() == CPU:
+        raise NotImplementedError("block contraction runs on one GPU; move both operands there")
+    if T._is_block and (T._bonds[ax].qnums() != D._bonds[dax].qnums() or
+                        T._bonds[ax].getDegeneracies() != D._bonds[dax].getDegeneracies()):
+        raise ValueError("the contracted bond of the diagonal UniTensor differs from its partner")
+    out = T.clone_meta()
+    out._bonds = list(T._bonds)
+    out._bonds[ax] = D._bonds[1 - dax].clone()
+    out._labels = list(T._labels)
+    out._labels[ax] = D._labels[1 - dax]
+    if T._is_block:
+        st = torch.empty_like(T._storage)
+        for bi in range(T.Nblocks()):
+            q = int(T._itoi[bi][ax])
+            d = D.get_block_(q)
+            shp = [1] * T.rank()
+            shp[ax] = -1
+            blk = T.get_block_(bi)
+            o = int(T._offsets[bi])
+            st[o:o + blk.numel()].view(blk.shape).copy_(blk * d.view(shp))
+        out._storage = st
+    else:
+        t = T._block.contiguous()
+        shp = [1] * T.rank()
+        shp[ax] = -1
+        v = t.torch() * D._block.torch().view(shp)
+        out._block = Tensor.from_storage(v.reshape(-1).contiguous(), t.shape())
+    order = [out._labels.index(l) for l in out_labels]
+    if order != list(range(out.rank())):
+        out = out.permute(order, rowrank)
+    out._rowrank = rowrank
     return out
 
 
