@@ -34,7 +34,7 @@ from .heff import Heff2
 from .heisenberg import MPO_GROUPS, mpo_bond, phys_bond, xxz_mpo_block
 from .lanczos import ground_state_mixed, lanczos_gnd_native
 from .mps import bond_sectors
-from .svd import Gesvd, Svd_truncate
+from .svd import Gesvd, Svd_truncate, orthogonal_split, truncated_split
 from .unitensor import BD_BRA, BD_KET, Bond, Contract, UniTensor
 
 
@@ -104,16 +104,19 @@ def random_mps(L: int, chi: int, seed: int = 1, device: int = 0, Q: int = 0) -> 
     return As
 
 
-def right_canonicalize(As: List[UniTensor]) -> List[UniTensor]:
-    """Gesvd from the right: A_k <- vT (right-orthonormal), A_{k-1} <- A_{k-1}.U.S; A_0 then
-    normalised."""
+def right_canonicalize(As: List[UniTensor], svd: str = "gram") -> List[UniTensor]:
+    """SVD from the right: A_k <- vT (right-orthonormal), A_{k-1} <- A_{k-1}.U.S; A_0 then
+    normalised.  ``svd`` as in DMRG."""
     As = list(As)
     for k in range(len(As) - 1, 0, -1):
         A = As[k].clone_meta()
         A.set_rowrank_(1)
-        S, U, vT = Gesvd(A)
+        if svd == "gram":
+            US, vT = orthogonal_split(A)                      # US: (a_k, _aux_R)
+        else:
+            S, U, vT = Gesvd(A)
+            US = Contract(U, S)                               # (a_k, _aux_R)
         As[k] = vT.relabel(As[k].labels()).set_rowrank_(2)
-        US = Contract(U, S)                                   # (a_k, _aux_R)
         prev = Contract(As[k - 1], US.relabel([As[k].labels()[0], "~aux"]))
         As[k - 1] = prev.relabel(As[k - 1].labels()).set_rowrank_(2)
     As[0] = As[0] / As[0].Norm()
@@ -125,15 +128,17 @@ class DMRG:
 
     def __init__(self, L: int, chi: int, J: float = 1.0, Jz: float = 1.0, seed: int = 1,
                  device: int = 0, maxit: int = 30, cvg: float = 1e-14, prec: str = "fp64",
-                 mix: bool = True, Q: int = 0):
+                 mix: bool = True, Q: int = 0, svd: str = "gram"):
         if prec not in ("fp64", "mixed"):
             raise ValueError("prec is 'fp64' or 'mixed'")
+        if svd not in ("gram", "gesvd"):
+            raise ValueError("svd is 'gram' or 'gesvd'")
         if L < 2:
             raise ValueError("DMRG needs at least 2 sites")
         self.L, self.chi, self.device = L, chi, device
-        self.maxit, self.cvg, self.prec, self.mix = maxit, cvg, prec, mix
+        self.maxit, self.cvg, self.prec, self.mix, self.svd = maxit, cvg, prec, mix, svd
         self.M = xxz_mpo_block(J, Jz, device=device)
-        self.As = right_canonicalize(random_mps(L, chi, seed, device, Q))
+        self.As = right_canonicalize(random_mps(L, chi, seed, device, Q), svd)
         self.LR: List[Optional[UniTensor]] = [None] * (L + 1)
         self.LR[0] = _boundary(True, device)
         self.LR[L] = _boundary(False, device, -Q)
@@ -182,14 +187,19 @@ class DMRG:
         d = A.shape()[1]
         new_dim = min(dim_l * d, dim_r * d, self.chi)
         psi.set_rowrank_(2)
-        s, U, vT = Svd_truncate(psi, new_dim)
-        s = s / s.Norm()
-        if absorb_left:
-            self.As[p] = Contract(U, s).relabel(A.labels()).set_rowrank_(2)
-            self.As[p + 1] = vT.relabel(B.labels()).set_rowrank_(2)
+        if self.svd == "gram":
+            s, X, Y = truncated_split(psi, new_dim, "left" if absorb_left else "right")
+            nrm = s.Norm()
+            if absorb_left:
+                X = X / nrm
+            else:
+                Y = Y / nrm
         else:
-            self.As[p] = U.relabel(A.labels()).set_rowrank_(2)
-            self.As[p + 1] = Contract(s, vT).relabel(B.labels()).set_rowrank_(2)
+            s, U, vT = Svd_truncate(psi, new_dim)
+            s = s / s.Norm()
+            X, Y = (Contract(U, s), vT) if absorb_left else (U, Contract(s, vT))
+        self.As[p] = X.relabel(A.labels()).set_rowrank_(2)
+        self.As[p + 1] = Y.relabel(B.labels()).set_rowrank_(2)
         torch.cuda.synchronize(self.device)
         t1 = time.perf_counter()
         if absorb_left:

@@ -56,6 +56,14 @@ def bond_sectors(L: int, k: int, chi: int, Q: int = 0) -> Tuple[List[int], List[
     if rem > 0:
         order = np.argsort(-(ideal - base), kind="stable")
         base[order[:rem]] += 1
+    while rem < 0:
+        # the floor of 1 per charge overshot chi: take the excess from the largest sectors
+        order = np.argsort(-base, kind="stable")
+        take = min(-rem, int(np.count_nonzero(base > 1)))
+        if take == 0:
+            break
+        base[order[:take]] -= 1
+        rem += take
     keep = base > 0
     return [q for q, k_ in zip(qs, keep) if k_], base[keep].astype(int).tolist()
 

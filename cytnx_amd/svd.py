@@ -80,10 +80,11 @@ def _diag_from_blocks(bond: Bond, blocks: List[torch.Tensor], dtype: int,
 
 
 class _Sector:
-    __slots__ = ("key", "rows", "cols", "rsz", "csz", "s", "u", "vh")
+    __slots__ = ("key", "rows", "cols", "rsz", "csz", "s", "u", "vh", "m")
 
 
-def _block_gesvd(T: UniTensor, is_U: bool, is_vT: bool) -> List[_Sector]:
+def _sector_matrices(T: UniTensor) -> List[_Sector]:
+    """Every sector's grid of blocks as one device matrix (sec.m), keys ascending."""
     r, rank = T.rowrank(), T.rank()
     bonds = T._bonds
     itoi = T._itoi.astype(np.int64)
@@ -114,16 +115,177 @@ def _block_gesvd(T: UniTensor, is_U: bool, is_vT: bool) -> List[_Sector]:
                 T.get_block_(int(b)).reshape(rsz[i], csz[j])
         sec = _Sector()
         sec.key, sec.rows, sec.cols, sec.rsz, sec.csz = int(key), rows, cols, rsz, csz
+        sec.m = M
+        sec.s = sec.u = sec.vh = None
+        sectors.append(sec)
+    return sectors
+
+
+def _block_gesvd(T: UniTensor, is_U: bool, is_vT: bool) -> List[_Sector]:
+    """Thin SVD of every sector matrix (LAPACK-style gesvd through torch.linalg.svd)."""
+    sectors = _sector_matrices(T)
+    for sec in sectors:
         if is_U or is_vT:
-            U, s, Vh = torch.linalg.svd(M, full_matrices=False)
+            U, s, Vh = torch.linalg.svd(sec.m, full_matrices=False)
             sec.u = U if is_U else None
             sec.vh = Vh if is_vT else None
         else:
-            s = torch.linalg.svdvals(M)
-            sec.u = sec.vh = None
+            s = torch.linalg.svdvals(sec.m)
         sec.s = s
-        sectors.append(sec)
+        sec.m = None
     return sectors
+
+
+def _svd_factor(sec: _Sector, side: str) -> None:
+    """The factors of _gram_factor from a thin SVD of one sector matrix."""
+    U, s, Vh = torch.linalg.svd(sec.m, full_matrices=False)
+    sec.s = s
+    if side == "left":
+        sec.u = U
+        sec.vh = s.to(Vh.dtype)[:, None] * Vh
+    else:
+        sec.vh = Vh
+        sec.u = U * s.to(U.dtype)[None, :]
+    sec.m = None
+
+
+def _lq_factor(sectors: List[_Sector]) -> None:
+    """M = X Q per sector with Q (k x n) row-orthonormal, k = min(m, n): a QR of M^H
+    (M^H = Q1 R1, X = R1^H, Q = Q1^H), batched by padded size; zero padding leaves the first
+    k columns of Q1 in the true rows.  sec.s is set to ones (no singular values)."""
+    buckets: Dict[Tuple[int, int], List[int]] = {}
+    for i, sec in enumerate(sectors):
+        m, n = sec.m.shape
+        buckets.setdefault((_pad_size(n), _pad_size(m)), []).append(i)
+    for (np_, mp), idx in sorted(buckets.items()):
+        M0 = sectors[idx[0]].m
+        Hb = torch.zeros((len(idx), np_, mp), dtype=M0.dtype, device=M0.device)
+        for b, i in enumerate(idx):
+            m, n = sectors[i].m.shape
+            Hb[b, :n, :m] = sectors[i].m.conj().transpose(0, 1)
+        Q1, R1 = torch.linalg.qr(Hb, mode="reduced")
+        for b, i in enumerate(idx):
+            sec = sectors[i]
+            m, n = sec.m.shape
+            k = min(m, n)
+            sec.vh = Q1[b, :n, :k].conj().transpose(0, 1).contiguous()
+            sec.u = R1[b, :k, :m].conj().transpose(0, 1).contiguous()
+            sec.s = torch.ones(k, dtype=sec.m.real.dtype, device=sec.m.device)
+            sec.m = None
+
+
+def orthogonal_split(T: UniTensor) -> List[UniTensor]:
+    """T = X . Q with Q right-orthonormal, no truncation: [X (rows, _aux_R), Q (_aux_R,
+    columns)] with the aux bond of Gesvd (sector keys ascending, min(m, n) each) -- the
+    right-canonicalisation step, by a batched LQ instead of an SVD."""
+    _check_input(T, "orthogonal_split")
+    if not T.is_blockform():
+        raise NotImplementedError("orthogonal_split runs on block UniTensors")
+    sectors = _sector_matrices(T)
+    _lq_factor(sectors)
+    _, X, Q = _assemble_block(T, sectors, [int(sec.s.numel()) for sec in sectors], True, True)
+    X._labels[-1] = "_aux_R"
+    return [X, Q]
+
+
+def _pad_size(n: int) -> int:
+    p = 8
+    while p < n:
+        p *= 2
+    return p
+
+
+def _gram_factor(sectors: List[_Sector], side: str) -> None:
+    """Density-matrix factorisation of every sector matrix M (m x n), k = min(m, n):
+
+    side "left":  eigh of M M^H -> U (m x k, orthonormal), sigma = sqrt(lambda), and the exact
+                  product U^H M (k x n) in sec.vh;
+    side "right": eigh of M^H M -> V (n x k, orthonormal), vT = V^H in sec.vh, and the exact
+                  product M V (m x k) in sec.u.
+
+    Sectors are batched by the Gram dimension rounded up to a power of two; the padding part of
+    each Gram matrix gets -1 on its diagonal so its eigenvectors sort below every true one (the
+    true eigenvalues are >= 0) and the top k eigenvectors live in the true rows.  Eigenpairs
+    come out in descending order.  sigma is accurate to about eps * sigma_max^2 / sigma
+    (absolute); the factor kept orthonormal is orthonormal to working precision and the other
+    factor is a product, never a division by sigma."""
+    buckets: Dict[int, List[int]] = {}
+    for i, sec in enumerate(sectors):
+        g = sec.m.shape[0] if side == "left" else sec.m.shape[1]
+        buckets.setdefault(_pad_size(g), []).append(i)
+    for gp, idx in sorted(buckets.items()):
+        other = max(sectors[i].m.shape[1] if side == "left" else sectors[i].m.shape[0]
+                    for i in idx)
+        M0 = sectors[idx[0]].m
+        shape = (len(idx), gp, other) if side == "left" else (len(idx), other, gp)
+        Mb = torch.zeros(shape, dtype=M0.dtype, device=M0.device)
+        for b, i in enumerate(idx):
+            m, n = sectors[i].m.shape
+            Mb[b, :m, :n] = sectors[i].m
+        if side == "left":
+            G = torch.bmm(Mb, Mb.conj().transpose(1, 2).contiguous())
+        else:
+            G = torch.bmm(Mb.conj().transpose(1, 2).contiguous(), Mb)
+        for b, i in enumerate(idx):
+            g = sectors[i].m.shape[0] if side == "left" else sectors[i].m.shape[1]
+            if g < gp:
+                # distinct negative values: no repeated eigenvalues from the padding
+                pad = torch.arange(g, gp, device=G.device)
+                G[b, pad, pad] = (-1.0 - (pad - g).to(G.real.dtype) / gp).to(G.dtype)
+        try:
+            lam, vec = torch.linalg.eigh(G)
+        except torch.linalg.LinAlgError:
+            # the batched eigensolver did not converge on some Gram matrix (many repeated
+            # eigenvalues, e.g. a rank-deficient sector): thin SVD of these sectors instead
+            for i in idx:
+                _svd_factor(sectors[i], side)
+            continue
+        for b, i in enumerate(idx):
+            sec = sectors[i]
+            m, n = sec.m.shape
+            k = min(m, n)
+            top = torch.arange(gp - 1, gp - 1 - k, -1, device=M0.device)
+            sec.s = torch.sqrt(torch.clamp(lam[b, top], min=0.0))
+            if side == "left":
+                U = vec[b, :m][:, top].contiguous()
+                sec.u = U
+                sec.vh = U.conj().transpose(0, 1) @ sec.m
+            else:
+                V = vec[b, :n][:, top].contiguous()
+                sec.vh = V.conj().transpose(0, 1).contiguous()
+                sec.u = sec.m @ V
+            sec.m = None
+
+
+def truncated_split(T: UniTensor, keepdim: int, absorb: str, err: float = 0.0,
+                    mindim: int = 1) -> List[UniTensor]:
+    """The truncated factorisation T ~ X . Y of DMRG's bond update, with the block cut rule of
+    Svd_truncate on the singular values, by the density-matrix form (``_gram_factor``):
+
+    absorb "right": X = U (rows, _aux_L; left-orthonormal), Y = S vT (_aux_L, columns), as
+                    Svd_truncate followed by Contract(s, vT);
+    absorb "left":  X = U S (rows, _aux_R), Y = vT (_aux_R, columns; right-orthonormal), as
+                    Contract(U, s) and vT.
+
+    Returns [S, X, Y] (S the kept singular values, diagonal)."""
+    _check_input(T, "truncated_split")
+    if not T.is_blockform():
+        raise NotImplementedError("truncated_split runs on block UniTensors")
+    if absorb not in ("right", "left"):
+        raise ValueError("absorb is 'right' or 'left'")
+    if keepdim < 1:
+        raise ValueError("keepdim must be >=1")
+    sectors = _sector_matrices(T)
+    _gram_factor(sectors, "left" if absorb == "right" else "right")
+    svals = np.concatenate([sec.s.detach().cpu().numpy() for sec in sectors])
+    Smin, _, _ = _cut(svals, keepdim, err, mindim)
+    keep = [int(np.count_nonzero(sec.s.detach().cpu().numpy() >= Smin)) for sec in sectors]
+    S, X, Y = _assemble_block(T, sectors, keep, True, True)
+    if absorb == "right":
+        Y._labels[0] = "_aux_L"
+    else:
+        X._labels[-1] = "_aux_R"
+    return [S, X, Y]
 
 
 def _assemble_block(T: UniTensor, sectors: List[_Sector], keep: List[int], is_U: bool,

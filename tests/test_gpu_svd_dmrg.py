@@ -106,6 +106,49 @@ def test_block_svd_truncate_err_threshold():
     assert S.storage().numel() == 6
 
 
+@pytest.mark.parametrize("absorb", ["right", "left"])
+@pytest.mark.parametrize("complex_", [False, True])
+def test_truncated_split_matches_svd_truncate(absorb, complex_):
+    """The density-matrix split: the same kept values and the same rank-k product as
+    Svd_truncate, the kept factor orthonormal to working precision."""
+    from cytnx_amd.svd import truncated_split
+    T = _psi_like(40, 21, complex_)
+    keep = 17
+    S, U, vT = linalg.Svd_truncate(T, keep)
+    S2, X, Y = truncated_split(T, keep, absorb)
+    np.testing.assert_allclose(np.sort(S2.storage().cpu().numpy()),
+                               np.sort(S.storage().cpu().numpy()), rtol=1e-10)
+    want = Contract(Contract(U, S), vT).to_dense()
+    got = Contract(X, Y)
+    assert got.labels() == T.labels()
+    assert np.linalg.norm(got.to_dense() - want) <= 1e-10 * np.linalg.norm(want)
+    if absorb == "right":
+        assert X.labels()[-1] == "_aux_L" and Y.labels()[0] == "_aux_L"
+        Q = X.to_dense().reshape(-1, X.shape()[-1])
+        np.testing.assert_allclose(Q.conj().T @ Q, np.eye(Q.shape[1]), atol=1e-13)
+    else:
+        assert X.labels()[-1] == "_aux_R" and Y.labels()[0] == "_aux_R"
+        Q = Y.to_dense().reshape(Y.shape()[0], -1)
+        np.testing.assert_allclose(Q @ Q.conj().T, np.eye(Q.shape[0]), atol=1e-13)
+
+
+@pytest.mark.parametrize("rowrank", [1, 2])
+@pytest.mark.parametrize("complex_", [False, True])
+def test_orthogonal_split_reconstructs(rowrank, complex_):
+    from cytnx_amd.svd import orthogonal_split
+    T = _psi_like(40, 31 + rowrank, complex_)
+    T.set_rowrank_(rowrank)
+    X, Q = orthogonal_split(T)
+    S = linalg.Gesvd(T, False, False)[0]
+    assert Q.bonds()[0].qnums() == S.bonds()[0].qnums()
+    assert Q.bonds()[0].getDegeneracies() == S.bonds()[0].getDegeneracies()
+    R = Contract(X, Q)
+    assert R.labels() == T.labels()
+    assert _rel(_by_key(R), _by_key(T)) <= 1e-12
+    Qd = Q.to_dense().reshape(Q.shape()[0], -1)
+    np.testing.assert_allclose(Qd @ Qd.conj().T, np.eye(Qd.shape[0]), atol=1e-13)
+
+
 def test_dense_svd_truncate():
     rng = np.random.default_rng(4)
     a = rng.standard_normal((6, 5, 7))
@@ -151,6 +194,16 @@ def test_dmrg_xx_free_fermions_at_L20():
     ref = c[c < 0].sum()
     print("DMRG XX L=20 chi=32: E", E[-1], "free fermions", ref, flush=True)
     assert abs(E[-1] - ref) <= 1e-6
+
+
+def test_dmrg_density_matrix_split_matches_gesvd_split():
+    a = DMRG(12, 24, seed=9, maxit=30, svd="gesvd")
+    b = DMRG(12, 24, seed=9, maxit=30, svd="gram")
+    Ea = a.run(2)
+    Eb = b.run(2)
+    print("DMRG L=12 chi=24 gesvd", Ea[-1], "gram", Eb[-1], flush=True)
+    assert abs(Ea[-1] - Eb[-1]) <= 1e-10
+    assert a.bond_dims() == b.bond_dims()
 
 
 def test_dmrg_mixed_precision_matches_fp64():
