@@ -237,6 +237,83 @@ def bench_mps_gradient(torch, L=32, chi=512):
     return out
 
 
+def bench_dmrg(torch, L=100, chi=2048, sweeps=1, maxit=30, cpu=True):
+    """Two-site U(1) DMRG of the open Heisenberg chain: L sites, bond dimension chi, `sweeps`
+    full sweeps (left to right and back) from a random right-canonical MPS, `maxit` Lanczos
+    iterations per bond.  Run twice from the same seed: contractions in FP32 + FP64 refinement
+    (the reported run) and in FP64 (the energy reference); the truncation and environments are
+    FP64 in both.  The CPU side times the oracle's restated per-pair contraction chain for one
+    matvec at the centre bond of the final state and scales it by the run's matvec flops
+    (a sampled estimate of the Lanczos share only)."""
+    from cytnx_amd.dmrg import DMRG
+    out = {"L": L, "chi": chi, "sweeps": sweeps, "lanczos_iterations_per_bond": maxit}
+    runs = {}
+    for prec in ("mixed", "fp64"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dm = DMRG(L, chi, seed=1, maxit=maxit, prec=prec)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        dm.run(sweeps)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        st = dm.stats
+        runs[prec] = {"E": dm.energies[-1], "init_s": t1 - t0, "sweeps_s": t2 - t1,
+                      "wall_s": t2 - t0, "updates": st["updates"], "matvecs": st["matvecs"],
+                      "lanczos_s": st["lanczos"], "lanczos_tflops":
+                          st["flops"] / st["lanczos"] / 1e12 if st["lanczos"] else None,
+                      "heff_plan_s": st["heff"], "split_s": st["svd"], "env_s": st["env"],
+                      "max_bond": max(dm.bond_dims())}
+        log("dmrg", prec, json.dumps(runs[prec]))
+        last = dm
+    out["mixed"], out["fp64"] = runs["mixed"], runs["fp64"]
+    out["dE_mixed_vs_fp64"] = runs["mixed"]["E"] - runs["fp64"]["E"]
+    out["speedup_mixed_vs_fp64_lanczos"] = runs["fp64"]["lanczos_s"] / runs["mixed"]["lanczos_s"]
+    if cpu:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import pyoracle as po
+        from cytnx_amd.unitensor import Contract
+
+        def ob(ut):
+            blocks = [ut.get_block_(i).detach().cpu().numpy() for i in range(ut.Nblocks())]
+            bonds = [po.OBond(b.type(), b.qnums(), b.getDegeneracies()) for b in ut.bonds()]
+            return po.OBlockTensor(bonds, ut.labels(), ut.rowrank(), itoi=ut.itoi(),
+                                   blocks=blocks)
+
+        from cytnx_amd.dmrg import left_env_step
+        from cytnx_amd.heff import Heff2
+        # after the closing right-to-left sweep LR[k >= 1] are right environments: rebuild the
+        # left one of the centre bond from the boundary
+        p = L // 2 - 1
+        Lp = last.LR[0]
+        for k in range(p):
+            Lp = left_env_step(Lp, last.As[k], last.M)
+        psi = Contract(last.As[p], last.As[p + 1]).relabel(["-1", "-2", "-3", "-4"])
+        Lr = Lp.relabel(["-5", "-1", "0"])
+        Rr = last.LR[p + 2].relabel(["-7", "-4", "3"])
+        M1r = last.M.relabel(["-5", "-6", "-2", "1"])
+        M2r = last.M.relabel(["-6", "-7", "-3", "2"])
+        f = Heff2(psi, Lr, M1r, M2r, Rr).ref_flops
+        Lo, Ro, M1, M2, P = ob(Lr), ob(Rr), ob(M1r), ob(M2r), ob(psi)
+        ncpu = min(16, os.cpu_count() or 1)
+        blas = po.use_openblas(nthreads=ncpu)
+        t0 = time.perf_counter()
+        X = po.block_contract(Lo, P)
+        X = po.block_contract(X, M1)
+        X = po.block_contract(X, M2)
+        po.block_contract(X, Ro)
+        tc = time.perf_counter() - t0
+        po.use_reference_loops()
+        est = tc / f * last.stats["flops"]
+        out["cpu"] = {"matvec_s_at_centre": tc, "centre_flops": f, "cores": ncpu, "kind": "port",
+                      "blas": blas, "lanczos_s_estimate": est,
+                      "sample": "one centre-bond matvec of the oracle's restated "
+                                "BlockUniTensor::contract chain (FP64), scaled by the run's "
+                                "matvec flops"}
+        out["speedup_lanczos_vs_cpu_estimate"] = est / runs["mixed"]["lanczos_s"]
+    return out
+
+
 def cpu_baseline(problem, flops, seconds):
     import numpy as np
     sys.path.insert(0, os.path.join(HERE, "oracle"))
@@ -313,6 +390,10 @@ def main() -> int:
                     help="apply tilings from a --save-tiles file instead of tuning")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the permute sweep and the dense one-site Lanczos lines")
+    ap.add_argument("--dmrg-sweeps", type=int, default=1,
+                    help="full sweeps of the L=100, chi=2048 two-site DMRG line (0: skip)")
+    ap.add_argument("--dmrg-chi", type=int, default=2048)
+    ap.add_argument("--dmrg-L", type=int, default=100)
     ap.add_argument("--chain", default="mix", choices=["mix", "w12", "ref"],
                     help="matvec program: mix (two launches), w12 (three), ref (four, the "
                          "reference's contraction order)")
@@ -468,6 +549,12 @@ def main() -> int:
         log("MPS energy gradient")
         extra["mps_energy_gradient"] = bench_mps_gradient(torch)
         log("mps", json.dumps(extra["mps_energy_gradient"]))
+        if args.dmrg_sweeps > 0:
+            log("two-site DMRG L=%d chi=%d, %d sweep(s)" % (args.dmrg_L, args.dmrg_chi,
+                                                            args.dmrg_sweeps))
+            extra["dmrg"] = bench_dmrg(torch, args.dmrg_L, args.dmrg_chi, args.dmrg_sweeps,
+                                       cpu=not args.no_cpu)
+            log("dmrg", json.dumps(extra["dmrg"]))
 
     out = {
         "metric": METRIC,
