@@ -237,6 +237,20 @@ def bench_mps_gradient(torch, L=32, chi=512):
     return out
 
 
+def measured_traffic(chi, chain, prec, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary of this
+    workload (profiles/heff2_chi<chi>_traffic.json), or None when no measurement matches."""
+    path = os.path.join(HERE, "profiles", "heff2_chi%d_traffic.json" % chi)
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("chi") != chi or t.get("chain") != chain or t.get("prec") != prec:
+        return None
+    return t.get("hbm_bytes_per_launch", {}).get(kernel)
+
+
 def bench_dmrg(torch, L=100, chi=2048, sweeps=1, maxit=30, cpu=True):
     """Two-site U(1) DMRG of the open Heisenberg chain: L sites, bond dimension chi, `sweeps`
     full sweeps (left to right and back) from a random right-canonical MPS, `maxit` Lanczos
@@ -526,7 +540,8 @@ def main() -> int:
         achieved = step_flops[k_dom] / (dom_ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[main_mode],
                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[main_mode]}
-    roofline.update({"traffic": None,
+    roofline.update({"traffic": measured_traffic(args.chi, H.mode, main_mode,
+                                                 H.step_names[k_dom]),
                      "kernel": "chain step %d of %d (%s)" % (k_dom, len(step_flops),
                                                              H.step_names[k_dom]),
                      "kernel_flops": step_flops[k_dom], "kernel_bytes": sbytes[k_dom],
