@@ -145,10 +145,29 @@ def bench_permute(torch, nperm=32):
                 per = a.elapsed_time(b) * 1e-3 / (3 * nb)
                 gbs.append(2.0 * n * es / per / 1e9)
                 del g
+            # the same bytes moved by a plain device copy, timed the same way: the practical
+            # ceiling of one launch at this size
+            g = torch.cuda.CUDAGraph()
+            cs = torch.cuda.Stream()
+            with torch.cuda.graph(g, stream=cs):
+                for i in range(nb):
+                    dst[i * n:(i + 1) * n].copy_(src[i * n:(i + 1) * n])
+            g.replay()
+            torch.cuda.synchronize()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(3):
+                g.replay()
+            b.record()
+            torch.cuda.synchronize()
+            copy_gbs = 2.0 * n * es / (a.elapsed_time(b) * 1e-3 / (3 * nb)) / 1e9
+            del g
             key = "%s_%s" % ("x".join(map(str, shape)), name)
             res[key] = {"mean_GBs": float(np.mean(gbs)), "min_GBs": float(np.min(gbs)),
                         "max_GBs": float(np.max(gbs)),
                         "frac_of_8TBs": float(np.mean(gbs)) / PEAK_HBM_GBS,
+                        "copy_GBs": copy_gbs, "frac_of_copy": float(np.mean(gbs)) / copy_gbs,
                         "buffers_rotated": nb}
             del src, dst
     allg = [v["mean_GBs"] for v in res.values()]

@@ -8,7 +8,10 @@
 //   * an LDS-tiled transpose otherwise: a tile spans a run of the input's trailing axes ("A",
 //     contiguous in the input) by a run of the output's trailing axes ("B", contiguous in the
 //     output).  Reads stream along A, writes stream along B, the tile in LDS turns one into the
-//     other; per-tile offsets of the A and B indices are decoded once into LDS.
+//     other; per-tile offsets of the A and B indices are decoded once into LDS.  For elements of
+//     8 bytes or less each workgroup walks several tiles with the next tile's loads issued
+//     before the current tile's stores (software pipelining); 16-byte elements go one tile at
+//     a time.
 // The semantics are those of the reference's MoveMemoryCpu / MoveMemoryGpu
 // (src/backend/utils_internal_cpu/Movemem_cpu.cpp:18-211,
 //  src/backend/utils_internal_gpu/cuMovemem_gpu.cu:60-304): out = transpose(in, mapper).
@@ -27,6 +30,7 @@ struct alignas(16) U128 {
 constexpr int kMaxOuter = 14;
 constexpr int kMaxGroup = 8;
 constexpr int kThreads = 256;
+constexpr int kMaxTile = 256;  // tile extent bound along A and along B
 
 struct TileArgs {
   int na, nb, no;
@@ -174,70 +178,162 @@ __device__ __forceinline__ uint32_t decode32(uint32_t x, int n, const FastDiv *d
   return off;
 }
 
+struct TileAt {
+  uint32_t oin, oout, a0, b0, tw, th;
+};
+
+__device__ __forceinline__ TileAt tile_at(const TileArgs32 &p, uint32_t t) {
+  TileAt x;
+  uint32_t rem = t, q = p.dnca.div(rem);
+  const uint32_t ca = rem - q * p.dnca.d;
+  rem = q;
+  q = p.dncb.div(rem);
+  const uint32_t cb = rem - q * p.dncb.d;
+  rem = q;
+  x.oin = 0;
+  x.oout = 0;
+  for (int i = p.no - 1; i >= 0; --i) {
+    q = p.o_div[i].div(rem);
+    const uint32_t o = rem - q * p.o_div[i].d;
+    rem = q;
+    x.oin += o * p.o_istr[i];
+    x.oout += o * p.o_ostr[i];
+  }
+  x.a0 = ca * p.w;
+  x.b0 = cb * p.h;
+  x.tw = min((uint32_t)p.w, p.prodA - x.a0);
+  x.th = min((uint32_t)p.h, p.prodB - x.b0);
+  return x;
+}
+
+// One tile at a time: offsets, loads, LDS, stores (16-byte elements, where the pipelined form's
+// extra registers and offset tables cost more occupancy than the overlap gains).
 template <typename T, int MAXE>
-__global__ __launch_bounds__(kThreads) void permute_tiled32(const T *__restrict__ src,
-                                                            T *__restrict__ dst, TileArgs32 p) {
+__global__ __launch_bounds__(kThreads) void permute_tiled32_once(const T *__restrict__ src,
+                                                                 T *__restrict__ dst,
+                                                                 TileArgs32 p) {
   constexpr int PER = MAXE / kThreads;
   __shared__ T tile[MAXE + 64];
-  __shared__ uint32_t aoff[1024];
-  __shared__ uint32_t boff[1024];
+  __shared__ uint32_t aoff[kMaxTile];
+  __shared__ uint32_t boff[kMaxTile];
   const int w = p.w, h = p.h, ws = w + 1;
   const uint32_t wh = (uint32_t)(w * h);
   for (uint32_t t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
-    uint32_t rem = t, q = p.dnca.div(rem);
-    const uint32_t ca = rem - q * p.dnca.d;
-    rem = q;
-    q = p.dncb.div(rem);
-    const uint32_t cb = rem - q * p.dncb.d;
-    rem = q;
-    uint32_t oin = 0, oout = 0;
-    for (int i = p.no - 1; i >= 0; --i) {
-      q = p.o_div[i].div(rem);
-      const uint32_t x = rem - q * p.o_div[i].d;
-      rem = q;
-      oin += x * p.o_istr[i];
-      oout += x * p.o_ostr[i];
-    }
-    const uint32_t a0 = ca * w, b0 = cb * h;
-    const uint32_t tw = min((uint32_t)w, p.prodA - a0), th = min((uint32_t)h, p.prodB - b0);
+    const TileAt x = tile_at(p, t);
     __syncthreads();  // the previous tile is fully written out
     for (int i = threadIdx.x; i < w + h; i += kThreads) {
       if (i < w)
-        aoff[i] = decode32(a0 + i, p.na, p.a_div, p.a_ostr);
+        aoff[i] = decode32(x.a0 + i, p.na, p.a_div, p.a_ostr);
       else
-        boff[i - w] = decode32(b0 + (i - w), p.nb, p.b_div, p.b_istr);
+        boff[i - w] = decode32(x.b0 + (i - w), p.nb, p.b_div, p.b_istr);
     }
     __syncthreads();
-    // read: consecutive threads walk A, which is contiguous in the input
-    const T *s = src + oin + a0;
+    const T *s = src + x.oin + x.a0;
     T r[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t i = threadIdx.x + k * kThreads;
       const uint32_t b = p.dw.div(i), a = i - b * w;
-      if (i < wh && a < tw && b < th) r[k] = s[boff[b] + a];
+      if (i < wh && a < x.tw && b < x.th) r[k] = s[boff[b] + a];
     }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t i = threadIdx.x + k * kThreads;
       const uint32_t b = p.dw.div(i), a = i - b * w;
-      if (i < wh && a < tw && b < th) tile[b * ws + a] = r[k];
+      if (i < wh && a < x.tw && b < x.th) tile[b * ws + a] = r[k];
     }
     __syncthreads();
+    T *d = dst + x.oout + x.b0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t i = threadIdx.x + k * kThreads;
+      const uint32_t a = p.dh.div(i), b = i - a * h;
+      if (i < wh && a < x.tw && b < x.th) r[k] = tile[b * ws + a];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t i = threadIdx.x + k * kThreads;
+      const uint32_t a = p.dh.div(i), b = i - a * h;
+      if (i < wh && a < x.tw && b < x.th) d[aoff[a] + b] = r[k];
+    }
+  }
+}
+
+// Software-pipelined over the workgroup's tiles (t, t + grid, ...): the loads of tile t+grid
+// are issued right after tile t is staged in LDS, so they are in flight while tile t is written
+// out -- reads and writes overlap within each workgroup instead of alternating in lockstep
+// across the whole launch.  The per-tile offset tables are double-buffered for that.
+template <typename T, int MAXE>
+__global__ __launch_bounds__(kThreads) void permute_tiled32(const T *__restrict__ src,
+                                                            T *__restrict__ dst, TileArgs32 p) {
+  constexpr int PER = MAXE / kThreads;
+  __shared__ T tile[MAXE + 64];
+  __shared__ uint32_t aoff[2][kMaxTile];
+  __shared__ uint32_t boff[2][kMaxTile];
+  const int w = p.w, h = p.h, ws = w + 1;
+  const uint32_t wh = (uint32_t)(w * h);
+  uint32_t t = blockIdx.x;
+  if (t >= p.ntiles) return;  // uniform over the workgroup
+  auto decode_offsets = [&](int buf, const TileAt &x) {
+    for (int i = threadIdx.x; i < w + h; i += kThreads) {
+      if (i < w)
+        aoff[buf][i] = decode32(x.a0 + i, p.na, p.a_div, p.a_ostr);
+      else
+        boff[buf][i - w] = decode32(x.b0 + (i - w), p.nb, p.b_div, p.b_istr);
+    }
+  };
+  // read: consecutive threads walk A, which is contiguous in the input
+  auto load = [&](int buf, const TileAt &x, T (&r)[PER]) {
+    const T *s = src + x.oin + x.a0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t i = threadIdx.x + k * kThreads;
+      const uint32_t b = p.dw.div(i), a = i - b * w;
+      if (i < wh && a < x.tw && b < x.th) r[k] = s[boff[buf][b] + a];
+    }
+  };
+  TileAt cur = tile_at(p, t);
+  decode_offsets(0, cur);
+  __syncthreads();
+  T r[PER];
+  load(0, cur, r);
+  int buf = 0;
+  for (;;) {
+    const uint32_t tn = t + gridDim.x;
+    const bool more = tn < p.ntiles;  // uniform
+    TileAt nxt = cur;
+    if (more) {
+      nxt = tile_at(p, tn);
+      decode_offsets(buf ^ 1, nxt);  // that buffer was last read before the previous barrier
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t i = threadIdx.x + k * kThreads;
+      const uint32_t b = p.dw.div(i), a = i - b * w;
+      if (i < wh && a < cur.tw && b < cur.th) tile[b * ws + a] = r[k];
+    }
+    __syncthreads();  // tile staged; next offsets decoded
+    if (more) load(buf ^ 1, nxt, r);  // next tile's loads in flight during the writes below
     // write: consecutive threads walk B, which is contiguous in the output
-    T *d = dst + oout + b0;
+    T *d = dst + cur.oout + cur.b0;
+    T o[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t i = threadIdx.x + k * kThreads;
       const uint32_t a = p.dh.div(i), b = i - a * h;
-      if (i < wh && a < tw && b < th) r[k] = tile[b * ws + a];
+      if (i < wh && a < cur.tw && b < cur.th) o[k] = tile[b * ws + a];
     }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t i = threadIdx.x + k * kThreads;
       const uint32_t a = p.dh.div(i), b = i - a * h;
-      if (i < wh && a < tw && b < th) d[aoff[a] + b] = r[k];
+      if (i < wh && a < cur.tw && b < cur.th) d[aoff[buf][a] + b] = o[k];
     }
+    __syncthreads();  // tile and this buffer's offsets consumed
+    if (!more) break;
+    t = tn;
+    cur = nxt;
+    buf ^= 1;
   }
 }
 
@@ -395,9 +491,15 @@ int launch_tiled(const Reduced &r, const void *src, void *dst, hipStream_t s) {
   }
   p.no = no;
   constexpr int MAXE = (sizeof(T) >= 16) ? 1024 : 2048;
-  const int64_t wmax = std::max<int64_t>(1, 256 / es);
-  p.w = (int)std::min<int64_t>(prodA, wmax);
-  p.h = (int)std::min<int64_t>({prodB, (int64_t)(MAXE / (p.w + 1)), (int64_t)1024});
+  // Tile shape and grid, measured on the rank-6 sweep (16 MiB / 32 MiB tensors, 32 seeded
+  // permutations): 8-byte and smaller elements run the pipelined kernel on tiles 128 bytes wide
+  // with at most 1024 workgroups (several tiles each, so loads and stores overlap); 16-byte
+  // elements run one tile at a time, 256 bytes wide and 32 rows high.
+  const bool pipelined = es <= 8;
+  const int64_t wmax = std::max<int64_t>(1, (pipelined ? 128 : 256) / es);
+  const int64_t hmax = pipelined ? kMaxTile : 32;
+  p.w = (int)std::min<int64_t>({prodA, wmax, (int64_t)kMaxTile});
+  p.h = (int)std::min<int64_t>({prodB, (int64_t)(MAXE / (p.w + 1)), (int64_t)kMaxTile, hmax});
   p.prodA = prodA;
   p.prodB = prodB;
   p.nca = (prodA + p.w - 1) / p.w;
@@ -405,7 +507,12 @@ int launch_tiled(const Reduced &r, const void *src, void *dst, hipStream_t s) {
   int64_t outer = 1;
   for (int k = 0; k < no; ++k) outer *= p.o_ext[k];
   p.ntiles = outer * p.nca * p.ncb;
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(p.ntiles, 256 * 64));
+  // pipelined: about two tiles per workgroup, at least 1024 workgroups (a single round when the
+  // tensor is small), at most 16384
+  const int64_t grid_cap = pipelined
+                               ? std::min<int64_t>(256 * 64, std::max<int64_t>(1024, p.ntiles / 2))
+                               : 4096;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(p.ntiles, grid_cap));
   if (r.total < (int64_t(1) << 31)) {  // every offset and index fits in 32 bits
     TileArgs32 q{};
     q.na = p.na;
@@ -433,8 +540,12 @@ int launch_tiled(const Reduced &r, const void *src, void *dst, hipStream_t s) {
       q.o_istr[k] = (uint32_t)p.o_istr[k];
       q.o_ostr[k] = (uint32_t)p.o_ostr[k];
     }
-    hipLaunchKernelGGL((permute_tiled32<T, MAXE>), dim3(grid), dim3(kThreads), 0, s,
-                       (const T *)src, (T *)dst, q);
+    if (pipelined)
+      hipLaunchKernelGGL((permute_tiled32<T, MAXE>), dim3(grid), dim3(kThreads), 0, s,
+                         (const T *)src, (T *)dst, q);
+    else
+      hipLaunchKernelGGL((permute_tiled32_once<T, MAXE>), dim3(grid), dim3(kThreads), 0, s,
+                         (const T *)src, (T *)dst, q);
     return 0;
   }
   hipLaunchKernelGGL((permute_tiled<T, MAXE>), dim3(grid), dim3(kThreads), 0, s,
