@@ -482,12 +482,19 @@ def main() -> int:
         elif not args.no_tune:
             tuned[name] = H.tune(reps=20)
             log(name, "tuned", json.dumps(tuned[name]))
-        ms, wall = time_matvec(H, x, y, args.steps, args.warmup, torch, graph=True)
-        ms_eager, _ = time_matvec(H, x, y, args.steps, args.warmup, torch, graph=False)
+        ms_graph, wall_g = time_matvec(H, x, y, args.steps, args.warmup, torch, graph=True)
+        ms_eager, wall_e = time_matvec(H, x, y, args.steps, args.warmup, torch, graph=False)
         per = time_steps(H, x, y, max(20, args.steps // 4), torch)
+        # the reported rate is the faster way of issuing the K matvecs: the chain's launches
+        # enqueued on the stream (what the Lanczos driver does by default) or one captured-graph
+        # replay per matvec
+        launch = "stream" if ms_eager <= ms_graph else "graph"
+        ms = min(ms_eager, ms_graph)
         res[name] = {"ms_per_step": ms, "tflops": flops / (ms * 1e-3) / 1e12,
                      "pct_peak": 100.0 * flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS[name],
-                     "ms_per_step_host_launched": ms_eager, "step_ms": per, "wall_s": wall}
+                     "launch": launch, "ms_per_step_stream": ms_eager,
+                     "ms_per_step_graph": ms_graph, "step_ms": per,
+                     "wall_s": wall_e if launch == "stream" else wall_g}
         log(name, json.dumps(res[name]))
     if args.save_tiles:
         with open(args.save_tiles, "w") as f:
@@ -608,6 +615,7 @@ def main() -> int:
                    "chi": args.chi, "sectors": len(heisenberg.gaussian_sectors(args.chi)[0]),
                    "psi_elements": int(x.numel()), "block_pairs_per_step": H.pairs,
                    "flops_per_matvec": flops, "chain": H.mode, "kernels": H.step_names,
+                   "launch": res[main_mode]["launch"],
                    "kernel_flops": step_flops},
         "fp64": res.get("fp64"),
         "fp32": res["fp32"],

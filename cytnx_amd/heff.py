@@ -142,6 +142,9 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
     offs1p = _sector_offsets(Y.bonds()[pY["s1p"]])
     offs2p = _sector_offsets(Y.bonds()[pY["s2p"]])
     offw2 = _sector_offsets(Y.bonds()[pY["w2"]])
+    stL = [_row_strides(s) for s in Lp._shapes]
+    stP = [_row_strides(s) for s in psi._shapes]
+    stY = [_row_strides(s) for s in Y._shapes]
     L_by = {}
     for bi, t in enumerate(Lp._itoi):
         L_by.setdefault((int(t[pL["l"]]), int(t[pL["lp"]])), []).append(bi)
@@ -156,11 +159,11 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
         cand = []
         for pi in P_by.get(qc, []):
             tp, shp = psi._itoi[pi], psi._shapes[pi]
-            stp = _row_strides(shp)
+            stp = stP[pi]
             qa = int(tp[pP["l"]])
             for li in L_by.get((qa, qap), []):
                 tl, shl = Lp._itoi[li], Lp._shapes[li]
-                stl = _row_strides(shl)
+                stl = stL[li]
                 K = int(shl[pL["l"]])
                 if K != int(shp[pP["l"]]):
                     raise ValueError("Heff2 mix plan: L and psi disagree on a sector of l")
@@ -181,7 +184,7 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
         M = N = None
         for yi in ylist:
             ty, shy = Y._itoi[yi], Y._shapes[yi]
-            sty = _row_strides(shy)
+            sty = stY[yi]
             if sty[pY["r"]] != 1:
                 raise ValueError("Heff2 mix plan: r must be the contiguous axis of Y")
             M, N = int(shy[pY["lp"]]), int(shy[pY["r"]])

@@ -203,58 +203,61 @@ def _gram_factor(sectors: List[_Sector], side: str) -> None:
     side "right": eigh of M^H M -> V (n x k, orthonormal), vT = V^H in sec.vh, and the exact
                   product M V (m x k) in sec.u.
 
-    Sectors are batched by the Gram dimension rounded up to a power of two; the padding part of
-    each Gram matrix gets -1 on its diagonal so its eigenvectors sort below every true one (the
-    true eigenvalues are >= 0) and the top k eigenvectors live in the true rows.  Eigenpairs
-    come out in descending order.  sigma is accurate to about eps * sigma_max^2 / sigma
-    (absolute); the factor kept orthonormal is orthonormal to working precision and the other
-    factor is a product, never a division by sigma."""
-    buckets: Dict[int, List[int]] = {}
-    for i, sec in enumerate(sectors):
-        g = sec.m.shape[0] if side == "left" else sec.m.shape[1]
-        buckets.setdefault(_pad_size(g), []).append(i)
-    for gp, idx in sorted(buckets.items()):
-        other = max(sectors[i].m.shape[1] if side == "left" else sectors[i].m.shape[0]
-                    for i in idx)
-        M0 = sectors[idx[0]].m
-        shape = (len(idx), gp, other) if side == "left" else (len(idx), other, gp)
-        Mb = torch.zeros(shape, dtype=M0.dtype, device=M0.device)
-        for b, i in enumerate(idx):
-            m, n = sectors[i].m.shape
-            Mb[b, :m, :n] = sectors[i].m
-        if side == "left":
-            G = torch.bmm(Mb, Mb.conj().transpose(1, 2).contiguous())
+    All sectors go into one batch padded to the largest Gram dimension (one eigensolver call:
+    on this stack a call costs milliseconds whatever its size, so one padded call beats one per
+    size class); the padding part of each Gram matrix gets distinct values below -1 on its
+    diagonal, so its eigenvectors sort below every true one (the true eigenvalues are >= 0) and
+    the top k eigenvectors live in the true rows.  Eigenpairs come out in descending order.
+    sigma is accurate to about eps * sigma_max^2 / sigma (absolute); the factor kept orthonormal
+    is orthonormal to working precision and the other factor is a product, never a division by
+    sigma."""
+    if not sectors:
+        return
+    left = side == "left"
+    dims = [sec.m.shape for sec in sectors]
+    gs = [d[0] if left else d[1] for d in dims]
+    os_ = [d[1] if left else d[0] for d in dims]
+    gp, op = max(gs), max(os_)
+    M0 = sectors[0].m
+    Mb = torch.zeros((len(sectors), gp, op) if left else (len(sectors), op, gp),
+                     dtype=M0.dtype, device=M0.device)
+    for b, sec in enumerate(sectors):
+        m, n = sec.m.shape
+        Mb[b, :m, :n] = sec.m
+    MbH = Mb.conj().transpose(1, 2).contiguous()
+    G = torch.bmm(Mb, MbH) if left else torch.bmm(MbH, Mb)
+    # distinct negative values on the padding diagonal: no repeated eigenvalues from padding
+    gidx = torch.arange(gp, device=G.device)
+    glen = torch.tensor(gs, device=G.device)[:, None]
+    padv = -1.0 - (gidx[None, :] - glen).clamp(min=0).to(G.real.dtype) / gp
+    diag = torch.diagonal(G, dim1=1, dim2=2)
+    diag.copy_(torch.where(gidx[None, :] < glen, diag, padv.to(G.dtype)))
+    try:
+        lam, vec = torch.linalg.eigh(G)
+    except torch.linalg.LinAlgError:
+        # the batched eigensolver did not converge on some Gram matrix (many repeated
+        # eigenvalues, e.g. a rank-deficient sector): thin SVD of every sector instead
+        for sec in sectors:
+            _svd_factor(sec, side)
+        return
+    kmax = max(min(d) for d in dims)
+    sig = torch.sqrt(torch.clamp(lam.flip(-1)[:, :kmax], min=0.0))   # descending
+    vd = vec.flip(-1)[:, :, :kmax]                                     # matching vectors
+    if left:
+        other = torch.bmm(vd.conj().transpose(1, 2), Mb)               # U^H M  (B, kmax, op)
+    else:
+        other = torch.bmm(Mb, vd)                                      # M V    (B, op, kmax)
+    for b, sec in enumerate(sectors):
+        m, n = sec.m.shape
+        k = min(m, n)
+        sec.s = sig[b, :k]
+        if left:
+            sec.u = vd[b, :m, :k]
+            sec.vh = other[b, :k, :n]
         else:
-            G = torch.bmm(Mb.conj().transpose(1, 2).contiguous(), Mb)
-        for b, i in enumerate(idx):
-            g = sectors[i].m.shape[0] if side == "left" else sectors[i].m.shape[1]
-            if g < gp:
-                # distinct negative values: no repeated eigenvalues from the padding
-                pad = torch.arange(g, gp, device=G.device)
-                G[b, pad, pad] = (-1.0 - (pad - g).to(G.real.dtype) / gp).to(G.dtype)
-        try:
-            lam, vec = torch.linalg.eigh(G)
-        except torch.linalg.LinAlgError:
-            # the batched eigensolver did not converge on some Gram matrix (many repeated
-            # eigenvalues, e.g. a rank-deficient sector): thin SVD of these sectors instead
-            for i in idx:
-                _svd_factor(sectors[i], side)
-            continue
-        for b, i in enumerate(idx):
-            sec = sectors[i]
-            m, n = sec.m.shape
-            k = min(m, n)
-            top = torch.arange(gp - 1, gp - 1 - k, -1, device=M0.device)
-            sec.s = torch.sqrt(torch.clamp(lam[b, top], min=0.0))
-            if side == "left":
-                U = vec[b, :m][:, top].contiguous()
-                sec.u = U
-                sec.vh = U.conj().transpose(0, 1) @ sec.m
-            else:
-                V = vec[b, :n][:, top].contiguous()
-                sec.vh = V.conj().transpose(0, 1).contiguous()
-                sec.u = sec.m @ V
-            sec.m = None
+            sec.vh = torch.conj_physical(vd[b, :n, :k].transpose(0, 1))
+            sec.u = other[b, :m, :k]
+        sec.m = None
 
 
 def truncated_split(T: UniTensor, keepdim: int, absorb: str, err: float = 0.0,
@@ -277,9 +280,12 @@ def truncated_split(T: UniTensor, keepdim: int, absorb: str, err: float = 0.0,
         raise ValueError("keepdim must be >=1")
     sectors = _sector_matrices(T)
     _gram_factor(sectors, "left" if absorb == "right" else "right")
-    svals = np.concatenate([sec.s.detach().cpu().numpy() for sec in sectors])
+    lens = [int(sec.s.numel()) for sec in sectors]
+    svals = torch.cat([sec.s for sec in sectors]).detach().cpu().numpy()   # one transfer
     Smin, _, _ = _cut(svals, keepdim, err, mindim)
-    keep = [int(np.count_nonzero(sec.s.detach().cpu().numpy() >= Smin)) for sec in sectors]
+    bounds = np.concatenate([[0], np.cumsum(lens)]).astype(int)
+    keep = [int(np.count_nonzero(svals[bounds[i]:bounds[i + 1]] >= Smin))
+            for i in range(len(sectors))]
     S, X, Y = _assemble_block(T, sectors, keep, True, True)
     if absorb == "right":
         Y._labels[0] = "_aux_L"
