@@ -149,10 +149,20 @@ def _svd_factor(sec: _Sector, side: str) -> None:
     sec.m = None
 
 
+def _pad_size(n: int) -> int:
+    p = 8
+    while p < n:
+        p *= 2
+    return p
+
+
 def _lq_factor(sectors: List[_Sector]) -> None:
     """M = X Q per sector with Q (k x n) row-orthonormal, k = min(m, n): a QR of M^H
-    (M^H = Q1 R1, X = R1^H, Q = Q1^H), batched by padded size; zero padding leaves the first
-    k columns of Q1 in the true rows.  sec.s is set to ones (no singular values)."""
+    (M^H = Q1 R1, X = R1^H, Q = Q1^H), sectors batched by size rounded up to powers of two
+    (the batched QR's cost grows with the padding, unlike its per-call cost; measured: one call
+    padded to the largest sector is 2.7x slower at chi=2048); zero padding at the end of rows
+    and columns leaves the first k columns of Q1 in the true rows.  sec.s is set to ones (no
+    singular values)."""
     buckets: Dict[Tuple[int, int], List[int]] = {}
     for i, sec in enumerate(sectors):
         m, n = sec.m.shape
@@ -186,13 +196,6 @@ def orthogonal_split(T: UniTensor) -> List[UniTensor]:
     _, X, Q = _assemble_block(T, sectors, [int(sec.s.numel()) for sec in sectors], True, True)
     X._labels[-1] = "_aux_R"
     return [X, Q]
-
-
-def _pad_size(n: int) -> int:
-    p = 8
-    while p < n:
-        p *= 2
-    return p
 
 
 def _gram_factor(sectors: List[_Sector], side: str) -> None:
