@@ -68,11 +68,11 @@ struct CopyJob {
 // capturable) and the tuner can pick different shapes for FP64 and FP32.
 struct PlanVariant {
   int kind = -1;        // tile kind override, -1 = per-block default
-  int split_mode = -1;  // -1 automatic, 0 off, 1 on
+  int split_mode = -1;  // -1 automatic, 0 off, 1 on (four waves per tile), 2 eight waves
   std::vector<cxa::GTile> tiles;  // kXcds ranges of per_xcd tiles
   cxa::DevBuf d_tiles;
   int per_xcd = 0;
-  bool splitk = false;
+  int splitk = 0;  // waves per tile of the split-K launch, 0 = none (mix plans: 1 = four-wave)
   int tile_class = 0;
   int launch_kind = -1;  // mix plans: the one tile kind of the launch
 };
@@ -108,7 +108,7 @@ int build_variant(cxa_bc_plan *p, int prec) {
   if (p->mix) {
     v.launch_kind = mix_kind_ok(v.kind) ? v.kind : TK_1x1;
     v.tile_class = make_tiles(p->blks, p->segs, v.tiles, &v.per_xcd, v.launch_kind, false);
-    v.splitk = v.split_mode == 1;  // mix plans: four-wave workgroups (two slices per wave)
+    v.splitk = v.split_mode == 1 ? 1 : 0;  // mix plans: four-wave workgroups (two slices each)
     if (v.d_tiles.p) {
       CXA_HIP(hipFree(v.d_tiles.p));
       v.d_tiles.p = nullptr;
@@ -116,9 +116,12 @@ int build_variant(cxa_bc_plan *p, int prec) {
     return upload(v.d_tiles, v.tiles);
   }
   v.tile_class = make_tiles(p->blks, p->segs, v.tiles, &v.per_xcd, v.kind, real);
-  v.splitk = real && v.tile_class != 2 &&
-             (v.split_mode == 1 ||
-              (v.split_mode < 0 && prefer_splitk(p->blks, p->segs, v.tiles, v.tile_class)));
+  const bool split = real && v.tile_class != 2 &&
+                     (v.split_mode >= 1 || (v.split_mode < 0 && prefer_splitk(p->blks, p->segs,
+                                                                              v.tiles,
+                                                                              v.tile_class)));
+  // eight-wave split-K only for MFMA tiles (row tiles always split over four waves)
+  v.splitk = !split ? 0 : (v.split_mode == 2 && v.tile_class == 0 ? 8 : 4);
   if (v.d_tiles.p) {
     CXA_HIP(hipFree(v.d_tiles.p));
     v.d_tiles.p = nullptr;
@@ -441,9 +444,11 @@ extern "C" int cxa_block_contract_set_config(cxa_bc_plan *p, int prec, int kind,
   CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_set_config: null plan");
   CXA_CHECK(prec >= -1 && prec <= CXA_PREC_FP32X3, CXA_ERR_INVALID, "unknown precision %d", prec);
   CXA_CHECK(kind >= -1 && kind < TK_COUNT, CXA_ERR_INVALID, "unknown tile kind %d", kind);
-  CXA_CHECK(split >= -1 && split <= 1, CXA_ERR_INVALID, "split mode must be -1, 0 or 1");
+  CXA_CHECK(split >= -1 && split <= 2, CXA_ERR_INVALID, "split mode must be -1, 0, 1 or 2");
   CXA_CHECK(!p->mix || kind == -1 || mix_kind_ok(kind), CXA_ERR_UNSUPPORTED,
             "mix plans run 16x16, 32x16 or 16x32 tiles (kind %d)", kind);
+  CXA_CHECK(!p->mix || split <= 1, CXA_ERR_UNSUPPORTED,
+            "mix plans run eight- or four-wave workgroups (split 0 or 1)");
   for (int q = 0; q < 3; ++q) {
     if (prec >= 0 && q != prec) continue;
     if (p->var[q].kind == kind && p->var[q].split_mode == split && p->var[q].d_tiles.p) continue;
@@ -461,7 +466,7 @@ extern "C" int cxa_block_contract_get_config(const cxa_bc_plan *p, int prec, int
             "cxa_block_contract_get_config: bad arguments");
   const PlanVariant &v = p->var[prec];
   if (kind) *kind = v.kind;
-  if (split) *split = v.splitk ? 1 : 0;
+  if (split) *split = v.splitk == 8 ? 2 : (v.splitk ? 1 : 0);
   if (ntiles) {
     int n = 0;
     for (const GTile &t : v.tiles) n += t.blk >= 0;
@@ -541,7 +546,7 @@ int bc_dtype(const cxa_bc_plan *p) { return p->dtype; }
 
 extern "C" int cxa_block_contract_set_split(cxa_bc_plan *p, int mode) {
   CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_set_split: null plan");
-  CXA_CHECK(mode >= -1 && mode <= 1, CXA_ERR_INVALID, "split mode must be -1, 0 or 1");
+  CXA_CHECK(mode >= -1 && mode <= 2, CXA_ERR_INVALID, "split mode must be -1, 0, 1 or 2");
   for (int q = 0; q < 3; ++q) {
     p->var[q].split_mode = mode;
     int rc = build_variant(p, q);

@@ -281,8 +281,10 @@ __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
       }
 }
 
-// Split-K tile: every wave of the workgroup calls this for the same tile.
-template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, typename SEGS>
+// Split-K tile: every wave of the workgroup (WAVES of them) calls this for the same tile; wave
+// w takes the K chunks c with c % WAVES == w, and the partial tiles are summed in wave order.
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int WAVES,
+          typename SEGS>
 __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
                                              const char *__restrict__ Bbase,
                                              char *__restrict__ Cbase, const GBlk &blk,
@@ -291,7 +293,7 @@ __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
   RealAcc<PREC, MI, NI> acc;
   acc.zero();
   mfma_accumulate<PREC, TA, TB, MI, NI>(acc, Abase, Bbase, blk, segs, m0, n0, lane, wave,
-                                        kWaves);
+                                        WAVES);
 #pragma unroll
   for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -312,8 +314,9 @@ __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
           const int row = 16 * i + acc_row(RealAcc<PREC, MI, NI>::F64, g, v);
           const int col = 16 * jj + r;
           const int idx = (i * NI + jj) * 4 + v;
-          const double s = ((red[0][idx][lane] + red[1][idx][lane]) + red[2][idx][lane]) +
-                           red[3][idx][lane];
+          double s = red[0][idx][lane];
+#pragma unroll
+          for (int q = 1; q < WAVES; ++q) s += red[q][idx][lane];
           if (row < mt && col < nt)
             store_c(Cp + (int64_t)(m0 + row) * blk.ldc + (n0 + col), s, blk);
         }
@@ -585,11 +588,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict
 
 // grouped launch with workgroup split-K: one workgroup per tile (real dtypes; the mixed MFMA
 // class or the row class)
-template <int PREC, typename TA, typename TB, typename TC, int CLS>
-__global__ __launch_bounds__(64 * kWaves) void k_grouped_split(
+template <int PREC, typename TA, typename TB, typename TC, int CLS, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_grouped_split(
     const GTile *__restrict__ tiles, int per_xcd, const GBlk *__restrict__ blks,
     const GSeg *__restrict__ segs, const char *A, const char *B, char *C) {
-  __shared__ double red[kWaves][16][64];
+  static_assert(CLS != TC_ROWS || WAVES == kWaves, "row tiles split over four waves");
+  __shared__ double red[WAVES][16][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const GTile *range = tiles + (int64_t)(blockIdx.x % kXcds) * per_xcd;
@@ -604,16 +608,20 @@ __global__ __launch_bounds__(64 * kWaves) void k_grouped_split(
     } else {
       switch (tl.kind) {
         case TK_1x1:
-          wg_tile_real<PREC, TA, TB, TC, 1, 1>(A, B, C, blk, sa, tl.m0, tl.n0, lane, wave, red);
+          wg_tile_real<PREC, TA, TB, TC, 1, 1, WAVES>(A, B, C, blk, sa, tl.m0, tl.n0, lane, wave,
+                                                     red);
           break;
         case TK_2x1:
-          wg_tile_real<PREC, TA, TB, TC, 2, 1>(A, B, C, blk, sa, tl.m0, tl.n0, lane, wave, red);
+          wg_tile_real<PREC, TA, TB, TC, 2, 1, WAVES>(A, B, C, blk, sa, tl.m0, tl.n0, lane, wave,
+                                                     red);
           break;
         case TK_1x2:
-          wg_tile_real<PREC, TA, TB, TC, 1, 2>(A, B, C, blk, sa, tl.m0, tl.n0, lane, wave, red);
+          wg_tile_real<PREC, TA, TB, TC, 1, 2, WAVES>(A, B, C, blk, sa, tl.m0, tl.n0, lane, wave,
+                                                     red);
           break;
         default:
-          wg_tile_real<PREC, TA, TB, TC, 2, 2>(A, B, C, blk, sa, tl.m0, tl.n0, lane, wave, red);
+          wg_tile_real<PREC, TA, TB, TC, 2, 2, WAVES>(A, B, C, blk, sa, tl.m0, tl.n0, lane, wave,
+                                                     red);
           break;
       }
     }
@@ -867,11 +875,15 @@ int launch_t(const GemmLaunch &g, hipStream_t s) {
   const int wave_grid = g.grid > 0 ? g.grid : kXcds * std::min((P + kWaves - 1) / kWaves, 1024);
   if constexpr (!CPLX) {
     if (g.splitk && cls == TC_ROWS)
-      hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_ROWS>), dim3(wg_grid),
+      hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_ROWS, kWaves>), dim3(wg_grid),
                          dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
                          (const char *)g.A, (const char *)g.B, (char *)g.C);
+    else if (g.splitk == 8 && cls == TC_MIXED)
+      hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED, 8>), dim3(wg_grid),
+                         dim3(64 * 8), 0, s, g.tiles, P, g.blks, g.segs, (const char *)g.A,
+                         (const char *)g.B, (char *)g.C);
     else if (g.splitk && cls == TC_MIXED)
-      hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED>), dim3(wg_grid),
+      hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED, kWaves>), dim3(wg_grid),
                          dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
                          (const char *)g.A, (const char *)g.B, (char *)g.C);
     else if (cls == TC_ROWS)

@@ -73,7 +73,9 @@ struct GemmLaunch {
   int ntiles = 0;   // = kXcds * per_xcd
   int per_xcd = 0;  // tiles per XCD range (make_tiles)
   int grid = 0;     // workgroups (a multiple of kXcds); 0 = pick from per_xcd
-  bool splitk = false;  // one workgroup per tile, K chunks interleaved over its waves
+  // 0: one wave per tile; 4 or 8: one workgroup of that many waves per tile, K chunks
+  // interleaved over its waves (mix launches: nonzero = four-wave workgroups)
+  int splitk = 0;
   int tile_class = 0;   // the class make_tiles returned
   const MixOut *outs = nullptr;  // non-null: mix launch (blks are items, segs are slices)
   int conj = 0;  // complex dtypes: bit 0 conjugates A, bit 1 conjugates B

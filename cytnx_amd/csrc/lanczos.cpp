@@ -283,10 +283,11 @@ extern "C" int cxa_chain_tune(cxa_chain *c, const void *x, void *y, int reps, in
   CXA_CHECK(c->prec != CXA_PREC_FP32 || !c->fixed32.empty(), CXA_ERR_INVALID,
             "cxa_chain_tune: FP32 copies missing; call cxa_chain_prepare first");
   hipStream_t s = (hipStream_t)stream;
-  static const int cand[][2] = {{-1, -1}, {-1, 0}, {-1, 1}, {TK_1x1, 0}, {TK_1x1, 1},
-                                {TK_2x1, 0}, {TK_2x1, 1}, {TK_1x2, 0}, {TK_1x2, 1},
-                                {TK_2x2, 0}, {TK_2x2, 1}, {TK_4x2, 0}, {TK_ROWS, 0},
-                                {TK_ROWS, 1}};
+  static const int cand[][2] = {{-1, -1},    {-1, 0},     {-1, 1},     {-1, 2},
+                                {TK_1x1, 0}, {TK_1x1, 1}, {TK_1x1, 2}, {TK_2x1, 0},
+                                {TK_2x1, 1}, {TK_2x1, 2}, {TK_1x2, 0}, {TK_1x2, 1},
+                                {TK_1x2, 2}, {TK_2x2, 0}, {TK_2x2, 1}, {TK_2x2, 2},
+                                {TK_4x2, 0}, {TK_ROWS, 0}, {TK_ROWS, 1}};
   const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
   hipEvent_t e0, e1;
   CXA_HIP(hipEventCreate(&e0));
@@ -299,7 +300,8 @@ extern "C" int cxa_chain_tune(cxa_chain *c, const void *x, void *y, int reps, in
     for (int ci = 0; ci < ncand && rc == CXA_OK; ++ci) {
       // mix plans: explicit 16x16, 32x16 or 16x32 tiles; split 0 / 1 = eight / four waves
       if (bc_is_mix(plan) && (cand[ci][0] < 0 || cand[ci][0] == TK_2x2 ||
-                              cand[ci][0] == TK_4x2 || cand[ci][0] == TK_ROWS))
+                              cand[ci][0] == TK_4x2 || cand[ci][0] == TK_ROWS ||
+                              cand[ci][1] == 2))
         continue;
       if ((rc = cxa_block_contract_set_config(plan, c->prec, cand[ci][0], cand[ci][1]))) break;
       for (int w = 0; w < 2 && rc == CXA_OK; ++w) rc = chain_step(c, k, x, y, s);

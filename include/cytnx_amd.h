@@ -157,7 +157,8 @@ int cxa_block_contract_info(const cxa_bc_plan *plan, double *flops, int64_t *wor
                             int64_t *npairs);
 /* Override the tile shape of every tile (-1 = per-block default: MFMA 16x16..32x32 tiles,
  * row streaming for blocks with N <= 8) and the workgroup split-K mode (-1 automatic, 0 off,
- * 1 on); used by the tuner. */
+ * 1 on with four waves per tile, 2 on with eight waves per tile for MFMA tiles); used by the
+ * tuner.  get_config reports the split mode in the same 0 / 1 / 2 form. */
 int cxa_block_contract_set_tile(cxa_bc_plan *plan, int kind);
 int cxa_block_contract_set_split(cxa_bc_plan *plan, int mode);
 /* The same for one precision mode only (prec = -1: all); get returns the chosen shape, whether

@@ -125,6 +125,23 @@ def test_heff2_mix_every_tile_kind_matches_oracle(kind, split):
     assert relerr(out._storage.cpu().numpy(), ref.flat()) <= 1e-12
 
 
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 5e-6)])
+def test_heff2_R_step_every_tiling_matches_oracle(prec, tol):
+    """The grouped GEMM of the .R launch with every MFMA tile shape and split-K mode (none,
+    four-wave, eight-wave workgroups) and the 64x32 shape, at chi=256."""
+    psi, L, M1, M2, R = problem(256, seed=29)
+    H = Heff2(psi, L, M1, M2, R)
+    H.set_precision(prec)
+    ref = oracle_heff2(psi, L, M1, M2, R).flat()
+    cases = [(k, s) for k in (0, 1, 2, 3) for s in (0, 1, 2)] + [(4, 0)]
+    for kind, split in cases:
+        H.set_tiles([{"kind": -1, "split": -1}, {"kind": kind, "split": split}])
+        out = H.matvec(psi)
+        err = relerr(out._storage.cpu().numpy(), ref)
+        print("R step kind", kind, "split", split, "prec", prec, "err", err, flush=True)
+        assert err <= tol
+
+
 @pytest.mark.parametrize("prec,tol", [(1, 5e-6), (2, 5e-6)])
 def test_heff2_reduced_precision_error_levels(prec, tol):
     # chi=384: sectors up to 31 wide, so the MFMA tiles (not only row streaming) run
