@@ -224,3 +224,111 @@ def read_unitensor(path: str) -> SavedUniTensor:
     if c.pos != len(c.buf):
         raise ValueError("trailing bytes in Cytnx file")
     return ut
+
+
+# ------------------------------------------------------------------------------------------
+# Writer: the inverse of the reader, following the reference's _Save chain (UniTensor::_Save,
+# src/UniTensor.cpp:68-110; BlockUniTensor::_save_dispatch, src/BlockUniTensor.cpp:1618-1635;
+# Bond::_Save, src/Bond.cpp:500-548; Tensor::_Save, src/Tensor.cpp:472-493; Storage::_Save,
+# src/backend/Storage.cpp:161-193).  Tensors are written contiguous (identity mapper) and every
+# Storage as a host storage (device -1), so the file loads on any build.
+# ------------------------------------------------------------------------------------------
+DTYPE_OF_NP = {np.dtype(v): k for k, v in NP_DTYPE.items()}
+
+
+def _w(out: bytearray, fmt: str, *vals) -> None:
+    out += struct.pack("<" + fmt, *vals)
+
+
+def _write_storage(out: bytearray, data: np.ndarray) -> None:
+    flat = np.ascontiguousarray(data).reshape(-1)
+    dtype = DTYPE_OF_NP[flat.dtype]
+    _w(out, "IQIi", 999, flat.size, dtype, -1)
+    out += flat.astype(flat.dtype.newbyteorder("<"), copy=False).tobytes()
+
+
+def _write_tensor(out: bytearray, arr: np.ndarray, legacy: bool = False) -> None:
+    arr = np.ascontiguousarray(arr)
+    rank = arr.ndim
+    if legacy:
+        _w(out, "IQQ", 888, rank, 1)
+    else:
+        _w(out, "IIQQ", 889, 1, rank, 1)
+    if rank:
+        ident = list(range(rank))
+        _w(out, "%dQ" % rank, *[int(s) for s in arr.shape])
+        _w(out, "%dQ" % rank, *ident)
+        _w(out, "%dQ" % rank, *ident)
+    _write_storage(out, arr)
+
+
+def _write_symmetry(out: bytearray, s: SavedSymmetry) -> None:
+    _w(out, "Iii", 777, s.stype, s.n)
+
+
+def _write_bond(out: bytearray, b: SavedBond) -> None:
+    _w(out, "I", 666)
+    ver = 1 if b.degs else 0
+    nsym = len(b.syms)
+    _w(out, "iQiQ", ver, len(b.qnums) if ver else b.dim, b.btype, nsym)
+    for q in b.qnums:
+        _w(out, "%dq" % nsym, *q)
+    if ver:
+        _w(out, "%dQ" % len(b.degs), *b.degs)
+    for s in b.syms:
+        _write_symmetry(out, s)
+
+
+def unitensor_bytes(ut: SavedUniTensor, legacy: bool = False) -> bytes:
+    """The file image of `ut` (blocks given as logical row-major arrays in ut.blocks).
+
+    The default is the versioned format the reference writes today (UniTensor magic 556 and
+    Tensor magic 889, each followed by format version 1).  `legacy=True` writes the unversioned
+    format (magics 555 / 888, no version words) that older Cytnx releases write and read; the
+    reference's loader accepts both (src/UniTensor.cpp:112-125, src/Tensor.cpp:516-530)."""
+    out = bytearray()
+    if legacy:
+        _w(out, "I", 555)
+    else:
+        _w(out, "II", 556, 1)
+    _w(out, "i???q", ut.uten_type, ut.is_braket_form, ut.is_tag, ut.is_diag, ut.rowrank)
+    name = ut.name.encode()
+    _w(out, "I", len(name))
+    out += name
+    labels = [l.encode() for l in ut.labels]
+    _w(out, "Q", len(labels))
+    for l in labels:
+        _w(out, "Q", len(l))
+    for l in labels:
+        out += l
+    for b in ut.bonds:
+        _write_bond(out, b)
+    if ut.uten_type == UTEN_DENSE:
+        _write_tensor(out, ut.blocks[0], legacy)
+    elif ut.uten_type == UTEN_BLOCK:
+        if not ut.labels and not legacy:
+            syms = ut.syms_cache or []
+            _w(out, "Q", len(syms))
+            for s in syms:
+                _write_symmetry(out, s)
+        _w(out, "Q", len(ut.blocks))
+        for t in ut.itoi:
+            if t:
+                _w(out, "%dQ" % len(t), *t)
+        for blk in ut.blocks:
+            _write_tensor(out, blk, legacy)
+    else:
+        raise ValueError("only dense and block UniTensors are written")
+    return bytes(out)
+
+
+def write_unitensor(path: str, ut: SavedUniTensor, legacy: bool = False) -> None:
+    with open(path, "wb") as f:
+        f.write(unitensor_bytes(ut, legacy))
+
+
+def tensor_bytes(arr: np.ndarray, legacy: bool = False) -> bytes:
+    """The .cytn image of a logical row-major array (Tensor::_Save, src/Tensor.cpp:472-493)."""
+    out = bytearray()
+    _write_tensor(out, arr, legacy)
+    return bytes(out)

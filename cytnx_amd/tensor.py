@@ -216,6 +216,23 @@ class Tensor:
         t = self.contiguous()
         return t._storage.reshape(t._shape)
 
+    # ---------------------------------------------------------------- files
+    def Save(self, fname: str) -> None:
+        """Tensor::Save (src/Tensor.cpp:452-470): '.cytn' is appended to a name without an
+        extension; the tensor is written contiguous."""
+        import os
+        from . import fileio
+        if not os.path.splitext(fname)[1]:
+            fname = fname + ".cytn"
+        with open(fname, "wb") as f:
+            f.write(fileio.tensor_bytes(self.numpy()))
+
+    @classmethod
+    def Load(cls, fname: str, device: int = CPU) -> "Tensor":
+        """Tensor::Load: the stored (possibly lazily permuted) tensor, contiguous."""
+        from . import fileio
+        return cls.from_numpy(fileio.read_tensor(fname).logical(), device)
+
     def __repr__(self) -> str:
         return "Tensor(shape=%s, dtype=%d, device=%d, contiguous=%s)" % (
             self._shape, self.dtype(), self.device(), self._contiguous)

@@ -417,14 +417,14 @@ class UniTensor:
     def from_saved(cls, ut: fileio.SavedUniTensor, device: int = CPU) -> "UniTensor":
         if ut.uten_type == fileio.UTEN_DENSE:
             bonds = [Bond(b.dim) for b in ut.bonds]
-            out = cls(bonds, ut.labels, ut.rowrank, ut.blocks[0].dtype, device)
+            out = cls(bonds, ut.labels, ut.rowrank, ut.blocks[0].dtype, device, name=ut.name)
             out._block = Tensor.from_numpy(ut.blocks[0].logical(), device)
             return out
         if ut.uten_type != fileio.UTEN_BLOCK:
             raise NotImplementedError("only dense and block UniTensors are supported")
         bonds = [Bond(b.btype, [q[0] for q in b.qnums], b.degs) for b in ut.bonds]
         dtype = ut.blocks[0].dtype if ut.blocks else Type_F64
-        out = cls(bonds, ut.labels, ut.rowrank, dtype, device)
+        out = cls(bonds, ut.labels, ut.rowrank, dtype, device, name=ut.name)
         want = {tuple(int(x) for x in t): i for i, t in enumerate(out._itoi)}
         for t, blk in zip(ut.itoi, ut.blocks):
             out.put_block(blk.logical(), want[tuple(t)])
@@ -433,6 +433,45 @@ class UniTensor:
     @classmethod
     def Load(cls, path: str, device: int = CPU) -> "UniTensor":
         return cls.from_saved(fileio.read_unitensor(path), device)
+
+    def _is_braket_form(self) -> bool:
+        """UniTensor_base::_update_braket (include/UniTensor.hpp:70-86)."""
+        if not self._bonds or self._bonds[0].type() == BD_REG:
+            return False
+        return all(b.type() == (BD_KET if i < self._rowrank else BD_BRA)
+                   for i, b in enumerate(self._bonds))
+
+    def to_saved(self) -> fileio.SavedUniTensor:
+        """The file content of this UniTensor (blocks in their stored order)."""
+        u1 = [fileio.SavedSymmetry(fileio.SYM_U1, 1)]
+        if self._is_block:
+            bonds = [fileio.SavedBond(b.type(), [[q] for q in b.qnums()], b.getDegeneracies(),
+                                      list(u1), b.dim()) for b in self._bonds]
+            ut = fileio.SavedUniTensor(fileio.UTEN_BLOCK, self._is_braket_form(), True,
+                                       self.is_diag(), self._rowrank, self._name,
+                                       list(self._labels), bonds)
+            ut.itoi = self._itoi.astype(np.int64).tolist() if self.rank() else \
+                [[] for _ in range(self.Nblocks())]
+            ut.blocks = [self.get_block_(i).detach().cpu().numpy()
+                         for i in range(self.Nblocks())]
+            if not self._bonds:
+                ut.syms_cache = list(u1)
+            return ut
+        bonds = [fileio.SavedBond(b.type(), [], [], [], b.dim()) for b in self._bonds]
+        tag = bool(self._bonds) and self._bonds[0].type() != BD_REG
+        ut = fileio.SavedUniTensor(fileio.UTEN_DENSE, self._is_braket_form(), tag,
+                                   self.is_diag(), self._rowrank, self._name,
+                                   list(self._labels), bonds)
+        ut.blocks = [self._block.numpy()]
+        return ut
+
+    def Save(self, fname: str) -> None:
+        """UniTensor::Save (src/UniTensor.cpp:169-187): '.cytnx' is appended to a name without
+        an extension."""
+        import os
+        if not os.path.splitext(fname)[1]:
+            fname = fname + ".cytnx"
+        fileio.write_unitensor(fname, self.to_saved())
 
     # ------------------------------------------------------------------ permute
     def _resolve_mapper(self, mapper) -> List[int]:
