@@ -26,6 +26,14 @@
 #include "gemm.hpp"
 
 namespace cxa {
+
+// Sixteen zero bytes in device memory (zero-initialised, never written).  An operand element
+// outside its block is read from here (the address is selected, not the value), so fragment
+// loads need no branch and no wait at the load site; the MFMA that consumes them sees exact
+// zeros.  External linkage keeps the compiler from folding these loads to constants, which
+// would turn the selects back into branches around the loads.
+__device__ __align__(16) double gemm_zero_cell[2];
+
 namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -55,13 +63,43 @@ __device__ __forceinline__ int acc_row(bool f64, int g, int v) {
   return f64 ? g + 4 * v : 4 * g + v;
 }
 
+// Wait for every outstanding vector-memory load (s_waitcnt vmcnt(0); expcnt and lgkmcnt left
+// at their maxima).  Issued where a load pipeline ends, when every load it issued has already
+// been consumed, so it costs nothing at run time; it tells the compiler's wait analysis that no
+// fragment load is still in flight, which it cannot prove by itself across the ring's
+// conditional prefetches, and which it would otherwise answer with waits on the next tile's
+// first loads.
+__device__ __forceinline__ void drain_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Operand and result pointers live in the global address space: loads through them are
+// global_load (counted by vmcnt alone), where generic pointers would give flat loads that also
+// hold lgkmcnt, so every wait for a scalar descriptor load would drain the operand pipeline.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+
 // base + element offset computed on integers: a batched call passes absolute addresses (in
 // elements) as offsets from a null base, which pointer arithmetic may not express.  Offsets in
 // elements let one task list serve FP64 and FP32 copies of the same block layout.
 template <typename T, typename B>
-__device__ __forceinline__ T *at(B *base, int64_t elem_off) {
-  return reinterpret_cast<T *>(reinterpret_cast<uintptr_t>(base) +
-                               (uintptr_t)elem_off * (uintptr_t)sizeof(T));
+__device__ __forceinline__ gptr<T> at(B *base, int64_t elem_off) {
+  return (gptr<T>)(reinterpret_cast<uintptr_t>(base) +
+                   (uintptr_t)elem_off * (uintptr_t)sizeof(T));
+}
+
+template <typename T>
+__device__ __forceinline__ gptr<const T> zero_cell() {
+  static_assert(sizeof(T) <= sizeof(gemm_zero_cell), "zero cell too small");
+  return (gptr<const T>)(&gemm_zero_cell[0]);
+}
+
+// *p when `valid`, else 0, as one unconditional load.  The selected address passes through an
+// empty asm operand so that the compiler cannot split the load into "valid ? *p : 0" (a branch
+// around the load, which costs the load pipeline its in-order wait counts).
+template <typename T>
+__device__ __forceinline__ T load_or_zero(gptr<const T> p, bool valid) {
+  uint64_t a = reinterpret_cast<uint64_t>(valid ? p : zero_cell<T>());
+  __asm__("" : "+v"(a));
+  return *reinterpret_cast<gptr<const T>>(a);
 }
 
 struct SegArray {
@@ -99,29 +137,28 @@ struct RealAcc {
 };
 
 // Operand fragments of one 16-deep K chunk (per lane: 4 k values per 16-row / 16-column
-// group), split into hi/lo FP32 parts in the FP32X3 mode.
-template <int PREC, int MI, int NI>
+// group), held as loaded (operand dtype).  Rounding to FP32 and the FP32X3 hi/lo split happen
+// when the chunk is consumed, so the loads of the chunks in flight are not waited for early.
+template <typename TA, typename TB, int MI, int NI>
 struct Frag {
-  using op_t = typename std::conditional<PREC == CXA_PREC_FP64, double, float>::type;
-  op_t a[MI][4], b[NI][4], alo[MI][4], blo[NI][4];
+  TA a[MI][4];
+  TB b[NI][4];
 };
 
-template <int PREC, typename TA, typename TB, int MI, int NI>
-__device__ __forceinline__ void load_frag(Frag<PREC, MI, NI> &f, const TA *Ap, const TB *Bp,
-                                          const GSeg &sg, int k0, int mt, int nt, int r, int g) {
-  constexpr bool X3 = (PREC == CXA_PREC_FP32X3);
-  using op_t = typename Frag<PREC, MI, NI>::op_t;
+// `live` is false once the cursor has passed the end of its K range: every element then reads
+// the zero cell (the pipelines load only chunks inside the range, so this is a guard).
+template <typename TA, typename TB, int MI, int NI>
+__device__ __forceinline__ void load_frag(Frag<TA, TB, MI, NI> &f, gptr<const TA> Ap,
+                                          gptr<const TB> Bp, const GSeg &sg, int k0, int mt,
+                                          int nt, int r, int g, bool live) {
 #pragma unroll
   for (int i = 0; i < MI; ++i) {
     const int row = 16 * i + r;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int k = k0 + 4 * g + j;
-      const double v = (row < mt && k < sg.K)
-                           ? (double)Ap[(int64_t)row * sg.sam + (int64_t)k * sg.sak]
-                           : 0.0;
-      f.a[i][j] = (op_t)v;
-      if constexpr (X3) f.alo[i][j] = (float)(v - (double)(float)v);
+      f.a[i][j] = load_or_zero<TA>(Ap + ((int64_t)row * sg.sam + (int64_t)k * sg.sak),
+                                   live && row < mt && k < sg.K);
     }
   }
 #pragma unroll
@@ -130,34 +167,46 @@ __device__ __forceinline__ void load_frag(Frag<PREC, MI, NI> &f, const TA *Ap, c
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int k = k0 + 4 * g + j;
-      const double v = (col < nt && k < sg.K)
-                           ? (double)Bp[(int64_t)k * sg.sbk + (int64_t)col * sg.sbn]
-                           : 0.0;
-      f.b[i][j] = (op_t)v;
-      if constexpr (X3) f.blo[i][j] = (float)(v - (double)(float)v);
+      f.b[i][j] = load_or_zero<TB>(Bp + ((int64_t)k * sg.sbk + (int64_t)col * sg.sbn),
+                                   live && col < nt && k < sg.K);
     }
   }
 }
 
-template <int PREC, int MI, int NI>
+template <int PREC, typename TA, typename TB, int MI, int NI>
 __device__ __forceinline__ void mma_frag(RealAcc<PREC, MI, NI> &acc,
-                                         const Frag<PREC, MI, NI> &f) {
+                                         const Frag<TA, TB, MI, NI> &f) {
+  constexpr bool X3 = (PREC == CXA_PREC_FP32X3);
+  using op_t = typename std::conditional<PREC == CXA_PREC_FP64, double, float>::type;
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
+  for (int j = 0; j < 4; ++j) {
+    op_t a[MI], b[NI];
+    float alo[MI], blo[NI];
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+      a[i] = (op_t)f.a[i][j];
+      if constexpr (X3) alo[i] = (float)((double)f.a[i][j] - (double)a[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      b[i] = (op_t)f.b[i][j];
+      if constexpr (X3) blo[i] = (float)((double)f.b[i][j] - (double)b[i]);
+    }
 #pragma unroll
     for (int i = 0; i < MI; ++i)
 #pragma unroll
       for (int jj = 0; jj < NI; ++jj) {
         if constexpr (PREC == CXA_PREC_FP64) {
-          acc.hi[i][jj] = mma64(f.a[i][j], f.b[jj][j], acc.hi[i][jj]);
+          acc.hi[i][jj] = mma64(a[i], b[jj], acc.hi[i][jj]);
         } else {
-          acc.hi[i][jj] = mma32(f.a[i][j], f.b[jj][j], acc.hi[i][jj]);
-          if constexpr (PREC == CXA_PREC_FP32X3) {
-            acc.lo[i][jj] = mma32(f.a[i][j], f.blo[jj][j], acc.lo[i][jj]);
-            acc.lo[i][jj] = mma32(f.alo[i][j], f.b[jj][j], acc.lo[i][jj]);
+          acc.hi[i][jj] = mma32(a[i], b[jj], acc.hi[i][jj]);
+          if constexpr (X3) {
+            acc.lo[i][jj] = mma32(a[i], blo[jj], acc.lo[i][jj]);
+            acc.lo[i][jj] = mma32(alo[i], b[jj], acc.lo[i][jj]);
           }
         }
       }
+  }
 }
 
 // Chunks in flight per wave: enough loads outstanding to cover an L2/MALL round trip with a
@@ -176,14 +225,21 @@ struct ChunkCursor {
   int seg0, nseg, nslices, m0, n0;
   int s, k0;  // current segment and chunk start within it
   GSeg sg;
-  const TA *Ap;
-  const TB *Bp;
+  gptr<const TA> Ap;
+  gptr<const TB> Bp;
   __device__ __forceinline__ void enter(const char *Abase, const char *Bbase) {
     sg = segs(seg0 + s);
     Ap = at<const TA>(Abase, sg.a_off) + (int64_t)m0 * sg.sam;
     Bp = at<const TB>(Bbase, sg.b_off) + (int64_t)n0 * sg.sbn;
   }
-  // skip `nchunk` chunks forward (crossing segment ends)
+  __device__ __forceinline__ bool live() const { return s < nseg; }
+  template <int MI, int NI>
+  __device__ __forceinline__ void load(Frag<TA, TB, MI, NI> &f, int mt, int nt, int r,
+                                       int g) const {
+    load_frag<TA, TB, MI, NI>(f, Ap, Bp, sg, k0, mt, nt, r, g, live());
+  }
+  // skip `nchunk` chunks forward (crossing segment ends); past the last segment the cursor
+  // stays exhausted (live() false)
   __device__ __forceinline__ void skip(int nchunk, const char *Abase, const char *Bbase) {
     k0 += 16 * nchunk;
     bool moved = false;
@@ -205,6 +261,9 @@ struct ChunkCursor {
 // A ring of pipe_depth() fragment sets keeps that many chunks' loads in flight across the
 // whole range (segment boundaries included): the MFMAs of chunk c issue while chunks
 // c+1 .. c+depth-1 are still arriving, and chunk c+depth is loaded into the set just consumed.
+// Only real chunks are loaded: the launches are bound by the vector-memory address path, so
+// padding a wave's chunk count to whole rounds of the ring (which would let every wait count
+// be exact) costs more than the waits it saves.
 template <int PREC, typename TA, typename TB, int MI, int NI, typename SEGS>
 __device__ __forceinline__ void mfma_accumulate(RealAcc<PREC, MI, NI> &acc,
                                                 const char *__restrict__ Abase,
@@ -227,11 +286,11 @@ __device__ __forceinline__ void mfma_accumulate(RealAcc<PREC, MI, NI> &acc,
   ChunkCursor<TA, TB, SEGS> cur{segs, blk.seg0, blk.nseg, nslices, m0, n0, 0, 0};
   cur.enter(Abase, Bbase);
   cur.skip(slice, Abase, Bbase);  // also steps over empty leading segments
-  Frag<PREC, MI, NI> f[D];
+  Frag<TA, TB, MI, NI> f[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     if (d < total) {
-      load_frag<PREC, TA, TB, MI, NI>(f[d], cur.Ap, cur.Bp, cur.sg, cur.k0, mt, nt, r, g);
+      cur.load(f[d], mt, nt, r, g);
       cur.skip(nslices, Abase, Bbase);
     }
   }
@@ -240,18 +299,19 @@ __device__ __forceinline__ void mfma_accumulate(RealAcc<PREC, MI, NI> &acc,
     for (int d = 0; d < D; ++d) {
       const int c = base + d;
       if (c < total) {
-        mma_frag<PREC, MI, NI>(acc, f[d]);
+        mma_frag<PREC, TA, TB, MI, NI>(acc, f[d]);
         if (c + D < total) {
-          load_frag<PREC, TA, TB, MI, NI>(f[d], cur.Ap, cur.Bp, cur.sg, cur.k0, mt, nt, r, g);
+          cur.load(f[d], mt, nt, r, g);
           cur.skip(nslices, Abase, Bbase);
         }
       }
     }
   }
+  drain_loads();
 }
 
 template <typename TC>
-__device__ __forceinline__ void store_c(TC *dst, double val, const GBlk &blk) {
+__device__ __forceinline__ void store_c(gptr<TC> dst, double val, const GBlk &blk) {
   val *= blk.alpha;
   if (blk.beta != 0.0) val += blk.beta * (double)(*dst);
   *dst = (TC)val;
@@ -267,7 +327,7 @@ __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
   mfma_accumulate<PREC, TA, TB, MI, NI>(acc, Abase, Bbase, blk, segs, m0, n0, lane, 0, 1);
   const int r = lane & 15, g = lane >> 4;
   const int mt = min(16 * MI, blk.M - m0), nt = min(16 * NI, blk.N - n0);
-  TC *Cp = at<TC>(Cbase, blk.c_off);
+  gptr<TC> Cp = at<TC>(Cbase, blk.c_off);
 #pragma unroll
   for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -304,7 +364,7 @@ __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
   if (wave == 0) {
     const int r = lane & 15, g = lane >> 4;
     const int mt = min(16 * MI, blk.M - m0), nt = min(16 * NI, blk.N - n0);
-    TC *Cp = at<TC>(Cbase, blk.c_off);
+    gptr<TC> Cp = at<TC>(Cbase, blk.c_off);
 #pragma unroll
     for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -366,12 +426,13 @@ __device__ __forceinline__ void rows_accumulate(double (&acc)[kRowsN], const cha
   lc.sg = segs(blk.seg0);
   lc.skip(slice);  // also steps over empty leading segments
   RowCursor<SEGS> cc = lc;
-  double a[kRowsDepth];
+  TA a[kRowsDepth];  // as loaded; widened to FP64 where consumed
 #pragma unroll
   for (int d = 0; d < kRowsDepth; ++d) {
     if (d < total) {
-      const TA *Ap = at<const TA>(Abase, lc.sg.a_off);
-      a[d] = valid ? (double)Ap[(int64_t)row * lc.sg.sam + (int64_t)lc.k * lc.sg.sak] : 0.0;
+      a[d] = load_or_zero<TA>(at<const TA>(Abase, lc.sg.a_off) +
+                                  ((int64_t)row * lc.sg.sam + (int64_t)lc.k * lc.sg.sak),
+                              valid);
       lc.skip(nslices);
     }
   }
@@ -380,19 +441,21 @@ __device__ __forceinline__ void rows_accumulate(double (&acc)[kRowsN], const cha
     for (int d = 0; d < kRowsDepth; ++d) {
       const int c = base + d;
       if (c < total) {
-        const TB *bk = at<const TB>(Bbase, cc.sg.b_off) + (int64_t)cc.k * cc.sg.sbk;
+        gptr<const TB> bk = at<const TB>(Bbase, cc.sg.b_off) + (int64_t)cc.k * cc.sg.sbk;
 #pragma unroll
         for (int n = 0; n < kRowsN; ++n)
-          if (n < N) acc[n] += a[d] * (double)bk[(int64_t)n * cc.sg.sbn];
+          if (n < N) acc[n] += (double)a[d] * (double)bk[(int64_t)n * cc.sg.sbn];
         cc.skip(nslices);
         if (c + kRowsDepth < total) {
-          const TA *Ap = at<const TA>(Abase, lc.sg.a_off);
-          a[d] = valid ? (double)Ap[(int64_t)row * lc.sg.sam + (int64_t)lc.k * lc.sg.sak] : 0.0;
+          a[d] = load_or_zero<TA>(at<const TA>(Abase, lc.sg.a_off) +
+                                      ((int64_t)row * lc.sg.sam + (int64_t)lc.k * lc.sg.sak),
+                                  valid);
           lc.skip(nslices);
         }
       }
     }
   }
+  drain_loads();
 }
 
 template <typename TA, typename TB, typename TC, typename SEGS>
@@ -405,7 +468,7 @@ __device__ __forceinline__ void wave_rows(const char *__restrict__ Abase,
   double acc[kRowsN];
   rows_accumulate<TA, TB>(acc, Abase, Bbase, blk, segs, row, valid, 0, 1);
   if (!valid) return;
-  TC *Cp = at<TC>(Cbase, blk.c_off) + (int64_t)row * blk.ldc;
+  gptr<TC> Cp = at<TC>(Cbase, blk.c_off) + (int64_t)row * blk.ldc;
 #pragma unroll
   for (int n = 0; n < kRowsN; ++n)
     if (n < blk.N) store_c(Cp + n, acc[n], blk);
@@ -425,7 +488,7 @@ __device__ __forceinline__ void wg_rows(const char *__restrict__ Abase,
   for (int n = 0; n < kRowsN; ++n) red[wave][n][lane] = acc[n];
   __syncthreads();
   if (wave == 0 && valid) {
-    TC *Cp = at<TC>(Cbase, blk.c_off) + (int64_t)row * blk.ldc;
+    gptr<TC> Cp = at<TC>(Cbase, blk.c_off) + (int64_t)row * blk.ldc;
 #pragma unroll
     for (int n = 0; n < kRowsN; ++n)
       if (n < blk.N)
@@ -447,6 +510,10 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
   constexpr bool F64 = std::is_same<CT, cd>::value;
   using acc_t = typename std::conditional<F64, f64x4, f32x4>::type;
   using op_t = typename std::conditional<F64, double, float>::type;
+  // complex elements move as two-wide vectors (x = real, y = imaginary; same size and
+  // alignment as CT), which load and store through global-address-space pointers
+  typedef op_t c2 __attribute__((ext_vector_type(2)));
+  static_assert(sizeof(c2) == sizeof(CT), "complex element layout");
   const op_t sa = (conj & 1) ? op_t(-1) : op_t(1), sb = (conj & 2) ? op_t(-1) : op_t(1);
   const int r = lane & 15, g = lane >> 4;
   const int mt = min(16 * MI, blk.M - m0);
@@ -461,21 +528,19 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
     }
   for (int s = 0; s < blk.nseg; ++s) {
     const GSeg sg = segs(blk.seg0 + s);
-    const CT *Ap = at<const CT>(Abase, sg.a_off) + (int64_t)m0 * sg.sam;
-    const CT *Bp = at<const CT>(Bbase, sg.b_off) + (int64_t)n0 * sg.sbn;
+    gptr<const c2> Ap = at<const c2>(Abase, sg.a_off) + (int64_t)m0 * sg.sam;
+    gptr<const c2> Bp = at<const c2>(Bbase, sg.b_off) + (int64_t)n0 * sg.sbn;
     const int K = sg.K;
     for (int k0 = 0; k0 < K; k0 += 16) {
-      op_t ar[MI][4], ai[MI][4], br[NI][4], bi[NI][4];
+      c2 va[MI][4], vb[NI][4];  // every load of the chunk issued before the first use
 #pragma unroll
       for (int i = 0; i < MI; ++i) {
         const int row = 16 * i + r;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int k = k0 + 4 * g + j;
-          CT v{0, 0};
-          if (row < mt && k < K) v = Ap[(int64_t)row * sg.sam + (int64_t)k * sg.sak];
-          ar[i][j] = v.re;
-          ai[i][j] = sa * v.im;
+          va[i][j] = load_or_zero<c2>(Ap + ((int64_t)row * sg.sam + (int64_t)k * sg.sak),
+                                      row < mt && k < K);
         }
       }
 #pragma unroll
@@ -484,12 +549,25 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int k = k0 + 4 * g + j;
-          CT v{0, 0};
-          if (col < nt && k < K) v = Bp[(int64_t)k * sg.sbk + (int64_t)col * sg.sbn];
-          br[i][j] = v.re;
-          bi[i][j] = sb * v.im;
+          vb[i][j] = load_or_zero<c2>(Bp + ((int64_t)k * sg.sbk + (int64_t)col * sg.sbn),
+                                      col < nt && k < K);
         }
       }
+      op_t ar[MI][4], ai[MI][4], br[NI][4], bi[NI][4];
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          ar[i][j] = va[i][j].x;
+          ai[i][j] = sa * va[i][j].y;
+        }
+#pragma unroll
+      for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          br[i][j] = vb[i][j].x;
+          bi[i][j] = sb * vb[i][j].y;
+        }
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -510,7 +588,7 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
           }
     }
   }
-  CT *Cp = at<CT>(Cbase, blk.c_off);
+  gptr<c2> Cp = at<c2>(Cbase, blk.c_off);
   const double alpha = blk.alpha, beta = blk.beta;
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -522,15 +600,15 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
         const int col = 16 * jj + r;
         if (row < mt && col < nt) {
           double re = alpha * (double)accr[i][jj][v], im = alpha * (double)acci[i][jj][v];
-          CT *dst = Cp + (int64_t)(m0 + row) * blk.ldc + (n0 + col);
+          gptr<c2> dst = Cp + (int64_t)(m0 + row) * blk.ldc + (n0 + col);
           if (beta != 0.0) {
-            const CT o = *dst;
-            re += beta * (double)o.re;
-            im += beta * (double)o.im;
+            const c2 o = *dst;
+            re += beta * (double)o.x;
+            im += beta * (double)o.y;
           }
-          CT w;
-          w.re = (decltype(w.re))re;
-          w.im = (decltype(w.im))im;
+          c2 w;
+          w.x = (op_t)re;
+          w.y = (op_t)im;
           *dst = w;
         }
       }
@@ -671,37 +749,42 @@ __device__ __forceinline__ void mfma_accumulate_multi(RealAcc<PREC, MI, NI> (&ac
       cs = q;
       left = nc[q];
     }
-  Frag<PREC, MI, NI> f[D];
+  Frag<TA, TB, MI, NI> f[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     if (d < total) {
-      load_frag<PREC, TA, TB, MI, NI>(f[d], cur.Ap, cur.Bp, cur.sg, cur.k0, mt, nt, r, g);
+      cur.load(f[d], mt, nt, r, g);
       cur.skip(1, Abase, Bbase);
     }
   }
+  // chunk -> accumulator bookkeeping
+  auto consume = [&](const Frag<TA, TB, MI, NI> &fr) {
+#pragma unroll
+    for (int q = 0; q < NACC; ++q)
+      if (q == cs) mma_frag<PREC, TA, TB, MI, NI>(acc[q], fr);
+    if (--left == 0) {
+#pragma unroll
+      for (int q = 1; q < NACC; ++q)
+        if (left == 0 && cs == q - 1) {
+          cs = q;
+          left = nc[q];
+        }
+    }
+  };
   for (int base = 0; base < total; base += D) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const int c = base + d;
       if (c < total) {
-#pragma unroll
-        for (int q = 0; q < NACC; ++q)
-          if (q == cs) mma_frag<PREC, MI, NI>(acc[q], f[d]);
-        if (--left == 0) {
-#pragma unroll
-          for (int q = 1; q < NACC; ++q)
-            if (left == 0 && cs == q - 1) {
-              cs = q;
-              left = nc[q];
-            }
-        }
+        consume(f[d]);
         if (c + D < total) {
-          load_frag<PREC, TA, TB, MI, NI>(f[d], cur.Ap, cur.Bp, cur.sg, cur.k0, mt, nt, r, g);
+          cur.load(f[d], mt, nt, r, g);
           cur.skip(1, Abase, Bbase);
         }
       }
     }
   }
+  drain_loads();
 }
 
 template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int WAVES>
@@ -750,7 +833,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mix(const GTile *__restrict__ ti
     const int mt = min(16 * MI, it.M - tl.m0), nt = min(16 * NI, it.N - tl.n0);
     for (int j = wave; j < it.nout; j += WAVES) {
       const MixOut &o = outs[it.out0 + j];
-      TC *Cp = at<TC>(C, o.c_off);
+      gptr<TC> Cp = at<TC>(C, o.c_off);
 #pragma unroll
       for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -813,6 +896,8 @@ int launch_mix(const GemmLaunch &g, hipStream_t s) {
   if (p == CXA_PREC_FP32) {
     if (a == CXA_F32 && b == CXA_F64 && c == CXA_F32)
       return launch_mix_t<CXA_PREC_FP32, float, double, float>(g, s);
+    if (a == CXA_F64 && b == CXA_F32 && c == CXA_F32)  // psi as the left operand
+      return launch_mix_t<CXA_PREC_FP32, double, float, float>(g, s);
     if (a == CXA_F32 && b == CXA_F32 && c == CXA_F32)
       return launch_mix_t<CXA_PREC_FP32, float, float, float>(g, s);
     if (a == CXA_F32 && b == CXA_F64 && c == CXA_F64)

@@ -118,17 +118,29 @@ def _sector_offsets(b: Bond) -> np.ndarray:
     return np.concatenate([[0], np.cumsum(degs)[:-1]]).astype(np.int64)
 
 
+def psi_is_left(Y: UniTensor, labY: dict) -> bool:
+    """Whether a mix launch writing Y takes psi as its left operand: Y's l' leg comes after its
+    r leg (l' contiguous), so each item computes the transposed tile."""
+    return Y.labels().index(labY["lp"]) > Y.labels().index(labY["r"])
+
+
 def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: np.ndarray,
                     Y: UniTensor, labY: dict, live: Optional[np.ndarray] = None):
     """Task arrays of the mix launch Y = W12 applied to (Lp . psi), never storing Lp . psi.
 
     Lp: (l, w, l') environment blocks; psi: the two-site state (l, s1, s2, r); W12: the dense
-    two-site MPO with axes (w, s1, s1', w'', s2, s2'); Y: the output layout, blocks
-    (l', s1', s2', w'', r) with r contiguous.  labL / labP / labY map the roles (l, w, lp /
-    l, s1, s2, r / lp, s1p, s2p, w2, r) to each tensor's labels.  One item per (l', r) sector
-    pair of Y; its slices are the products  X[l', r] = sum_l Lp[l, w, l'] psi[l, s1, s2, r]
-    for every (w, s1, s2) element that reaches one of the item's outputs, and each output
-    Y[l', s1', s2', w'', r] is the sum over slices of W12[w, s1, s1', w'', s2, s2'] * X.
+    two-site MPO with axes (w, s1, s1', w'', s2, s2'); Y: the output layout, blocks with the
+    legs (l', s1', s2', w'', r) in any order, r or l' contiguous.  labL / labP / labY map the
+    roles (l, w, lp / l, s1, s2, r / lp, s1p, s2p, w2, r) to each tensor's labels.  One item
+    per (l', r) sector pair of Y; its slices are the products
+    X[l', r] = sum_l Lp[l, w, l'] psi[l, s1, s2, r]  for every (w, s1, s2) element that
+    reaches one of the item's outputs, and each output Y[l', s1', s2', w'', r] is the sum over
+    slices of W12[w, s1, s1', w'', s2, s2'] * X.
+
+    The contiguous axis of Y is the launch tile's column axis.  With r contiguous (r after l'
+    in Y's label order) the items compute X[l', r] with A = Lp and B = psi; with l' contiguous
+    (l' after r) they compute the transposed tile X[r, l'] with A = psi and B = Lp, and the
+    launch takes psi as its left operand.
 
     Returns (items (n,4), slices (m,7), outs (k,2), coef (k,MIX_SLICES)) as int64 / float64
     arrays, or raises MixUnsupported.  With live (a bool array of Y.Nblocks()), marks the Y
@@ -136,6 +148,8 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
     pL = {k: Lp.labels().index(labL[k]) for k in ("l", "w", "lp")}
     pP = {k: psi.labels().index(labP[k]) for k in ("l", "s1", "s2", "r")}
     pY = {k: Y.labels().index(labY[k]) for k in ("lp", "s1p", "s2p", "w2", "r")}
+    psi_left = psi_is_left(Y, labY)
+    col = "lp" if psi_left else "r"
     offw = _sector_offsets(Lp.bonds()[pL["w"]])
     offs1 = _sector_offsets(psi.bonds()[pP["s1"]])
     offs2 = _sector_offsets(psi.bonds()[pP["s2"]])
@@ -170,30 +184,37 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
                 for iw in range(int(shl[pL["w"]])):
                     for i1 in range(int(shp[pP["s1"]])):
                         for i2 in range(int(shp[pP["s2"]])):
-                            desc = (int(Lp._offsets[li] + iw * stl[pL["w"]]),
-                                    int(psi._offsets[pi] + i1 * stp[pP["s1"]]
-                                        + i2 * stp[pP["s2"]]),
-                                    int(stl[pL["lp"]]), int(stl[pL["l"]]),
-                                    int(stp[pP["l"]]), int(stp[pP["r"]]), K)
+                            a_l = int(Lp._offsets[li] + iw * stl[pL["w"]])
+                            a_p = int(psi._offsets[pi] + i1 * stp[pP["s1"]]
+                                      + i2 * stp[pP["s2"]])
+                            if psi_left:  # X[r, l'] = psi^T . Lp
+                                desc = (a_p, a_l, int(stp[pP["r"]]), int(stp[pP["l"]]),
+                                        int(stl[pL["l"]]), int(stl[pL["lp"]]), K)
+                                mn = (int(shp[pP["r"]]), int(shl[pL["lp"]]))
+                            else:  # X[l', r] = Lp^T . psi
+                                desc = (a_l, a_p, int(stl[pL["lp"]]), int(stl[pL["l"]]),
+                                        int(stp[pP["l"]]), int(stp[pP["r"]]), K)
+                                mn = (int(shl[pL["lp"]]), int(shp[pP["r"]]))
                             g = (int(offw[int(tl[pL["w"]])] + iw),
                                  int(offs1[int(tp[pP["s1"]])] + i1),
                                  int(offs2[int(tp[pP["s2"]])] + i2))
-                            cand.append((desc, g, int(shl[pL["lp"]]), int(shp[pP["r"]])))
+                            cand.append((desc, g, mn[0], mn[1]))
         olist = []
         oblock = []
         M = N = None
         for yi in ylist:
             ty, shy = Y._itoi[yi], Y._shapes[yi]
             sty = stY[yi]
-            if sty[pY["r"]] != 1:
-                raise ValueError("Heff2 mix plan: r must be the contiguous axis of Y")
-            M, N = int(shy[pY["lp"]]), int(shy[pY["r"]])
+            if sty[pY[col]] != 1:
+                raise ValueError("Heff2 mix plan: %s must be the contiguous axis of Y" % col)
+            row = "r" if psi_left else "lp"
+            M, N = int(shy[pY[row]]), int(shy[pY[col]])
             for i1p in range(int(shy[pY["s1p"]])):
                 for i2p in range(int(shy[pY["s2p"]])):
                     for j in range(int(shy[pY["w2"]])):
                         c_off = int(Y._offsets[yi] + i1p * sty[pY["s1p"]]
                                     + i2p * sty[pY["s2p"]] + j * sty[pY["w2"]])
-                        olist.append((c_off, int(sty[pY["lp"]]),
+                        olist.append((c_off, int(sty[pY[row]]),
                                       (int(offs1p[int(ty[pY["s1p"]])] + i1p),
                                        int(offw2[int(ty[pY["w2"]])] + j),
                                        int(offs2p[int(ty[pY["s2p"]])] + i2p))))
@@ -422,12 +443,20 @@ class Heff2(_HeffBase):
         s2p = [x for x in M2.labels() if x not in (lab_w2, s2[0]) and x not in M1.labels()]
         if len(s1) != 1 or len(s2) != 1 or len(s1p) != 1 or len(s2p) != 1:
             raise MixUnsupported("Heff2 mix plan: unexpected network labels")
-        # Y (l', s1', s2', w'', r); the labels of its two contracted legs sort as (w'', r), so
-        # the contraction over (w'', r) reads Y and R (w'', r, r') as strided matrices
+        # The labels of Y's two contracted legs sort as (w'', r), so the contraction over
+        # (w'', r) reads Y and R (w'', r, r') as strided matrices.  Y's leg order is
+        # (w'', r, l', s1', s2') when every block of it has one-dimensional physical legs: l'
+        # is then contiguous, the contraction with R reads Y's rows (l') with unit stride, and
+        # the mix launch computes transposed tiles (psi as its left operand).  Otherwise Y is
+        # (l', s1', s2', w'', r) with r contiguous.  Either order gives the output legs
+        # (l', s1', s2', r'), psi's layout.
         kw, kr = "~mix0", "~mix1"
         zb = {lab: Zf.bonds()[i] for i, lab in enumerate(Zf.labels())}
-        Y = UniTensor([zb[lab_lp], zb[s1p[0]], zb[s2p[0]], zb[lab_w2], zb[lab_r]],
-                      [lab_lp, s1p[0], s2p[0], kw, kr], 3, 3, dev, _alloc=False)
+        Y = UniTensor([zb[lab_w2], zb[lab_r], zb[lab_lp], zb[s1p[0]], zb[s2p[0]]],
+                      [kw, kr, lab_lp, s1p[0], s2p[0]], 2, 3, dev, _alloc=False)
+        if Y.Nblocks() == 0 or np.any(Y._shapes[:, 3] * Y._shapes[:, 4] != 1):
+            Y = UniTensor([zb[lab_lp], zb[s1p[0]], zb[s2p[0]], zb[lab_w2], zb[lab_r]],
+                          [lab_lp, s1p[0], s2p[0], kw, kr], 3, 3, dev, _alloc=False)
         Wd = W12.to_dense()
         wl = W12.labels()
         Wd = np.ascontiguousarray(np.transpose(
@@ -441,13 +470,16 @@ class Heff2(_HeffBase):
         mix_plan_arrays(*args, Y, ylab, live)
         Y = subset_blocks(Y, live)
         arrays = mix_plan_arrays(*args, Y, ylab)
-        check_mix_bounds(arrays, Lp._storage.numel(), meta_elems(psi_m), meta_elems(Y))
+        psi_left = psi_is_left(Y, ylab)
+        na, nb = Lp._storage.numel(), meta_elems(psi_m)
+        check_mix_bounds(arrays, *((nb, na) if psi_left else (na, nb)), meta_elems(Y))
         plan = _MixPlan(arrays, dev)
         Rpp = Rp.permute([lab_w2, lab_r, lab_rp]).relabel([kw, kr, lab_rp])
         O, c1, c2 = contract_meta(Y, Rpp, 3, dev)
         if O.itoi() != psi.itoi() or [b.dim() for b in O.bonds()] != psi.shape():
             raise ValueError("Heff2 mix chain: output block layout differs from psi's")
-        return [(plan, Lp, True, Y), (plan_contraction(Y, Rpp, O, c1, c2, 3), Rpp, False, O)]
+        return [(plan, Lp, not psi_left, Y),
+                (plan_contraction(Y, Rpp, O, c1, c2, 3), Rpp, False, O)]
 
     @property
     def flops(self) -> float:

@@ -10,7 +10,8 @@ import pytest
 import torch
 
 from cytnx_amd import heisenberg
-from cytnx_amd.heff import MIX_SLICES, check_mix_bounds, mix_plan_arrays, subset_blocks
+from cytnx_amd.heff import (MIX_SLICES, check_mix_bounds, mix_plan_arrays, psi_is_left,
+                             subset_blocks)
 from cytnx_amd.unitensor import UniTensor
 
 
@@ -131,3 +132,43 @@ def test_mix_plan_xxz_item_bounds():
     # the weights are W12's 30 nonzero elements, each used by every item it reaches
     assert np.count_nonzero(W) == 30
     assert set(np.unique(coef[coef != 0]).round(12)) <= set(np.unique(W[W != 0]).round(12))
+
+
+@pytest.mark.parametrize("chi", [8, 20])
+def test_mix_plan_transposed_layout(chi):
+    """Y laid out (w'', r, l', s1', s2'): l' is contiguous, the items compute the transposed
+    tiles X[r, l'] with psi as the left operand, and Y still equals the dense closed form."""
+    psi, L, M1, M2, R = heisenberg.two_site_problem(chi, seed=chi + 7, hermitian=True)
+    Lb = L.bonds()
+    Lp = _meta([Lb[1], Lb[0], Lb[2]], ["-1", "-5", "0"])
+    Ld = L.to_dense()
+    Lflat = _fill_from_dense(Lp, np.transpose(Ld, (1, 0, 2)))
+    P = psi.storage().numpy()
+    W = np.einsum("awsx,wvty->asxvty", M1.to_dense(), M2.to_dense())
+    Y = _meta([M2.bonds()[1], psi.bonds()[3], Lb[2], M1.bonds()[3], M2.bonds()[3]],
+              ["k0", "k1", "0", "1", "2"])
+    assert np.all(Y._shapes[:, 3] * Y._shapes[:, 4] == 1)
+    labY = {"lp": "0", "s1p": "1", "s2p": "2", "w2": "k0", "r": "k1"}
+    assert psi_is_left(Y, labY)
+    labs = ({"l": "-1", "w": "-5", "lp": "0"}, {"l": "-1", "s1": "-2", "s2": "-3", "r": "-4"})
+    live = np.zeros(Y.Nblocks(), dtype=bool)
+    mix_plan_arrays(Lp, labs[0], psi, labs[1], W, Y, labY, live)
+    Yl = subset_blocks(Y, live)
+    arrays = mix_plan_arrays(Lp, labs[0], psi, labs[1], W, Yl, labY)
+    ny = int(sum(np.prod(s.astype(np.int64)) for s in Yl._shapes))
+    check_mix_bounds(arrays, P.size, Lflat.size, ny)  # A = psi, B = Lp
+    with pytest.raises(ValueError):  # B (Lp) reads reach past a one-element buffer
+        check_mix_bounds(arrays, P.size, 1, ny)
+    got = _run_mix(arrays, P, Lflat, ny)
+    assert not np.isnan(got).any()
+    dense = np.einsum("asxvty,alp,lstr->vrpxy", W, Ld, psi.to_dense(), optimize=True)
+    want = _fill_from_dense(Yl, dense)
+    assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+    # the same items as the r-contiguous layout, with M and N exchanged
+    Yr = _meta([Lb[2], M1.bonds()[3], M2.bonds()[3], M2.bonds()[1], psi.bonds()[3]],
+               ["0", "1", "2", "k0", "k1"])
+    assert not psi_is_left(Yr, labY)
+    live_r = np.zeros(Yr.Nblocks(), dtype=bool)
+    mix_plan_arrays(Lp, labs[0], psi, labs[1], W, Yr, labY, live_r)
+    items_r = mix_plan_arrays(Lp, labs[0], psi, labs[1], W, subset_blocks(Yr, live_r), labY)[0]
+    assert sorted(map(tuple, items_r[:, [1, 0, 2, 3]])) == sorted(map(tuple, arrays[0]))
