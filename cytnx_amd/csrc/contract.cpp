@@ -2,6 +2,7 @@
 // contraction planner that turns every charge-matched block pair of a BlockUniTensor
 // contraction into segments of one grouped GEMM launch.
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -492,7 +493,10 @@ namespace cxa {
 // dtype: FP32 copies of the same layout, see the chain).  Offsets are in elements, so the same
 // task list addresses either copy.  Workspace offsets are in elements of the copied operand.
 int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, int a_dt,
-                  int b_dt, int c_dt, int prec, void *workspace, hipStream_t s, int conj) {
+                  int b_dt, int c_dt, int prec, void *workspace, hipStream_t s, int conj,
+                  const GSeg *segs, bool b_k4) {
+  CXA_CHECK(!b_k4 || (segs && !p->copyB && !is_complex(p->dtype)), CXA_ERR_INVALID,
+            "block contract exec: packed B needs remapped segments of a real, uncopied B");
   CXA_CHECK(p, CXA_ERR_INVALID, "block contract exec: null plan");
   CXA_CHECK(!(p->copyA || p->copyB) || workspace, CXA_ERR_INVALID,
             "block contract exec: this plan needs a workspace of %lld bytes",
@@ -524,7 +528,8 @@ int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, i
   const PlanVariant &v = p->var[prec_eff];
   g.tiles = (const GTile *)v.d_tiles.p;
   g.blks = (const GBlk *)p->d_blks.p;
-  g.segs = (const GSeg *)p->d_segs.p;
+  g.segs = segs ? segs : (const GSeg *)p->d_segs.p;
+  g.b_k4 = b_k4;
   g.ntiles = (int)v.tiles.size();
   g.per_xcd = v.per_xcd;
   g.splitk = v.splitk;
@@ -535,6 +540,46 @@ int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, i
     g.kind0 = v.launch_kind;
   }
   return launch_grouped_gemm(g, s);
+}
+
+int bc_pack_b(const cxa_bc_plan *p, int64_t b_elems, std::vector<GSeg> &segs,
+              std::vector<PackRegion> &regions, int64_t *packed_elems) {
+  CXA_CHECK(p && packed_elems, CXA_ERR_INVALID, "bc_pack_b: bad arguments");
+  if (p->copyB || is_complex(p->dtype))
+    return fail(CXA_ERR_UNSUPPORTED, "bc_pack_b: B is permuted into a workspace or complex");
+  segs = p->segs;
+  regions.clear();
+  std::map<std::array<int64_t, 5>, int64_t> at;  // view -> packed offset
+  int64_t total = 0;
+  for (const GBlk &b : p->blks) {
+    for (int s = b.seg0; s < b.seg0 + b.nseg; ++s) {
+      GSeg &sg = segs[s];
+      const int64_t K = sg.K, N = b.N;
+      if (K > 0 && N > 0) {
+        CXA_CHECK(sg.b_off >= 0 && sg.sbk >= 0 && sg.sbn >= 0 &&
+                      sg.b_off + (K - 1) * sg.sbk + (N - 1) * sg.sbn < b_elems,
+                  CXA_ERR_INVALID, "bc_pack_b: segment %d reads outside B", s);
+      }
+      const std::array<int64_t, 5> key{sg.b_off, sg.sbk, sg.sbn, K, N};
+      auto it = at.find(key);
+      int64_t off;
+      if (it != at.end()) {
+        off = it->second;
+      } else {
+        off = total;
+        at.emplace(key, off);
+        CXA_CHECK(K <= INT32_MAX && N <= INT32_MAX, CXA_ERR_UNSUPPORTED,
+                  "bc_pack_b: view too large");
+        regions.push_back(PackRegion{off, sg.b_off, sg.sbk, sg.sbn, (int32_t)K, (int32_t)N});
+        total += (K + 3) / 4 * N * 4;
+      }
+      sg.b_off = off;
+      sg.sbk = 4 * N;
+      sg.sbn = 4;
+    }
+  }
+  *packed_elems = total;
+  return CXA_OK;
 }
 
 bool bc_is_mix(const cxa_bc_plan *p) { return p->mix; }

@@ -27,12 +27,12 @@
 
 namespace cxa {
 
-// Sixteen zero bytes in device memory (zero-initialised, never written).  An operand element
+// Thirty-two zero bytes in device memory (zero-initialised, never written).  An operand element
 // outside its block is read from here (the address is selected, not the value), so fragment
 // loads need no branch and no wait at the load site; the MFMA that consumes them sees exact
 // zeros.  External linkage keeps the compiler from folding these loads to constants, which
 // would turn the selects back into branches around the loads.
-__device__ __align__(16) double gemm_zero_cell[2];
+__device__ __align__(32) double gemm_zero_cell[4];
 
 namespace {
 
@@ -146,8 +146,10 @@ struct Frag {
 };
 
 // `live` is false once the cursor has passed the end of its K range: every element then reads
-// the zero cell (the pipelines load only chunks inside the range, so this is a guard).
-template <typename TA, typename TB, int MI, int NI>
+// the zero cell (the pipelines load only chunks inside the range, so this is a guard).  With
+// BK4 the B operand is k4-packed (element (k, n) at (k >> 2) * sbk + n * 4 + (k & 3) from the
+// segment's origin, whole quads): the 4 k of a lane's column are one vector load.
+template <typename TA, typename TB, int MI, int NI, bool BK4 = false>
 __device__ __forceinline__ void load_frag(Frag<TA, TB, MI, NI> &f, gptr<const TA> Ap,
                                           gptr<const TB> Bp, const GSeg &sg, int k0, int mt,
                                           int nt, int r, int g, bool live) {
@@ -161,14 +163,30 @@ __device__ __forceinline__ void load_frag(Frag<TA, TB, MI, NI> &f, gptr<const TA
                                    live && row < mt && k < sg.K);
     }
   }
+  if constexpr (BK4) {
+    typedef TB tb4 __attribute__((ext_vector_type(4)));
+    const bool quad = live && k0 + 4 * g < sg.K;  // quads past K are not stored
+    const int64_t qoff = (int64_t)((k0 >> 2) + g) * sg.sbk;
 #pragma unroll
-  for (int i = 0; i < NI; ++i) {
-    const int col = 16 * i + r;
+    for (int i = 0; i < NI; ++i) {
+      const int col = 16 * i + r;
+      const tb4 v = load_or_zero<tb4>((gptr<const tb4>)(Bp + (qoff + (int64_t)col * 4)),
+                                      quad && col < nt);
+      f.b[i][0] = v.x;
+      f.b[i][1] = v.y;
+      f.b[i][2] = v.z;
+      f.b[i][3] = v.w;
+    }
+  } else {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = k0 + 4 * g + j;
-      f.b[i][j] = load_or_zero<TB>(Bp + ((int64_t)k * sg.sbk + (int64_t)col * sg.sbn),
-                                   live && col < nt && k < sg.K);
+    for (int i = 0; i < NI; ++i) {
+      const int col = 16 * i + r;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = k0 + 4 * g + j;
+        f.b[i][j] = load_or_zero<TB>(Bp + ((int64_t)k * sg.sbk + (int64_t)col * sg.sbn),
+                                     live && col < nt && k < sg.K);
+      }
     }
   }
 }
@@ -233,10 +251,10 @@ struct ChunkCursor {
     Bp = at<const TB>(Bbase, sg.b_off) + (int64_t)n0 * sg.sbn;
   }
   __device__ __forceinline__ bool live() const { return s < nseg; }
-  template <int MI, int NI>
+  template <int MI, int NI, bool BK4>
   __device__ __forceinline__ void load(Frag<TA, TB, MI, NI> &f, int mt, int nt, int r,
                                        int g) const {
-    load_frag<TA, TB, MI, NI>(f, Ap, Bp, sg, k0, mt, nt, r, g, live());
+    load_frag<TA, TB, MI, NI, BK4>(f, Ap, Bp, sg, k0, mt, nt, r, g, live());
   }
   // skip `nchunk` chunks forward (crossing segment ends); past the last segment the cursor
   // stays exhausted (live() false)
@@ -264,7 +282,7 @@ struct ChunkCursor {
 // Only real chunks are loaded: the launches are bound by the vector-memory address path, so
 // padding a wave's chunk count to whole rounds of the ring (which would let every wait count
 // be exact) costs more than the waits it saves.
-template <int PREC, typename TA, typename TB, int MI, int NI, typename SEGS>
+template <int PREC, typename TA, typename TB, int MI, int NI, bool BK4, typename SEGS>
 __device__ __forceinline__ void mfma_accumulate(RealAcc<PREC, MI, NI> &acc,
                                                 const char *__restrict__ Abase,
                                                 const char *__restrict__ Bbase, const GBlk &blk,
@@ -290,7 +308,7 @@ __device__ __forceinline__ void mfma_accumulate(RealAcc<PREC, MI, NI> &acc,
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     if (d < total) {
-      cur.load(f[d], mt, nt, r, g);
+      cur.template load<MI, NI, BK4>(f[d], mt, nt, r, g);
       cur.skip(nslices, Abase, Bbase);
     }
   }
@@ -301,7 +319,7 @@ __device__ __forceinline__ void mfma_accumulate(RealAcc<PREC, MI, NI> &acc,
       if (c < total) {
         mma_frag<PREC, TA, TB, MI, NI>(acc, f[d]);
         if (c + D < total) {
-          cur.load(f[d], mt, nt, r, g);
+          cur.template load<MI, NI, BK4>(f[d], mt, nt, r, g);
           cur.skip(nslices, Abase, Bbase);
         }
       }
@@ -317,14 +335,15 @@ __device__ __forceinline__ void store_c(gptr<TC> dst, double val, const GBlk &bl
   *dst = (TC)val;
 }
 
-template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, typename SEGS>
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, bool BK4,
+          typename SEGS>
 __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
                                                const char *__restrict__ Bbase,
                                                char *__restrict__ Cbase, const GBlk &blk,
                                                SEGS segs, int m0, int n0, int lane) {
   RealAcc<PREC, MI, NI> acc;
   acc.zero();
-  mfma_accumulate<PREC, TA, TB, MI, NI>(acc, Abase, Bbase, blk, segs, m0, n0, lane, 0, 1);
+  mfma_accumulate<PREC, TA, TB, MI, NI, BK4>(acc, Abase, Bbase, blk, segs, m0, n0, lane, 0, 1);
   const int r = lane & 15, g = lane >> 4;
   const int mt = min(16 * MI, blk.M - m0), nt = min(16 * NI, blk.N - n0);
   gptr<TC> Cp = at<TC>(Cbase, blk.c_off);
@@ -343,7 +362,7 @@ __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
 
 // Split-K tile: every wave of the workgroup (WAVES of them) calls this for the same tile; wave
 // w takes the K chunks c with c % WAVES == w, and the partial tiles are summed in wave order.
-template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int WAVES,
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int WAVES, bool BK4,
           typename SEGS>
 __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
                                              const char *__restrict__ Bbase,
@@ -352,7 +371,7 @@ __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
                                              double (*red)[16][64]) {
   RealAcc<PREC, MI, NI> acc;
   acc.zero();
-  mfma_accumulate<PREC, TA, TB, MI, NI>(acc, Abase, Bbase, blk, segs, m0, n0, lane, wave,
+  mfma_accumulate<PREC, TA, TB, MI, NI, BK4>(acc, Abase, Bbase, blk, segs, m0, n0, lane, wave,
                                         WAVES);
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -407,7 +426,7 @@ struct RowCursor {
 // (B rows are wave-uniform reads).
 constexpr int kRowsDepth = 8;
 
-template <typename TA, typename TB, typename SEGS>
+template <typename TA, typename TB, bool BK4, typename SEGS>
 __device__ __forceinline__ void rows_accumulate(double (&acc)[kRowsN], const char *Abase,
                                                 const char *Bbase, const GBlk &blk, SEGS segs,
                                                 int row, bool valid, int slice, int nslices) {
@@ -441,7 +460,9 @@ __device__ __forceinline__ void rows_accumulate(double (&acc)[kRowsN], const cha
     for (int d = 0; d < kRowsDepth; ++d) {
       const int c = base + d;
       if (c < total) {
-        gptr<const TB> bk = at<const TB>(Bbase, cc.sg.b_off) + (int64_t)cc.k * cc.sg.sbk;
+        gptr<const TB> bk = at<const TB>(Bbase, cc.sg.b_off) +
+                            (BK4 ? (int64_t)(cc.k >> 2) * cc.sg.sbk + (cc.k & 3)
+                                 : (int64_t)cc.k * cc.sg.sbk);
 #pragma unroll
         for (int n = 0; n < kRowsN; ++n)
           if (n < N) acc[n] += (double)a[d] * (double)bk[(int64_t)n * cc.sg.sbn];
@@ -458,7 +479,7 @@ __device__ __forceinline__ void rows_accumulate(double (&acc)[kRowsN], const cha
   drain_loads();
 }
 
-template <typename TA, typename TB, typename TC, typename SEGS>
+template <typename TA, typename TB, typename TC, bool BK4, typename SEGS>
 __device__ __forceinline__ void wave_rows(const char *__restrict__ Abase,
                                           const char *__restrict__ Bbase,
                                           char *__restrict__ Cbase, const GBlk &blk, SEGS segs,
@@ -466,7 +487,7 @@ __device__ __forceinline__ void wave_rows(const char *__restrict__ Abase,
   const int row = m0 + lane;
   const bool valid = row < blk.M;
   double acc[kRowsN];
-  rows_accumulate<TA, TB>(acc, Abase, Bbase, blk, segs, row, valid, 0, 1);
+  rows_accumulate<TA, TB, BK4>(acc, Abase, Bbase, blk, segs, row, valid, 0, 1);
   if (!valid) return;
   gptr<TC> Cp = at<TC>(Cbase, blk.c_off) + (int64_t)row * blk.ldc;
 #pragma unroll
@@ -475,7 +496,7 @@ __device__ __forceinline__ void wave_rows(const char *__restrict__ Abase,
 }
 
 // Split-K row tile: the four waves take interleaved k values of the same 64 rows.
-template <typename TA, typename TB, typename TC, typename SEGS>
+template <typename TA, typename TB, typename TC, bool BK4, typename SEGS>
 __device__ __forceinline__ void wg_rows(const char *__restrict__ Abase,
                                         const char *__restrict__ Bbase, char *__restrict__ Cbase,
                                         const GBlk &blk, SEGS segs, int m0, int lane, int wave,
@@ -483,7 +504,7 @@ __device__ __forceinline__ void wg_rows(const char *__restrict__ Abase,
   const int row = m0 + lane;
   const bool valid = row < blk.M;
   double acc[kRowsN];
-  rows_accumulate<TA, TB>(acc, Abase, Bbase, blk, segs, row, valid, wave, kWaves);
+  rows_accumulate<TA, TB, BK4>(acc, Abase, Bbase, blk, segs, row, valid, wave, kWaves);
 #pragma unroll
   for (int n = 0; n < kRowsN; ++n) red[wave][n][lane] = acc[n];
   __syncthreads();
@@ -618,7 +639,8 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
 // register budget (occupancy matters most for the memory-bound row tiles).
 enum TileClass { TC_MIXED = 0, TC_ROWS = 1, TC_BIG = 2 };
 
-template <int PREC, typename TA, typename TB, typename TC, bool CPLX, int CLS, typename SEGS>
+template <int PREC, typename TA, typename TB, typename TC, bool CPLX, int CLS, bool BK4,
+          typename SEGS>
 __device__ __forceinline__ void run_tile(int kind, const char *A, const char *B, char *C,
                                          const GBlk &blk, SEGS segs, int m0, int n0, int lane,
                                          int conj) {
@@ -630,22 +652,26 @@ __device__ __forceinline__ void run_tile(int kind, const char *A, const char *B,
       default: wave_tile_cplx<TA, 2, 2>(A, B, C, blk, segs, m0, n0, lane, conj); break;
     }
   } else if constexpr (CLS == TC_ROWS) {
-    wave_rows<TA, TB, TC>(A, B, C, blk, segs, m0, lane);
+    wave_rows<TA, TB, TC, BK4>(A, B, C, blk, segs, m0, lane);
   } else if constexpr (CLS == TC_BIG) {
-    wave_tile_real<PREC, TA, TB, TC, 4, 2>(A, B, C, blk, segs, m0, n0, lane);
+    wave_tile_real<PREC, TA, TB, TC, 4, 2, BK4>(A, B, C, blk, segs, m0, n0, lane);
   } else {
     switch (kind) {
-      case TK_1x1: wave_tile_real<PREC, TA, TB, TC, 1, 1>(A, B, C, blk, segs, m0, n0, lane); break;
-      case TK_2x1: wave_tile_real<PREC, TA, TB, TC, 2, 1>(A, B, C, blk, segs, m0, n0, lane); break;
-      case TK_1x2: wave_tile_real<PREC, TA, TB, TC, 1, 2>(A, B, C, blk, segs, m0, n0, lane); break;
-      default: wave_tile_real<PREC, TA, TB, TC, 2, 2>(A, B, C, blk, segs, m0, n0, lane); break;
+      case TK_1x1: wave_tile_real<PREC, TA, TB, TC, 1, 1, BK4>(A, B, C, blk, segs, m0, n0, lane);
+        break;
+      case TK_2x1: wave_tile_real<PREC, TA, TB, TC, 2, 1, BK4>(A, B, C, blk, segs, m0, n0, lane);
+        break;
+      case TK_1x2: wave_tile_real<PREC, TA, TB, TC, 1, 2, BK4>(A, B, C, blk, segs, m0, n0, lane);
+        break;
+      default: wave_tile_real<PREC, TA, TB, TC, 2, 2, BK4>(A, B, C, blk, segs, m0, n0, lane);
+        break;
     }
   }
 }
 
 // grouped launch: tiles, blocks and segments come from device arrays; one wave per tile.
 // Workgroup w runs on XCD w % kXcds and takes tiles from that XCD's range of the list.
-template <int PREC, typename TA, typename TB, typename TC, bool CPLX, int CLS>
+template <int PREC, typename TA, typename TB, typename TC, bool CPLX, int CLS, bool BK4>
 __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict__ tiles,
                                                         int per_xcd, const GBlk *__restrict__ blks,
                                                         const GSeg *__restrict__ segs,
@@ -659,14 +685,15 @@ __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict
     const GTile tl = range[t];
     if (tl.blk < 0) continue;  // padding
     const GBlk blk = blks[tl.blk];
-    run_tile<PREC, TA, TB, TC, CPLX, CLS>(tl.kind, A, B, C, blk, SegArray{segs}, tl.m0, tl.n0,
+    run_tile<PREC, TA, TB, TC, CPLX, CLS, BK4>(tl.kind, A, B, C, blk, SegArray{segs}, tl.m0,
+                                               tl.n0,
                                           lane, conj);
   }
 }
 
 // grouped launch with workgroup split-K: one workgroup per tile (real dtypes; the mixed MFMA
 // class or the row class)
-template <int PREC, typename TA, typename TB, typename TC, int CLS, int WAVES>
+template <int PREC, typename TA, typename TB, typename TC, int CLS, int WAVES, bool BK4>
 __global__ __launch_bounds__(64 * WAVES) void k_grouped_split(
     const GTile *__restrict__ tiles, int per_xcd, const GBlk *__restrict__ blks,
     const GSeg *__restrict__ segs, const char *A, const char *B, char *C) {
@@ -682,23 +709,27 @@ __global__ __launch_bounds__(64 * WAVES) void k_grouped_split(
     const GBlk blk = blks[tl.blk];
     const SegArray sa{segs};
     if constexpr (CLS == TC_ROWS) {
-      wg_rows<TA, TB, TC>(A, B, C, blk, sa, tl.m0, lane, wave, red);
+      wg_rows<TA, TB, TC, BK4>(A, B, C, blk, sa, tl.m0, lane, wave, red);
     } else {
       switch (tl.kind) {
         case TK_1x1:
-          wg_tile_real<PREC, TA, TB, TC, 1, 1, WAVES>(A, B, C, blk, sa, tl.m0, tl.n0, lane, wave,
+          wg_tile_real<PREC, TA, TB, TC, 1, 1, WAVES, BK4>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
+                                                          wave,
                                                      red);
           break;
         case TK_2x1:
-          wg_tile_real<PREC, TA, TB, TC, 2, 1, WAVES>(A, B, C, blk, sa, tl.m0, tl.n0, lane, wave,
+          wg_tile_real<PREC, TA, TB, TC, 2, 1, WAVES, BK4>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
+                                                          wave,
                                                      red);
           break;
         case TK_1x2:
-          wg_tile_real<PREC, TA, TB, TC, 1, 2, WAVES>(A, B, C, blk, sa, tl.m0, tl.n0, lane, wave,
+          wg_tile_real<PREC, TA, TB, TC, 1, 2, WAVES, BK4>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
+                                                          wave,
                                                      red);
           break;
         default:
-          wg_tile_real<PREC, TA, TB, TC, 2, 2, WAVES>(A, B, C, blk, sa, tl.m0, tl.n0, lane, wave,
+          wg_tile_real<PREC, TA, TB, TC, 2, 2, WAVES, BK4>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
+                                                          wave,
                                                      red);
           break;
       }
@@ -721,7 +752,7 @@ struct SegStride {
 
 // Segment i of `segs` (nseg <= NACC) accumulates into acc[i]; one load pipeline runs across
 // all of them (as in mfma_accumulate).
-template <int PREC, typename TA, typename TB, int MI, int NI, int NACC, typename SEGS>
+template <int PREC, typename TA, typename TB, int MI, int NI, int NACC, bool BK4, typename SEGS>
 __device__ __forceinline__ void mfma_accumulate_multi(RealAcc<PREC, MI, NI> (&acc)[NACC],
                                                       const char *__restrict__ Abase,
                                                       const char *__restrict__ Bbase, SEGS segs,
@@ -753,7 +784,7 @@ __device__ __forceinline__ void mfma_accumulate_multi(RealAcc<PREC, MI, NI> (&ac
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     if (d < total) {
-      cur.load(f[d], mt, nt, r, g);
+      cur.template load<MI, NI, BK4>(f[d], mt, nt, r, g);
       cur.skip(1, Abase, Bbase);
     }
   }
@@ -778,7 +809,7 @@ __device__ __forceinline__ void mfma_accumulate_multi(RealAcc<PREC, MI, NI> (&ac
       if (c < total) {
         consume(f[d]);
         if (c + D < total) {
-          cur.load(f[d], mt, nt, r, g);
+          cur.template load<MI, NI, BK4>(f[d], mt, nt, r, g);
           cur.skip(1, Abase, Bbase);
         }
       }
@@ -787,7 +818,7 @@ __device__ __forceinline__ void mfma_accumulate_multi(RealAcc<PREC, MI, NI> (&ac
   drain_loads();
 }
 
-template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int WAVES>
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int WAVES, bool BK4>
 __global__ __launch_bounds__(64 * WAVES) void k_mix(const GTile *__restrict__ tiles,
                                                    int per_xcd, const GBlk *__restrict__ items,
                                                    const GSeg *__restrict__ slices,
@@ -812,7 +843,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mix(const GTile *__restrict__ ti
       RealAcc<PREC, MI, NI> acc[NACC];
 #pragma unroll
       for (int q = 0; q < NACC; ++q) acc[q].zero();
-      mfma_accumulate_multi<PREC, TA, TB, MI, NI, NACC>(
+      mfma_accumulate_multi<PREC, TA, TB, MI, NI, NACC, BK4>(
           acc, A, B, SegStride{slices, it.seg0 + wave, WAVES}, mine, it.M, it.N, tl.m0, tl.n0,
           lane);
 #pragma unroll
@@ -855,17 +886,36 @@ __global__ __launch_bounds__(64 * WAVES) void k_mix(const GTile *__restrict__ ti
   }
 }
 
-template <int PREC, typename TA, typename TB, typename TC, int MI, int NI>
-void launch_mix_kind(const GemmLaunch &g, int grid, hipStream_t s) {
+// Packed (k4) B operands are built for the fixed operands of a chain: FP32 copies in FP32 mode,
+// the FP64 tensors otherwise; only those kernel variants exist.
+template <int PREC, typename TB>
+constexpr bool k4_ok() {
+  return PREC == CXA_PREC_FP32 ? std::is_same<TB, float>::value
+                               : std::is_same<TB, double>::value;
+}
+
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, bool BK4>
+void launch_mix_kind_t(const GemmLaunch &g, int grid, hipStream_t s) {
   const char *A = (const char *)g.A, *B = (const char *)g.B;
   char *C = (char *)g.C;
   if (g.splitk)  // four waves, two slices each
-    hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, MI, NI, 4>), dim3(grid), dim3(64 * 4), 0, s,
-                       g.tiles, g.per_xcd, g.blks, g.segs, g.outs, A, B, C);
+    hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, MI, NI, 4, BK4>), dim3(grid), dim3(64 * 4), 0,
+                       s, g.tiles, g.per_xcd, g.blks, g.segs, g.outs, A, B, C);
   else  // one wave per slice
-    hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, MI, NI, kMixSlices>), dim3(grid),
+    hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, MI, NI, kMixSlices, BK4>), dim3(grid),
                        dim3(64 * kMixSlices), 0, s, g.tiles, g.per_xcd, g.blks, g.segs, g.outs,
                        A, B, C);
+}
+
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI>
+void launch_mix_kind(const GemmLaunch &g, int grid, hipStream_t s) {
+  if constexpr (k4_ok<PREC, TB>()) {
+    if (g.b_k4) {
+      launch_mix_kind_t<PREC, TA, TB, TC, MI, NI, true>(g, grid, s);
+      return;
+    }
+  }
+  launch_mix_kind_t<PREC, TA, TB, TC, MI, NI, false>(g, grid, s);
 }
 
 template <int PREC, typename TA, typename TB, typename TC>
@@ -874,6 +924,8 @@ int launch_mix_t(const GemmLaunch &g, hipStream_t s) {
   if (g.ntiles != kXcds * g.per_xcd)
     return fail(CXA_ERR_INVALID, "mix GEMM: tile list of %d is not %d ranges of %d", g.ntiles,
                 kXcds, g.per_xcd);
+  if (g.b_k4 && !k4_ok<PREC, TB>())
+    return fail(CXA_ERR_UNSUPPORTED, "mix GEMM: no packed-B variant for these dtypes");
   const int grid = kXcds * std::min(g.per_xcd, 2048);
   switch (g.kind0) {
     case TK_1x1: launch_mix_kind<PREC, TA, TB, TC, 1, 1>(g, grid, s); break;
@@ -921,14 +973,42 @@ __global__ __launch_bounds__(64 * kWaves) void k_single(GBlk blk, GSeg seg, int 
   const int tc = kind == TK_1x1 || kind == TK_2x1 ? 16 : 32;
   for (int t = blockIdx.x * kWaves + wave; t < ntiles; t += gridDim.x * kWaves) {
     const int m0 = (t / tiles_n) * tr, n0 = (t % tiles_n) * tc;
-    run_tile<PREC, TA, TB, TC, CPLX, TC_MIXED>(kind, A, B, C, blk, SegOne{seg}, m0, n0, lane,
+    run_tile<PREC, TA, TB, TC, CPLX, TC_MIXED, false>(kind, A, B, C, blk, SegOne{seg}, m0, n0,
+                                                      lane,
                                                conj);
   }
+}
+
+// the real-dtype grouped launches of one class / split mode
+template <int PREC, typename TA, typename TB, typename TC, bool BK4>
+void launch_real(const GemmLaunch &g, int cls, int wg_grid, int wave_grid, hipStream_t s) {
+  const int P = g.per_xcd;
+  const char *A = (const char *)g.A, *B = (const char *)g.B;
+  char *C = (char *)g.C;
+  if (g.splitk && cls == TC_ROWS)
+    hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_ROWS, kWaves, BK4>), dim3(wg_grid),
+                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C);
+  else if (g.splitk == 8 && cls == TC_MIXED)
+    hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED, 8, BK4>), dim3(wg_grid),
+                       dim3(64 * 8), 0, s, g.tiles, P, g.blks, g.segs, A, B, C);
+  else if (g.splitk && cls == TC_MIXED)
+    hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED, kWaves, BK4>), dim3(wg_grid),
+                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C);
+  else if (cls == TC_ROWS)
+    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_ROWS, BK4>), dim3(wave_grid),
+                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, g.conj);
+  else if (cls == TC_BIG)
+    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_BIG, BK4>), dim3(wave_grid),
+                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, g.conj);
+  else
+    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_MIXED, BK4>), dim3(wave_grid),
+                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, g.conj);
 }
 
 template <int PREC, typename TA, typename TB, typename TC, bool CPLX>
 int launch_t(const GemmLaunch &g, hipStream_t s) {
   if (g.single) {
+    if (g.b_k4) return fail(CXA_ERR_INVALID, "single-block GEMM: packed B is not supported");
     const GBlk &b = g.blk0;
     if (b.M <= 0 || b.N <= 0) return CXA_OK;
     int kind = g.kind0 >= 0 ? g.kind0 : default_kind(b.M, b.N);
@@ -959,32 +1039,17 @@ int launch_t(const GemmLaunch &g, hipStream_t s) {
   const int wg_grid = g.grid > 0 ? g.grid : kXcds * std::min(P, 2048);
   const int wave_grid = g.grid > 0 ? g.grid : kXcds * std::min((P + kWaves - 1) / kWaves, 1024);
   if constexpr (!CPLX) {
-    if (g.splitk && cls == TC_ROWS)
-      hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_ROWS, kWaves>), dim3(wg_grid),
-                         dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
-                         (const char *)g.A, (const char *)g.B, (char *)g.C);
-    else if (g.splitk == 8 && cls == TC_MIXED)
-      hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED, 8>), dim3(wg_grid),
-                         dim3(64 * 8), 0, s, g.tiles, P, g.blks, g.segs, (const char *)g.A,
-                         (const char *)g.B, (char *)g.C);
-    else if (g.splitk && cls == TC_MIXED)
-      hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED, kWaves>), dim3(wg_grid),
-                         dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
-                         (const char *)g.A, (const char *)g.B, (char *)g.C);
-    else if (cls == TC_ROWS)
-      hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_ROWS>), dim3(wave_grid),
-                         dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
-                         (const char *)g.A, (const char *)g.B, (char *)g.C, g.conj);
-    else if (cls == TC_BIG)
-      hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_BIG>), dim3(wave_grid),
-                         dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
-                         (const char *)g.A, (const char *)g.B, (char *)g.C, g.conj);
-    else
-      hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_MIXED>), dim3(wave_grid),
-                         dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
-                         (const char *)g.A, (const char *)g.B, (char *)g.C, g.conj);
+    if (g.b_k4) {
+      if constexpr (k4_ok<PREC, TB>())
+        launch_real<PREC, TA, TB, TC, true>(g, cls, wg_grid, wave_grid, s);
+      else
+        return fail(CXA_ERR_UNSUPPORTED, "grouped GEMM: no packed-B variant for these dtypes");
+    } else {
+      launch_real<PREC, TA, TB, TC, false>(g, cls, wg_grid, wave_grid, s);
+    }
   } else {
-    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, true, TC_MIXED>), dim3(wave_grid),
+    if (g.b_k4) return fail(CXA_ERR_UNSUPPORTED, "grouped GEMM: packed B is real-only");
+    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, true, TC_MIXED, false>), dim3(wave_grid),
                        dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
                        (const char *)g.A, (const char *)g.B, (char *)g.C, g.conj);
   }

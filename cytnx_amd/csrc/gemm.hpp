@@ -79,6 +79,11 @@ struct GemmLaunch {
   int tile_class = 0;   // the class make_tiles returned
   const MixOut *outs = nullptr;  // non-null: mix launch (blks are items, segs are slices)
   int conj = 0;  // complex dtypes: bit 0 conjugates A, bit 1 conjugates B
+  // B is k4-packed (real dtypes, B = FP32 in FP32 mode or FP64 otherwise): element (k, n) of a
+  // segment sits at b_off + (k >> 2) * sbk + n * 4 + (k & 3), b_off and sbk are multiples of
+  // 4, sbn is 4, and every k-quad of a segment is whole (zero past K), so a lane reads the 4
+  // consecutive k of one column with one vector load
+  bool b_k4 = false;
   // single-block mode: one block with one segment passed by value, no device task arrays
   bool single = false;
   GBlk blk0{};

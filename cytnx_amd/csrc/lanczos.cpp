@@ -152,16 +152,46 @@ struct cxa_chain {
   std::vector<int> fixed_left;
   std::vector<int64_t> fixed_elems, out_elems;
   std::vector<void *> mid64, mid32, fixed32;
+  // Steps whose fixed operand is B read it k4-packed (one vector load per lane and k-quad):
+  // the plan's segments remapped into the packed buffer (device), the packed FP64 copy (made
+  // at creation) and FP32 copy (made by cxa_chain_prepare for FP32), null where not packed.
+  std::vector<void *> packed_segs, packed64, packed32;
+  std::vector<int64_t> packed_elems;
+  std::vector<std::vector<cxa::PackRegion>> pack_regions;
   void *ws = nullptr;  // permute workspace (max over steps), 16 bytes per element
   int prec = CXA_PREC_FP64;
   double flops = 0;
   ~cxa_chain() {
-    for (auto *v : {&mid64, &mid32, &fixed32})
+    for (auto *v : {&mid64, &mid32, &fixed32, &packed_segs, &packed64, &packed32})
       for (void *p : *v)
         if (p) (void)hipFree(p);
     if (ws) (void)hipFree(ws);
   }
 };
+
+namespace {
+
+// Fill a packed copy of step k's fixed operand (FP64 source) on stream s and wait for it (the
+// region table is a temporary device buffer).
+int fill_packed(const cxa_chain *c, int k, void *dst, bool f32, hipStream_t s) {
+  const std::vector<cxa::PackRegion> &reg = c->pack_regions[k];
+  if (reg.empty()) return CXA_OK;
+  void *d_reg = nullptr;
+  CXA_HIP(hipMalloc(&d_reg, reg.size() * sizeof(cxa::PackRegion)));
+  int rc = CXA_OK;
+  if (hipMemcpyAsync(d_reg, reg.data(), reg.size() * sizeof(cxa::PackRegion),
+                     hipMemcpyHostToDevice, s) != hipSuccess)
+    rc = cxa::fail(CXA_ERR_HIP, "packed copy: region upload failed");
+  if (rc == CXA_OK)
+    rc = cxa::vec_pack_b((const double *)c->fixed[k], dst, f32,
+                         (const cxa::PackRegion *)d_reg, (int)reg.size(), s);
+  if (hipStreamSynchronize(s) != hipSuccess && rc == CXA_OK)
+    rc = cxa::fail(CXA_ERR_HIP, "packed copy: stream synchronisation failed");
+  (void)hipFree(d_reg);
+  return rc;
+}
+
+}  // namespace
 
 extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const void *const *fixed,
                                 const int *fixed_is_left, const int64_t *fixed_elems,
@@ -189,6 +219,33 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
   for (int k = 0; k + 1 < nsteps; ++k)
     CXA_HIP(hipMalloc(&c->mid64[k], std::max<int64_t>(1, out_elems[k]) * sizeof(double)));
   if (wsmax > 0) CXA_HIP(hipMalloc(&c->ws, wsmax * 16));
+  // k4-packed copies of the fixed B operands (read once here: the chain keeps using these
+  // copies, so a fixed operand changed after creation needs a new chain)
+  c->packed_segs.assign(nsteps, nullptr);
+  c->packed64.assign(nsteps, nullptr);
+  c->packed32.assign(nsteps, nullptr);
+  c->packed_elems.assign(nsteps, 0);
+  c->pack_regions.assign(nsteps, {});
+  bool synced = false;
+  for (int k = 0; k < nsteps; ++k) {
+    if (c->fixed_left[k]) continue;
+    std::vector<cxa::GSeg> segs;
+    int64_t elems = 0;
+    if (cxa::bc_pack_b(plans[k], fixed_elems[k], segs, c->pack_regions[k], &elems) != CXA_OK)
+      continue;  // this plan reads B through a workspace: unpacked
+    if (!synced) {  // the fixed operands may come from any stream
+      CXA_HIP(hipDeviceSynchronize());
+      synced = true;
+    }
+    c->packed_elems[k] = elems;
+    CXA_HIP(hipMalloc(&c->packed_segs[k], std::max<size_t>(1, segs.size()) * sizeof(cxa::GSeg)));
+    if (!segs.empty())
+      CXA_HIP(hipMemcpy(c->packed_segs[k], segs.data(), segs.size() * sizeof(cxa::GSeg),
+                        hipMemcpyHostToDevice));
+    CXA_HIP(hipMalloc(&c->packed64[k], std::max<int64_t>(1, elems) * sizeof(double)));
+    int rc = fill_packed(c.get(), k, c->packed64[k], false, nullptr);
+    if (rc) return rc;
+  }
   *out = c.release();
   return CXA_OK;
 }
@@ -204,6 +261,13 @@ extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
   c->fixed32.assign(c->nsteps, nullptr);
   c->mid32.assign(c->mid64.size(), nullptr);
   for (int k = 0; k < c->nsteps; ++k) {
+    if (c->packed_segs[k]) {  // packed steps keep only the packed FP32 copy
+      CXA_HIP(hipMalloc(&c->packed32[k],
+                        std::max<int64_t>(1, c->packed_elems[k]) * sizeof(float)));
+      int rc = fill_packed(c, k, c->packed32[k], true, s);
+      if (rc) return rc;
+      continue;
+    }
     CXA_HIP(hipMalloc(&c->fixed32[k], std::max<int64_t>(1, c->fixed_elems[k]) * sizeof(float)));
     int rc = vec_cast_f64_f32((const double *)c->fixed[k], (float *)c->fixed32[k],
                               c->fixed_elems[k], s);
@@ -223,11 +287,16 @@ int chain_step(cxa_chain *c, int k, const void *x, void *y, hipStream_t s) {
   const int in_dt = (k == 0 || !f32) ? CXA_F64 : CXA_F32;
   void *outp = last ? y : (f32 ? c->mid32[k] : c->mid64[k]);
   const int out_dt = (last || !f32) ? CXA_F64 : CXA_F32;
-  const void *fx = f32 ? c->fixed32[k] : c->fixed[k];
   const int fx_dt = f32 ? CXA_F32 : CXA_F64;
   if (c->fixed_left[k])
-    return bc_exec_typed(c->plans[k], fx, in, outp, fx_dt, in_dt, out_dt, c->prec, c->ws, s);
-  return bc_exec_typed(c->plans[k], in, fx, outp, in_dt, fx_dt, out_dt, c->prec, c->ws, s);
+    return bc_exec_typed(c->plans[k], f32 ? c->fixed32[k] : c->fixed[k], in, outp, fx_dt, in_dt,
+                         out_dt, c->prec, c->ws, s);
+  if (c->packed_segs[k])
+    return bc_exec_typed(c->plans[k], in, f32 ? c->packed32[k] : c->packed64[k], outp, in_dt,
+                         fx_dt, out_dt, c->prec, c->ws, s, 0,
+                         (const cxa::GSeg *)c->packed_segs[k], true);
+  return bc_exec_typed(c->plans[k], in, f32 ? c->fixed32[k] : c->fixed[k], outp, in_dt, fx_dt,
+                       out_dt, c->prec, c->ws, s);
 }
 
 int chain_apply(cxa_chain *c, const void *x, void *y, hipStream_t s) {

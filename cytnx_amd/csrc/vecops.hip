@@ -5,6 +5,7 @@
 // with a fixed number of partial sums, so results are deterministic run to run.
 #include "cxa_common.hpp"
 #include "vecops.hpp"
+#include "contract.hpp"
 
 namespace cxa {
 namespace {
@@ -273,6 +274,40 @@ int vec_combine(double *out, const double *V, const double *coef, int k, int64_t
 int vec_cast_f64_f32(const double *x, float *y, int64_t n, hipStream_t s) {
   if (n <= 0) return CXA_OK;
   hipLaunchKernelGGL(k_cast_f64_f32, dim3(grid_of(n)), dim3(kT), 0, s, x, y, n);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+// One workgroup per region (grid-stride over regions): packed element e of a region holds
+// source element (k, n) with k = 4 * (e / 4N) + e % 4, n = (e % 4N) / 4, or zero for k >= K.
+// Consecutive threads write consecutive packed elements.
+template <typename T>
+__global__ __launch_bounds__(kT) void k_pack_b(const double *__restrict__ src, T *__restrict__ dst,
+                                               const PackRegion *__restrict__ regions,
+                                               int nregions) {
+  for (int r = blockIdx.x; r < nregions; r += gridDim.x) {
+    const PackRegion R = regions[r];
+    const int64_t row = 4 * (int64_t)R.N;
+    const int64_t nel = (int64_t)((R.K + 3) / 4) * row;
+    for (int64_t e = threadIdx.x; e < nel; e += kT) {
+      const int64_t q = e / row;
+      const int64_t rem = e - q * row;
+      const int64_t k = 4 * q + (rem & 3), n = rem >> 2;
+      dst[R.dst_off + e] = k < R.K ? (T)src[R.src_off + k * R.sbk + n * R.sbn] : (T)0;
+    }
+  }
+}
+
+int vec_pack_b(const double *src, void *dst, bool dst_f32, const PackRegion *regions,
+               int nregions, hipStream_t s) {
+  if (nregions <= 0) return CXA_OK;
+  const int grid = nregions < 4096 ? nregions : 4096;
+  if (dst_f32)
+    hipLaunchKernelGGL(k_pack_b<float>, dim3(grid), dim3(kT), 0, s, src, (float *)dst, regions,
+                       nregions);
+  else
+    hipLaunchKernelGGL(k_pack_b<double>, dim3(grid), dim3(kT), 0, s, src, (double *)dst,
+                       regions, nregions);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }

@@ -45,4 +45,10 @@ int vec_combine(double *out, const double *V, const double *coef, int k, int64_t
                 hipStream_t s);
 int vec_cast_f64_f32(const double *x, float *y, int64_t n, hipStream_t s);
 
+// Fill the k4-packed copy of a B operand (regions of contract.hpp's PackRegion, nregions of
+// them, on the device): dst = FP64 or FP32 (dst_f32), src = FP64.
+struct PackRegion;
+int vec_pack_b(const double *src, void *dst, bool dst_f32, const PackRegion *regions,
+               int nregions, hipStream_t s);
+
 }  // namespace cxa
