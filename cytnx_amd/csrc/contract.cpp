@@ -494,9 +494,16 @@ namespace cxa {
 // task list addresses either copy.  Workspace offsets are in elements of the copied operand.
 int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, int a_dt,
                   int b_dt, int c_dt, int prec, void *workspace, hipStream_t s, int conj,
-                  const GSeg *segs, bool b_k4) {
-  CXA_CHECK(!b_k4 || (segs && !p->copyB && !is_complex(p->dtype)), CXA_ERR_INVALID,
-            "block contract exec: packed B needs remapped segments of a real, uncopied B");
+                  const ExecLayout *lay) {
+  const ExecLayout none;
+  if (!lay) lay = &none;
+  CXA_CHECK(!(lay->a_k4 || lay->b_k4) || (lay->segs && !is_complex(p->dtype)),
+            CXA_ERR_INVALID, "block contract exec: packed operands need remapped segments of a "
+                             "real plan");
+  CXA_CHECK(!(lay->a_k4 && p->copyA) && !(lay->b_k4 && p->copyB), CXA_ERR_INVALID,
+            "block contract exec: a packed operand cannot also be permuted into the workspace");
+  CXA_CHECK(!lay->c_k4 || (p->mix && lay->outs), CXA_ERR_INVALID,
+            "block contract exec: packed outputs need a mix plan and its packed outputs");
   CXA_CHECK(p, CXA_ERR_INVALID, "block contract exec: null plan");
   CXA_CHECK(!(p->copyA || p->copyB) || workspace, CXA_ERR_INVALID,
             "block contract exec: this plan needs a workspace of %lld bytes",
@@ -528,39 +535,48 @@ int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, i
   const PlanVariant &v = p->var[prec_eff];
   g.tiles = (const GTile *)v.d_tiles.p;
   g.blks = (const GBlk *)p->d_blks.p;
-  g.segs = segs ? segs : (const GSeg *)p->d_segs.p;
-  g.b_k4 = b_k4;
+  g.segs = lay->segs ? lay->segs : (const GSeg *)p->d_segs.p;
+  g.a_k4 = lay->a_k4;
+  g.b_k4 = lay->b_k4;
+  g.c_k4 = lay->c_k4;
   g.ntiles = (int)v.tiles.size();
   g.per_xcd = v.per_xcd;
   g.splitk = v.splitk;
   g.tile_class = v.tile_class;
   g.conj = is_complex(p->dtype) ? conj : 0;
   if (p->mix) {
-    g.outs = (const MixOut *)p->d_outs.p;
+    g.outs = lay->outs ? lay->outs : (const MixOut *)p->d_outs.p;
     g.kind0 = v.launch_kind;
   }
   return launch_grouped_gemm(g, s);
 }
 
-int bc_pack_b(const cxa_bc_plan *p, int64_t b_elems, std::vector<GSeg> &segs,
-              std::vector<PackRegion> &regions, int64_t *packed_elems) {
-  CXA_CHECK(p && packed_elems, CXA_ERR_INVALID, "bc_pack_b: bad arguments");
-  if (p->copyB || is_complex(p->dtype))
-    return fail(CXA_ERR_UNSUPPORTED, "bc_pack_b: B is permuted into a workspace or complex");
-  segs = p->segs;
+namespace {
+
+// Shared body of bc_pack_a / bc_pack_b: `which` 0 packs the A views (free index m, extent the
+// block's M), 1 the B views (free index n, extent N).
+int pack_views(const cxa_bc_plan *p, int which, int64_t elems, std::vector<GSeg> &segs,
+               std::vector<PackRegion> &regions, int64_t *packed_elems) {
+  CXA_CHECK(p && packed_elems, CXA_ERR_INVALID, "pack views: bad arguments");
+  if (segs.empty()) segs = p->segs;
+  CXA_CHECK(segs.size() == p->segs.size(), CXA_ERR_INVALID, "pack views: segment count");
+  if ((which == 0 ? p->copyA : p->copyB) || is_complex(p->dtype))
+    return fail(CXA_ERR_UNSUPPORTED, "pack views: operand permuted into a workspace or complex");
   regions.clear();
   std::map<std::array<int64_t, 5>, int64_t> at;  // view -> packed offset
   int64_t total = 0;
   for (const GBlk &b : p->blks) {
     for (int s = b.seg0; s < b.seg0 + b.nseg; ++s) {
       GSeg &sg = segs[s];
-      const int64_t K = sg.K, N = b.N;
-      if (K > 0 && N > 0) {
-        CXA_CHECK(sg.b_off >= 0 && sg.sbk >= 0 && sg.sbn >= 0 &&
-                      sg.b_off + (K - 1) * sg.sbk + (N - 1) * sg.sbn < b_elems,
-                  CXA_ERR_INVALID, "bc_pack_b: segment %d reads outside B", s);
+      int64_t &off0 = which == 0 ? sg.a_off : sg.b_off;
+      int64_t &sk = which == 0 ? sg.sak : sg.sbk;
+      int64_t &sx = which == 0 ? sg.sam : sg.sbn;
+      const int64_t K = sg.K, X = which == 0 ? b.M : b.N;
+      if (K > 0 && X > 0) {
+        CXA_CHECK(off0 >= 0 && sk >= 0 && sx >= 0 && off0 + (K - 1) * sk + (X - 1) * sx < elems,
+                  CXA_ERR_INVALID, "pack views: segment %d reads outside its operand", s);
       }
-      const std::array<int64_t, 5> key{sg.b_off, sg.sbk, sg.sbn, K, N};
+      const std::array<int64_t, 5> key{off0, sk, sx, K, X};
       auto it = at.find(key);
       int64_t off;
       if (it != at.end()) {
@@ -568,17 +584,69 @@ int bc_pack_b(const cxa_bc_plan *p, int64_t b_elems, std::vector<GSeg> &segs,
       } else {
         off = total;
         at.emplace(key, off);
-        CXA_CHECK(K <= INT32_MAX && N <= INT32_MAX, CXA_ERR_UNSUPPORTED,
-                  "bc_pack_b: view too large");
-        regions.push_back(PackRegion{off, sg.b_off, sg.sbk, sg.sbn, (int32_t)K, (int32_t)N});
-        total += (K + 3) / 4 * N * 4;
+        CXA_CHECK(K <= INT32_MAX && X <= INT32_MAX, CXA_ERR_UNSUPPORTED,
+                  "pack views: view too large");
+        regions.push_back(PackRegion{off, off0, sk, sx, (int32_t)K, (int32_t)X});
+        total += (K + 3) / 4 * X * 4;
       }
-      sg.b_off = off;
-      sg.sbk = 4 * N;
-      sg.sbn = 4;
+      off0 = off;
+      sk = 4 * X;
+      sx = 4;
     }
   }
   *packed_elems = total;
+  return CXA_OK;
+}
+
+}  // namespace
+
+int bc_pack_b(const cxa_bc_plan *p, int64_t elems, std::vector<GSeg> &segs,
+              std::vector<PackRegion> &regions, int64_t *packed_elems) {
+  return pack_views(p, 1, elems, segs, regions, packed_elems);
+}
+
+int bc_pack_a(const cxa_bc_plan *p, int64_t elems, std::vector<GSeg> &segs,
+              std::vector<PackRegion> &regions, int64_t *packed_elems) {
+  return pack_views(p, 0, elems, segs, regions, packed_elems);
+}
+
+int bc_mix_pack_outs(const cxa_bc_plan *mix, const std::vector<PackRegion> &regions,
+                     std::vector<MixOut> &outs) {
+  CXA_CHECK(mix && mix->mix, CXA_ERR_INVALID, "bc_mix_pack_outs: not a mix plan");
+  // regions by source start (views of distinct blocks do not overlap)
+  std::vector<const PackRegion *> by(regions.size());
+  for (size_t i = 0; i < regions.size(); ++i) by[i] = &regions[i];
+  std::sort(by.begin(), by.end(),
+            [](const PackRegion *x, const PackRegion *y) { return x->src_off < y->src_off; });
+  outs = mix->outs;
+  for (const GBlk &it : mix->blks) {
+    for (int j = it.out0; j < it.out0 + it.nout; ++j) {
+      MixOut &o = outs[j];
+      auto up = std::upper_bound(by.begin(), by.end(), o.c_off,
+                                 [](int64_t v, const PackRegion *r) { return v < r->src_off; });
+      if (up == by.begin())
+        return fail(CXA_ERR_UNSUPPORTED, "mix output %d lies before every region", j);
+      const PackRegion &R = **(up - 1);
+      const int64_t d = o.c_off - R.src_off;
+      // strides of extent-1 indices are arbitrary (the planner reports 0 for them)
+      int64_t k0 = 0;
+      bool ok = it.N == R.X && (R.X == 1 || R.sx == 1);
+      if (ok && R.K == 1) {
+        ok = d == 0 && it.M == 1;
+      } else if (ok) {
+        ok = R.sk > 0 && d >= 0 && d % R.sk == 0 && (it.M == 1 || o.ldc == R.sk);
+        if (ok) k0 = d / R.sk;
+      }
+      if (!ok || k0 + it.M > R.K)
+        return fail(CXA_ERR_UNSUPPORTED,
+                    "mix output %d (c_off %lld, ldc %lld, item %d x %d) does not map onto the "
+                    "packed region at %lld (K %d, X %d, sk %lld, sx %lld)",
+                    j, (long long)o.c_off, (long long)o.ldc, it.M, it.N, (long long)R.src_off,
+                    R.K, R.X, (long long)R.sk, (long long)R.sx);
+      o.c_off = R.dst_off;
+      o.ldc = k0;
+    }
+  }
   return CXA_OK;
 }
 

@@ -11,27 +11,44 @@
 
 namespace cxa {
 
-// conj (complex plans): bit 0 conjugates L, bit 1 conjugates R as they are read.  segs / b_k4:
-// run the plan's tiles with another segment array whose B views are k4-packed (bc_pack_b).
-int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, int a_dt,
-                  int b_dt, int c_dt, int prec, void *workspace, hipStream_t s, int conj = 0,
-                  const GSeg *segs = nullptr, bool b_k4 = false);
-
-// One distinct B view of a plan and where its packed copy goes: source element (k, n) at
-// src_off + k * sbk + n * sbn, packed element (k, n) at dst_off + (k >> 2) * 4N + 4n + (k & 3)
-// for k < 4 * ceil(K / 4) (zero for k >= K).
-struct PackRegion {
-  int64_t dst_off, src_off, sbk, sbn;
-  int32_t K, N;
+// Operand layouts of an execution other than the plan's own (null: the plan's): segments with
+// k4-packed views (GemmLaunch::a_k4 / b_k4), mix outputs written k4-packed (c_k4).
+struct ExecLayout {
+  const GSeg *segs = nullptr;
+  const MixOut *outs = nullptr;
+  bool a_k4 = false, b_k4 = false, c_k4 = false;
 };
 
-// The k4-packed form of a real plan's B operand (read directly from the caller's buffer, no
-// permute copy): the plan's segments with their B views remapped into the packed buffer
-// (b_off = region start, sbk = 4N, sbn = 4), the regions to fill (one per distinct view, each
-// a multiple of 4 elements), and the packed buffer's size.  Views are checked against the B
-// buffer's b_elems elements.  CXA_ERR_UNSUPPORTED for plans that cannot be packed.
-int bc_pack_b(const cxa_bc_plan *p, int64_t b_elems, std::vector<GSeg> &segs,
+// conj (complex plans): bit 0 conjugates L, bit 1 conjugates R as they are read.
+int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, int a_dt,
+                  int b_dt, int c_dt, int prec, void *workspace, hipStream_t s, int conj = 0,
+                  const ExecLayout *lay = nullptr);
+
+// One distinct operand view of a plan and where its packed copy goes: source element (k, x) at
+// src_off + k * sk + x * sx (x = n of a B view, m of an A view), packed element (k, x) at
+// dst_off + (k >> 2) * 4X + 4x + (k & 3) for k < 4 * ceil(K / 4) (zero for k >= K).
+struct PackRegion {
+  int64_t dst_off, src_off, sk, sx;
+  int32_t K, X;
+};
+
+// The k4-packed form of a real plan's B (bc_pack_b) or A (bc_pack_a) operand, read directly
+// from the caller's buffer (no permute copy): `segs` (in: the segments to remap, empty for the
+// plan's own; out: those views remapped into the packed buffer, offset = region start, stride
+// along k = 4X, along the free index 4), the regions (one per distinct view, each a multiple
+// of 4 elements) and the packed buffer's size.  Views are checked against the operand buffer's
+// `elems` elements.  CXA_ERR_UNSUPPORTED for plans that cannot be packed.
+int bc_pack_b(const cxa_bc_plan *p, int64_t elems, std::vector<GSeg> &segs,
               std::vector<PackRegion> &regions, int64_t *packed_elems);
+int bc_pack_a(const cxa_bc_plan *p, int64_t elems, std::vector<GSeg> &segs,
+              std::vector<PackRegion> &regions, int64_t *packed_elems);
+
+// A mix plan's outputs redirected into the packed A regions of the launch that reads them
+// (regions from bc_pack_a on that launch's plan): every output must be rows k0.. k0+M-1 of
+// one region read with unit stride along x (c_off = region start + k0 * sk, ldc = sk,
+// N = X).  On success outs holds the packed outputs (c_off = region's dst_off, ldc = k0).
+int bc_mix_pack_outs(const cxa_bc_plan *mix, const std::vector<PackRegion> &regions,
+                     std::vector<MixOut> &outs);
 bool bc_is_mix(const cxa_bc_plan *p);
 int64_t bc_workspace_elems(const cxa_bc_plan *p);
 double bc_flops(const cxa_bc_plan *p);

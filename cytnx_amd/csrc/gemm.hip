@@ -49,6 +49,10 @@ struct alignas(8) cf {
 constexpr int kWaves = 4;
 constexpr int kRowsN = 8;  // widest block the row-streaming path takes
 
+// Operand layouts of a launch (template bit set PK): B, A k4-packed (GemmLaunch::b_k4 / a_k4),
+// mix outputs written k4-packed (GemmLaunch::c_k4).
+constexpr int kPackB = 1, kPackA = 2, kPackC = 4;
+
 __device__ __forceinline__ f64x4 mma64(double a, double b, f64x4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -147,23 +151,40 @@ struct Frag {
 
 // `live` is false once the cursor has passed the end of its K range: every element then reads
 // the zero cell (the pipelines load only chunks inside the range, so this is a guard).  With
-// BK4 the B operand is k4-packed (element (k, n) at (k >> 2) * sbk + n * 4 + (k & 3) from the
-// segment's origin, whole quads): the 4 k of a lane's column are one vector load.
-template <typename TA, typename TB, int MI, int NI, bool BK4 = false>
+// kPackA / kPackB in PK the A / B operand is k4-packed (element (m, k) at (k >> 2) * sak +
+// m * 4 + (k & 3), element (k, n) at (k >> 2) * sbk + n * 4 + (k & 3) from the segment's
+// origin, whole quads): the 4 k of a lane's row or column are one vector load.
+template <typename TA, typename TB, int MI, int NI, int PK = 0>
 __device__ __forceinline__ void load_frag(Frag<TA, TB, MI, NI> &f, gptr<const TA> Ap,
                                           gptr<const TB> Bp, const GSeg &sg, int k0, int mt,
                                           int nt, int r, int g, bool live) {
+  if constexpr ((PK & kPackA) != 0) {
+    typedef TA ta4 __attribute__((ext_vector_type(4)));
+    const bool quad = live && k0 + 4 * g < sg.K;
+    const int64_t qoff = (int64_t)((k0 >> 2) + g) * sg.sak;
 #pragma unroll
-  for (int i = 0; i < MI; ++i) {
-    const int row = 16 * i + r;
+    for (int i = 0; i < MI; ++i) {
+      const int row = 16 * i + r;
+      const ta4 v = load_or_zero<ta4>((gptr<const ta4>)(Ap + (qoff + (int64_t)row * 4)),
+                                      quad && row < mt);
+      f.a[i][0] = v.x;
+      f.a[i][1] = v.y;
+      f.a[i][2] = v.z;
+      f.a[i][3] = v.w;
+    }
+  } else {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = k0 + 4 * g + j;
-      f.a[i][j] = load_or_zero<TA>(Ap + ((int64_t)row * sg.sam + (int64_t)k * sg.sak),
-                                   live && row < mt && k < sg.K);
+    for (int i = 0; i < MI; ++i) {
+      const int row = 16 * i + r;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = k0 + 4 * g + j;
+        f.a[i][j] = load_or_zero<TA>(Ap + ((int64_t)row * sg.sam + (int64_t)k * sg.sak),
+                                     live && row < mt && k < sg.K);
+      }
     }
   }
-  if constexpr (BK4) {
+  if constexpr ((PK & kPackB) != 0) {
     typedef TB tb4 __attribute__((ext_vector_type(4)));
     const bool quad = live && k0 + 4 * g < sg.K;  // quads past K are not stored
     const int64_t qoff = (int64_t)((k0 >> 2) + g) * sg.sbk;
@@ -251,10 +272,10 @@ struct ChunkCursor {
     Bp = at<const TB>(Bbase, sg.b_off) + (int64_t)n0 * sg.sbn;
   }
   __device__ __forceinline__ bool live() const { return s < nseg; }
-  template <int MI, int NI, bool BK4>
+  template <int MI, int NI, int PK>
   __device__ __forceinline__ void load(Frag<TA, TB, MI, NI> &f, int mt, int nt, int r,
                                        int g) const {
-    load_frag<TA, TB, MI, NI, BK4>(f, Ap, Bp, sg, k0, mt, nt, r, g, live());
+    load_frag<TA, TB, MI, NI, PK>(f, Ap, Bp, sg, k0, mt, nt, r, g, live());
   }
   // skip `nchunk` chunks forward (crossing segment ends); past the last segment the cursor
   // stays exhausted (live() false)
@@ -282,7 +303,7 @@ struct ChunkCursor {
 // Only real chunks are loaded: the launches are bound by the vector-memory address path, so
 // padding a wave's chunk count to whole rounds of the ring (which would let every wait count
 // be exact) costs more than the waits it saves.
-template <int PREC, typename TA, typename TB, int MI, int NI, bool BK4, typename SEGS>
+template <int PREC, typename TA, typename TB, int MI, int NI, int PK, typename SEGS>
 __device__ __forceinline__ void mfma_accumulate(RealAcc<PREC, MI, NI> &acc,
                                                 const char *__restrict__ Abase,
                                                 const char *__restrict__ Bbase, const GBlk &blk,
@@ -308,7 +329,7 @@ __device__ __forceinline__ void mfma_accumulate(RealAcc<PREC, MI, NI> &acc,
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     if (d < total) {
-      cur.template load<MI, NI, BK4>(f[d], mt, nt, r, g);
+      cur.template load<MI, NI, PK>(f[d], mt, nt, r, g);
       cur.skip(nslices, Abase, Bbase);
     }
   }
@@ -319,7 +340,7 @@ __device__ __forceinline__ void mfma_accumulate(RealAcc<PREC, MI, NI> &acc,
       if (c < total) {
         mma_frag<PREC, TA, TB, MI, NI>(acc, f[d]);
         if (c + D < total) {
-          cur.template load<MI, NI, BK4>(f[d], mt, nt, r, g);
+          cur.template load<MI, NI, PK>(f[d], mt, nt, r, g);
           cur.skip(nslices, Abase, Bbase);
         }
       }
@@ -335,7 +356,7 @@ __device__ __forceinline__ void store_c(gptr<TC> dst, double val, const GBlk &bl
   *dst = (TC)val;
 }
 
-template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, bool BK4,
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int PK,
           typename SEGS>
 __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
                                                const char *__restrict__ Bbase,
@@ -343,7 +364,7 @@ __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
                                                SEGS segs, int m0, int n0, int lane) {
   RealAcc<PREC, MI, NI> acc;
   acc.zero();
-  mfma_accumulate<PREC, TA, TB, MI, NI, BK4>(acc, Abase, Bbase, blk, segs, m0, n0, lane, 0, 1);
+  mfma_accumulate<PREC, TA, TB, MI, NI, PK>(acc, Abase, Bbase, blk, segs, m0, n0, lane, 0, 1);
   const int r = lane & 15, g = lane >> 4;
   const int mt = min(16 * MI, blk.M - m0), nt = min(16 * NI, blk.N - n0);
   gptr<TC> Cp = at<TC>(Cbase, blk.c_off);
@@ -362,7 +383,7 @@ __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
 
 // Split-K tile: every wave of the workgroup (WAVES of them) calls this for the same tile; wave
 // w takes the K chunks c with c % WAVES == w, and the partial tiles are summed in wave order.
-template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int WAVES, bool BK4,
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int WAVES, int PK,
           typename SEGS>
 __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
                                              const char *__restrict__ Bbase,
@@ -371,7 +392,7 @@ __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
                                              double (*red)[16][64]) {
   RealAcc<PREC, MI, NI> acc;
   acc.zero();
-  mfma_accumulate<PREC, TA, TB, MI, NI, BK4>(acc, Abase, Bbase, blk, segs, m0, n0, lane, wave,
+  mfma_accumulate<PREC, TA, TB, MI, NI, PK>(acc, Abase, Bbase, blk, segs, m0, n0, lane, wave,
                                         WAVES);
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -426,7 +447,13 @@ struct RowCursor {
 // (B rows are wave-uniform reads).
 constexpr int kRowsDepth = 8;
 
-template <typename TA, typename TB, bool BK4, typename SEGS>
+// element offset of k along a row of A (k4-packed: quad stride sak, 4 consecutive k adjacent)
+template <int PK>
+__device__ __forceinline__ int64_t a_koff(int k, int64_t sak) {
+  return (PK & kPackA) ? (int64_t)(k >> 2) * sak + (k & 3) : (int64_t)k * sak;
+}
+
+template <typename TA, typename TB, int PK, typename SEGS>
 __device__ __forceinline__ void rows_accumulate(double (&acc)[kRowsN], const char *Abase,
                                                 const char *Bbase, const GBlk &blk, SEGS segs,
                                                 int row, bool valid, int slice, int nslices) {
@@ -449,8 +476,8 @@ __device__ __forceinline__ void rows_accumulate(double (&acc)[kRowsN], const cha
 #pragma unroll
   for (int d = 0; d < kRowsDepth; ++d) {
     if (d < total) {
-      a[d] = load_or_zero<TA>(at<const TA>(Abase, lc.sg.a_off) +
-                                  ((int64_t)row * lc.sg.sam + (int64_t)lc.k * lc.sg.sak),
+      a[d] = load_or_zero<TA>(at<const TA>(Abase, lc.sg.a_off) + ((int64_t)row * lc.sg.sam +
+                                                                  a_koff<PK>(lc.k, lc.sg.sak)),
                               valid);
       lc.skip(nslices);
     }
@@ -461,16 +488,17 @@ __device__ __forceinline__ void rows_accumulate(double (&acc)[kRowsN], const cha
       const int c = base + d;
       if (c < total) {
         gptr<const TB> bk = at<const TB>(Bbase, cc.sg.b_off) +
-                            (BK4 ? (int64_t)(cc.k >> 2) * cc.sg.sbk + (cc.k & 3)
+                            ((PK & kPackB) ? (int64_t)(cc.k >> 2) * cc.sg.sbk + (cc.k & 3)
                                  : (int64_t)cc.k * cc.sg.sbk);
 #pragma unroll
         for (int n = 0; n < kRowsN; ++n)
           if (n < N) acc[n] += (double)a[d] * (double)bk[(int64_t)n * cc.sg.sbn];
         cc.skip(nslices);
         if (c + kRowsDepth < total) {
-          a[d] = load_or_zero<TA>(at<const TA>(Abase, lc.sg.a_off) +
-                                      ((int64_t)row * lc.sg.sam + (int64_t)lc.k * lc.sg.sak),
-                                  valid);
+          a[d] = load_or_zero<TA>(
+              at<const TA>(Abase, lc.sg.a_off) +
+                  ((int64_t)row * lc.sg.sam + a_koff<PK>(lc.k, lc.sg.sak)),
+              valid);
           lc.skip(nslices);
         }
       }
@@ -479,7 +507,7 @@ __device__ __forceinline__ void rows_accumulate(double (&acc)[kRowsN], const cha
   drain_loads();
 }
 
-template <typename TA, typename TB, typename TC, bool BK4, typename SEGS>
+template <typename TA, typename TB, typename TC, int PK, typename SEGS>
 __device__ __forceinline__ void wave_rows(const char *__restrict__ Abase,
                                           const char *__restrict__ Bbase,
                                           char *__restrict__ Cbase, const GBlk &blk, SEGS segs,
@@ -487,7 +515,7 @@ __device__ __forceinline__ void wave_rows(const char *__restrict__ Abase,
   const int row = m0 + lane;
   const bool valid = row < blk.M;
   double acc[kRowsN];
-  rows_accumulate<TA, TB, BK4>(acc, Abase, Bbase, blk, segs, row, valid, 0, 1);
+  rows_accumulate<TA, TB, PK>(acc, Abase, Bbase, blk, segs, row, valid, 0, 1);
   if (!valid) return;
   gptr<TC> Cp = at<TC>(Cbase, blk.c_off) + (int64_t)row * blk.ldc;
 #pragma unroll
@@ -496,7 +524,7 @@ __device__ __forceinline__ void wave_rows(const char *__restrict__ Abase,
 }
 
 // Split-K row tile: the four waves take interleaved k values of the same 64 rows.
-template <typename TA, typename TB, typename TC, bool BK4, typename SEGS>
+template <typename TA, typename TB, typename TC, int PK, typename SEGS>
 __device__ __forceinline__ void wg_rows(const char *__restrict__ Abase,
                                         const char *__restrict__ Bbase, char *__restrict__ Cbase,
                                         const GBlk &blk, SEGS segs, int m0, int lane, int wave,
@@ -504,7 +532,7 @@ __device__ __forceinline__ void wg_rows(const char *__restrict__ Abase,
   const int row = m0 + lane;
   const bool valid = row < blk.M;
   double acc[kRowsN];
-  rows_accumulate<TA, TB, BK4>(acc, Abase, Bbase, blk, segs, row, valid, wave, kWaves);
+  rows_accumulate<TA, TB, PK>(acc, Abase, Bbase, blk, segs, row, valid, wave, kWaves);
 #pragma unroll
   for (int n = 0; n < kRowsN; ++n) red[wave][n][lane] = acc[n];
   __syncthreads();
@@ -639,7 +667,7 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
 // register budget (occupancy matters most for the memory-bound row tiles).
 enum TileClass { TC_MIXED = 0, TC_ROWS = 1, TC_BIG = 2 };
 
-template <int PREC, typename TA, typename TB, typename TC, bool CPLX, int CLS, bool BK4,
+template <int PREC, typename TA, typename TB, typename TC, bool CPLX, int CLS, int PK,
           typename SEGS>
 __device__ __forceinline__ void run_tile(int kind, const char *A, const char *B, char *C,
                                          const GBlk &blk, SEGS segs, int m0, int n0, int lane,
@@ -652,18 +680,18 @@ __device__ __forceinline__ void run_tile(int kind, const char *A, const char *B,
       default: wave_tile_cplx<TA, 2, 2>(A, B, C, blk, segs, m0, n0, lane, conj); break;
     }
   } else if constexpr (CLS == TC_ROWS) {
-    wave_rows<TA, TB, TC, BK4>(A, B, C, blk, segs, m0, lane);
+    wave_rows<TA, TB, TC, PK>(A, B, C, blk, segs, m0, lane);
   } else if constexpr (CLS == TC_BIG) {
-    wave_tile_real<PREC, TA, TB, TC, 4, 2, BK4>(A, B, C, blk, segs, m0, n0, lane);
+    wave_tile_real<PREC, TA, TB, TC, 4, 2, PK>(A, B, C, blk, segs, m0, n0, lane);
   } else {
     switch (kind) {
-      case TK_1x1: wave_tile_real<PREC, TA, TB, TC, 1, 1, BK4>(A, B, C, blk, segs, m0, n0, lane);
+      case TK_1x1: wave_tile_real<PREC, TA, TB, TC, 1, 1, PK>(A, B, C, blk, segs, m0, n0, lane);
         break;
-      case TK_2x1: wave_tile_real<PREC, TA, TB, TC, 2, 1, BK4>(A, B, C, blk, segs, m0, n0, lane);
+      case TK_2x1: wave_tile_real<PREC, TA, TB, TC, 2, 1, PK>(A, B, C, blk, segs, m0, n0, lane);
         break;
-      case TK_1x2: wave_tile_real<PREC, TA, TB, TC, 1, 2, BK4>(A, B, C, blk, segs, m0, n0, lane);
+      case TK_1x2: wave_tile_real<PREC, TA, TB, TC, 1, 2, PK>(A, B, C, blk, segs, m0, n0, lane);
         break;
-      default: wave_tile_real<PREC, TA, TB, TC, 2, 2, BK4>(A, B, C, blk, segs, m0, n0, lane);
+      default: wave_tile_real<PREC, TA, TB, TC, 2, 2, PK>(A, B, C, blk, segs, m0, n0, lane);
         break;
     }
   }
@@ -671,7 +699,7 @@ __device__ __forceinline__ void run_tile(int kind, const char *A, const char *B,
 
 // grouped launch: tiles, blocks and segments come from device arrays; one wave per tile.
 // Workgroup w runs on XCD w % kXcds and takes tiles from that XCD's range of the list.
-template <int PREC, typename TA, typename TB, typename TC, bool CPLX, int CLS, bool BK4>
+template <int PREC, typename TA, typename TB, typename TC, bool CPLX, int CLS, int PK>
 __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict__ tiles,
                                                         int per_xcd, const GBlk *__restrict__ blks,
                                                         const GSeg *__restrict__ segs,
@@ -685,7 +713,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict
     const GTile tl = range[t];
     if (tl.blk < 0) continue;  // padding
     const GBlk blk = blks[tl.blk];
-    run_tile<PREC, TA, TB, TC, CPLX, CLS, BK4>(tl.kind, A, B, C, blk, SegArray{segs}, tl.m0,
+    run_tile<PREC, TA, TB, TC, CPLX, CLS, PK>(tl.kind, A, B, C, blk, SegArray{segs}, tl.m0,
                                                tl.n0,
                                           lane, conj);
   }
@@ -693,7 +721,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict
 
 // grouped launch with workgroup split-K: one workgroup per tile (real dtypes; the mixed MFMA
 // class or the row class)
-template <int PREC, typename TA, typename TB, typename TC, int CLS, int WAVES, bool BK4>
+template <int PREC, typename TA, typename TB, typename TC, int CLS, int WAVES, int PK>
 __global__ __launch_bounds__(64 * WAVES) void k_grouped_split(
     const GTile *__restrict__ tiles, int per_xcd, const GBlk *__restrict__ blks,
     const GSeg *__restrict__ segs, const char *A, const char *B, char *C) {
@@ -709,26 +737,26 @@ __global__ __launch_bounds__(64 * WAVES) void k_grouped_split(
     const GBlk blk = blks[tl.blk];
     const SegArray sa{segs};
     if constexpr (CLS == TC_ROWS) {
-      wg_rows<TA, TB, TC, BK4>(A, B, C, blk, sa, tl.m0, lane, wave, red);
+      wg_rows<TA, TB, TC, PK>(A, B, C, blk, sa, tl.m0, lane, wave, red);
     } else {
       switch (tl.kind) {
         case TK_1x1:
-          wg_tile_real<PREC, TA, TB, TC, 1, 1, WAVES, BK4>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
+          wg_tile_real<PREC, TA, TB, TC, 1, 1, WAVES, PK>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
                                                           wave,
                                                      red);
           break;
         case TK_2x1:
-          wg_tile_real<PREC, TA, TB, TC, 2, 1, WAVES, BK4>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
+          wg_tile_real<PREC, TA, TB, TC, 2, 1, WAVES, PK>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
                                                           wave,
                                                      red);
           break;
         case TK_1x2:
-          wg_tile_real<PREC, TA, TB, TC, 1, 2, WAVES, BK4>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
+          wg_tile_real<PREC, TA, TB, TC, 1, 2, WAVES, PK>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
                                                           wave,
                                                      red);
           break;
         default:
-          wg_tile_real<PREC, TA, TB, TC, 2, 2, WAVES, BK4>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
+          wg_tile_real<PREC, TA, TB, TC, 2, 2, WAVES, PK>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
                                                           wave,
                                                      red);
           break;
@@ -752,7 +780,7 @@ struct SegStride {
 
 // Segment i of `segs` (nseg <= NACC) accumulates into acc[i]; one load pipeline runs across
 // all of them (as in mfma_accumulate).
-template <int PREC, typename TA, typename TB, int MI, int NI, int NACC, bool BK4, typename SEGS>
+template <int PREC, typename TA, typename TB, int MI, int NI, int NACC, int PK, typename SEGS>
 __device__ __forceinline__ void mfma_accumulate_multi(RealAcc<PREC, MI, NI> (&acc)[NACC],
                                                       const char *__restrict__ Abase,
                                                       const char *__restrict__ Bbase, SEGS segs,
@@ -784,7 +812,7 @@ __device__ __forceinline__ void mfma_accumulate_multi(RealAcc<PREC, MI, NI> (&ac
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     if (d < total) {
-      cur.template load<MI, NI, BK4>(f[d], mt, nt, r, g);
+      cur.template load<MI, NI, PK>(f[d], mt, nt, r, g);
       cur.skip(1, Abase, Bbase);
     }
   }
@@ -809,7 +837,7 @@ __device__ __forceinline__ void mfma_accumulate_multi(RealAcc<PREC, MI, NI> (&ac
       if (c < total) {
         consume(f[d]);
         if (c + D < total) {
-          cur.template load<MI, NI, BK4>(f[d], mt, nt, r, g);
+          cur.template load<MI, NI, PK>(f[d], mt, nt, r, g);
           cur.skip(1, Abase, Bbase);
         }
       }
@@ -818,7 +846,7 @@ __device__ __forceinline__ void mfma_accumulate_multi(RealAcc<PREC, MI, NI> (&ac
   drain_loads();
 }
 
-template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int WAVES, bool BK4>
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int WAVES, int PK>
 __global__ __launch_bounds__(64 * WAVES) void k_mix(const GTile *__restrict__ tiles,
                                                    int per_xcd, const GBlk *__restrict__ items,
                                                    const GSeg *__restrict__ slices,
@@ -843,7 +871,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mix(const GTile *__restrict__ ti
       RealAcc<PREC, MI, NI> acc[NACC];
 #pragma unroll
       for (int q = 0; q < NACC; ++q) acc[q].zero();
-      mfma_accumulate_multi<PREC, TA, TB, MI, NI, NACC, BK4>(
+      mfma_accumulate_multi<PREC, TA, TB, MI, NI, NACC, PK>(
           acc, A, B, SegStride{slices, it.seg0 + wave, WAVES}, mine, it.M, it.N, tl.m0, tl.n0,
           lane);
 #pragma unroll
@@ -861,6 +889,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_mix(const GTile *__restrict__ ti
       }
     }
     __syncthreads();
+    // Each lane writes rows 16i + 4g .. 16i + 4g + 3 of its column: the slice tiles are in LDS,
+    // so the values come from whichever lane and register held those rows (the FP64 MFMA
+    // layout spreads four consecutive rows over the four lane groups), and a packed output
+    // gets them as one k-quad store where the quad is aligned and whole.
     const int mt = min(16 * MI, it.M - tl.m0), nt = min(16 * NI, it.N - tl.n0);
     for (int j = wave; j < it.nout; j += WAVES) {
       const MixOut &o = outs[it.out0 + j];
@@ -868,19 +900,49 @@ __global__ __launch_bounds__(64 * WAVES) void k_mix(const GTile *__restrict__ ti
 #pragma unroll
       for (int i = 0; i < MI; ++i)
 #pragma unroll
-        for (int jj = 0; jj < NI; ++jj)
+        for (int jj = 0; jj < NI; ++jj) {
+          const int col = 16 * jj + r;
+          const int row0 = 16 * i + 4 * g;
+          TC vals[4];
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int e = (i * NI + jj) * 4 + v;
+            // row row0 + v sits in lane group gs, register slot vs (acc_row(F64, gs, vs))
+            const int gs = F64 ? v : g, vs = F64 ? g : v;
+            const int e = (i * NI + jj) * 4 + vs;
+            const int src = r + 16 * gs;
             double val = 0.0;
 #pragma unroll
             for (int q = 0; q < kMixSlices; ++q)
-              if (q < it.nseg) val += o.coef[q] * (double)xs[q][e][lane];
-            const int row = 16 * i + acc_row(F64, g, v);
-            const int col = 16 * jj + r;
-            if (row < mt && col < nt)
-              Cp[(int64_t)(tl.m0 + row) * o.ldc + (tl.n0 + col)] = (TC)val;
+              if (q < it.nseg) val += o.coef[q] * (double)xs[q][e][src];
+            vals[v] = (TC)val;
           }
+          if (col < nt) {
+            if constexpr ((PK & kPackC) != 0) {  // c_off: region base, ldc: first k
+              const int64_t kk = o.ldc + tl.m0 + row0;
+              const int64_t cpos = (int64_t)(tl.n0 + col) * 4;
+              const int64_t qs = 4 * (int64_t)it.N;
+              if ((kk & 3) == 0 && row0 + 3 < mt) {
+                typedef TC tc4 __attribute__((ext_vector_type(4)));
+                tc4 w;
+                w.x = vals[0];
+                w.y = vals[1];
+                w.z = vals[2];
+                w.w = vals[3];
+                *(gptr<tc4>)(Cp + ((kk >> 2) * qs + cpos)) = w;
+              } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                  if (row0 + v < mt)
+                    Cp[((kk + v) >> 2) * qs + cpos + ((kk + v) & 3)] = vals[v];
+              }
+            } else {
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                if (row0 + v < mt)
+                  Cp[(int64_t)(tl.m0 + row0 + v) * o.ldc + (tl.n0 + col)] = vals[v];
+            }
+          }
+        }
     }
     __syncthreads();  // the next tile reuses xs
   }
@@ -894,28 +956,33 @@ constexpr bool k4_ok() {
                                : std::is_same<TB, double>::value;
 }
 
-template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, bool BK4>
+template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int PK>
 void launch_mix_kind_t(const GemmLaunch &g, int grid, hipStream_t s) {
   const char *A = (const char *)g.A, *B = (const char *)g.B;
   char *C = (char *)g.C;
   if (g.splitk)  // four waves, two slices each
-    hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, MI, NI, 4, BK4>), dim3(grid), dim3(64 * 4), 0,
+    hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, MI, NI, 4, PK>), dim3(grid), dim3(64 * 4), 0,
                        s, g.tiles, g.per_xcd, g.blks, g.segs, g.outs, A, B, C);
   else  // one wave per slice
-    hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, MI, NI, kMixSlices, BK4>), dim3(grid),
+    hipLaunchKernelGGL((k_mix<PREC, TA, TB, TC, MI, NI, kMixSlices, PK>), dim3(grid),
                        dim3(64 * kMixSlices), 0, s, g.tiles, g.per_xcd, g.blks, g.segs, g.outs,
                        A, B, C);
 }
 
+// mix layouts: plain, packed B, packed B with packed outputs (launch_mix_t checks the flags)
 template <int PREC, typename TA, typename TB, typename TC, int MI, int NI>
 void launch_mix_kind(const GemmLaunch &g, int grid, hipStream_t s) {
   if constexpr (k4_ok<PREC, TB>()) {
+    if (g.b_k4 && g.c_k4) {
+      launch_mix_kind_t<PREC, TA, TB, TC, MI, NI, kPackB | kPackC>(g, grid, s);
+      return;
+    }
     if (g.b_k4) {
-      launch_mix_kind_t<PREC, TA, TB, TC, MI, NI, true>(g, grid, s);
+      launch_mix_kind_t<PREC, TA, TB, TC, MI, NI, kPackB>(g, grid, s);
       return;
     }
   }
-  launch_mix_kind_t<PREC, TA, TB, TC, MI, NI, false>(g, grid, s);
+  launch_mix_kind_t<PREC, TA, TB, TC, MI, NI, 0>(g, grid, s);
 }
 
 template <int PREC, typename TA, typename TB, typename TC>
@@ -924,8 +991,8 @@ int launch_mix_t(const GemmLaunch &g, hipStream_t s) {
   if (g.ntiles != kXcds * g.per_xcd)
     return fail(CXA_ERR_INVALID, "mix GEMM: tile list of %d is not %d ranges of %d", g.ntiles,
                 kXcds, g.per_xcd);
-  if (g.b_k4 && !k4_ok<PREC, TB>())
-    return fail(CXA_ERR_UNSUPPORTED, "mix GEMM: no packed-B variant for these dtypes");
+  if ((g.b_k4 && !k4_ok<PREC, TB>()) || (g.c_k4 && !g.b_k4) || g.a_k4)
+    return fail(CXA_ERR_UNSUPPORTED, "mix GEMM: no variant for these packed layouts / dtypes");
   const int grid = kXcds * std::min(g.per_xcd, 2048);
   switch (g.kind0) {
     case TK_1x1: launch_mix_kind<PREC, TA, TB, TC, 1, 1>(g, grid, s); break;
@@ -973,42 +1040,43 @@ __global__ __launch_bounds__(64 * kWaves) void k_single(GBlk blk, GSeg seg, int 
   const int tc = kind == TK_1x1 || kind == TK_2x1 ? 16 : 32;
   for (int t = blockIdx.x * kWaves + wave; t < ntiles; t += gridDim.x * kWaves) {
     const int m0 = (t / tiles_n) * tr, n0 = (t % tiles_n) * tc;
-    run_tile<PREC, TA, TB, TC, CPLX, TC_MIXED, false>(kind, A, B, C, blk, SegOne{seg}, m0, n0,
+    run_tile<PREC, TA, TB, TC, CPLX, TC_MIXED, 0>(kind, A, B, C, blk, SegOne{seg}, m0, n0,
                                                       lane,
                                                conj);
   }
 }
 
 // the real-dtype grouped launches of one class / split mode
-template <int PREC, typename TA, typename TB, typename TC, bool BK4>
+template <int PREC, typename TA, typename TB, typename TC, int PK>
 void launch_real(const GemmLaunch &g, int cls, int wg_grid, int wave_grid, hipStream_t s) {
   const int P = g.per_xcd;
   const char *A = (const char *)g.A, *B = (const char *)g.B;
   char *C = (char *)g.C;
   if (g.splitk && cls == TC_ROWS)
-    hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_ROWS, kWaves, BK4>), dim3(wg_grid),
+    hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_ROWS, kWaves, PK>), dim3(wg_grid),
                        dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C);
   else if (g.splitk == 8 && cls == TC_MIXED)
-    hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED, 8, BK4>), dim3(wg_grid),
+    hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED, 8, PK>), dim3(wg_grid),
                        dim3(64 * 8), 0, s, g.tiles, P, g.blks, g.segs, A, B, C);
   else if (g.splitk && cls == TC_MIXED)
-    hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED, kWaves, BK4>), dim3(wg_grid),
+    hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED, kWaves, PK>), dim3(wg_grid),
                        dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C);
   else if (cls == TC_ROWS)
-    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_ROWS, BK4>), dim3(wave_grid),
+    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_ROWS, PK>), dim3(wave_grid),
                        dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, g.conj);
   else if (cls == TC_BIG)
-    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_BIG, BK4>), dim3(wave_grid),
+    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_BIG, PK>), dim3(wave_grid),
                        dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, g.conj);
   else
-    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_MIXED, BK4>), dim3(wave_grid),
+    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_MIXED, PK>), dim3(wave_grid),
                        dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, g.conj);
 }
 
 template <int PREC, typename TA, typename TB, typename TC, bool CPLX>
 int launch_t(const GemmLaunch &g, hipStream_t s) {
   if (g.single) {
-    if (g.b_k4) return fail(CXA_ERR_INVALID, "single-block GEMM: packed B is not supported");
+    if (g.a_k4 || g.b_k4 || g.c_k4)
+      return fail(CXA_ERR_INVALID, "single-block GEMM: packed layouts are not supported");
     const GBlk &b = g.blk0;
     if (b.M <= 0 || b.N <= 0) return CXA_OK;
     int kind = g.kind0 >= 0 ? g.kind0 : default_kind(b.M, b.N);
@@ -1039,17 +1107,25 @@ int launch_t(const GemmLaunch &g, hipStream_t s) {
   const int wg_grid = g.grid > 0 ? g.grid : kXcds * std::min(P, 2048);
   const int wave_grid = g.grid > 0 ? g.grid : kXcds * std::min((P + kWaves - 1) / kWaves, 1024);
   if constexpr (!CPLX) {
-    if (g.b_k4) {
+    if (g.c_k4 || (g.a_k4 && !g.b_k4))
+      return fail(CXA_ERR_UNSUPPORTED, "grouped GEMM: no variant for these packed layouts");
+    if (g.a_k4) {
+      if constexpr (k4_ok<PREC, TA>() && k4_ok<PREC, TB>())
+        launch_real<PREC, TA, TB, TC, kPackA | kPackB>(g, cls, wg_grid, wave_grid, s);
+      else
+        return fail(CXA_ERR_UNSUPPORTED, "grouped GEMM: no packed-A variant for these dtypes");
+    } else if (g.b_k4) {
       if constexpr (k4_ok<PREC, TB>())
-        launch_real<PREC, TA, TB, TC, true>(g, cls, wg_grid, wave_grid, s);
+        launch_real<PREC, TA, TB, TC, kPackB>(g, cls, wg_grid, wave_grid, s);
       else
         return fail(CXA_ERR_UNSUPPORTED, "grouped GEMM: no packed-B variant for these dtypes");
     } else {
-      launch_real<PREC, TA, TB, TC, false>(g, cls, wg_grid, wave_grid, s);
+      launch_real<PREC, TA, TB, TC, 0>(g, cls, wg_grid, wave_grid, s);
     }
   } else {
-    if (g.b_k4) return fail(CXA_ERR_UNSUPPORTED, "grouped GEMM: packed B is real-only");
-    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, true, TC_MIXED, false>), dim3(wave_grid),
+    if (g.a_k4 || g.b_k4 || g.c_k4)
+      return fail(CXA_ERR_UNSUPPORTED, "grouped GEMM: packed layouts are real-only");
+    hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, true, TC_MIXED, 0>), dim3(wave_grid),
                        dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
                        (const char *)g.A, (const char *)g.B, (char *)g.C, g.conj);
   }

@@ -84,6 +84,12 @@ struct GemmLaunch {
   // 4, sbn is 4, and every k-quad of a segment is whole (zero past K), so a lane reads the 4
   // consecutive k of one column with one vector load
   bool b_k4 = false;
+  // A is k4-packed (needs b_k4): element (m, k) at a_off + (k >> 2) * sak + m * 4 + (k & 3)
+  bool a_k4 = false;
+  // mix launches (needs b_k4): outputs written k4-packed, each MixOut's c_off the region base
+  // and ldc the k of its first row: output (row, col) at c_off + (k >> 2) * 4N + col * 4 +
+  // (k & 3) with k = ldc + row and N the item's columns
+  bool c_k4 = false;
   // single-block mode: one block with one segment passed by value, no device task arrays
   bool single = false;
   GBlk blk0{};

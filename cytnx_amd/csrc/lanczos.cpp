@@ -158,11 +158,16 @@ struct cxa_chain {
   std::vector<void *> packed_segs, packed64, packed32;
   std::vector<int64_t> packed_elems;
   std::vector<std::vector<cxa::PackRegion>> pack_regions;
+  // A mix step followed by a step that reads its output as A: the output is written k4-packed
+  // (packed_outs: the mix plan's outputs redirected, device) and read as a packed A.
+  std::vector<void *> packed_outs;
+  std::vector<int> a_packed, c_packed;
+  std::vector<int64_t> mid_elems;  // elements of each intermediate buffer (packed: padded)
   void *ws = nullptr;  // permute workspace (max over steps), 16 bytes per element
   int prec = CXA_PREC_FP64;
   double flops = 0;
   ~cxa_chain() {
-    for (auto *v : {&mid64, &mid32, &fixed32, &packed_segs, &packed64, &packed32})
+    for (auto *v : {&mid64, &mid32, &fixed32, &packed_segs, &packed64, &packed32, &packed_outs})
       for (void *p : *v)
         if (p) (void)hipFree(p);
     if (ws) (void)hipFree(ws);
@@ -226,13 +231,21 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
   c->packed32.assign(nsteps, nullptr);
   c->packed_elems.assign(nsteps, 0);
   c->pack_regions.assign(nsteps, {});
+  c->packed_outs.assign(nsteps, nullptr);
+  c->a_packed.assign(nsteps, 0);
+  c->c_packed.assign(nsteps, 0);
+  c->mid_elems.assign(c->mid64.size(), 0);
+  for (size_t k = 0; k < c->mid64.size(); ++k) c->mid_elems[k] = std::max<int64_t>(1, out_elems[k]);
+  std::vector<std::vector<cxa::GSeg>> segs_host(nsteps);
   bool synced = false;
   for (int k = 0; k < nsteps; ++k) {
     if (c->fixed_left[k]) continue;
-    std::vector<cxa::GSeg> segs;
+    std::vector<cxa::GSeg> &segs = segs_host[k];
     int64_t elems = 0;
-    if (cxa::bc_pack_b(plans[k], fixed_elems[k], segs, c->pack_regions[k], &elems) != CXA_OK)
+    if (cxa::bc_pack_b(plans[k], fixed_elems[k], segs, c->pack_regions[k], &elems) != CXA_OK) {
+      segs.clear();
       continue;  // this plan reads B through a workspace: unpacked
+    }
     if (!synced) {  // the fixed operands may come from any stream
       CXA_HIP(hipDeviceSynchronize());
       synced = true;
@@ -245,6 +258,33 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
     CXA_HIP(hipMalloc(&c->packed64[k], std::max<int64_t>(1, elems) * sizeof(double)));
     int rc = fill_packed(c.get(), k, c->packed64[k], false, nullptr);
     if (rc) return rc;
+  }
+  // a mix step whose output the next step reads as its (running) A operand: write it packed
+  for (int k = 0; k + 1 < nsteps; ++k) {
+    if (!cxa::bc_is_mix(plans[k]) || !c->packed_segs[k] || c->fixed_left[k + 1] ||
+        !c->packed_segs[k + 1])
+      continue;
+    std::vector<cxa::GSeg> segs = segs_host[k + 1];  // B already remapped
+    std::vector<cxa::PackRegion> regions;
+    std::vector<cxa::MixOut> outs;
+    int64_t elems = 0;
+    if (cxa::bc_pack_a(plans[k + 1], out_elems[k], segs, regions, &elems) != CXA_OK ||
+        cxa::bc_mix_pack_outs(plans[k], regions, outs) != CXA_OK)
+      continue;
+    CXA_HIP(hipMemcpy(c->packed_segs[k + 1], segs.data(), segs.size() * sizeof(cxa::GSeg),
+                      hipMemcpyHostToDevice));
+    CXA_HIP(hipMalloc(&c->packed_outs[k], std::max<size_t>(1, outs.size()) * sizeof(cxa::MixOut)));
+    if (!outs.empty())
+      CXA_HIP(hipMemcpy(c->packed_outs[k], outs.data(), outs.size() * sizeof(cxa::MixOut),
+                        hipMemcpyHostToDevice));
+    // the packed intermediate: quads past K are never written and stay zero
+    c->mid_elems[k] = std::max<int64_t>(1, elems);
+    CXA_HIP(hipFree(c->mid64[k]));
+    c->mid64[k] = nullptr;
+    CXA_HIP(hipMalloc(&c->mid64[k], c->mid_elems[k] * sizeof(double)));
+    CXA_HIP(hipMemset(c->mid64[k], 0, c->mid_elems[k] * sizeof(double)));
+    c->a_packed[k + 1] = 1;
+    c->c_packed[k] = 1;
   }
   *out = c.release();
   return CXA_OK;
@@ -273,8 +313,11 @@ extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
                               c->fixed_elems[k], s);
     if (rc) return rc;
   }
-  for (size_t k = 0; k < c->mid32.size(); ++k)
-    CXA_HIP(hipMalloc(&c->mid32[k], std::max<int64_t>(1, c->out_elems[k]) * sizeof(float)));
+  for (size_t k = 0; k < c->mid32.size(); ++k) {
+    CXA_HIP(hipMalloc(&c->mid32[k], c->mid_elems[k] * sizeof(float)));
+    if (c->c_packed[k])  // the padding of a packed intermediate must read as zero
+      CXA_HIP(hipMemsetAsync(c->mid32[k], 0, c->mid_elems[k] * sizeof(float), s));
+  }
   return CXA_OK;
 }
 
@@ -291,10 +334,18 @@ int chain_step(cxa_chain *c, int k, const void *x, void *y, hipStream_t s) {
   if (c->fixed_left[k])
     return bc_exec_typed(c->plans[k], f32 ? c->fixed32[k] : c->fixed[k], in, outp, fx_dt, in_dt,
                          out_dt, c->prec, c->ws, s);
-  if (c->packed_segs[k])
+  if (c->packed_segs[k]) {
+    cxa::ExecLayout lay;
+    lay.segs = (const cxa::GSeg *)c->packed_segs[k];
+    lay.b_k4 = true;
+    lay.a_k4 = c->a_packed[k] != 0;
+    if (c->c_packed[k]) {
+      lay.outs = (const cxa::MixOut *)c->packed_outs[k];
+      lay.c_k4 = true;
+    }
     return bc_exec_typed(c->plans[k], in, f32 ? c->packed32[k] : c->packed64[k], outp, in_dt,
-                         fx_dt, out_dt, c->prec, c->ws, s, 0,
-                         (const cxa::GSeg *)c->packed_segs[k], true);
+                         fx_dt, out_dt, c->prec, c->ws, s, 0, &lay);
+  }
   return bc_exec_typed(c->plans[k], in, f32 ? c->fixed32[k] : c->fixed[k], outp, in_dt, fx_dt,
                        out_dt, c->prec, c->ws, s);
 }

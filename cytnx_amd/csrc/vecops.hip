@@ -279,7 +279,7 @@ int vec_cast_f64_f32(const double *x, float *y, int64_t n, hipStream_t s) {
 }
 
 // One workgroup per region (grid-stride over regions): packed element e of a region holds
-// source element (k, n) with k = 4 * (e / 4N) + e % 4, n = (e % 4N) / 4, or zero for k >= K.
+// source element (k, x) with k = 4 * (e / 4X) + e % 4, x = (e % 4X) / 4, or zero for k >= K.
 // Consecutive threads write consecutive packed elements.
 template <typename T>
 __global__ __launch_bounds__(kT) void k_pack_b(const double *__restrict__ src, T *__restrict__ dst,
@@ -287,13 +287,13 @@ __global__ __launch_bounds__(kT) void k_pack_b(const double *__restrict__ src, T
                                                int nregions) {
   for (int r = blockIdx.x; r < nregions; r += gridDim.x) {
     const PackRegion R = regions[r];
-    const int64_t row = 4 * (int64_t)R.N;
+    const int64_t row = 4 * (int64_t)R.X;
     const int64_t nel = (int64_t)((R.K + 3) / 4) * row;
     for (int64_t e = threadIdx.x; e < nel; e += kT) {
       const int64_t q = e / row;
       const int64_t rem = e - q * row;
-      const int64_t k = 4 * q + (rem & 3), n = rem >> 2;
-      dst[R.dst_off + e] = k < R.K ? (T)src[R.src_off + k * R.sbk + n * R.sbn] : (T)0;
+      const int64_t k = 4 * q + (rem & 3), x = rem >> 2;
+      dst[R.dst_off + e] = k < R.K ? (T)src[R.src_off + k * R.sk + x * R.sx] : (T)0;
     }
   }
 }

@@ -464,17 +464,21 @@ class Heff2(_HeffBase):
         args = (Lp, {"l": lab_l, "w": lab_w, "lp": lab_lp},
                 psi_m, {"l": lab_l, "s1": s1[0], "s2": s2[0], "r": lab_r}, Wd)
         ylab = {"lp": lab_lp, "s1p": s1p[0], "s2p": s2p[0], "w2": kw, "r": kr}
-        # Y blocks that no (w, s1, s2) slice reaches are zero for every psi: leave them out of
-        # Y, so neither the mix launch nor the contraction with R touches them
+        # Y blocks that no (w, s1, s2) slice reaches are zero for every psi, and Y blocks whose
+        # (w'', r) sector pair no block of R has are never read: leave both out of Y, so
+        # neither launch touches them
+        Rpp = Rp.permute([lab_w2, lab_r, lab_rp]).relabel([kw, kr, lab_rp])
         live = np.zeros(Y.Nblocks(), dtype=bool)
         mix_plan_arrays(*args, Y, ylab, live)
-        Y = subset_blocks(Y, live)
+        rk = {(int(t[0]), int(t[1])) for t in Rpp._itoi}
+        pw, pr = Y.labels().index(kw), Y.labels().index(kr)
+        read = np.array([(int(t[pw]), int(t[pr])) in rk for t in Y._itoi], dtype=bool)
+        Y = subset_blocks(Y, live & read)
         arrays = mix_plan_arrays(*args, Y, ylab)
         psi_left = psi_is_left(Y, ylab)
         na, nb = Lp._storage.numel(), meta_elems(psi_m)
         check_mix_bounds(arrays, *((nb, na) if psi_left else (na, nb)), meta_elems(Y))
         plan = _MixPlan(arrays, dev)
-        Rpp = Rp.permute([lab_w2, lab_r, lab_rp]).relabel([kw, kr, lab_rp])
         O, c1, c2 = contract_meta(Y, Rpp, 3, dev)
         if O.itoi() != psi.itoi() or [b.dim() for b in O.bonds()] != psi.shape():
             raise ValueError("Heff2 mix chain: output block layout differs from psi's")

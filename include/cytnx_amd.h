@@ -233,6 +233,10 @@ typedef int (*cxa_matvec_fn)(void *ctx, const void *x, void *y, cxa_stream_t str
  * nothing.  Plans must be real FP64 and must outlive the chain.  Precision (cxa_chain_prepare)
  * selects FP64 (drop-in), FP32 (FP32 copies of the fixed operands and intermediates; input and
  * output stay FP64) or FP32X3.  cxa_chain_matvec is a cxa_matvec_fn with ctx = the chain.
+ * A fixed operand on the plan's right (B) is read at cxa_chain_create into a k-quad packed
+ * copy (and at cxa_chain_prepare into its FP32 form); the chain uses those copies, so a fixed
+ * operand changed after creation needs a new chain.  cxa_chain_create synchronises the device
+ * once when it makes such copies.
  */
 typedef struct cxa_chain cxa_chain;
 int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const void *const *fixed,
