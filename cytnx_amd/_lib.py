@@ -76,6 +76,7 @@ def _declare(L):
         "cxa_chain_step": (ci, [vp, ci, vp, vp, vp]),
         "cxa_chain_tune": (ci, [vp, vp, vp, ci, intp, intp, dblp, vp]),
         "cxa_chain_flops": (ci, [vp, dblp]),
+        "cxa_chain_layout": (ci, [vp, ci, intp]),
         "cxa_chain_destroy": (ci, [vp]),
         "cxa_chain_graph_create": (ci, [vp, vp, vp, ci, ctypes.POINTER(vp)]),
         "cxa_graph_launch": (ci, [vp, vp]),

@@ -381,6 +381,14 @@ extern "C" int cxa_chain_step(void *ctx, int step, const void *x, void *y, cxa_s
   return chain_step(c, step, x, y, (hipStream_t)stream);
 }
 
+extern "C" int cxa_chain_layout(const cxa_chain *c, int step, int *flags) {
+  CXA_CHECK(c && flags && step >= 0 && step < c->nsteps, CXA_ERR_INVALID,
+            "cxa_chain_layout: bad arguments");
+  *flags = (c->packed_segs[step] ? 1 : 0) | (c->a_packed[step] ? 2 : 0) |
+           (c->c_packed[step] ? 4 : 0);
+  return CXA_OK;
+}
+
 extern "C" int cxa_chain_flops(const cxa_chain *c, double *flops) {
   CXA_CHECK(c && flops, CXA_ERR_INVALID, "cxa_chain_flops: bad arguments");
   *flops = c->flops;

@@ -55,6 +55,13 @@ class _Chain:
         self.flops = f.value
         self.prec = None
 
+    def layout(self, step: int) -> int:
+        """Operand layouts of a step (cxa_chain_layout): bit 0 packed fixed B, bit 1 packed
+        running A, bit 2 outputs written packed for the next step."""
+        f = ctypes.c_int(0)
+        _lib.check(_lib.lib().cxa_chain_layout(self.handle, int(step), ctypes.byref(f)))
+        return f.value
+
     def prepare(self, prec: int) -> None:
         if self.prec != prec:
             _lib.check(_lib.lib().cxa_chain_prepare(self.handle, prec,

@@ -253,6 +253,13 @@ int cxa_chain_step(void *chain, int step, const void *x, void *y, cxa_stream_t s
 int cxa_chain_tune(cxa_chain *chain, const void *x, void *y, int reps, int *kinds, int *splits,
                    double *ms, cxa_stream_t stream);
 int cxa_chain_flops(const cxa_chain *chain, double *flops);
+/*
+ * Operand layouts step `step` runs with (no counterpart in the reference: a diagnostic of this
+ * library's packing): bit 0 its fixed B operand is read k4-packed, bit 1 its running A
+ * operand (the previous step's output) is read k4-packed, bit 2 its outputs are written
+ * k4-packed for the next step.
+ */
+int cxa_chain_layout(const cxa_chain *chain, int step, int *flags);
 int cxa_chain_destroy(cxa_chain *chain);
 
 /*

@@ -63,6 +63,9 @@ def test_heff2_fp64_matches_oracle_chain(chi, mode):
     assert ref.itoi == psi.itoi()
     got = out._storage.cpu().numpy()
     assert relerr(got, ref.flat()) <= 1e-12
+    if mode == "mix":
+        # both fixed operands read k4-packed, Y written packed by the mix and read packed by R
+        assert [H.chain.layout(k) for k in range(2)] == [1 | 4, 1 | 2]
 
 
 def test_heff2_dense_closed_form_and_symmetry():
