@@ -104,6 +104,103 @@ def random_mps(L: int, chi: int, seed: int = 1, device: int = 0, Q: int = 0) -> 
     return As
 
 
+def _orthonormal_rows(Mb: torch.Tensor, rows: torch.Tensor, tol: float = 1e-14,
+                      maxit: int = 80) -> torch.Tensor:
+    """The polar factor of each full-row-rank matrix of the batch Mb (B x mp x np, zero padded;
+    rows[b, i] = 1 on its true rows): X with orthonormal rows spanning the same row space.
+    Newton-Schulz iteration X <- 1.5 X - 0.5 (X X^T) X from X = M / |M|_F (every singular
+    value in (0, 1]), which maps each singular value s to 1.5 s - 0.5 s^3 (to 1, quadratically
+    near it) and keeps the zero padding; only batched GEMMs.  Stops once max |X X^T - I| over
+    the true rows is below tol; a batch that has not converged after maxit steps (a nearly
+    rank-deficient matrix) is finished with a QR of X^T instead."""
+    X = Mb / torch.linalg.vector_norm(Mb, dim=(1, 2), keepdim=True)
+    eye = torch.diag_embed(rows)
+    for it in range(maxit):
+        XXt = torch.bmm(X, X.transpose(1, 2).contiguous())
+        if it % 4 == 0 and float((XXt - eye).abs().max()) <= tol:
+            return X
+        X = 1.5 * X - 0.5 * torch.bmm(XXt, X)
+    XXt = torch.bmm(X, X.transpose(1, 2).contiguous())
+    if float((XXt - eye).abs().max()) <= tol:
+        return X
+    Q1, _ = torch.linalg.qr(X.transpose(1, 2).contiguous(), mode="reduced")
+    return Q1.transpose(1, 2) * rows[:, :, None]
+
+
+def random_right_canonical_mps(L: int, chi: int, seed: int = 1, device: int = 0,
+                               Q: int = 0) -> List[UniTensor]:
+    """A random right-canonical MPS (A_1..A_{L-1} right isometries, A_0 of unit norm) at the
+    bond dimensions of mps.bond_sectors, reduced from the right to what each site can hold
+    (a sector of a_k keeps at most as many states as its rows can be orthonormal: the columns
+    (s, a_{k+1}) reaching it) -- the bond structure a right-to-left LQ sweep of random_mps
+    produces.  Each site's sector matrices (rows a_k, columns (s, a_{k+1})) are iid N(0,1)
+    orthonormalised (_orthonormal_rows); the sites are independent, so the sector matrices of
+    all sites go through one batched iteration per padded size class."""
+    from .svd import _pad_size, _sector_matrices
+    rng = np.random.default_rng(seed)
+    phys = phys_bond().redirect()
+    target = []
+    for k in range(L + 1):
+        q, d = bond_sectors(L, k, chi, Q)
+        target.append(Bond(BD_KET, [[-x] for x in q], d))
+    bonds: List[Optional[Bond]] = [None] * (L + 1)
+    bonds[0], bonds[L] = target[0], target[L]
+    for k in range(L - 1, 0, -1):
+        T = UniTensor([target[k], phys, bonds[k + 1].redirect()], ["a", "s", "b"], 1, 3, -1,
+                      _alloc=False)
+        avail = np.zeros(len(target[k].qnums()), dtype=np.int64)
+        for t, shp in zip(T._itoi, T._shapes):
+            avail[int(t[0])] += int(shp[1]) * int(shp[2])
+        degs = np.minimum(np.asarray(target[k].getDegeneracies(), dtype=np.int64), avail)
+        qs = [q for q, d in zip(target[k].qnums(), degs) if d > 0]
+        bonds[k] = Bond(BD_KET, [[q] for q in qs], [int(d) for d in degs if d > 0])
+    As = []
+    for k in range(L):
+        A = UniTensor([bonds[k], phys, bonds[k + 1].redirect()],
+                      ["a%d" % k, "s%d" % k, "a%d" % (k + 1)], 1, 3, -1, _alloc=False)
+        n = int(sum(np.prod(s.astype(np.int64)) for s in A._shapes))
+        A._storage = torch.from_numpy(rng.standard_normal(n)).to("cuda:%d" % device)
+        As.append(A)
+    # every sector matrix of A_1..A_{L-1}: rows a_k (m), columns (s, a_{k+1}) (n >= m)
+    work = []
+    for k in range(1, L):
+        for sec in _sector_matrices(As[k]):
+            m, n = sec.m.shape
+            if m > n:
+                raise RuntimeError("[Internal ERROR] a sector has more rows than columns")
+            work.append((k, sec))
+    buckets: Dict[tuple, list] = {}
+    for i, (_, sec) in enumerate(work):
+        m, n = sec.m.shape
+        buckets.setdefault((_pad_size(n), _pad_size(m)), []).append(i)
+    for (np_, mp), idx in sorted(buckets.items()):
+        Mb = torch.zeros((len(idx), mp, np_), dtype=torch.float64, device="cuda:%d" % device)
+        rows = torch.zeros((len(idx), mp), dtype=torch.float64, device=Mb.device)
+        for b, i in enumerate(idx):
+            m, n = work[i][1].m.shape
+            Mb[b, :m, :n] = work[i][1].m
+            rows[b, :m] = 1.0
+        Xb = _orthonormal_rows(Mb, rows)
+        for b, i in enumerate(idx):
+            k, sec = work[i]
+            m, n = sec.m.shape
+            sec.m = Xb[b, :m, :n]  # m x n, orthonormal rows
+    # write the orthonormal rows back into the same blocks
+    for k, sec in work:
+        A = As[k]
+        coff = np.concatenate([[0], np.cumsum(sec.csz)]).astype(int)
+        pos = {tuple(int(x) for x in t): bi for bi, t in enumerate(A._itoi)}
+        for j, ct in enumerate(sec.cols):
+            bi = pos[tuple(sec.rows[0]) + tuple(ct)]
+            o = int(A._offsets[bi])
+            size = int(np.prod(A._shapes[bi].astype(np.int64)))
+            A._storage[o:o + size] = sec.m[:, coff[j]:coff[j + 1]].reshape(-1)
+    for A in As:
+        A.set_rowrank_(2)
+    As[0] = As[0] / As[0].Norm()
+    return As
+
+
 def right_canonicalize(As: List[UniTensor], svd: str = "gram") -> List[UniTensor]:
     """SVD from the right: A_k <- vT (right-orthonormal), A_{k-1} <- A_{k-1}.U.S; A_0 then
     normalised.  ``svd`` as in DMRG."""
@@ -138,7 +235,7 @@ class DMRG:
         self.L, self.chi, self.device = L, chi, device
         self.maxit, self.cvg, self.prec, self.mix, self.svd = maxit, cvg, prec, mix, svd
         self.M = xxz_mpo_block(J, Jz, device=device)
-        self.As = right_canonicalize(random_mps(L, chi, seed, device, Q), svd)
+        self.As = random_right_canonical_mps(L, chi, seed, device, Q)
         self.LR: List[Optional[UniTensor]] = [None] * (L + 1)
         self.LR[0] = _boundary(True, device)
         self.LR[L] = _boundary(False, device, -Q)

@@ -214,3 +214,21 @@ def test_dmrg_mixed_precision_matches_fp64():
     print("DMRG L=12 chi=32 fp64", Ea[-1], "mixed", Eb[-1], flush=True)
     assert abs(Ea[-1] - Eb[-1]) <= 1e-9
     assert abs(Ea[-1] - _heisenberg_ed(12)) <= 1e-6
+
+
+def test_random_right_canonical_mps():
+    """The DMRG starting state: every A_k (k >= 1) a right isometry (A A^T = I over (s, a_{k+1}))
+    and A_0 of unit norm, with the bond dimensions the right-to-left LQ sweep of random_mps
+    produces."""
+    from cytnx_amd.dmrg import random_mps, random_right_canonical_mps, right_canonicalize
+    L, chi = 12, 16
+    As = random_right_canonical_mps(L, chi, seed=4, device=0)
+    Bs = right_canonicalize(random_mps(L, chi, 4, 0), "gram")
+    assert [A.shape() for A in As] == [B.shape() for B in Bs]
+    assert abs(float(As[0].Norm()) - 1.0) <= 1e-13
+    for k in range(1, L):
+        D = As[k].to_dense()
+        M = D.reshape(D.shape[0], -1)
+        err = np.abs(M @ M.T - np.eye(M.shape[0])).max()
+        print("site", k, "shape", D.shape, "max |A A^T - I|", err, flush=True)
+        assert err <= 1e-13
