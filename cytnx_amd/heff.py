@@ -132,7 +132,8 @@ def psi_is_left(Y: UniTensor, labY: dict) -> bool:
 
 
 def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: np.ndarray,
-                    Y: UniTensor, labY: dict, live: Optional[np.ndarray] = None):
+                    Y: UniTensor, labY: dict, live: Optional[np.ndarray] = None,
+                    read: Optional[np.ndarray] = None):
     """Task arrays of the mix launch Y = W12 applied to (Lp . psi), never storing Lp . psi.
 
     Lp: (l, w, l') environment blocks; psi: the two-site state (l, s1, s2, r); W12: the dense
@@ -151,7 +152,10 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
 
     Returns (items (n,4), slices (m,7), outs (k,2), coef (k,MIX_SLICES)) as int64 / float64
     arrays, or raises MixUnsupported.  With live (a bool array of Y.Nblocks()), marks the Y
-    blocks that some slice reaches (the others are zero for every input)."""
+    blocks that some slice reaches (the others are zero for every input).  With read (a bool
+    array of Y.Nblocks(): the blocks the next launch reads), the plan is made in one pass for
+    the pruned layout subset_blocks(Y, keep), keep = reached and read: the other blocks get no
+    outputs, the offsets address the pruned layout, and live (if given) receives keep."""
     pL = {k: Lp.labels().index(labL[k]) for k in ("l", "w", "lp")}
     pP = {k: psi.labels().index(labP[k]) for k in ("l", "s1", "s2", "r")}
     pY = {k: Y.labels().index(labY[k]) for k in ("lp", "s1p", "s2p", "w2", "r")}
@@ -175,7 +179,8 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
     Y_by = {}
     for yi, t in enumerate(Y._itoi):
         Y_by.setdefault((int(t[pY["lp"]]), int(t[pY["r"]])), []).append(yi)
-    items, slices, outs, coefs = [], [], [], []
+    items, slices, outs, coefs, out_block, records = [], [], [], [], [], []
+    keep_blocks = np.zeros(Y.Nblocks(), dtype=bool)
     for (qap, qc), ylist in Y_by.items():
         cand = []
         for pi in P_by.get(qc, []):
@@ -219,8 +224,8 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
             for i1p in range(int(shy[pY["s1p"]])):
                 for i2p in range(int(shy[pY["s2p"]])):
                     for j in range(int(shy[pY["w2"]])):
-                        c_off = int(Y._offsets[yi] + i1p * sty[pY["s1p"]]
-                                    + i2p * sty[pY["s2p"]] + j * sty[pY["w2"]])
+                        c_off = int(i1p * sty[pY["s1p"]] + i2p * sty[pY["s2p"]]
+                                    + j * sty[pY["w2"]])  # within block yi
                         olist.append((c_off, int(sty[pY[row]]),
                                       (int(offs1p[int(ty[pY["s1p"]])] + i1p),
                                        int(offw2[int(ty[pY["w2"]])] + j),
@@ -230,10 +235,22 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
         for o, (_, _, (s1p, w2, s2p)) in enumerate(olist):
             for s, (_, (w, s1, s2), _, _) in enumerate(cand):
                 C[o, s] = W12[w, s1, s1p, w2, s2, s2p]
-        if live is not None:
-            for o, yi in enumerate(oblock):
-                if np.any(C[o] != 0.0):
-                    live[yi] = True
+        reached = {yi for o, yi in enumerate(oblock) if np.any(C[o] != 0.0)}
+        if live is not None and read is None:
+            for yi in reached:
+                live[yi] = True
+        if read is not None:
+            # one pass for the pruned layout: only blocks that are reached and read keep their
+            # outputs (all of them, zero rows included: the block is read)
+            kept = [o for o, yi in enumerate(oblock) if yi in reached and read[yi]]
+            for yi in reached:
+                if read[yi]:
+                    keep_blocks[yi] = True
+            if not kept:
+                continue
+            olist = [olist[o] for o in kept]
+            oblock = [oblock[o] for o in kept]
+            C = C[kept]
         keep = np.any(C != 0.0, axis=0) if len(olist) else np.zeros(len(cand), dtype=bool)
         cand = [c for c, k in zip(cand, keep) if k]
         C = C[:, keep]
@@ -243,13 +260,36 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
         for c in cand:
             if (c[2], c[3]) != (M, N):
                 raise ValueError("Heff2 mix plan: slice shape differs from its outputs'")
-        items.append((M, N, len(cand), len(olist)))
-        slices.extend(c[0] for c in cand)
-        for o, (c_off, ldc, _) in enumerate(olist):
-            outs.append((c_off, ldc))
+        rows = []
+        for o in range(len(olist)):
             row = np.zeros(MIX_SLICES)
             row[:len(cand)] = C[o]
-            coefs.append(row)
+            rows.append(row)
+        # items in the order of their first block in the layout the plan addresses
+        records.append((min(oblock) if oblock else len(records), (M, N, len(cand), len(olist)),
+                        [c[0] for c in cand], [(c_off, ldc) for c_off, ldc, _ in olist],
+                        list(oblock), rows))
+    if read is not None:
+        records.sort(key=lambda rec: rec[0])
+    for _, item, sl, ou, ob, rows in records:
+        items.append(item)
+        slices.extend(sl)
+        outs.extend(ou)
+        out_block.extend(ob)
+        coefs.extend(rows)
+    # block offsets of the layout the outputs address
+    if read is not None:
+        sizes = np.prod(Y._shapes.astype(np.int64), axis=1) if Y.Nblocks() else \
+            np.zeros(0, dtype=np.int64)
+        base = np.zeros(Y.Nblocks(), dtype=np.int64)
+        ks = sizes * keep_blocks
+        if len(ks):
+            base[1:] = np.cumsum(ks)[:-1]
+        if live is not None:
+            live[:] = keep_blocks
+    else:
+        base = np.asarray(Y._offsets, dtype=np.int64)
+    outs = [(int(base[b]) + c, l) for (c, l), b in zip(outs, out_block)]
     as64 = lambda rows, w: np.asarray(rows, dtype=np.int64).reshape(-1, w)
     return (as64(items, 4), as64(slices, 7), as64(outs, 2),
             np.asarray(coefs, dtype=np.float64).reshape(-1, MIX_SLICES))
@@ -475,13 +515,12 @@ class Heff2(_HeffBase):
         # (w'', r) sector pair no block of R has are never read: leave both out of Y, so
         # neither launch touches them
         Rpp = Rp.permute([lab_w2, lab_r, lab_rp]).relabel([kw, kr, lab_rp])
-        live = np.zeros(Y.Nblocks(), dtype=bool)
-        mix_plan_arrays(*args, Y, ylab, live)
         rk = {(int(t[0]), int(t[1])) for t in Rpp._itoi}
         pw, pr = Y.labels().index(kw), Y.labels().index(kr)
         read = np.array([(int(t[pw]), int(t[pr])) in rk for t in Y._itoi], dtype=bool)
-        Y = subset_blocks(Y, live & read)
-        arrays = mix_plan_arrays(*args, Y, ylab)
+        keep = np.zeros(Y.Nblocks(), dtype=bool)
+        arrays = mix_plan_arrays(*args, Y, ylab, keep, read)
+        Y = subset_blocks(Y, keep)
         psi_left = psi_is_left(Y, ylab)
         na, nb = Lp._storage.numel(), meta_elems(psi_m)
         check_mix_bounds(arrays, *((nb, na) if psi_left else (na, nb)), meta_elems(Y))

@@ -172,3 +172,33 @@ def test_mix_plan_transposed_layout(chi):
     mix_plan_arrays(Lp, labs[0], psi, labs[1], W, Yr, labY, live_r)
     items_r = mix_plan_arrays(Lp, labs[0], psi, labs[1], W, subset_blocks(Yr, live_r), labY)[0]
     assert sorted(map(tuple, items_r[:, [1, 0, 2, 3]])) == sorted(map(tuple, arrays[0]))
+
+
+@pytest.mark.parametrize("chi", [8, 20])
+@pytest.mark.parametrize("transposed", [False, True])
+def test_mix_plan_single_pass_equals_two_passes(chi, transposed):
+    """mix_plan_arrays(..., read=mask) plans the pruned layout in one pass: the arrays and the
+    kept-block mask equal those of the two-pass form (live mask, subset_blocks, plan again)."""
+    psi, L, M1, M2, R = heisenberg.two_site_problem(chi, seed=chi + 3, hermitian=True)
+    Lb = L.bonds()
+    Lp = _meta([Lb[1], Lb[0], Lb[2]], ["-1", "-5", "0"])
+    W = np.einsum("awsx,wvty->asxvty", M1.to_dense(), M2.to_dense())
+    if transposed:
+        Y = _meta([M2.bonds()[1], psi.bonds()[3], Lb[2], M1.bonds()[3], M2.bonds()[3]],
+                  ["k0", "k1", "0", "1", "2"])
+    else:
+        Y = _meta([Lb[2], M1.bonds()[3], M2.bonds()[3], M2.bonds()[1], psi.bonds()[3]],
+                  ["0", "1", "2", "k0", "k1"])
+    labs = ({"l": "-1", "w": "-5", "lp": "0"}, {"l": "-1", "s1": "-2", "s2": "-3", "r": "-4"},
+            {"lp": "0", "s1p": "1", "s2p": "2", "w2": "k0", "r": "k1"})
+    rng = np.random.default_rng(chi)
+    for read in (np.ones(Y.Nblocks(), dtype=bool), rng.random(Y.Nblocks()) < 0.8):
+        live = np.zeros(Y.Nblocks(), dtype=bool)
+        mix_plan_arrays(Lp, labs[0], psi, labs[1], W, Y, labs[2], live)
+        two = mix_plan_arrays(Lp, labs[0], psi, labs[1], W, subset_blocks(Y, live & read),
+                              labs[2])
+        keep = np.zeros(Y.Nblocks(), dtype=bool)
+        one = mix_plan_arrays(Lp, labs[0], psi, labs[1], W, Y, labs[2], keep, read)
+        assert np.array_equal(keep, live & read)
+        for a, b in zip(one, two):
+            assert a.shape == b.shape and np.array_equal(a, b)
