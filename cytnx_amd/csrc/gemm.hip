@@ -867,6 +867,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_mix(const GTile *__restrict__ ti
     if (tl.blk < 0) continue;  // padding (uniform over the workgroup)
     const GBlk it = items[tl.blk];
     const int mine = it.nseg > wave ? (it.nseg - wave + WAVES - 1) / WAVES : 0;
+    // this wave's first output record, loaded before the K loop so that its latency overlaps
+    // the loop instead of following it
+    MixOut o_first{};
+    if (wave < it.nout) o_first = outs[it.out0 + wave];
     if (mine > 0) {
       RealAcc<PREC, MI, NI> acc[NACC];
 #pragma unroll
@@ -895,7 +899,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mix(const GTile *__restrict__ ti
     // gets them as one k-quad store where the quad is aligned and whole.
     const int mt = min(16 * MI, it.M - tl.m0), nt = min(16 * NI, it.N - tl.n0);
     for (int j = wave; j < it.nout; j += WAVES) {
-      const MixOut &o = outs[it.out0 + j];
+      const MixOut o = j == wave ? o_first : outs[it.out0 + j];
       gptr<TC> Cp = at<TC>(C, o.c_off);
 #pragma unroll
       for (int i = 0; i < MI; ++i)
