@@ -85,6 +85,11 @@ def _declare(L):
         "cxa_lanczos_run": (ci, [vp, vp, vp, vp, ctypes.c_double, ci, ci, ci, dblp, vp, intp,
                                  dblp, dblp, vp]),
         "cxa_lanczos_destroy": (ci, [vp]),
+        "cxa_lanczos_exp": (ci, [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ci, vp, intp,
+                                 vp]),
+        "cxa_lanczos_eigsh": (ci, [vp, vp, vp, vp, ci, ci, ctypes.c_double, ci, dblp, vp, intp,
+                                   intp, vp]),
+        "cxa_tridiag_eigh": (ci, [ci, dblp, dblp, dblp, dblp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -47,6 +47,53 @@ int sturm_count(int n, const double *a, const double *b2, double x, double pivmi
   return cnt;
 }
 
+// Sturm counts at three shifts in one pass: three independent recurrences, so the divisions of
+// one overlap those of the others instead of waiting on each other.
+void sturm_count3(int n, const double *a, const double *b2, const double *x, double pivmin,
+                  int *cnt) {
+  double q[3];
+  int c[3] = {0, 0, 0};
+  for (int k = 0; k < 3; ++k) {
+    q[k] = a[0] - x[k];
+    if (std::fabs(q[k]) < pivmin) q[k] = -pivmin;
+    c[k] += q[k] < 0;
+  }
+  for (int i = 1; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      q[k] = (a[i] - x[k]) - b2[i - 1] / q[k];
+      if (std::fabs(q[k]) < pivmin) q[k] = -pivmin;
+      c[k] += q[k] < 0;
+    }
+  }
+  for (int k = 0; k < 3; ++k) cnt[k] = c[k];
+}
+
+// Smallest eigenvalue inside the bracket [lo, hi] (count(lo) == 0 <= 1 <= count(hi)) by
+// quadrisection: each pass keeps the quarter that holds the eigenvalue, until the bracket is
+// two ulps wide relative to its ends.
+double lowest_in(int n, const double *a, const double *b2, double lo, double hi, double pivmin) {
+  const double eps = 2.220446049250313e-16;
+  for (int it = 0; it < 200; ++it) {
+    if (hi - lo <= 2.0 * eps * std::fmax(std::fabs(lo), std::fabs(hi))) break;
+    const double x[3] = {lo + 0.25 * (hi - lo), 0.5 * (lo + hi), hi - 0.25 * (hi - lo)};
+    if (!(x[0] > lo && x[2] < hi)) break;  // no representable interior points left
+    int c[3];
+    sturm_count3(n, a, b2, x, pivmin, c);
+    if (c[0] >= 1) {
+      hi = x[0];
+    } else if (c[1] >= 1) {
+      lo = x[0];
+      hi = x[1];
+    } else if (c[2] >= 1) {
+      lo = x[1];
+      hi = x[2];
+    } else {
+      lo = x[2];
+    }
+  }
+  return 0.5 * (lo + hi);
+}
+
 // solve (T - lam I) x = r in place (r -> x) with partial pivoting
 void gt_solve(int n, const double *a, const double *b, double lam, double tiny,
               std::vector<double> &r) {
@@ -85,6 +132,50 @@ void gt_solve(int n, const double *a, const double *b, double lam, double tiny,
   for (int i = n - 3; i >= 0; --i) r[i] = (r[i] - du[i] * r[i + 1] - du2[i] * r[i + 2]) / d[i];
 }
 
+// Lowest eigenvalue of the symmetric tridiagonal matrix with diagonal a and off-diagonal b.
+// Without `upper` the bracket is the Gershgorin interval.  With `upper` = the lowest eigenvalue of
+// the leading (n-1) x (n-1) block (what the previous Lanczos step computed) the bracket is
+// narrowed: by Cauchy interlacing the eigenvalue lies below `upper`, and by Weyl's inequality
+// it lies above min(upper, a[n-1]) - |b[n-2]|; a first guess of `upper - step` (step: a few times
+// the previous change) is tried before that bound.  Every end is confirmed by a Sturm count and
+// the Gershgorin end is kept where a confirmation fails, so the result does not depend on the hint
+// beyond rounding.
+int tridiag_lowest(int n, const double *a, const double *b, const double *upper, double step,
+                   double *E, double *nrm_out) {
+  std::vector<double> b2(n - 1);
+  double lo = a[0], hi = a[0], nrm = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(b[i]) : 0.0);
+    lo = std::fmin(lo, a[i] - r);
+    hi = std::fmax(hi, a[i] + r);
+    nrm = std::fmax(nrm, std::fabs(a[i]) + r);
+  }
+  for (int i = 0; i < n; ++i)
+    CXA_CHECK(std::isfinite(a[i]) && (i + 1 == n || std::isfinite(b[i])), CXA_ERR_NUMERIC,
+              "cxa_tridiag_ground: non-finite entries");
+  for (int i = 0; i + 1 < n; ++i) b2[i] = b[i] * b[i];
+  const double eps = 2.220446049250313e-16;
+  // smallest pivot magnitude allowed in the Sturm recurrence (keeps b^2/q finite)
+  const double pivmin = 1e-290 * std::fmax(1.0, nrm * nrm);
+  lo -= eps * nrm + pivmin;
+  hi += eps * nrm + pivmin;
+  if (upper && std::isfinite(*upper)) {
+    const double u = *upper;
+    const double slack = 8.0 * eps * (nrm + std::fabs(u)) + pivmin;
+    const double h = u + slack;
+    if (h < hi && sturm_count(n, a, b2.data(), h, pivmin) >= 1) hi = h;
+    const double guess = hi - std::fmax(4.0 * step, 64.0 * eps * std::fmax(nrm, 1.0));
+    const double weyl = std::fmin(u, a[n - 1]) - std::fabs(b[n - 2]) - slack;
+    if (guess > lo && guess < hi && sturm_count(n, a, b2.data(), guess, pivmin) == 0)
+      lo = guess;
+    else if (weyl > lo && weyl < hi && sturm_count(n, a, b2.data(), weyl, pivmin) == 0)
+      lo = weyl;
+  }
+  *E = lowest_in(n, a, b2.data(), lo, hi, pivmin);
+  if (nrm_out) *nrm_out = nrm;
+  return CXA_OK;
+}
+
 }  // namespace
 
 extern "C" int cxa_tridiag_ground(int n, const double *a, const double *b, double *E,
@@ -96,34 +187,10 @@ extern "C" int cxa_tridiag_ground(int n, const double *a, const double *b, doubl
     if (z) z[0] = 1.0;
     return CXA_OK;
   }
-  std::vector<double> b2(n - 1);
-  double lo = a[0], hi = a[0], nrm = 0.0;
-  for (int i = 0; i < n; ++i) {
-    const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(b[i]) : 0.0);
-    lo = std::fmin(lo, a[i] - r);
-    hi = std::fmax(hi, a[i] + r);
-    nrm = std::fmax(nrm, std::fabs(a[i]) + r);
-  }
-  for (int i = 0; i + 1 < n; ++i) {
-    CXA_CHECK(std::isfinite(b[i]) && std::isfinite(a[i]), CXA_ERR_NUMERIC,
-              "cxa_tridiag_ground: non-finite entries");
-    b2[i] = b[i] * b[i];
-  }
   const double eps = 2.220446049250313e-16;
-  // smallest pivot magnitude allowed in the Sturm recurrence (keeps b^2/q finite)
-  const double pivmin = 1e-290 * std::fmax(1.0, nrm * nrm);
-  lo -= eps * nrm + pivmin;
-  hi += eps * nrm + pivmin;
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (mid <= lo || mid >= hi) break;
-    if (hi - lo <= 2.0 * eps * std::fmax(std::fabs(lo), std::fabs(hi))) break;
-    if (sturm_count(n, a, b2.data(), mid, pivmin) >= 1)
-      hi = mid;
-    else
-      lo = mid;
-  }
-  const double lam = 0.5 * (lo + hi);
+  double lam = 0.0, nrm = 0.0;
+  int rc = tridiag_lowest(n, a, b, nullptr, 0.0, &lam, &nrm);
+  if (rc) return rc;
   *E = lam;
   if (!z) return CXA_OK;
   std::vector<double> r(n, 1.0 / std::sqrt((double)n));
@@ -614,6 +681,156 @@ int capture_iterations(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, 
 
 }  // namespace
 
+namespace {
+
+// The stream a run enqueues on: the caller's, or (graphs on the null stream, which cannot be
+// captured) the context's own, joined to the caller's with an event at the end.
+struct RunStream {
+  hipStream_t s = nullptr, ws = nullptr;
+  hipEvent_t ev = nullptr;
+};
+
+int run_begin(cxa_lanczos *L, bool use_graph, hipStream_t s, RunStream &r) {
+  r.s = r.ws = s;
+  if (use_graph && s == nullptr) {
+    if (!L->own) CXA_HIP(hipStreamCreateWithFlags(&L->own, hipStreamNonBlocking));
+    r.ws = L->own;
+    CXA_HIP(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
+    CXA_HIP(hipEventRecord(r.ev, s));
+    CXA_HIP(hipStreamWaitEvent(r.ws, r.ev, 0));
+  }
+  return CXA_OK;
+}
+
+int run_end(RunStream &r) {
+  if (r.ws != r.s) {
+    CXA_HIP(hipEventRecord(r.ev, r.ws));
+    CXA_HIP(hipStreamWaitEvent(r.s, r.ev, 0));
+    CXA_HIP(hipEventDestroy(r.ev));
+    r.ev = nullptr;
+  } else {
+    CXA_HIP(hipStreamSynchronize(r.ws));  // the pinned coefficients are reused by the next run
+  }
+  return CXA_OK;
+}
+
+// psi_0 = Tin / |Tin| into V[0] and cur, then (graphs) the iterations captured afresh: the matvec
+// behind (fn, ctx) may have changed since the last run (a chain's precision, its buffers).  One
+// iteration, and a whole batch of them so that a full batch is one host launch.
+int run_prologue(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, const double *Tin, int reorth,
+                 bool use_graph, int batch, int maxiter, hipStream_t ws) {
+  int rc;
+  if ((rc = vec_counter_set(L->counter, 0, ws))) return rc;
+  if ((rc = vec_dot(Tin, Tin, L->n, L->part, L->nrm(), nullptr, 0, 1, ws))) return rc;
+  if ((rc = vec_normalize_init(L->cur, L->V, Tin, L->n, L->nrm(), ws))) return rc;
+  if (use_graph) {
+    L->drop_graphs();
+    if ((rc = capture_iterations(L, matvec, ctx, reorth, 1, ws, &L->gexec1))) return rc;
+    if (batch > 1 && maxiter >= batch) {
+      if ((rc = capture_iterations(L, matvec, ctx, reorth, batch, ws, &L->gexecB))) return rc;
+      L->gbatch = batch;
+    }
+  }
+  return CXA_OK;
+}
+
+// Iterations [it, it + nb), then alphas / betas [0, it + nb) copied into the pinned host buffers
+// hA / hB; returns after they have arrived.
+int run_batch(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, int reorth, bool use_graph, int it,
+              int nb, hipStream_t ws) {
+  int rc;
+  if (use_graph && nb == L->gbatch) {
+    CXA_HIP(hipGraphLaunch(L->gexecB, ws));
+  } else {
+    for (int b = 0; b < nb; ++b) {
+      if (use_graph) {
+        CXA_HIP(hipGraphLaunch(L->gexec1, ws));
+      } else if ((rc = lanczos_body(L, matvec, ctx, reorth, it + b, ws))) {
+        return rc;
+      }
+    }
+  }
+  const int m1 = L->maxiter + 1, upto = it + nb;
+  CXA_HIP(hipMemcpyAsync(L->hbuf, L->As(), sizeof(double) * upto, hipMemcpyDeviceToHost, ws));
+  CXA_HIP(hipMemcpyAsync(L->hbuf + m1, L->Bs(), sizeof(double) * upto, hipMemcpyDeviceToHost,
+                         ws));
+  CXA_HIP(hipStreamSynchronize(ws));
+  return CXA_OK;
+}
+
+// out = sum_{j < k} coef[j] V_j with coef given on the host (staged through the pinned buffer).
+int combine_host_coef(cxa_lanczos *L, const double *coef, int k, double *out, hipStream_t ws) {
+  double *hz = L->hbuf + 2 * (L->maxiter + 1);
+  CXA_HIP(hipStreamSynchronize(ws));  // the staging buffer may still feed an earlier upload
+  std::memcpy(hz, coef, sizeof(double) * k);
+  CXA_HIP(hipMemcpyAsync(L->coef, hz, sizeof(double) * k, hipMemcpyHostToDevice, ws));
+  return vec_combine(out, L->V, L->coef, k, L->n, ws);
+}
+
+// Eigen-decomposition of a symmetric n x n matrix A (row-major, destroyed) by cyclic Jacobi
+// rotations (Golub & Van Loan, Matrix Computations, 4th ed., Algorithms 8.5.1 and 8.5.3):
+// eigenvalues ascending in w, the eigenvector of w[k] in column k of Z (row-major n x n).
+void sym_eig(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z) {
+  std::vector<double> Q((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
+  auto a = [&](int i, int j) -> double & { return A[(size_t)i * n + j]; };
+  auto q = [&](int i, int j) -> double & { return Q[(size_t)i * n + j]; };
+  double fro = 0.0;
+  for (double v : A) fro += v * v;
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    double off = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) off += a(i, j) * a(i, j);
+    if (off <= 1e-32 * fro) break;
+    for (int p = 0; p < n; ++p) {
+      for (int r = p + 1; r < n; ++r) {
+        const double apr = a(p, r);
+        if (apr == 0.0) continue;
+        // the rotation that zeroes a(p, r): t = tan(theta), the root of smaller magnitude
+        const double tau = (a(r, r) - a(p, p)) / (2.0 * apr);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+        const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double kp = a(k, p), kr = a(k, r);
+          a(k, p) = c * kp - sn * kr;
+          a(k, r) = sn * kp + c * kr;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double pk = a(p, k), rk = a(r, k);
+          a(p, k) = c * pk - sn * rk;
+          a(r, k) = sn * pk + c * rk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double kp = q(k, p), kr = q(k, r);
+          q(k, p) = c * kp - sn * kr;
+          q(k, r) = sn * kp + c * kr;
+        }
+      }
+    }
+  }
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return a(x, x) < a(y, y); });
+  w.resize(n);
+  Z.assign((size_t)n * n, 0.0);
+  for (int k = 0; k < n; ++k) {
+    w[k] = a(ord[k], ord[k]);
+    for (int i = 0; i < n; ++i) Z[(size_t)i * n + k] = q(i, ord[k]);
+  }
+}
+
+// the symmetric tridiagonal matrix (a, b) of size n as a dense row-major matrix
+std::vector<double> tridiag_dense(int n, const double *a, const double *b) {
+  std::vector<double> T((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    T[(size_t)i * n + i] = a[i];
+    if (i + 1 < n) T[(size_t)i * n + i + 1] = T[(size_t)(i + 1) * n + i] = b[i];
+  }
+  return T;
+}
+
+}  // namespace
+
 // flags: bit 0 = FP64 full re-orthogonalisation, bit 1 = replay captured iteration graphs
 extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, const double *Tin,
                                double CvgCrit, int maxiter, int batch, int flags, double *E_out,
@@ -625,58 +842,23 @@ extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
   CXA_CHECK(maxiter <= L->maxiter, CXA_ERR_INVALID,
             "cxa_lanczos_run: Maxiter %d above the context's %d", maxiter, L->maxiter);
   if (batch < 1) batch = 1;
-  const int reorth = flags & 1, use_graph = (flags >> 1) & 1;
-  hipStream_t s = (hipStream_t)stream;
-  hipStream_t ws = s;
-  hipEvent_t ev = nullptr;
-  if (use_graph && s == nullptr) {  // the null stream cannot be captured: use our own
-    if (!L->own) CXA_HIP(hipStreamCreateWithFlags(&L->own, hipStreamNonBlocking));
-    ws = L->own;
-    CXA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    CXA_HIP(hipEventRecord(ev, s));
-    CXA_HIP(hipStreamWaitEvent(ws, ev, 0));
-  }
-  const int64_t n = L->n;
+  const int reorth = flags & 1;
+  const bool use_graph = (flags >> 1) & 1;
+  RunStream r;
   int rc;
-  // psi_1 = Tin / |Tin|
-  if ((rc = vec_counter_set(L->counter, 0, ws))) return rc;
-  if ((rc = vec_dot(Tin, Tin, n, L->part, L->nrm(), nullptr, 0, 1, ws))) return rc;
-  if ((rc = vec_normalize_init(L->cur, L->V, Tin, n, L->nrm(), ws))) return rc;
-
-  // The iterations are captured afresh on every run (the matvec behind (fn, ctx) may have
-  // changed: a chain's precision, its buffers): one iteration, and a whole batch of them so that
-  // a full batch is one host launch.
-  if (use_graph) {
-    L->drop_graphs();
-    if ((rc = capture_iterations(L, matvec, ctx, reorth, 1, ws, &L->gexec1))) return rc;
-    if (batch > 1 && maxiter >= batch) {
-      if ((rc = capture_iterations(L, matvec, ctx, reorth, batch, ws, &L->gexecB))) return rc;
-      L->gbatch = batch;
-    }
-  }
+  if ((rc = run_begin(L, use_graph, (hipStream_t)stream, r))) return rc;
+  const hipStream_t ws = r.ws;
+  if ((rc = run_prologue(L, matvec, ctx, Tin, reorth, use_graph, batch, maxiter, ws))) return rc;
 
   const int m1 = L->maxiter + 1;
-  double *hA = L->hbuf, *hB = L->hbuf + m1, *hz = L->hbuf + 2 * m1;
+  const double *hA = L->hbuf, *hB = L->hbuf + m1;
   std::vector<double> z(m1);
-  double E = 0.0, eig = 0.0;
+  double E = 0.0, eig = 0.0, dE = 0.0;
   int done = -1, it = 0;
   while (done < 0 && it < maxiter) {
     const int nb = std::min(batch, maxiter - it);
-    if (use_graph && nb == L->gbatch) {
-      CXA_HIP(hipGraphLaunch(L->gexecB, ws));
-    } else {
-      for (int b = 0; b < nb; ++b) {
-        if (use_graph) {
-          CXA_HIP(hipGraphLaunch(L->gexec1, ws));
-        } else if ((rc = lanczos_body(L, matvec, ctx, reorth, it + b, ws))) {
-          return rc;
-        }
-      }
-    }
+    if ((rc = run_batch(L, matvec, ctx, reorth, use_graph, it, nb, ws))) return rc;
     const int upto = it + nb;  // alphas/betas [0, upto) are valid after the batch
-    CXA_HIP(hipMemcpyAsync(hA, L->As(), sizeof(double) * upto, hipMemcpyDeviceToHost, ws));
-    CXA_HIP(hipMemcpyAsync(hB, L->Bs(), sizeof(double) * upto, hipMemcpyDeviceToHost, ws));
-    CXA_HIP(hipStreamSynchronize(ws));
     for (int j = it; j < upto && done < 0; ++j) {
       if (j == 0) {
         E = hA[0];
@@ -684,7 +866,8 @@ extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
         if (hB[0] == 0.0) done = 0;  // Tin is an eigenvector
         continue;
       }
-      rc = cxa_tridiag_ground(j + 1, hA, hB, &eig, nullptr);
+      // E (the lowest eigenvalue of the previous tridiagonal) bounds this one from above
+      rc = tridiag_lowest(j + 1, hA, hB, &E, dE, &eig, nullptr);
       if (rc) {  // the reference stops on a failed tridiagonal solve
         done = j - 1;
         break;
@@ -697,6 +880,7 @@ extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
         done = j;
         break;
       }
+      dE = std::fabs(E - eig);
       E = eig;
     }
     it = upto;
@@ -708,17 +892,189 @@ extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
   if (niter) *niter = k;
   if (alphas) std::memcpy(alphas, hA, sizeof(double) * k);
   if (betas) std::memcpy(betas, hB, sizeof(double) * k);
-  if (evec) {
-    std::memcpy(hz, z.data(), sizeof(double) * k);
-    CXA_HIP(hipMemcpyAsync(L->coef, hz, sizeof(double) * k, hipMemcpyHostToDevice, ws));
-    if ((rc = vec_combine(evec, L->V, L->coef, k, n, ws))) return rc;
-  }
-  if (ws != s) {
-    CXA_HIP(hipEventRecord(ev, ws));
-    CXA_HIP(hipStreamWaitEvent(s, ev, 0));
-    CXA_HIP(hipEventDestroy(ev));
-  } else {
-    CXA_HIP(hipStreamSynchronize(ws));  // the pinned coefficients are reused by the next run
-  }
+  if (evec && (rc = combine_host_coef(L, z.data(), k, evec, ws))) return rc;
+  return run_end(r);
+}
+
+extern "C" int cxa_tridiag_eigh(int n, const double *a, const double *b, double *w, double *z) {
+  CXA_CHECK(n >= 1 && a && w && (n == 1 || b), CXA_ERR_INVALID,
+            "cxa_tridiag_eigh: bad arguments");
+  for (int i = 0; i < n; ++i)
+    CXA_CHECK(std::isfinite(a[i]) && (i + 1 == n || std::isfinite(b[i])), CXA_ERR_NUMERIC,
+              "cxa_tridiag_eigh: non-finite entries");
+  std::vector<double> T = tridiag_dense(n, a, b), ev, Z;
+  sym_eig(n, T, ev, Z);
+  std::memcpy(w, ev.data(), sizeof(double) * n);
+  if (z)  // row k = the eigenvector of w[k]
+    for (int k = 0; k < n; ++k)
+      for (int i = 0; i < n; ++i) z[(size_t)k * n + i] = Z[(size_t)i * n + k];
   return CXA_OK;
+}
+
+// Lanczos_Exp (src/linalg/Lanczos_Exp.cpp:266-390): the plain three-term recurrence from
+// Tin / |Tin|; at step i (i >= 1) the step stops before v_i when beta_i is at the round-off
+// level 100 eps max(1, |alpha|, |alpha| + beta_i + beta_{i-1}, ...) sqrt(i), otherwise after v_i
+// when |exp(tau T_{i+1})[i, 0]| < CvgCrit or i = min(Maxiter, n) - 1; then
+// out = V_k exp(tau T_k) (V_k^T Tin).  Real tau, FP64.
+extern "C" int cxa_lanczos_exp(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, const double *Tin,
+                               double tau, double CvgCrit, int Maxiter, double *out, int *niter,
+                               cxa_stream_t stream) {
+  CXA_CHECK(L && matvec && Tin && out, CXA_ERR_INVALID, "cxa_lanczos_exp: bad arguments");
+  CXA_CHECK(CvgCrit > 0, CXA_ERR_INVALID, "[ERROR][Lanczos_Exp] converge criteria must >0");
+  CXA_CHECK(Maxiter >= 2, CXA_ERR_INVALID, "[ERROR][Lanczos_Exp] Maxiter must >1");
+  CXA_CHECK(std::isfinite(tau), CXA_ERR_INVALID, "[ERROR][Lanczos_Exp] tau must be finite");
+  const int imp = (int)std::min<int64_t>(Maxiter, L->n);  // Krylov vectors at most
+  CXA_CHECK(imp <= L->maxiter, CXA_ERR_INVALID,
+            "cxa_lanczos_exp: %d Krylov vectors above the context's %d", imp, L->maxiter);
+  const double eps = 2.220446049250313e-16;
+  RunStream r;
+  int rc;
+  if ((rc = run_begin(L, false, (hipStream_t)stream, r))) return rc;
+  const hipStream_t ws = r.ws;
+  if ((rc = run_prologue(L, matvec, ctx, Tin, 0, false, 1, imp, ws))) return rc;
+  const int m1 = L->maxiter + 1;
+  const double *hA = L->hbuf, *hB = L->hbuf + m1;
+  // B = exp(tau T_m) from the eigen-decomposition of the symmetric tridiagonal T_m
+  std::vector<double> B;
+  auto expm = [&](int m) {
+    std::vector<double> T = tridiag_dense(m, hA, hB), w, Z;
+    sym_eig(m, T, w, Z);
+    B.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) {
+        double acc = 0.0;
+        for (int k = 0; k < m; ++k)
+          acc += Z[(size_t)i * m + k] * std::exp(tau * w[k]) * Z[(size_t)j * m + k];
+        B[(size_t)i * m + j] = acc;
+      }
+  };
+  int size = -1, it = 0;
+  double scale = 1.0;
+  const int batch = 8;
+  while (size < 0) {
+    // one more iteration than the host checks: alpha_i of step i comes from iteration i
+    const int nb = std::min(batch, imp - it);
+    if (nb <= 0) break;
+    if ((rc = run_batch(L, matvec, ctx, 0, false, it, nb, ws))) return rc;
+    const int upto = it + nb;
+    for (int i = std::max(it, 1); i < upto && size < 0; ++i) {
+      if (i == 1) scale = std::max(std::fabs(hA[0]), 1.0);
+      const double beta = hB[i - 1], beta_prev = i >= 2 ? hB[i - 2] : 0.0;
+      const double local = std::fabs(hA[i - 1]) + std::fabs(beta) + std::fabs(beta_prev);
+      scale = std::max(scale, std::max(local, 1.0));
+      if (!(beta > 100.0 * eps * scale * std::sqrt((double)i))) {
+        size = i;  // breakdown: v_0 .. v_{i-1}
+        break;
+      }
+      expm(i + 1);
+      if (std::fabs(B[(size_t)i * (i + 1)]) < CvgCrit || i == imp - 1) size = i + 1;
+    }
+    it = upto;
+  }
+  if (size < 0) size = std::max(1, std::min(it, imp));
+  if ((int)B.size() != size * size) expm(size);
+  // c = V^T Tin on the device, then y = B c on the host, out = V y
+  if ((rc = vec_project(L->V, Tin, L->n, size, L->part, L->h(), ws))) return rc;
+  double *hc = L->hbuf + 2 * m1;
+  CXA_HIP(hipMemcpyAsync(hc, L->h(), sizeof(double) * size, hipMemcpyDeviceToHost, ws));
+  CXA_HIP(hipStreamSynchronize(ws));
+  std::vector<double> y(size, 0.0);
+  for (int i = 0; i < size; ++i)
+    for (int j = 0; j < size; ++j) y[i] += B[(size_t)i * size + j] * hc[j];
+  if ((rc = combine_host_coef(L, y.data(), size, out, ws))) return rc;
+  if (niter) *niter = size;
+  return run_end(r);
+}
+
+// k extreme eigenpairs of a symmetric operator: which 0 = "SA" (smallest algebraic), 1 = "LA"
+// (largest algebraic), 2 = "LM" (largest magnitude), the contract of linalg::Lanczos(Hop, Tin,
+// which, ...) (src/linalg/Lanczos.cpp:249-402).  Lanczos with full FP64 re-orthogonalisation
+// from Tin / |Tin| (no restarts: the basis grows to at most min(Maxiter, n) vectors); after each
+// batch the Ritz pairs of T_m are checked with the residual bound |beta_m z_i[m - 1]| <=
+// tol max(eps^(2/3), |theta_i|) (tol <= 0: machine epsilon), the test of ARPACK's dsconv.  The
+// run also ends at an invariant subspace holding k pairs.  Eigenvalues sorted as the reference
+// sorts them (SA ascending, LA descending, LM by descending magnitude); evecs (device, may be
+// NULL) holds the k Ritz vectors as rows; nconv = pairs that met the bound.
+extern "C" int cxa_lanczos_eigsh(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx,
+                                 const double *Tin, int which, int k, double tol, int Maxiter,
+                                 double *evals, double *evecs, int *niter, int *nconv,
+                                 cxa_stream_t stream) {
+  CXA_CHECK(L && matvec && Tin && evals, CXA_ERR_INVALID, "cxa_lanczos_eigsh: bad arguments");
+  CXA_CHECK(which >= 0 && which <= 2, CXA_ERR_INVALID,
+            "[ERROR][Lanczos] 'which' should be 'LM', 'LA', 'SA'");
+  CXA_CHECK(k >= 1, CXA_ERR_INVALID, "[ERROR][Lanczos] k should be >0");
+  CXA_CHECK(k <= L->n, CXA_ERR_INVALID,
+            "[ERROR][Lanczos] k can only be up to total dimension of input vector D");
+  CXA_CHECK(Maxiter >= 1, CXA_ERR_INVALID, "cxa_lanczos_eigsh: Maxiter must be positive");
+  const int cap = (int)std::min<int64_t>({(int64_t)Maxiter, L->n, (int64_t)L->maxiter});
+  CXA_CHECK(cap >= k, CXA_ERR_INVALID,
+            "cxa_lanczos_eigsh: %d Krylov vectors allowed, fewer than k = %d", cap, k);
+  const double eps = 2.220446049250313e-16, eps23 = std::cbrt(eps * eps);
+  if (!(tol > 0)) tol = eps;
+  RunStream r;
+  int rc;
+  if ((rc = run_begin(L, false, (hipStream_t)stream, r))) return rc;
+  const hipStream_t ws = r.ws;
+  if ((rc = run_prologue(L, matvec, ctx, Tin, 1, false, 1, cap, ws))) return rc;
+  const int m1 = L->maxiter + 1;
+  const double *hA = L->hbuf, *hB = L->hbuf + m1;
+  std::vector<double> w, Z;
+  std::vector<int> pick;
+  int m = 0, conv = 0, it = 0;
+  const int batch = std::max(8, k);
+  // the wanted k of the m Ritz values, in the reference's output order
+  auto select = [&](int mm) {
+    std::vector<int> idx(mm);
+    for (int i = 0; i < mm; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+      if (which == 0) return w[x] < w[y];
+      if (which == 1) return w[x] > w[y];
+      return std::fabs(w[x]) > std::fabs(w[y]);
+    });
+    idx.resize(std::min(k, mm));
+    return idx;
+  };
+  for (;;) {
+    const int nb = std::min(batch, cap - it);
+    if ((rc = run_batch(L, matvec, ctx, 1, false, it, nb, ws))) return rc;
+    const int upto = it + nb;
+    // an invariant subspace ends the basis early: beta_j at round-off level
+    m = upto;
+    double tnorm = 0.0;
+    for (int j = 0; j < upto; ++j)
+      tnorm = std::max(tnorm, std::fabs(hA[j]) + std::fabs(hB[j]) + (j ? std::fabs(hB[j - 1]) : 0));
+    bool invariant = false;
+    for (int j = 0; j < upto; ++j)
+      if (!(hB[j] > 100.0 * eps * std::max(tnorm, 1.0) * std::sqrt((double)(j + 1)))) {
+        m = j + 1;
+        invariant = true;
+        break;
+      }
+    std::vector<double> T = tridiag_dense(m, hA, hB);
+    sym_eig(m, T, w, Z);
+    pick = select(m);
+    conv = 0;
+    const double bm = invariant ? 0.0 : hB[m - 1];
+    for (int i : pick)
+      if (std::fabs(bm * Z[(size_t)(m - 1) * m + i]) <= tol * std::max(eps23, std::fabs(w[i])))
+        ++conv;
+    it = upto;
+    if ((int)pick.size() == k && conv == k) break;
+    CXA_CHECK(!(invariant && m < k), CXA_ERR_NUMERIC,
+              "[ERROR][Lanczos] the start vector spans an invariant subspace of dimension %d < k "
+              "= %d",
+              m, k);
+    if (invariant || it >= cap) break;
+  }
+  for (int i = 0; i < k; ++i) evals[i] = w[pick[i]];
+  if (evecs) {
+    std::vector<double> y(m);
+    for (int i = 0; i < k; ++i) {
+      for (int j = 0; j < m; ++j) y[j] = Z[(size_t)j * m + pick[i]];
+      if ((rc = combine_host_coef(L, y.data(), m, evecs + (size_t)i * L->n, ws))) return rc;
+    }
+  }
+  if (niter) *niter = m;
+  if (nconv) *nconv = conv;
+  return run_end(r);
 }

@@ -3,6 +3,8 @@
 // through a device-side iteration counter, so an iteration enqueues without any host round trip
 // and the same launch sequence can be captured once and replayed.  Reductions are two-stage
 // with a fixed number of partial sums, so results are deterministic run to run.
+#include <algorithm>
+
 #include "cxa_common.hpp"
 #include "vecops.hpp"
 #include "contract.hpp"
@@ -93,19 +95,33 @@ __global__ __launch_bounds__(kT) void k_three_term(double *__restrict__ w,
   if (threadIdx.x == 0) part_b[blockIdx.x] = s;
 }
 
-// h_j partial sums: part[j * P + block] = sum over the block's slice of V_j * w, j <= it
+// h_j partial sums: part[j * P + block] = sum over the block's slice of V_j * w, j <= it.
+// Block (x, y) takes the basis vectors in groups of four, groups y, y + gridDim.y, ...: with the
+// iteration known at launch there is one row of blocks per group; a captured launch (iteration
+// read on the device) has a fixed number of rows that walk the groups.  Each partial sums the
+// same elements in the same order whatever the grid.
 __global__ __launch_bounds__(kT) void k_proj_partials(const double *__restrict__ V,
                                                       const double *__restrict__ w, int64_t n,
                                                       IterIdx iter, double *__restrict__ part) {
   const int it = iter.get();
-  const int j = blockIdx.y;
-  if (j > it) return;
-  const double *v = V + (int64_t)j * n;
-  double s = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
-    s += v[i] * w[i];
-  s = block_sum(s);
-  if (threadIdx.x == 0) part[(int64_t)j * gridDim.x + blockIdx.x] = s;
+  const int64_t P = gridDim.x;
+  for (int j0 = 4 * blockIdx.y; j0 <= it; j0 += 4 * gridDim.y) {
+    const int nj = min(4, it + 1 - j0);
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += P * kT) {
+      const double wi = w[i];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q < nj) s[q] += V[(int64_t)(j0 + q) * n + i] * wi;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < nj) {  // uniform over the block
+        const double t = block_sum(s[q]);
+        if (threadIdx.x == 0) part[(int64_t)(j0 + q) * P + blockIdx.x] = t;
+      }
+    }
+  }
 }
 
 __global__ __launch_bounds__(kT) void k_proj_finalize(const double *__restrict__ part, int P,
@@ -223,9 +239,21 @@ int vec_reorth(double *w, const double *V, int64_t n, int max_j, IterIdx iter, d
                double *h, double *part_out, hipStream_t s) {
   // with the iteration known on the host only the live projections are launched
   const int nj = iter.p ? max_j : iter.v + 1;
-  hipLaunchKernelGGL(k_proj_partials, dim3(kPartials, nj), dim3(kT), 0, s, V, w, n, iter, part);
+  const int groups = iter.p ? std::min(8, (max_j + 3) / 4) : (nj + 3) / 4;
+  hipLaunchKernelGGL(k_proj_partials, dim3(kPartials, groups), dim3(kT), 0, s, V, w, n, iter,
+                     part);
   hipLaunchKernelGGL(k_proj_finalize, dim3(nj), dim3(kT), 0, s, part, kPartials, iter, h);
   hipLaunchKernelGGL(k_proj_apply, dim3(kPartials), dim3(kT), 0, s, w, V, n, h, iter, part_out);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_project(const double *V, const double *x, int64_t n, int k, double *part, double *h,
+                hipStream_t s) {
+  const IterIdx iter{nullptr, k - 1};
+  hipLaunchKernelGGL(k_proj_partials, dim3(kPartials, (k + 3) / 4), dim3(kT), 0, s, V, x, n, iter,
+                     part);
+  hipLaunchKernelGGL(k_proj_finalize, dim3(k), dim3(kT), 0, s, part, kPartials, iter, h);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }

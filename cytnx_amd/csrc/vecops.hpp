@@ -32,6 +32,9 @@ int vec_three_term(double *w, const double *cur, const double *V, int64_t n,
 // part_out = partials of the new |w|^2; part: kPartials * max_j scratch
 int vec_reorth(double *w, const double *V, int64_t n, int max_j, IterIdx iter, double *part,
                double *h, double *part_out, hipStream_t s);
+// h[j] = V_j . x for j < k (no update of x); part: kPartials * k scratch
+int vec_project(const double *V, const double *x, int64_t n, int k, double *part, double *h,
+                hipStream_t s);
 int vec_finalize(const double *part, double *out, const int *counter, int offset, int mode,
                  hipStream_t s);
 // Bs[it] = sqrt(sum(part_b)); cur = V[it+1] = w / Bs[it]

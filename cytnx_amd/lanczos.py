@@ -113,17 +113,56 @@ def lanczos_gnd_native(op, psi0: torch.Tensor, CvgCrit: float = 1e-14, Maxiter: 
     return E.value, evec, k, alphas[:k].copy(), betas[:k].copy()
 
 
+def _flat_vector(T) -> Tuple[torch.Tensor, object]:
+    """(flat FP64 device data, a contiguous template with the same metadata) of a UniTensor (block
+    or dense) or a Tensor."""
+    from .tensor import Tensor
+    from .unitensor import UniTensor
+    if isinstance(T, UniTensor):
+        t = T.clone()
+        t.contiguous_()
+        flat = t.storage().reshape(-1)
+    elif isinstance(T, Tensor):
+        t = T.contiguous()
+        flat = t.storage().reshape(-1)
+    else:
+        raise TypeError("Tin must be a UniTensor or a Tensor")
+    if flat.dtype != torch.float64 or flat.device.type != "cuda":
+        raise TypeError("the GPU Lanczos runs real FP64 operators on device tensors")
+    return flat, t
+
+
+def _wrap_like(template, x: torch.Tensor):
+    """A tensor with the template's metadata over the flat data x (no copy)."""
+    from .tensor import Tensor
+    from .unitensor import UniTensor
+    if isinstance(template, UniTensor):
+        v = template.clone_meta()
+        if template._is_block:
+            v._storage = x
+        else:
+            v._block = Tensor.from_storage(x, template._block.shape())
+        return v
+    return Tensor.from_storage(x, template.shape())
+
+
+def _flat_result(out) -> torch.Tensor:
+    from .tensor import Tensor
+    if isinstance(out, Tensor):
+        return out.contiguous().storage().reshape(-1)
+    out.contiguous_()
+    return out.storage().reshape(-1)
+
+
 def _python_matvec(Hop, template):
-    n = template._storage.numel()
+    n = int(template.storage().numel())  # the template is contiguous
 
     def cb(_ctx, xp, yp, _stream):
         try:
             x = torch.as_tensor(_DevView(xp, n), device="cuda")
             y = torch.as_tensor(_DevView(yp, n), device="cuda")
-            v = template.clone_meta()
-            v._storage = x
-            out = Hop.matvec(v)
-            y.copy_(out._storage.reshape(-1))
+            out = Hop.matvec(_wrap_like(template, x))
+            y.copy_(_flat_result(out))
             return 0
         except Exception as e:  # reported as a failed matvec
             _python_matvec.last_error = e
@@ -132,39 +171,151 @@ def _python_matvec(Hop, template):
     return _MATVEC_T(cb)
 
 
-def Lanczos(Hop: LinOp, Tin, method: str = "Gnd", CvgCrit: float = 1.0e-14,
-            Maxiter: int = 10000, k: int = 1, is_V: bool = True, is_row: bool = True,
-            max_krydim: int = 0, verbose: bool = False):
-    """linalg::Lanczos for UniTensors, method "Gnd" (returns [energy, eigvector])."""
-    if method != "Gnd":
-        raise NotImplementedError("only method='Gnd' (Lanczos_Gnd_Ut) is on the hot path")
-    if Tin.dtype() != 3:
-        raise TypeError("the GPU Lanczos runs real FP64 operators")
-    cvg = float(CvgCrit)
-    x0 = Tin._storage.reshape(-1)
+def _matvec_of(Hop, template):
+    """(fnptr, ctx) of the matvec, plus the Python callback object to keep alive."""
     from .heff import _HeffBase
     if isinstance(Hop, _HeffBase):
-        E, v, k_, _, _ = lanczos_gnd_native(Hop, x0, cvg, Maxiter)
-    else:
-        cb = _python_matvec(Hop, Tin)
-        fn = ctypes.cast(cb, ctypes.c_void_p).value
-        try:
-            E, v, k_, _, _ = lanczos_gnd_native(None, x0, cvg, Maxiter, batch=1, fn_ctx=(fn, None))
-        except _lib.CxaError as err:
-            inner = getattr(_python_matvec, "last_error", None)
-            if inner is not None:
-                _python_matvec.last_error = None
-                raise inner from err
-            raise
+        return Hop.native_matvec(), None
+    cb = _python_matvec(Hop, template)
+    return (ctypes.cast(cb, ctypes.c_void_p).value, None), cb
+
+
+def _reraise_matvec(err):
+    inner = getattr(_python_matvec, "last_error", None)
+    if inner is not None:
+        _python_matvec.last_error = None
+        raise inner from err
+    raise err
+
+
+_WHICH = {"SA": 0, "LA": 1, "LM": 2}
+
+
+def Lanczos(Hop: LinOp, Tin, method: str = "Gnd", CvgCrit: Optional[float] = None,
+            Maxiter: int = 10000, k: int = 1, is_V: bool = True, is_row: bool = True,
+            max_krydim: int = 0, verbose: bool = False, which: Optional[str] = None,
+            ncv: int = 0):
+    """linalg::Lanczos.
+
+    * ``method="Gnd"`` (src/linalg/Lanczos.cpp:135-179, Lanczos_Gnd_Ut.cpp): ``[energy,
+      eigvector]``, CvgCrit default 1e-14.
+    * ``which="SA" | "LA" | "LM"`` (src/linalg/Lanczos.cpp:453-583, the ARPACK overload,
+      pybind/linalg_py.cpp:722-740): ``k`` extreme eigenpairs, CvgCrit default 0 (machine
+      precision).  UniTensor input: ``[eigenvalues, eigvector_1, ..., eigvector_k]``; Tensor
+      input: ``[eigenvalues, eigenvectors]`` (shape (n,) for k = 1, (k, n) otherwise).
+    """
+    if which is not None:
+        return _lanczos_which(Hop, Tin, which, Maxiter, 0.0 if CvgCrit is None else CvgCrit, k,
+                              is_V, ncv, verbose)
+    if method != "Gnd":
+        raise NotImplementedError("method='Gnd', or which='SA'/'LA'/'LM' for k eigenpairs")
+    if Tin.dtype() != 3:
+        raise TypeError("the GPU Lanczos runs real FP64 operators")
+    cvg = 1.0e-14 if CvgCrit is None else float(CvgCrit)
+    x0, tmpl = _flat_vector(Tin)
+    fn_ctx, keep = _matvec_of(Hop, tmpl)
+    try:
+        if keep is None:
+            E, v, k_, _, _ = lanczos_gnd_native(Hop, x0, cvg, Maxiter)
+        else:
+            E, v, k_, _, _ = lanczos_gnd_native(None, x0, cvg, Maxiter, batch=1, fn_ctx=fn_ctx)
+    except _lib.CxaError as err:
+        _reraise_matvec(err)
     from .unitensor import _scalar_block
     e = _scalar_block(3, x0.device.index)
     e._storage.fill_(E)
     out = [e]
     if is_V:
-        ev = Tin.clone_meta()
-        ev._storage = v
-        out.append(ev)
+        out.append(_wrap_like(tmpl, v))
     return out
+
+
+def _lanczos_which(Hop, Tin, which, Maxiter, CvgCrit, k, is_V, ncv, verbose):
+    """k extreme eigenpairs on the device (cxa_lanczos_eigsh)."""
+    import warnings
+    from .tensor import Tensor
+    from .unitensor import Bond, UniTensor
+    if which not in _WHICH:
+        raise ValueError("[ERROR][Lanczos] 'which' should be 'LM', 'LA', 'SA'")
+    x0, tmpl = _flat_vector(Tin)
+    n = x0.numel()
+    k = int(k)
+    if k < 1:
+        raise ValueError("[ERROR][Lanczos] k should be >0")
+    if k > n:
+        raise ValueError("[ERROR][Lanczos] k can only be up to total dimension of input vector D")
+    if CvgCrit < 0:
+        raise ValueError("[ERROR][Lanczos] cvg_crit should be >= 0")
+    if ncv != 0 and (ncv < 2 + k or ncv > n):
+        raise ValueError("[ERROR][Lanczos] ncv should be 2+k<=ncv<=nx")
+    if Maxiter < 1:
+        raise ValueError("[ERROR][Lanczos] Maxiter must be positive")
+    dev = x0.device.index
+    # no restarts: the basis may grow to n vectors, bounded by device memory
+    cap = max(2, _krylov_cap(n, n))
+    ctx = _context(n, cap, dev)
+    fn_ctx, keep = _matvec_of(Hop, tmpl)
+    evals = np.zeros(k)
+    evecs = torch.empty((k, n), dtype=torch.float64, device=x0.device) if is_V else None
+    niter = ctypes.c_int(0)
+    nconv = ctypes.c_int(0)
+    try:
+        _lib.check(_lib.lib().cxa_lanczos_eigsh(
+            ctx, fn_ctx[0], fn_ctx[1], x0.data_ptr(), _WHICH[which], k, float(CvgCrit),
+            int(min(cap, n)), evals.ctypes.data_as(_lib.dblp),
+            evecs.data_ptr() if evecs is not None else None, ctypes.byref(niter),
+            ctypes.byref(nconv), stream_handle(dev)))
+    except _lib.CxaError as err:
+        _reraise_matvec(err)
+    del keep
+    if nconv.value < k and verbose:
+        warnings.warn("[WARNING][Lanczos]: %d of %d eigenpairs within the tolerance after %d "
+                      "Krylov vectors" % (nconv.value, k, niter.value))
+    vals = torch.tensor(evals, dtype=torch.float64, device=x0.device)
+    if isinstance(Tin, UniTensor):
+        ev = UniTensor([Bond(k)], rowrank=1, dtype=3, device=dev)
+        ev._block = Tensor.from_storage(vals, [k])
+        out = [ev]
+        if is_V:
+            out += [_wrap_like(tmpl, evecs[i].clone()) for i in range(k)]
+        return out
+    out = [Tensor.from_storage(vals, [k])]
+    if is_V:
+        out.append(Tensor.from_storage(evecs.reshape(-1), [n] if k == 1 else [k, n]))
+    return out
+
+
+def Lanczos_Exp(Hop: LinOp, v, tau, CvgCrit: float = 1.0e-14, Maxiter: int = 10000,
+                verbose: bool = False):
+    """exp(tau H) v by Lanczos (src/linalg/Lanczos_Exp.cpp:266-450), real tau, FP64 on the
+    device (cxa_lanczos_exp).  Returns a tensor with v's metadata."""
+    if isinstance(tau, complex):
+        if tau.imag != 0:
+            raise NotImplementedError("Lanczos_Exp on this path takes a real tau (FP64 operators)")
+        tau = tau.real
+    if CvgCrit <= 0:
+        raise ValueError("[ERROR][Lanczos_Exp] converge criteria must >0")
+    if Maxiter < 2:
+        raise ValueError("[ERROR][Lanczos_Exp] Maxiter must >1")
+    x0, tmpl = _flat_vector(v)
+    n = x0.numel()
+    dev = x0.device.index
+    imp = min(int(Maxiter), n)
+    ctx = _context(n, max(2, imp), dev)
+    fn_ctx, keep = _matvec_of(Hop, tmpl)
+    out = torch.empty_like(x0)
+    niter = ctypes.c_int(0)
+    try:
+        _lib.check(_lib.lib().cxa_lanczos_exp(ctx, fn_ctx[0], fn_ctx[1], x0.data_ptr(),
+                                              float(tau), float(CvgCrit), int(Maxiter),
+                                              out.data_ptr(), ctypes.byref(niter),
+                                              stream_handle(dev)))
+    except _lib.CxaError as err:
+        _reraise_matvec(err)
+    del keep
+    if verbose:
+        print("Lanczos_Exp: %d Krylov vectors" % niter.value)
+    return _wrap_like(tmpl, out)
 
 
 def ground_state_mixed(op, psi0: torch.Tensor, CvgCrit: float = 1e-12, Maxiter: int = 1000,
@@ -189,4 +340,4 @@ def ground_state_mixed(op, psi0: torch.Tensor, CvgCrit: float = 1e-12, Maxiter: 
     return E, v2, n1, n2
 
 
-__all__ = ["LinOp", "Lanczos", "lanczos_gnd_native", "ground_state_mixed"]
+__all__ = ["LinOp", "Lanczos", "Lanczos_Exp", "lanczos_gnd_native", "ground_state_mixed"]

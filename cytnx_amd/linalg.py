@@ -176,4 +176,17 @@ def Svd_truncate(T, keepdim: int, err: float = 0.0, is_UvT: bool = True, return_
     return _svdt(T, keepdim, err, is_UvT, return_err, mindim)
 
 
-__all__ = ["Matmul", "Tensordot", "Gemm_Batch", "Gesvd", "Svd", "Svd_truncate"]
+def Lanczos(*args, **kwargs):
+    """linalg::Lanczos: method="Gnd", or which="SA"/"LA"/"LM" (cytnx_amd.lanczos.Lanczos)."""
+    from .lanczos import Lanczos as _lanczos
+    return _lanczos(*args, **kwargs)
+
+
+def Lanczos_Exp(*args, **kwargs):
+    """linalg::Lanczos_Exp for a real tau (cytnx_amd.lanczos.Lanczos_Exp)."""
+    from .lanczos import Lanczos_Exp as _lanczos_exp
+    return _lanczos_exp(*args, **kwargs)
+
+
+__all__ = ["Matmul", "Tensordot", "Gemm_Batch", "Gesvd", "Svd", "Svd_truncate", "Lanczos",
+           "Lanczos_Exp"]

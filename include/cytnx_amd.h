@@ -218,6 +218,14 @@ int cxa_nrm2(int64_t n, const double *x, double *result, void *workspace, cxa_st
 int cxa_tridiag_ground(int n, const double *a, const double *b, double *E, double *z);
 
 /*
+ * All eigenpairs of the same symmetric tridiagonal matrix (host): linalg::Tridiag(..., is_V)
+ * (src/linalg/Tridiag.cpp:10-104, LAPACKE_dstev in
+ * src/backend/linalg_internal_cpu/Tridiag_internal.cpp:9-42).  w[n] ascending; z (may be NULL,
+ * n x n) row k = the unit eigenvector of w[k] (sign arbitrary).  Cyclic Jacobi rotations.
+ */
+int cxa_tridiag_eigh(int n, const double *a, const double *b, double *w, double *z);
+
+/*
  * Matvec callback used by the Lanczos driver: y = H x for FP64 device vectors, enqueued on
  * `stream`; returns a CXA status.  The LinOp::matvec plug-in point (include/LinOp.hpp:163-167).
  */
@@ -291,6 +299,32 @@ int cxa_lanczos_run(cxa_lanczos *ctx, cxa_matvec_fn matvec, void *matvec_ctx, co
                     double CvgCrit, int Maxiter, int batch, int flags, double *E_out, double *evec,
                     int *niter, double *alphas, double *betas, cxa_stream_t stream);
 int cxa_lanczos_destroy(cxa_lanczos *ctx);
+
+/*
+ * Lanczos_Exp(Hop, Tin, tau, CvgCrit, Maxiter) for a real tau on FP64 device vectors
+ * (src/linalg/Lanczos_Exp.cpp:266-390, :404-450): the three-term recurrence from Tin / |Tin|,
+ * stopping before v_i at a round-off beta_i (100 eps times the largest |alpha| + beta seen, times
+ * sqrt(i)) or after v_i once |exp(tau T)[i, 0]| < CvgCrit or i = min(Maxiter, n) - 1; then
+ * out = V exp(tau T) (V^T Tin) (device).  niter = Krylov vectors used.  The context must hold
+ * min(Maxiter, n) vectors.
+ */
+int cxa_lanczos_exp(cxa_lanczos *ctx, cxa_matvec_fn matvec, void *matvec_ctx, const double *Tin,
+                    double tau, double CvgCrit, int Maxiter, double *out, int *niter,
+                    cxa_stream_t stream);
+
+/*
+ * k extreme eigenpairs of a real symmetric operator: linalg::Lanczos(Hop, Tin, which, maxiter,
+ * cvg_crit, k, is_V, ncv) (src/linalg/Lanczos.cpp:249-402, :453-583), which the reference runs
+ * with ARPACK dsaupd/dseupd.  which: 0 = "SA", 1 = "LA", 2 = "LM".  Lanczos with full FP64
+ * re-orthogonalisation (no restarts: up to min(Maxiter, n, context) basis vectors); converged
+ * when every wanted Ritz pair has |beta_m z_i[m-1]| <= tol max(eps^(2/3), |theta_i|) (ARPACK's
+ * dsconv test; tol <= 0 means machine epsilon).  evals[k] (host) ordered as the reference orders
+ * them (SA ascending, LA descending, LM by descending magnitude); evecs (device, may be NULL,
+ * k x n) the Ritz vectors as rows; niter = basis size; nconv = pairs within the bound.
+ */
+int cxa_lanczos_eigsh(cxa_lanczos *ctx, cxa_matvec_fn matvec, void *matvec_ctx, const double *Tin,
+                      int which, int k, double tol, int Maxiter, double *evals, double *evecs,
+                      int *niter, int *nconv, cxa_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -183,14 +183,15 @@ class DenseOp(LinOp):
 
     def matvec(self, v):
         out = v.clone_meta()
-        x = Tensor.from_storage(v._storage.reshape(-1), [self.nx(), 1])
-        out._storage = linalg.Matmul(self.M, x).storage().reshape(-1)
+        x = Tensor.from_storage(v.storage().reshape(-1), [self.nx(), 1])
+        out._block = linalg.Matmul(self.M, x)
         return out
 
 
 def _dense_ut(vec):
     ut = UniTensor([Bond(len(vec)), Bond(1)], ["b", "c"], 1, 3, 0)
-    ut._storage = torch.from_numpy(np.asarray(vec, dtype=np.float64)).cuda()
+    ut._block = Tensor.from_storage(torch.from_numpy(np.asarray(vec, dtype=np.float64)).cuda(),
+                                    [len(vec), 1])
     return ut
 
 
@@ -210,7 +211,7 @@ def test_lanczos_block_operator_fixture():
     E0 = min(np.linalg.eigvalsh(b)[0] for b in blocks)
     rng = np.random.default_rng(9)
     E, v = Lanczos(DenseOp(H), _dense_ut(rng.standard_normal(27)), "Gnd", 1e-15, 10000)
-    vv = v._storage.cpu().numpy()
+    vv = v.storage().reshape(-1).cpu().numpy()
     assert abs(E.item() - E0) < 1e-12
     assert np.linalg.norm(H @ vv - E.item() * vv) < 1e-6
     # the oracle's restated recurrence reaches the same energy
