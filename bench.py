@@ -549,6 +549,27 @@ def main() -> int:
         lanczos["speedup_fp32_refine_vs_fp64"] = (
             min(lanczos["time_fp64_s"], lanczos["time_fp64_graph_s"]) /
             min(lanczos["time_fp32_refine_s"], lanczos["time_fp32_refine_graph_s"]))
+        # the other drivers on the same chain: the four lowest eigenpairs (which="SA", k=4) and
+        # one imaginary-time step exp(-0.001 H) psi (Lanczos_Exp; tau |H| ~ 1, at most 200 vectors)
+        from cytnx_amd.lanczos import eigsh_native, exp_native
+        eigsh_native(H, x0, "SA", 4, 1e-10, False)  # warm-up (context allocation)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev, _, m_sa, nconv = eigsh_native(H, x0, "SA", 4, 1e-10, True)
+        torch.cuda.synchronize()
+        t_sa = time.perf_counter() - t0
+        lanczos["which_SA_k4"] = {"tol": 1e-10, "eigenvalues": [float(e) for e in ev],
+                                  "krylov_vectors": m_sa, "nconv": nconv, "time_s": t_sa,
+                                  "E0_minus_gnd": float(ev[0] - E64)}
+        exp_native(H, x0, -0.001, 1e-12, 200)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        y_exp, m_exp = exp_native(H, x0, -0.001, 1e-12, 200)
+        torch.cuda.synchronize()
+        t_exp = time.perf_counter() - t0
+        lanczos["exp_tau_m0.001"] = {
+            "CvgCrit": 1e-12, "krylov_vectors": m_exp, "time_s": t_exp,
+            "norm_ratio": float(torch.linalg.vector_norm(y_exp) / torch.linalg.vector_norm(x0))}
         log("lanczos", json.dumps(lanczos))
 
     main_mode = "fp32"

@@ -767,66 +767,86 @@ int combine_host_coef(cxa_lanczos *L, const double *coef, int k, double *out, hi
   return vec_combine(out, L->V, L->coef, k, L->n, ws);
 }
 
-// Eigen-decomposition of a symmetric n x n matrix A (row-major, destroyed) by cyclic Jacobi
-// rotations (Golub & Van Loan, Matrix Computations, 4th ed., Algorithms 8.5.1 and 8.5.3):
-// eigenvalues ascending in w, the eigenvector of w[k] in column k of Z (row-major n x n).
-void sym_eig(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z) {
-  std::vector<double> Q((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
-  auto a = [&](int i, int j) -> double & { return A[(size_t)i * n + j]; };
-  auto q = [&](int i, int j) -> double & { return Q[(size_t)i * n + j]; };
-  double fro = 0.0;
-  for (double v : A) fro += v * v;
-  for (int sweep = 0; sweep < 64; ++sweep) {
-    double off = 0.0;
-    for (int i = 0; i < n; ++i)
-      for (int j = i + 1; j < n; ++j) off += a(i, j) * a(i, j);
-    if (off <= 1e-32 * fro) break;
-    for (int p = 0; p < n; ++p) {
-      for (int r = p + 1; r < n; ++r) {
-        const double apr = a(p, r);
-        if (apr == 0.0) continue;
-        // the rotation that zeroes a(p, r): t = tan(theta), the root of smaller magnitude
-        const double tau = (a(r, r) - a(p, p)) / (2.0 * apr);
-        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
-        const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c;
-        for (int k = 0; k < n; ++k) {
-          const double kp = a(k, p), kr = a(k, r);
-          a(k, p) = c * kp - sn * kr;
-          a(k, r) = sn * kp + c * kr;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double pk = a(p, k), rk = a(r, k);
-          a(p, k) = c * pk - sn * rk;
-          a(r, k) = sn * pk + c * rk;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double kp = q(k, p), kr = q(k, r);
-          q(k, p) = c * kp - sn * kr;
-          q(k, r) = sn * kp + c * kr;
-        }
+// Eigenvalues of the symmetric tridiagonal matrix (diagonal a[n], off-diagonal b[n-1]) by the
+// implicit symmetric QR iteration with Wilkinson shifts (Golub & Van Loan, Matrix Computations,
+// 4th ed., Algorithms 8.3.2 and 8.3.3): each step chases the bulge of one shifted QR step down
+// the unreduced block with plane rotations, and off-diagonals below eps (|d_i| + |d_i+1|) are
+// set to zero.  w = eigenvalues ascending.  Of the orthogonal matrix of eigenvectors Z (column k
+// belongs to w[k]) only the rows listed in `rows` are accumulated, into zr (rows.size() x n,
+// row-major): the whole Z costs O(n^3), a few rows O(n^2).
+void tridiag_qr(int n, const double *a, const double *b, std::vector<double> &w,
+                const std::vector<int> &rows, std::vector<double> &zr) {
+  const double eps = 2.220446049250313e-16;
+  std::vector<double> d(a, a + n), e(n, 0.0);
+  for (int i = 0; i + 1 < n; ++i) e[i] = b[i];
+  const int nr = (int)rows.size();
+  zr.assign((size_t)nr * n, 0.0);
+  for (int r = 0; r < nr; ++r) zr[(size_t)r * n + rows[r]] = 1.0;
+  auto rotate = [&](int k, double c, double s) {  // Z <- Z G on columns k, k + 1
+    for (int r = 0; r < nr; ++r) {
+      double *z = &zr[(size_t)r * n];
+      const double zk = z[k], zk1 = z[k + 1];
+      z[k] = c * zk - s * zk1;
+      z[k + 1] = s * zk + c * zk1;
+    }
+  };
+  int h = n - 1, guard = 0;
+  while (h > 0 && guard++ < 60 * n + 60) {
+    if (std::fabs(e[h - 1]) <= eps * (std::fabs(d[h - 1]) + std::fabs(d[h]))) {
+      e[h - 1] = 0.0;
+      --h;
+      continue;
+    }
+    int l = h - 1;
+    while (l > 0 && std::fabs(e[l - 1]) > eps * (std::fabs(d[l - 1]) + std::fabs(d[l]))) --l;
+    if (l > 0) e[l - 1] = 0.0;
+    // Wilkinson shift: the eigenvalue of the trailing 2 x 2 closer to d[h]
+    const double delta = 0.5 * (d[h - 1] - d[h]);
+    const double rad = std::hypot(delta, e[h - 1]);
+    const double mu = d[h] - e[h - 1] * (e[h - 1] / (delta + (delta >= 0 ? rad : -rad)));
+    double x = d[l] - mu, z = e[l], bulge = 0.0;
+    for (int k = l; k < h; ++k) {
+      // rotation in the (k, k+1) plane with G^T [x; z] = [r; 0]
+      const double r = std::hypot(x, z);
+      double c = 1.0, s = 0.0;
+      if (r != 0.0) {
+        c = x / r;
+        s = -z / r;
       }
+      if (k > l) e[k - 1] = r;  // x, z were (T[k-1][k], bulge T[k-1][k+1])
+      const double ak = d[k], bk = e[k], ck = d[k + 1];
+      d[k] = c * c * ak - 2.0 * c * s * bk + s * s * ck;
+      d[k + 1] = s * s * ak + 2.0 * c * s * bk + c * c * ck;
+      e[k] = (ak - ck) * c * s + bk * (c * c - s * s);
+      if (k + 1 < h) {
+        bulge = -s * e[k + 1];
+        e[k + 1] *= c;
+      }
+      rotate(k, c, s);
+      x = e[k];
+      z = bulge;
     }
   }
   std::vector<int> ord(n);
   for (int i = 0; i < n; ++i) ord[i] = i;
-  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return a(x, x) < a(y, y); });
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return d[x] < d[y]; });
   w.resize(n);
-  Z.assign((size_t)n * n, 0.0);
-  for (int k = 0; k < n; ++k) {
-    w[k] = a(ord[k], ord[k]);
-    for (int i = 0; i < n; ++i) Z[(size_t)i * n + k] = q(i, ord[k]);
+  std::vector<double> tmp(n);
+  for (int k = 0; k < n; ++k) w[k] = d[ord[k]];
+  for (int r = 0; r < nr; ++r) {
+    double *z = &zr[(size_t)r * n];
+    for (int k = 0; k < n; ++k) tmp[k] = z[ord[k]];
+    std::copy(tmp.begin(), tmp.end(), z);
   }
 }
 
-// the symmetric tridiagonal matrix (a, b) of size n as a dense row-major matrix
-std::vector<double> tridiag_dense(int n, const double *a, const double *b) {
-  std::vector<double> T((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) {
-    T[(size_t)i * n + i] = a[i];
-    if (i + 1 < n) T[(size_t)i * n + i + 1] = T[(size_t)(i + 1) * n + i] = b[i];
-  }
-  return T;
+// every row of Z (the full eigenvector matrix, row-major, Z[i * n + k] = component i of
+// eigenvector k)
+void tridiag_qr_full(int n, const double *a, const double *b, std::vector<double> &w,
+                     std::vector<double> &Z) {
+  std::vector<int> rows(n);
+  for (int i = 0; i < n; ++i) rows[i] = i;
+  tridiag_qr(n, a, b, w, rows, Z);
 }
 
 }  // namespace
@@ -902,8 +922,8 @@ extern "C" int cxa_tridiag_eigh(int n, const double *a, const double *b, double 
   for (int i = 0; i < n; ++i)
     CXA_CHECK(std::isfinite(a[i]) && (i + 1 == n || std::isfinite(b[i])), CXA_ERR_NUMERIC,
               "cxa_tridiag_eigh: non-finite entries");
-  std::vector<double> T = tridiag_dense(n, a, b), ev, Z;
-  sym_eig(n, T, ev, Z);
+  std::vector<double> ev, Z;
+  tridiag_qr_full(n, a, b, ev, Z);
   std::memcpy(w, ev.data(), sizeof(double) * n);
   if (z)  // row k = the eigenvector of w[k]
     for (int k = 0; k < n; ++k)
@@ -934,19 +954,13 @@ extern "C" int cxa_lanczos_exp(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
   if ((rc = run_prologue(L, matvec, ctx, Tin, 0, false, 1, imp, ws))) return rc;
   const int m1 = L->maxiter + 1;
   const double *hA = L->hbuf, *hB = L->hbuf + m1;
-  // B = exp(tau T_m) from the eigen-decomposition of the symmetric tridiagonal T_m
-  std::vector<double> B;
-  auto expm = [&](int m) {
-    std::vector<double> T = tridiag_dense(m, hA, hB), w, Z;
-    sym_eig(m, T, w, Z);
-    B.assign((size_t)m * m, 0.0);
-    for (int i = 0; i < m; ++i)
-      for (int j = 0; j < m; ++j) {
-        double acc = 0.0;
-        for (int k = 0; k < m; ++k)
-          acc += Z[(size_t)i * m + k] * std::exp(tau * w[k]) * Z[(size_t)j * m + k];
-        B[(size_t)i * m + j] = acc;
-      }
+  // exp(tau T_m)[m-1][0] from rows 0 and m-1 of the eigenvectors of the tridiagonal T_m
+  auto corner = [&](int m) {
+    std::vector<double> w, zr;
+    tridiag_qr(m, hA, hB, w, {0, m - 1}, zr);
+    double acc = 0.0;
+    for (int k = 0; k < m; ++k) acc += zr[(size_t)m + k] * std::exp(tau * w[k]) * zr[k];
+    return acc;
   };
   int size = -1, it = 0;
   double scale = 1.0;
@@ -966,21 +980,25 @@ extern "C" int cxa_lanczos_exp(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
         size = i;  // breakdown: v_0 .. v_{i-1}
         break;
       }
-      expm(i + 1);
-      if (std::fabs(B[(size_t)i * (i + 1)]) < CvgCrit || i == imp - 1) size = i + 1;
+      if (std::fabs(corner(i + 1)) < CvgCrit || i == imp - 1) size = i + 1;
     }
     it = upto;
   }
   if (size < 0) size = std::max(1, std::min(it, imp));
-  if ((int)B.size() != size * size) expm(size);
   // c = V^T Tin on the device, then y = B c on the host, out = V y
   if ((rc = vec_project(L->V, Tin, L->n, size, L->part, L->h(), ws))) return rc;
   double *hc = L->hbuf + 2 * m1;
   CXA_HIP(hipMemcpyAsync(hc, L->h(), sizeof(double) * size, hipMemcpyDeviceToHost, ws));
   CXA_HIP(hipStreamSynchronize(ws));
-  std::vector<double> y(size, 0.0);
+  // y = exp(tau T) c = Z diag(exp(tau w)) Z^T c
+  std::vector<double> w, Z, t(size, 0.0), y(size, 0.0);
+  tridiag_qr_full(size, hA, hB, w, Z);
+  for (int k = 0; k < size; ++k) {
+    for (int j = 0; j < size; ++j) t[k] += Z[(size_t)j * size + k] * hc[j];
+    t[k] *= std::exp(tau * w[k]);
+  }
   for (int i = 0; i < size; ++i)
-    for (int j = 0; j < size; ++j) y[i] += B[(size_t)i * size + j] * hc[j];
+    for (int k = 0; k < size; ++k) y[i] += Z[(size_t)i * size + k] * t[k];
   if ((rc = combine_host_coef(L, y.data(), size, out, ws))) return rc;
   if (niter) *niter = size;
   return run_end(r);
@@ -1050,14 +1068,12 @@ extern "C" int cxa_lanczos_eigsh(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx
         invariant = true;
         break;
       }
-    std::vector<double> T = tridiag_dense(m, hA, hB);
-    sym_eig(m, T, w, Z);
+    tridiag_qr(m, hA, hB, w, {m - 1}, Z);  // eigenvalues and the last row of Z
     pick = select(m);
     conv = 0;
     const double bm = invariant ? 0.0 : hB[m - 1];
     for (int i : pick)
-      if (std::fabs(bm * Z[(size_t)(m - 1) * m + i]) <= tol * std::max(eps23, std::fabs(w[i])))
-        ++conv;
+      if (std::fabs(bm * Z[i]) <= tol * std::max(eps23, std::fabs(w[i]))) ++conv;
     it = upto;
     if ((int)pick.size() == k && conv == k) break;
     CXA_CHECK(!(invariant && m < k), CXA_ERR_NUMERIC,
@@ -1068,6 +1084,7 @@ extern "C" int cxa_lanczos_eigsh(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx
   }
   for (int i = 0; i < k; ++i) evals[i] = w[pick[i]];
   if (evecs) {
+    tridiag_qr_full(m, hA, hB, w, Z);  // the same eigenvalue order as the checks
     std::vector<double> y(m);
     for (int i = 0; i < k; ++i) {
       for (int j = 0; j < m; ++j) y[j] = Z[(size_t)j * m + pick[i]];

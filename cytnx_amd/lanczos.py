@@ -191,6 +191,44 @@ def _reraise_matvec(err):
 _WHICH = {"SA": 0, "LA": 1, "LM": 2}
 
 
+def eigsh_native(op, psi0: torch.Tensor, which: str = "SA", k: int = 1, tol: float = 0.0,
+                 want_vectors: bool = True, fn_ctx=None):
+    """k extreme eigenpairs of a flat FP64 device operator (cxa_lanczos_eigsh).  Returns
+    (eigenvalues (numpy, reference order), eigenvectors (k x n device tensor or None), Krylov
+    vectors used, pairs within the tolerance).  ``op`` is a planned Heff unless ``fn_ctx``
+    gives (fnptr, ctx)."""
+    n = psi0.numel()
+    dev = psi0.device.index
+    cap = max(2, _krylov_cap(n, n))  # no restarts: up to n vectors, bounded by memory
+    ctx = _context(n, cap, dev)
+    fn, mctx = fn_ctx if fn_ctx is not None else op.native_matvec()
+    evals = np.zeros(k)
+    evecs = torch.empty((k, n), dtype=torch.float64, device=psi0.device) if want_vectors else None
+    niter = ctypes.c_int(0)
+    nconv = ctypes.c_int(0)
+    _lib.check(_lib.lib().cxa_lanczos_eigsh(
+        ctx, fn, mctx, psi0.data_ptr(), _WHICH[which], int(k), float(tol), int(min(cap, n)),
+        evals.ctypes.data_as(_lib.dblp), evecs.data_ptr() if evecs is not None else None,
+        ctypes.byref(niter), ctypes.byref(nconv), stream_handle(dev)))
+    return evals, evecs, niter.value, nconv.value
+
+
+def exp_native(op, psi0: torch.Tensor, tau: float, CvgCrit: float = 1e-14,
+               Maxiter: int = 10000, fn_ctx=None):
+    """exp(tau H) psi0 for a flat FP64 device operator (cxa_lanczos_exp): (out, Krylov vectors
+    used)."""
+    n = psi0.numel()
+    dev = psi0.device.index
+    ctx = _context(n, max(2, min(int(Maxiter), n)), dev)
+    fn, mctx = fn_ctx if fn_ctx is not None else op.native_matvec()
+    out = torch.empty_like(psi0)
+    niter = ctypes.c_int(0)
+    _lib.check(_lib.lib().cxa_lanczos_exp(ctx, fn, mctx, psi0.data_ptr(), float(tau),
+                                          float(CvgCrit), int(Maxiter), out.data_ptr(),
+                                          ctypes.byref(niter), stream_handle(dev)))
+    return out, niter.value
+
+
 def Lanczos(Hop: LinOp, Tin, method: str = "Gnd", CvgCrit: Optional[float] = None,
             Maxiter: int = 10000, k: int = 1, is_V: bool = True, is_row: bool = True,
             max_krydim: int = 0, verbose: bool = False, which: Optional[str] = None,
@@ -250,27 +288,17 @@ def _lanczos_which(Hop, Tin, which, Maxiter, CvgCrit, k, is_V, ncv, verbose):
         raise ValueError("[ERROR][Lanczos] ncv should be 2+k<=ncv<=nx")
     if Maxiter < 1:
         raise ValueError("[ERROR][Lanczos] Maxiter must be positive")
-    dev = x0.device.index
-    # no restarts: the basis may grow to n vectors, bounded by device memory
-    cap = max(2, _krylov_cap(n, n))
-    ctx = _context(n, cap, dev)
     fn_ctx, keep = _matvec_of(Hop, tmpl)
-    evals = np.zeros(k)
-    evecs = torch.empty((k, n), dtype=torch.float64, device=x0.device) if is_V else None
-    niter = ctypes.c_int(0)
-    nconv = ctypes.c_int(0)
     try:
-        _lib.check(_lib.lib().cxa_lanczos_eigsh(
-            ctx, fn_ctx[0], fn_ctx[1], x0.data_ptr(), _WHICH[which], k, float(CvgCrit),
-            int(min(cap, n)), evals.ctypes.data_as(_lib.dblp),
-            evecs.data_ptr() if evecs is not None else None, ctypes.byref(niter),
-            ctypes.byref(nconv), stream_handle(dev)))
+        evals, evecs, niter, nconv = eigsh_native(None, x0, which, k, CvgCrit, is_V,
+                                                  fn_ctx=fn_ctx)
     except _lib.CxaError as err:
         _reraise_matvec(err)
     del keep
-    if nconv.value < k and verbose:
+    if nconv < k and verbose:
         warnings.warn("[WARNING][Lanczos]: %d of %d eigenpairs within the tolerance after %d "
-                      "Krylov vectors" % (nconv.value, k, niter.value))
+                      "Krylov vectors" % (nconv, k, niter))
+    dev = x0.device.index
     vals = torch.tensor(evals, dtype=torch.float64, device=x0.device)
     if isinstance(Tin, UniTensor):
         ev = UniTensor([Bond(k)], rowrank=1, dtype=3, device=dev)
@@ -298,23 +326,14 @@ def Lanczos_Exp(Hop: LinOp, v, tau, CvgCrit: float = 1.0e-14, Maxiter: int = 100
     if Maxiter < 2:
         raise ValueError("[ERROR][Lanczos_Exp] Maxiter must >1")
     x0, tmpl = _flat_vector(v)
-    n = x0.numel()
-    dev = x0.device.index
-    imp = min(int(Maxiter), n)
-    ctx = _context(n, max(2, imp), dev)
     fn_ctx, keep = _matvec_of(Hop, tmpl)
-    out = torch.empty_like(x0)
-    niter = ctypes.c_int(0)
     try:
-        _lib.check(_lib.lib().cxa_lanczos_exp(ctx, fn_ctx[0], fn_ctx[1], x0.data_ptr(),
-                                              float(tau), float(CvgCrit), int(Maxiter),
-                                              out.data_ptr(), ctypes.byref(niter),
-                                              stream_handle(dev)))
+        out, niter = exp_native(None, x0, tau, CvgCrit, Maxiter, fn_ctx=fn_ctx)
     except _lib.CxaError as err:
         _reraise_matvec(err)
     del keep
     if verbose:
-        print("Lanczos_Exp: %d Krylov vectors" % niter.value)
+        print("Lanczos_Exp: %d Krylov vectors" % niter)
     return _wrap_like(tmpl, out)
 
 
@@ -340,4 +359,5 @@ def ground_state_mixed(op, psi0: torch.Tensor, CvgCrit: float = 1e-12, Maxiter: 
     return E, v2, n1, n2
 
 
-__all__ = ["LinOp", "Lanczos", "Lanczos_Exp", "lanczos_gnd_native", "ground_state_mixed"]
+__all__ = ["LinOp", "Lanczos", "Lanczos_Exp", "lanczos_gnd_native", "eigsh_native",
+           "exp_native", "ground_state_mixed"]

@@ -221,7 +221,8 @@ int cxa_tridiag_ground(int n, const double *a, const double *b, double *E, doubl
  * All eigenpairs of the same symmetric tridiagonal matrix (host): linalg::Tridiag(..., is_V)
  * (src/linalg/Tridiag.cpp:10-104, LAPACKE_dstev in
  * src/backend/linalg_internal_cpu/Tridiag_internal.cpp:9-42).  w[n] ascending; z (may be NULL,
- * n x n) row k = the unit eigenvector of w[k] (sign arbitrary).  Cyclic Jacobi rotations.
+ * n x n) row k = the unit eigenvector of w[k] (sign arbitrary).  Implicit symmetric QR
+ * iteration with Wilkinson shifts.
  */
 int cxa_tridiag_eigh(int n, const double *a, const double *b, double *w, double *z);
 
