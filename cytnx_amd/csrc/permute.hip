@@ -31,6 +31,7 @@ constexpr int kMaxOuter = 14;
 constexpr int kMaxGroup = 8;
 constexpr int kThreads = 256;
 constexpr int kMaxTile = 256;  // tile extent bound along A and along B
+constexpr int kMinTile = 1024;  // elements a tile should span when the axes allow
 
 struct TileArgs {
   int na, nb, no;
@@ -346,24 +347,30 @@ struct RowArgs32 {
 };
 
 // Innermost axis kept: output vector v (V elements) is vector v % vpr of output row v / vpr;
-// the row's input offset is decoded once per vector with FastDiv.  U vectors per thread, all
-// loads issued before the stores.
+// the row's input offset is decoded once per vector with FastDiv.  U vectors per thread: all U
+// offsets are decoded first and the U loads then issued back to back with no branch between
+// them (a load placed between two decode loops is waited for at the next loop's header, which
+// serialises the loads); a vector past the end reads vector 0 and is not stored.
 template <typename T, typename VT, int U>
 __global__ __launch_bounds__(kThreads) void permute_rows32(const T *__restrict__ src,
                                                            T *__restrict__ dst, RowArgs32 p) {
   constexpr int V = sizeof(VT) / sizeof(T);
   const uint32_t G = gridDim.x * kThreads;
   for (uint32_t v0 = blockIdx.x * kThreads + threadIdx.x; v0 < p.nvec; v0 += U * G) {
-    VT r[U];
+    uint32_t in[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint32_t v = v0 + u * G;
-      if (v < p.nvec) {
-        const uint32_t row = p.dvpr.div(v), j = v - row * p.dvpr.d;
-        const uint32_t in = decode32(row, p.nd, p.n_div, p.n_istr) + j * V;
-        r[u] = *reinterpret_cast<const VT *>(src + in);
-      }
+      const uint32_t v = v0 + u * G < p.nvec ? v0 + u * G : 0;
+      const uint32_t row = p.dvpr.div(v), j = v - row * p.dvpr.d;
+      in[u] = decode32(row, p.nd, p.n_div, p.n_istr) + j * V;
     }
+    VT r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) r[u] = *reinterpret_cast<const VT *>(src + in[u]);
+    // a use of every loaded value here keeps the loads ahead of the conditional stores (the
+    // compiler would otherwise sink each load into its store's branch, one after another)
+#pragma unroll
+    for (int u = 0; u < U; ++u) __asm__ volatile("" ::"v"(r[u].lo), "v"(r[u].hi));
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t v = v0 + u * G;
@@ -459,9 +466,12 @@ int launch_tiled(const Reduced &r, const void *src, void *dst, hipStream_t s) {
   int B[kMaxGroup], nb = 0;  // old axes in output order, innermost first
   int64_t prodB = 1;
   bool inB[16] = {false};
+  // 8-byte and smaller elements: B grows past the target while A is short (A stops at the
+  // output-fastest axis), so that a tile still spans kMinTile elements
+  const bool pipelined = es <= 8;
   for (int j = r_ - 1; j >= 0 && nb < kMaxGroup; --j) {
     const int x = r.perm[j];
-    if (inA[x] || prodB >= target) break;
+    if (inA[x] || (prodB >= target && (!pipelined || prodA * prodB >= kMinTile))) break;
     B[nb++] = x;
     inB[x] = true;
     prodB *= r.shape[x];
@@ -495,11 +505,22 @@ int launch_tiled(const Reduced &r, const void *src, void *dst, hipStream_t s) {
   // permutations): 8-byte and smaller elements run the pipelined kernel on tiles 128 bytes wide
   // with at most 1024 workgroups (several tiles each, so loads and stores overlap); 16-byte
   // elements run one tile at a time, 256 bytes wide and 32 rows high.
-  const bool pipelined = es <= 8;
   const int64_t wmax = std::max<int64_t>(1, (pipelined ? 128 : 256) / es);
   const int64_t hmax = pipelined ? kMaxTile : 32;
   p.w = (int)std::min<int64_t>({prodA, wmax, (int64_t)kMaxTile});
   p.h = (int)std::min<int64_t>({prodB, (int64_t)(MAXE / (p.w + 1)), (int64_t)kMaxTile, hmax});
+  // A short output run (B stopped at an axis already in A) leaves tiles of a few hundred bytes,
+  // whose offset tables and barriers then cost more than their data: widen such a tile along A
+  // (the height stays, so tile rows keep their alignment).  Measured on the rank-6 sweep: below
+  // 1024 elements for the pipelined kernel, below 256 for 16-byte elements (whose 16 x 32 tiles
+  // are faster than wider ones).
+  const int64_t small = pipelined ? kMinTile : kMinTile / 4;
+  if ((int64_t)p.w * p.h < small && p.w < prodA) {
+    const int64_t w2 = std::min<int64_t>({prodA, (int64_t)kMaxTile,
+                                          std::max<int64_t>(p.w, kMinTile / p.h),
+                                          (int64_t)(MAXE / p.h) - 1});
+    if (w2 > p.w) p.w = (int)w2;
+  }
   p.prodA = prodA;
   p.prodB = prodB;
   p.nca = (prodA + p.w - 1) / p.w;

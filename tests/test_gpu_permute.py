@@ -38,7 +38,9 @@ def _gpu_permute(a: np.ndarray, perm) -> np.ndarray:
     return t.numpy()
 
 
-@pytest.mark.parametrize("shape", SWEEP_SHAPES)
+# the sweep shapes, and ragged ones whose tiles end partway (short output runs, small axes next
+# to the contiguous ones)
+@pytest.mark.parametrize("shape", SWEEP_SHAPES + [(33, 4, 17, 4, 5, 4), (3, 64, 5, 2, 40, 2)])
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128])
 def test_permute_sweep_bit_exact(shape, dtype):
     rng = np.random.default_rng(7)
