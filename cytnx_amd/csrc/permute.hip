@@ -516,9 +516,14 @@ int launch_tiled(const Reduced &r, const void *src, void *dst, hipStream_t s) {
   // are faster than wider ones).
   const int64_t small = pipelined ? kMinTile : kMinTile / 4;
   if ((int64_t)p.w * p.h < small && p.w < prodA) {
-    const int64_t w2 = std::min<int64_t>({prodA, (int64_t)kMaxTile,
-                                          std::max<int64_t>(p.w, kMinTile / p.h),
-                                          (int64_t)(MAXE / p.h) - 1});
+    int64_t w2 = std::min<int64_t>({prodA, (int64_t)kMaxTile,
+                                    std::max<int64_t>(p.w, kMinTile / p.h),
+                                    (int64_t)(MAXE / p.h) - 1});
+    if (w2 < prodA) {  // a partial width: a power of two, so tile rows stay aligned
+      int64_t pw = 1;
+      while (pw * 2 <= w2) pw *= 2;
+      w2 = pw;
+    }
     if (w2 > p.w) p.w = (int)w2;
   }
   p.prodA = prodA;
