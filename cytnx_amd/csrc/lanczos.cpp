@@ -647,17 +647,17 @@ int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, int it
                  hipStream_t s) {
   const IterIdx iter{it < 0 ? L->counter : nullptr, it < 0 ? 0 : it};
   double *part_a = L->part, *part_b = L->part + kPartials;
+  // cur holds the previous residual u (V[0] at it = 0); w = H u
   int rc = fn(ctx, L->cur, L->w, (cxa_stream_t)s);
   if (rc) return rc;
-  // alpha partials; alpha with the three-term update; (re-orthogonalisation); beta with the next
-  // basis vector
-  if ((rc = vec_dot_partials(L->w, L->cur, L->n, part_a, s))) return rc;
-  if ((rc = vec_three_term(L->w, L->cur, L->V, L->n, part_a, L->As(), L->Bs(), iter, part_b, s)))
+  // beta of u and V[it] = u / beta with the alpha partials; alpha with the residual (into cur);
+  // (re-orthogonalisation of the residual against V[0..it])
+  if ((rc = vec_dot_basis(L->w, L->cur, L->V, L->n, part_b, L->Bs(), iter, part_a, s))) return rc;
+  if ((rc = vec_residual(L->w, L->cur, L->V, L->n, part_a, L->As(), L->Bs(), iter, part_b, s)))
     return rc;
-  if (reorth && (rc = vec_reorth(L->w, L->V, L->n, L->maxiter + 1, iter, L->part, L->h(),
+  if (reorth && (rc = vec_reorth(L->cur, L->V, L->n, L->maxiter + 1, iter, L->part, L->h(),
                                  part_b, s)))
     return rc;
-  if ((rc = vec_next(L->cur, L->V, L->w, L->n, part_b, L->Bs(), iter, s))) return rc;
   return it < 0 ? vec_counter_add(L->counter, 1, s) : CXA_OK;
 }
 
@@ -751,6 +751,10 @@ int run_batch(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, int reorth, bool 
     }
   }
   const int m1 = L->maxiter + 1, upto = it + nb;
+  // beta of the batch's last residual (the next iteration's first launch would record it)
+  if ((rc = vec_finalize(L->part + kPartials, L->Bs(), use_graph ? L->counter : nullptr,
+                         use_graph ? -1 : upto - 1, 1, ws)))
+    return rc;
   CXA_HIP(hipMemcpyAsync(L->hbuf, L->As(), sizeof(double) * upto, hipMemcpyDeviceToHost, ws));
   CXA_HIP(hipMemcpyAsync(L->hbuf + m1, L->Bs(), sizeof(double) * upto, hipMemcpyDeviceToHost,
                          ws));

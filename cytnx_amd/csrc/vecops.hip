@@ -69,27 +69,54 @@ __device__ __forceinline__ double all_sum_partials(const double *__restrict__ pa
   return bc;
 }
 
-// w -= (alpha * cur + beta * prev) with alpha = the dot product whose partials are in part_a
-// (recorded in As[it] by block 0), beta = Bs[it - 1] (0 when it == 0), prev = V[it - 1];
-// partial sums of the updated w^2 go to part_b
-__global__ __launch_bounds__(kT) void k_three_term(double *__restrict__ w,
-                                                   const double *__restrict__ cur,
-                                                   const double *__restrict__ V, int64_t n,
-                                                   const double *__restrict__ part_a,
-                                                   double *__restrict__ As,
-                                                   const double *__restrict__ Bs, IterIdx iter,
-                                                   double *__restrict__ part_b) {
+// One Lanczos iteration is two launches after the matvec y = H u.  The matvec input u is the
+// previous iteration's residual, not yet divided by its norm beta (it = 0: u = V[0], beta = 1);
+// the division happens in the first launch, so no launch of its own normalises the next vector.
+//
+// First: beta = sqrt(sum of part_b) (recorded in Bs[it - 1] by block 0), V[it] = u / beta, and
+// partials of (y / beta) . V[it] into part_a.
+__global__ __launch_bounds__(kT) void k_dot_basis(const double *__restrict__ y,
+                                                  const double *__restrict__ u,
+                                                  double *__restrict__ V, int64_t n,
+                                                  const double *__restrict__ part_b,
+                                                  double *__restrict__ Bs, IterIdx iter,
+                                                  double *__restrict__ part_a) {
+  const int it = iter.get();
+  const double b = it > 0 ? sqrt(all_sum_partials(part_b, kPartials)) : 1.0;
+  if (it > 0 && blockIdx.x == 0 && threadIdx.x == 0) Bs[it - 1] = b;
+  double *v = V + (int64_t)it * n;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const double vi = u[i] / b;
+    if (it > 0) v[i] = vi;
+    s += (y[i] / b) * vi;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) part_a[blockIdx.x] = s;
+}
+
+// Second: alpha = sum of part_a (recorded in As[it] by block 0); the residual
+// u = y / beta - (alpha V[it] + beta V[it - 1]) (no V[-1] term at it = 0), the next matvec input;
+// partials of |u|^2 into part_b.
+__global__ __launch_bounds__(kT) void k_residual(const double *__restrict__ y,
+                                                 double *__restrict__ u,
+                                                 const double *__restrict__ V, int64_t n,
+                                                 const double *__restrict__ part_a,
+                                                 double *__restrict__ As,
+                                                 const double *__restrict__ Bs, IterIdx iter,
+                                                 double *__restrict__ part_b) {
   const int it = iter.get();
   const double a = all_sum_partials(part_a, kPartials);
   if (blockIdx.x == 0 && threadIdx.x == 0) As[it] = a;
-  const double b = it > 0 ? Bs[it - 1] : 0.0;
+  const double b = it > 0 ? Bs[it - 1] : 1.0;
+  const double bp = it > 0 ? b : 0.0;
+  const double *cur = V + (int64_t)it * n;
   const double *prev = it > 0 ? V + (int64_t)(it - 1) * n : cur;
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
-    const double t = a * cur[i] + b * prev[i];
-    const double v = w[i] - t;
-    w[i] = v;
-    s += v * v;
+    const double t = y[i] / b - (a * cur[i] + bp * prev[i]);
+    u[i] = t;
+    s += t * t;
   }
   s = block_sum(s);
   if (threadIdx.x == 0) part_b[blockIdx.x] = s;
@@ -151,23 +178,6 @@ __global__ __launch_bounds__(kT) void k_proj_apply(double *__restrict__ w,
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// beta = |w| from the partials in part_b (recorded in Bs[it] by block 0), then
-// cur = V[it + 1] = w / beta
-__global__ __launch_bounds__(kT) void k_next(double *__restrict__ cur, double *__restrict__ V,
-                                             const double *__restrict__ w, int64_t n,
-                                             const double *__restrict__ part_b,
-                                             double *__restrict__ Bs, IterIdx iter) {
-  const int it = iter.get();
-  const double b = sqrt(all_sum_partials(part_b, kPartials));
-  if (blockIdx.x == 0 && threadIdx.x == 0) Bs[it] = b;
-  double *dst = V + (int64_t)(it + 1) * n;
-  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
-    const double v = w[i] / b;
-    cur[i] = v;
-    dst[i] = v;
-  }
-}
-
 __global__ void k_counter_set(int *counter, int v) { *counter = v; }
 __global__ void k_counter_add(int *counter, int d) { *counter += d; }
 
@@ -226,11 +236,18 @@ int vec_dot_partials(const double *x, const double *y, int64_t n, double *part, 
   return CXA_OK;
 }
 
-int vec_three_term(double *w, const double *cur, const double *V, int64_t n,
-                   const double *part_a, double *As, const double *Bs, IterIdx iter,
-                   double *part_b, hipStream_t s) {
-  hipLaunchKernelGGL(k_three_term, dim3(kPartials), dim3(kT), 0, s, w, cur, V, n, part_a, As, Bs,
-                     iter, part_b);
+int vec_dot_basis(const double *y, const double *u, double *V, int64_t n, const double *part_b,
+                  double *Bs, IterIdx iter, double *part_a, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_basis, dim3(kPartials), dim3(kT), 0, s, y, u, V, n, part_b, Bs, iter,
+                     part_a);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_residual(const double *y, double *u, const double *V, int64_t n, const double *part_a,
+                 double *As, const double *Bs, IterIdx iter, double *part_b, hipStream_t s) {
+  hipLaunchKernelGGL(k_residual, dim3(kPartials), dim3(kT), 0, s, y, u, V, n, part_a, As, Bs, iter,
+                     part_b);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }
@@ -262,13 +279,6 @@ int vec_finalize(const double *part, double *out, const int *counter, int offset
                  hipStream_t s) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kT), 0, s, part, kPartials, out, counter, offset,
                      mode);
-  CXA_HIP(hipGetLastError());
-  return CXA_OK;
-}
-
-int vec_next(double *cur, double *V, const double *w, int64_t n, const double *part_b,
-             double *Bs, IterIdx iter, hipStream_t s) {
-  hipLaunchKernelGGL(k_next, dim3(grid_of(n)), dim3(kT), 0, s, cur, V, w, n, part_b, Bs, iter);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }

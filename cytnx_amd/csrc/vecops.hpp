@@ -24,10 +24,13 @@ struct IterIdx {
 // sums itself in a fixed order (no separate finishing launch).
 // part = kPartials partials of x.y
 int vec_dot_partials(const double *x, const double *y, int64_t n, double *part, hipStream_t s);
-// As[it] = sum(part_a); w -= As[it] cur + Bs[it-1] V[it-1]; part_b = partials of |w|^2
-int vec_three_term(double *w, const double *cur, const double *V, int64_t n,
-                   const double *part_a, double *As, const double *Bs, IterIdx iter,
-                   double *part_b, hipStream_t s);
+// After y = H u (u: the previous residual, unnormalised; it = 0: u = V[0]):
+// beta = sqrt(sum(part_b)) -> Bs[it-1]; V[it] = u / beta; part_a = partials of (y/beta).V[it]
+int vec_dot_basis(const double *y, const double *u, double *V, int64_t n, const double *part_b,
+                  double *Bs, IterIdx iter, double *part_a, hipStream_t s);
+// As[it] = sum(part_a); u = y/beta - As[it] V[it] - beta V[it-1]; part_b = partials of |u|^2
+int vec_residual(const double *y, double *u, const double *V, int64_t n, const double *part_a,
+                 double *As, const double *Bs, IterIdx iter, double *part_b, hipStream_t s);
 // w -= sum_{j <= it} (V_j . w) V_j (classical Gram-Schmidt against the stored basis);
 // part_out = partials of the new |w|^2; part: kPartials * max_j scratch
 int vec_reorth(double *w, const double *V, int64_t n, int max_j, IterIdx iter, double *part,
@@ -37,9 +40,6 @@ int vec_project(const double *V, const double *x, int64_t n, int k, double *part
                 hipStream_t s);
 int vec_finalize(const double *part, double *out, const int *counter, int offset, int mode,
                  hipStream_t s);
-// Bs[it] = sqrt(sum(part_b)); cur = V[it+1] = w / Bs[it]
-int vec_next(double *cur, double *V, const double *w, int64_t n, const double *part_b,
-             double *Bs, IterIdx iter, hipStream_t s);
 int vec_counter_add(int *counter, int d, hipStream_t s);
 int vec_counter_set(int *counter, int v, hipStream_t s);
 int vec_normalize_init(double *cur, double *V, const double *x, int64_t n, const double *nrm,
