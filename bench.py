@@ -430,6 +430,8 @@ def main() -> int:
     ap.add_argument("--chain", default="mix", choices=["mix", "w12", "ref"],
                     help="matvec program: mix (two launches), w12 (three), ref (four, the "
                          "reference's contraction order)")
+    ap.add_argument("--permute-only", action="store_true",
+                    help="run only the rank-6 permute sweep and print its JSON (for rocprofv3)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -448,6 +450,9 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
     torch.cuda.set_device(0)
+    if args.permute_only:
+        print(json.dumps({"permute_sweep": bench_permute(torch)}), flush=True)
+        return 0
     log("building chi=%d problem" % args.chi)
     host = heisenberg.two_site_problem(args.chi, args.seed, hermitian=True)
     problem = [t.to(0) for t in host]
