@@ -23,9 +23,10 @@
 namespace cxa {
 namespace {
 
-struct alignas(16) U128 {
-  uint64_t lo, hi;
-};
+// A 16-byte element (and the row copies' 16-byte vector): two 64-bit lanes of a vector type, so
+// that the kernels' register staging arrays of it stay in registers (an array of a 16-byte struct
+// was placed in scratch memory, which tripled the tiled kernels' memory traffic).
+typedef uint64_t U128 __attribute__((ext_vector_type(2)));
 
 constexpr int kMaxOuter = 14;
 constexpr int kMaxGroup = 8;
@@ -370,7 +371,7 @@ __global__ __launch_bounds__(kThreads) void permute_rows32(const T *__restrict__
     // a use of every loaded value here keeps the loads ahead of the conditional stores (the
     // compiler would otherwise sink each load into its store's branch, one after another)
 #pragma unroll
-    for (int u = 0; u < U; ++u) __asm__ volatile("" ::"v"(r[u].lo), "v"(r[u].hi));
+    for (int u = 0; u < U; ++u) __asm__ volatile("" ::"v"(r[u].x), "v"(r[u].y));
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t v = v0 + u * G;
