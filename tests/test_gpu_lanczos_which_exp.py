@@ -169,3 +169,28 @@ def test_exp_rejects_bad_arguments():
         Lanczos_Exp(op, dev_vec([1.0, 0.0]), 0.1, 1e-10, 1)
     with pytest.raises(NotImplementedError):
         Lanczos_Exp(op, dev_vec([1.0, 0.0]), 0.1j, 1e-10)
+
+
+@pytest.mark.parametrize("tau", [-0.01, 0.005])
+def test_exp_on_a_native_two_site_chain_vs_expm(tau):
+    """Lanczos_Exp with the planned chain as the matvec (no Python in the iteration) against
+    scipy.linalg.expm of the same operator's dense matrix."""
+    from scipy.linalg import expm
+    from cytnx_amd import heisenberg
+    from cytnx_amd.heff import Heff2
+    psi, L, M1, M2, R = [t.to(0) for t in heisenberg.two_site_problem(8, 5, True)]
+    H = Heff2(psi, L, M1, M2, R)
+    n = psi._storage.numel()
+    cols = []
+    for j in range(n):
+        e = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+        e[j] = 1
+        y = torch.empty_like(e)
+        H.apply_flat(e, y)
+        cols.append(y.cpu().numpy())
+    Hm = np.stack(cols, axis=1)
+    x = psi._storage.cpu().numpy()
+    out = Lanczos_Exp(H, psi, tau, 1e-13)
+    want = expm(tau * 0.5 * (Hm + Hm.T)) @ x
+    got = out._storage.cpu().numpy()
+    assert np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want)
