@@ -10,7 +10,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcytnx_amd.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_NOMEM, ERR_NUMERIC = 0, 1, 2, 3, 4, 5
 
@@ -66,8 +66,8 @@ def _declare(L):
         "cxa_mix_contract_plan": (ci, [ci, ci, i64p, i64, i64p, i64, i64p, dblp,
                                        ctypes.POINTER(vp)]),
         "cxa_vec_workspace_bytes": (ci, []),
-        "cxa_dot": (ci, [i64, vp, vp, vp, vp, vp]),
-        "cxa_nrm2": (ci, [i64, vp, vp, vp, vp]),
+        "cxa_dot": (ci, [ci, i64, vp, vp, ci, vp, vp, vp]),
+        "cxa_nrm2": (ci, [ci, i64, vp, vp, vp, vp]),
         "cxa_tridiag_ground": (ci, [ci, dblp, dblp, dblp, dblp]),
         "cxa_chain_create": (ci, [ci, ctypes.POINTER(vp), ctypes.POINTER(vp), intp, i64p, i64p,
                                   ctypes.POINTER(vp)]),

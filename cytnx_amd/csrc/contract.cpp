@@ -732,6 +732,7 @@ extern "C" int cxa_gemm_batch(int dtype, const char *transa, const char *transb,
   std::vector<GBlk> blks;
   std::vector<GSeg> segs;
   const int es = elem_size(dtype);
+  const bool cplx = dtype == CXA_C128 || dtype == CXA_C64;
   int64_t idx = 0;
   for (int64_t gi = 0; gi < group_count; ++gi) {
     const char ta = transa[gi], tb = transb[gi];
@@ -750,8 +751,15 @@ extern "C" int cxa_gemm_batch(int dtype, const char *transa, const char *transb,
       GBlk g{};
       g.c_off = (int64_t)((uintptr_t)c[idx] / es);
       g.ldc = ldc[gi];
-      g.alpha = alpha[gi];
-      g.beta = beta[gi];
+      if (cplx) {  // (re, im) pairs per group
+        g.alpha = alpha[2 * gi];
+        g.alpha_im = alpha[2 * gi + 1];
+        g.beta = beta[2 * gi];
+        g.beta_im = beta[2 * gi + 1];
+      } else {
+        g.alpha = alpha[gi];
+        g.beta = beta[gi];
+      }
       g.M = (int32_t)n[gi];
       g.N = (int32_t)m[gi];
       g.seg0 = (int32_t)segs.size();

@@ -638,7 +638,8 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
     }
   }
   gptr<c2> Cp = at<c2>(Cbase, blk.c_off);
-  const double alpha = blk.alpha, beta = blk.beta;
+  // complex alpha and beta (Gemm_Batch passes Scalars); contractions use alpha = 1, beta = 0
+  const double ar_ = blk.alpha, ai_ = blk.alpha_im, br_ = blk.beta, bi_ = blk.beta_im;
 #pragma unroll
   for (int i = 0; i < MI; ++i)
 #pragma unroll
@@ -648,12 +649,13 @@ __device__ __forceinline__ void wave_tile_cplx(const char *__restrict__ Abase,
         const int row = 16 * i + acc_row(F64, g, v);
         const int col = 16 * jj + r;
         if (row < mt && col < nt) {
-          double re = alpha * (double)accr[i][jj][v], im = alpha * (double)acci[i][jj][v];
+          const double xr = (double)accr[i][jj][v], xi = (double)acci[i][jj][v];
+          double re = ar_ * xr - ai_ * xi, im = ar_ * xi + ai_ * xr;
           gptr<c2> dst = Cp + (int64_t)(m0 + row) * blk.ldc + (n0 + col);
-          if (beta != 0.0) {
+          if (br_ != 0.0 || bi_ != 0.0) {
             const c2 o = *dst;
-            re += beta * (double)o.x;
-            im += beta * (double)o.y;
+            re += br_ * (double)o.x - bi_ * (double)o.y;
+            im += br_ * (double)o.y + bi_ * (double)o.x;
           }
           c2 w;
           w.x = (op_t)re;

@@ -32,6 +32,7 @@ struct GBlk {
   int32_t M, N;
   int32_t seg0, nseg;
   int32_t out0, nout;  // mix launches only: the item's outputs (MixOut)
+  double alpha_im, beta_im;  // imaginary parts of alpha / beta (complex dtypes only)
 };
 
 // Mix launches (a grouped GEMM with a linear-combination epilogue).  An item is a GBlk whose

@@ -212,6 +212,94 @@ __global__ __launch_bounds__(kT) void k_cast_f64_f32(const double *__restrict__ 
     y[i] = (float)x[i];
 }
 
+// ---- dtype-generic dot product / 2-norm (the Vectordot / Norm drop-ins) -------------------
+// Products are formed and summed in FP64 whatever the element type; the P partial sums (two
+// rows of P for complex: real parts, then imaginary parts) are reduced by one block in a fixed
+// order and the result is written in the element type (the norm in its real type).
+template <typename T>
+struct ElemIO {  // real element: value as FP64
+  static constexpr bool kCplx = false;
+  __device__ static double re(const T &v) { return (double)v; }
+  __device__ static double im(const T &) { return 0.0; }
+};
+struct cplx64 {
+  double re, im;
+};
+struct cplx32 {
+  float re, im;
+};
+template <>
+struct ElemIO<cplx64> {
+  static constexpr bool kCplx = true;
+  __device__ static double re(const cplx64 &v) { return v.re; }
+  __device__ static double im(const cplx64 &v) { return v.im; }
+};
+template <>
+struct ElemIO<cplx32> {
+  static constexpr bool kCplx = true;
+  __device__ static double re(const cplx32 &v) { return (double)v.re; }
+  __device__ static double im(const cplx32 &v) { return (double)v.im; }
+};
+
+// mode 0: x.y, mode 1: conj(x).y, mode 2: |x|^2 (y unused)
+template <typename T, int MODE>
+__global__ __launch_bounds__(kT) void k_vdot_partials(const T *__restrict__ x,
+                                                      const T *__restrict__ y, int64_t n,
+                                                      double *__restrict__ part) {
+  using IO = ElemIO<T>;
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const double xr = IO::re(x[i]), xi = IO::im(x[i]);
+    if constexpr (MODE == 2) {
+      sr += xr * xr + xi * xi;
+    } else {
+      const double yr = IO::re(y[i]), yi = IO::im(y[i]);
+      if constexpr (MODE == 1) {
+        sr += xr * yr + xi * yi;
+        si += xr * yi - xi * yr;
+      } else {
+        sr += xr * yr - xi * yi;
+        si += xr * yi + xi * yr;
+      }
+    }
+  }
+  sr = block_sum(sr);
+  if (threadIdx.x == 0) part[blockIdx.x] = sr;
+  if constexpr (IO::kCplx && MODE != 2) {
+    si = block_sum(si);
+    if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = si;
+  }
+}
+
+// result[0] (and result[1] for a complex dot) in the element type RT; sqrt for the norm
+template <typename RT, bool CPLX, bool SQRT>
+__global__ __launch_bounds__(kT) void k_vdot_finalize(const double *__restrict__ part, int P,
+                                                      RT *__restrict__ result) {
+  double sr = 0.0, si = 0.0;
+  for (int i = threadIdx.x; i < P; i += kT) sr += part[i];
+  sr = block_sum(sr);
+  if constexpr (CPLX) {
+    for (int i = threadIdx.x; i < P; i += kT) si += part[P + i];
+    si = block_sum(si);
+  }
+  if (threadIdx.x == 0) {
+    result[0] = (RT)(SQRT ? sqrt(sr) : sr);
+    if constexpr (CPLX) result[1] = (RT)si;
+  }
+}
+
+template <typename T, typename RT, int MODE>
+int launch_vdot(const void *x, const void *y, int64_t n, double *part, void *result,
+                hipStream_t s) {
+  constexpr bool cplx = ElemIO<T>::kCplx && MODE != 2;
+  hipLaunchKernelGGL((k_vdot_partials<T, MODE>), dim3(kPartials), dim3(kT), 0, s,
+                     (const T *)x, (const T *)y, n, part);
+  hipLaunchKernelGGL((k_vdot_finalize<RT, cplx, MODE == 2>), dim3(1), dim3(kT), 0, s,
+                     (const double *)part, kPartials, (RT *)result);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
 int grid_of(int64_t n) {
   int64_t g = (n + kT - 1) / kT;
   if (g > kPartials) g = kPartials;
@@ -353,22 +441,47 @@ int vec_pack_b(const double *src, void *dst, bool dst_f32, const PackRegion *reg
 }  // namespace cxa
 
 // ---- public entry points ---------------------------------------------------------------------
-// Replaces cuVectordot_internal_d (src/backend/linalg_internal_gpu/cuVectordot_internal.cu
-// :133-180) for real FP64 vectors: result[0] = sum_i x_i y_i, written to device memory.
-extern "C" int cxa_dot(int64_t n, const double *x, const double *y, double *result,
-                       void *workspace, cxa_stream_t stream) {
+// Replaces cuVectordot_internal_{d,f,cd,cf} (src/backend/linalg_internal_gpu/
+// cuVectordot_internal.cu:10-180, type Vectordotfunc_oii): result = sum_i x_i y_i, or with
+// is_conj sum_i conj(x_i) y_i (the zdotc / cdotc path), written to device memory in the dtype.
+extern "C" int cxa_dot(int dtype, int64_t n, const void *x, const void *y, int is_conj,
+                       void *result, void *workspace, cxa_stream_t stream) {
   CXA_CHECK(n >= 0 && result && workspace, CXA_ERR_INVALID, "cxa_dot: bad arguments");
   CXA_CHECK(n == 0 || (x && y), CXA_ERR_INVALID, "cxa_dot: null vector");
-  return cxa::vec_dot(x, y, n, (double *)workspace, result, nullptr, 0, 0, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  double *part = (double *)workspace;
+  switch (dtype) {
+    case CXA_F64: return cxa::launch_vdot<double, double, 0>(x, y, n, part, result, s);
+    case CXA_F32: return cxa::launch_vdot<float, float, 0>(x, y, n, part, result, s);
+    case CXA_C128:
+      return is_conj ? cxa::launch_vdot<cxa::cplx64, double, 1>(x, y, n, part, result, s)
+                     : cxa::launch_vdot<cxa::cplx64, double, 0>(x, y, n, part, result, s);
+    case CXA_C64:
+      return is_conj ? cxa::launch_vdot<cxa::cplx32, float, 1>(x, y, n, part, result, s)
+                     : cxa::launch_vdot<cxa::cplx32, float, 0>(x, y, n, part, result, s);
+    default:
+      return cxa::fail(CXA_ERR_UNSUPPORTED, "cxa_dot: dtype %d is not F64, F32, C128 or C64", dtype);
+  }
 }
 
-// Replaces cuNorm_internal_d (src/backend/linalg_internal_gpu/cuNorm_internal.cu:14-27):
-// result[0] = sqrt(sum_i x_i^2), written to device memory.
-extern "C" int cxa_nrm2(int64_t n, const double *x, double *result, void *workspace,
+// Replaces cuNorm_internal_{d,f,cd,cf} (src/backend/linalg_internal_gpu/cuNorm_internal.cu
+// :14-60, type Normfunc_oii): result[0] = sqrt(sum_i |x_i|^2) in the dtype's real type (double
+// for F64 / C128, float for F32 / C64), written to device memory.
+extern "C" int cxa_nrm2(int dtype, int64_t n, const void *x, void *result, void *workspace,
                         cxa_stream_t stream) {
   CXA_CHECK(n >= 0 && result && workspace, CXA_ERR_INVALID, "cxa_nrm2: bad arguments");
   CXA_CHECK(n == 0 || x, CXA_ERR_INVALID, "cxa_nrm2: null vector");
-  return cxa::vec_dot(x, x, n, (double *)workspace, result, nullptr, 0, 1, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  double *part = (double *)workspace;
+  switch (dtype) {
+    case CXA_F64: return cxa::launch_vdot<double, double, 2>(x, x, n, part, result, s);
+    case CXA_F32: return cxa::launch_vdot<float, float, 2>(x, x, n, part, result, s);
+    case CXA_C128: return cxa::launch_vdot<cxa::cplx64, double, 2>(x, x, n, part, result, s);
+    case CXA_C64: return cxa::launch_vdot<cxa::cplx32, float, 2>(x, x, n, part, result, s);
+    default:
+      return cxa::fail(CXA_ERR_UNSUPPORTED, "cxa_nrm2: dtype %d is not F64, F32, C128 or C64", dtype);
+  }
 }
 
-extern "C" int cxa_vec_workspace_bytes(void) { return cxa::kPartials * (int)sizeof(double); }
+// partial sums of a complex dot: two rows of kPartials doubles
+extern "C" int cxa_vec_workspace_bytes(void) { return 2 * cxa::kPartials * (int)sizeof(double); }

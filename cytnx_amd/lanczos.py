@@ -63,24 +63,45 @@ _MATVEC_T = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, cty
                              ctypes.c_void_p)
 
 
+CTX_CACHE_BYTES = 16 << 30  # Krylov bases kept alive between runs (all devices)
+
+
 def _context(n: int, maxiter: int, device: int):
+    """A Lanczos context of n-element vectors holding maxiter + 1 basis vectors, reused across
+    runs; cached contexts are released (all of them) once their bases would pass
+    CTX_CACHE_BYTES."""
     key = (n, maxiter, device)
     ctx = _CTX_CACHE.get(key)
     if ctx is None:
+        need = 8 * n * (maxiter + 1)
+        held = sum(8 * k[0] * (k[1] + 1) for k in _CTX_CACHE)
+        if _CTX_CACHE and held + need > CTX_CACHE_BYTES:
+            torch.cuda.synchronize()
+            for v in _CTX_CACHE.values():
+                _lib.lib().cxa_lanczos_destroy(v)
+            _CTX_CACHE.clear()
         h = ctypes.c_void_p(0)
         torch.cuda.set_device(device)
         _lib.check(_lib.lib().cxa_lanczos_create(n, maxiter, ctypes.byref(h)))
         ctx = h.value
-        if len(_CTX_CACHE) > 8:
-            for k, v in list(_CTX_CACHE.items()):
-                _lib.lib().cxa_lanczos_destroy(v)
-            _CTX_CACHE.clear()
         _CTX_CACHE[key] = ctx
     return ctx
 
 
-def _krylov_cap(n: int, maxiter: int, budget_bytes: int = 2 << 30) -> int:
+KRYLOV_BUDGET_BYTES = 8 << 30  # device memory one Krylov basis may take
+
+
+def _krylov_cap(n: int, maxiter: int, budget_bytes: int = KRYLOV_BUDGET_BYTES) -> int:
+    """Basis vectors a context holds: at most maxiter, within the memory budget."""
     return int(max(2, min(maxiter, budget_bytes // (8 * max(n, 1)) - 1)))
+
+
+class KrylovCapWarning(RuntimeWarning):
+    """A Lanczos run stopped at the memory cap of its Krylov basis before its criterion."""
+
+
+class KrylovCapError(RuntimeError):
+    """A Lanczos_Exp run needs more Krylov vectors than the memory budget allows."""
 
 
 def lanczos_gnd_native(op, psi0: torch.Tensor, CvgCrit: float = 1e-14, Maxiter: int = 10000,
@@ -110,6 +131,16 @@ def lanczos_gnd_native(op, psi0: torch.Tensor, CvgCrit: float = 1e-14, Maxiter: 
         ctypes.byref(E), evec.data_ptr(), ctypes.byref(niter),
         alphas.ctypes.data_as(_lib.dblp), betas.ctypes.data_as(_lib.dblp), stream_handle(dev)))
     k = niter.value
+    if cap < Maxiter and k >= cap and betas[k - 1] != 0.0:
+        from scipy.linalg import eigvalsh_tridiagonal
+        prev = eigvalsh_tridiagonal(alphas[:k - 1], betas[:k - 2], select="i",
+                                    select_range=(0, 0))[0] if k > 2 else alphas[0]
+        if not abs(E.value - prev) < CvgCrit:
+            import warnings
+            warnings.warn("Lanczos stopped after %d iterations, the Krylov basis the device "
+                          "memory budget allows (%d vectors of %d elements), with |dE| = %.3e "
+                          "above CvgCrit %.3e" % (k, cap, n, abs(E.value - prev), CvgCrit),
+                          KrylovCapWarning, stacklevel=2)
     return E.value, evec, k, alphas[:k].copy(), betas[:k].copy()
 
 
@@ -192,14 +223,15 @@ _WHICH = {"SA": 0, "LA": 1, "LM": 2}
 
 
 def eigsh_native(op, psi0: torch.Tensor, which: str = "SA", k: int = 1, tol: float = 0.0,
-                 want_vectors: bool = True, fn_ctx=None):
+                 want_vectors: bool = True, fn_ctx=None, Maxiter: int = 10000):
     """k extreme eigenpairs of a flat FP64 device operator (cxa_lanczos_eigsh).  Returns
     (eigenvalues (numpy, reference order), eigenvectors (k x n device tensor or None), Krylov
     vectors used, pairs within the tolerance).  ``op`` is a planned Heff unless ``fn_ctx``
-    gives (fnptr, ctx)."""
+    gives (fnptr, ctx).  The basis grows to at most min(Maxiter, n) vectors (no restarts),
+    bounded by the device memory budget."""
     n = psi0.numel()
     dev = psi0.device.index
-    cap = max(2, _krylov_cap(n, n))  # no restarts: up to n vectors, bounded by memory
+    cap = max(2, _krylov_cap(n, min(int(Maxiter), n)))
     ctx = _context(n, cap, dev)
     fn, mctx = fn_ctx if fn_ctx is not None else op.native_matvec()
     evals = np.zeros(k)
@@ -216,16 +248,24 @@ def eigsh_native(op, psi0: torch.Tensor, which: str = "SA", k: int = 1, tol: flo
 def exp_native(op, psi0: torch.Tensor, tau: float, CvgCrit: float = 1e-14,
                Maxiter: int = 10000, fn_ctx=None):
     """exp(tau H) psi0 for a flat FP64 device operator (cxa_lanczos_exp): (out, Krylov vectors
-    used)."""
+    used).  The basis holds at most min(Maxiter, n) vectors within the device memory budget; a
+    run that reaches that budget before the reference's stop test raises KrylovCapError."""
     n = psi0.numel()
     dev = psi0.device.index
-    ctx = _context(n, max(2, min(int(Maxiter), n)), dev)
+    want = max(2, min(int(Maxiter), n))
+    cap = _krylov_cap(n, want)
+    ctx = _context(n, cap, dev)
     fn, mctx = fn_ctx if fn_ctx is not None else op.native_matvec()
     out = torch.empty_like(psi0)
     niter = ctypes.c_int(0)
     _lib.check(_lib.lib().cxa_lanczos_exp(ctx, fn, mctx, psi0.data_ptr(), float(tau),
-                                          float(CvgCrit), int(Maxiter), out.data_ptr(),
+                                          float(CvgCrit), int(min(Maxiter, cap)), out.data_ptr(),
                                           ctypes.byref(niter), stream_handle(dev)))
+    if cap < want and niter.value >= cap:
+        raise KrylovCapError("Lanczos_Exp: the Krylov basis reached %d vectors of %d elements "
+                             "(the device memory budget, %d bytes) before the stop test; pass "
+                             "a smaller Maxiter or a smaller |tau|"
+                             % (cap, n, KRYLOV_BUDGET_BYTES))
     return out, niter.value
 
 
@@ -291,7 +331,7 @@ def _lanczos_which(Hop, Tin, which, Maxiter, CvgCrit, k, is_V, ncv, verbose):
     fn_ctx, keep = _matvec_of(Hop, tmpl)
     try:
         evals, evecs, niter, nconv = eigsh_native(None, x0, which, k, CvgCrit, is_V,
-                                                  fn_ctx=fn_ctx)
+                                                  fn_ctx=fn_ctx, Maxiter=Maxiter)
     except _lib.CxaError as err:
         _reraise_matvec(err)
     del keep

@@ -106,10 +106,11 @@ def Tensordot(Tl: Tensor, Tr: Tensor, idxl: Sequence[int], idxr: Sequence[int],
     return out
 
 
-def Gemm_Batch(alpha: Sequence[float], a: Sequence[Tensor], b: Sequence[Tensor],
-               beta: Sequence[float], c: Sequence[Tensor], group_size: Sequence[int]) -> None:
+def Gemm_Batch(alpha: Sequence[complex], a: Sequence[Tensor], b: Sequence[Tensor],
+               beta: Sequence[complex], c: Sequence[Tensor], group_size: Sequence[int]) -> None:
     """C_i = alpha_g A_i B_i + beta_g C_i for row-major rank-2 tensors, in place on c
-    (linalg::Gemm_Batch, src/linalg/Gemm_Batch.cpp:14-236; one grouped launch)."""
+    (linalg::Gemm_Batch, src/linalg/Gemm_Batch.cpp:14-236; one grouped launch).  alpha / beta
+    are the per-group Scalars: complex values for complex dtypes, real ones otherwise."""
     n = len(a)
     if not (len(b) == n and len(c) == n and sum(group_size) == n):
         raise ValueError("[Gemm_Batch] inconsistent batch sizes")
@@ -149,12 +150,98 @@ def Gemm_Batch(alpha: Sequence[float], a: Sequence[Tensor], b: Sequence[Tensor],
             idx += 1
     cnt = len(gsz)
     VP = ctypes.c_void_p * max(1, cnt)
-    D = ctypes.c_double * max(1, cnt)
+    if dt in (1, 2):  # complex: (re, im) per group
+        def pairs(v):
+            return [float(x) for z in v for x in (complex(z).real, complex(z).imag)]
+        alphas, betas = pairs(alphas), pairs(betas)
+    else:
+        for z in list(alphas) + list(betas):
+            if complex(z).imag != 0:
+                raise ValueError("[Gemm_Batch] complex alpha/beta need complex operands")
+        alphas = [complex(z).real for z in alphas]
+        betas = [complex(z).real for z in betas]
+    D = ctypes.c_double * max(1, len(alphas))
     tr = b"N" * max(1, cnt)
     _lib.check(_lib.lib().cxa_gemm_batch(
         dt, tr, tr, _lib.i64_array(ms), _lib.i64_array(ns), _lib.i64_array(ks), D(*alphas),
         VP(*flat_a), _lib.i64_array(lda), VP(*flat_b), _lib.i64_array(ldb), D(*betas),
         VP(*flat_c), _lib.i64_array(ldc), cnt, _lib.i64_array(gsz), stream_handle(dev)))
+
+
+_VEC_WS = {}
+
+
+def vec_workspace(dev: int) -> torch.Tensor:
+    """The reduction workspace of cxa_dot / cxa_nrm2 on a device (allocated once)."""
+    ws = _VEC_WS.get(dev)
+    if ws is None:
+        nb = _lib.lib().cxa_vec_workspace_bytes()
+        ws = torch.empty(nb, dtype=torch.uint8, device=torch_device(dev))
+        _VEC_WS[dev] = ws
+    return ws
+
+
+def _flat_data(T) -> torch.Tensor:
+    from .unitensor import UniTensor
+    if isinstance(T, UniTensor):
+        return T.storage().reshape(-1) if T._is_block else T._block.contiguous().storage()
+    return T.contiguous().storage().reshape(-1)
+
+
+def Vectordot(Tl: Tensor, Tr: Tensor, is_conj: bool = False) -> Tensor:
+    """linalg::Vectordot (src/linalg/Vectordot.cpp, backend Vectordotfunc_oii): sum_i Tl_i Tr_i
+    (conj(Tl_i) Tr_i with is_conj) of two rank-1 tensors of equal length, as a one-element
+    Tensor of the promoted dtype (cxa_dot, summed in FP64)."""
+    if Tl.rank() != 1 or Tr.rank() != 1:
+        raise ValueError("[Vectordot] the input Tensors should be rank-1")
+    if Tl.shape()[0] != Tr.shape()[0]:
+        raise ValueError("[Vectordot] the two input Tensors should have the same size")
+    dev = _require_device(Tl, Tr)
+    dt = type_promote(Tl.dtype(), Tr.dtype())
+    if dt not in _FLOAT_IDS:
+        raise NotImplementedError("integer Vectordot is outside the GPU hot path")
+    a = _as_dtype(Tl.contiguous(), dt).storage()
+    b = _as_dtype(Tr.contiguous(), dt).storage()
+    out = torch.empty(1, dtype=TORCH_OF[dt], device=torch_device(dev))
+    _lib.check(_lib.lib().cxa_dot(dt, a.numel(), a.data_ptr(), b.data_ptr(), int(bool(is_conj)),
+                                  out.data_ptr(), vec_workspace(dev).data_ptr(),
+                                  stream_handle(dev)))
+    return Tensor.from_storage(out, [1])
+
+
+def _nrm2_device(flat: torch.Tensor, dt: int, dev: int) -> torch.Tensor:
+    real = {1: torch.float64, 2: torch.float32, 3: torch.float64, 4: torch.float32}[dt]
+    out = torch.zeros(1, dtype=real, device=torch_device(dev))
+    if flat.numel():
+        _lib.check(_lib.lib().cxa_nrm2(dt, flat.numel(), flat.data_ptr(), out.data_ptr(),
+                                       vec_workspace(dev).data_ptr(), stream_handle(dev)))
+    return out
+
+
+def norm(T) -> float:
+    """linalg::norm of a Tensor or UniTensor (2-norm of all elements, cxa_nrm2), as a float."""
+    flat = _flat_data(T)
+    dev = T.device()
+    if dev == CPU:
+        raise NotImplementedError("cytnx_amd implements the Device.cuda branch; use a GPU tensor")
+    dt = T.dtype()
+    if dt not in _FLOAT_IDS:
+        raise NotImplementedError("integer norm is outside the GPU hot path")
+    return float(_nrm2_device(flat.contiguous(), dt, dev).item())
+
+
+def Norm(T) -> Tensor:
+    """linalg::Norm (deprecated in the reference in favour of norm): the 2-norm as a one-element
+    Tensor of the real dtype, computed on the device (cxa_nrm2, Normfunc_oii)."""
+    import warnings
+    warnings.warn("Norm() is deprecated, use norm() instead.", DeprecationWarning, stacklevel=2)
+    dev = T.device()
+    if dev == CPU:
+        raise NotImplementedError("cytnx_amd implements the Device.cuda branch; use a GPU tensor")
+    dt = T.dtype()
+    if dt not in _FLOAT_IDS:
+        raise NotImplementedError("integer Norm is outside the GPU hot path")
+    return Tensor.from_storage(_nrm2_device(_flat_data(T).contiguous(), dt, dev), [1])
 
 
 def Gesvd(T, is_U: bool = True, is_vT: bool = True):
@@ -188,5 +275,5 @@ def Lanczos_Exp(*args, **kwargs):
     return _lanczos_exp(*args, **kwargs)
 
 
-__all__ = ["Matmul", "Tensordot", "Gemm_Batch", "Gesvd", "Svd", "Svd_truncate", "Lanczos",
+__all__ = ["Matmul", "Tensordot", "Gemm_Batch", "Vectordot", "norm", "Norm", "Gesvd", "Svd", "Svd_truncate", "Lanczos",
            "Lanczos_Exp"]

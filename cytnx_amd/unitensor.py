@@ -618,8 +618,12 @@ class UniTensor:
 
     # ------------------------------------------------------------------ norms
     def Norm(self) -> float:
-        s = self.storage()
-        return float(torch.linalg.vector_norm(s).item()) if s.numel() else 0.0
+        """2-norm of all elements (BlockUniTensor::Norm, src/BlockUniTensor.cpp:1356-1372: the
+        blocks together), by cxa_nrm2 on the device."""
+        from .linalg import norm
+        if self.storage().numel() == 0:
+            return 0.0
+        return norm(self)
 
     norm = Norm
 

@@ -87,8 +87,10 @@ int cxa_matmul(int dtype, void *out, const void *inl, const void *inr, int64_t M
  * group_size[g] matrices i (numbered consecutively across groups):
  *   C_i = alpha[g] * op(A_i) * op(B_i) + beta[g] * C_i,  op = 'N' or 'T' (transa/transb[g]),
  *   op(A_i) is m[g] x k[g], op(B_i) is k[g] x n[g], leading dimensions lda/ldb/ldc[g].
- * alpha/beta are real.  a/b/c are host arrays of device pointers.  Synchronous with respect to
- * the stream: returns after the launch that reads the uploaded task list has finished.
+ * alpha/beta are the reference's per-group Scalars: group_count doubles for F64 / F32, and
+ * 2 * group_count doubles for C128 / C64 ((re, im) of group g at [2g], [2g + 1]).  a/b/c are
+ * host arrays of device pointers.  Synchronous with respect to the stream: returns after the
+ * launch that reads the uploaded task list has finished.
  */
 int cxa_gemm_batch(int dtype, const char *transa, const char *transb, const int64_t *m,
                    const int64_t *n, const int64_t *k, const double *alpha,
@@ -198,16 +200,20 @@ int cxa_mix_contract_plan(int dtype, int nitems, const int64_t *items, int64_t n
                           const double *coef, cxa_bc_plan **plan);
 
 /*
- * FP64 vector reductions into device memory (result[0]); workspace: device buffer of
- * cxa_vec_workspace_bytes() bytes.  Deterministic (fixed reduction order).
- * cxa_dot replaces cuVectordot_internal_d (src/backend/linalg_internal_gpu/
- * cuVectordot_internal.cu:133-180, type Vectordotfunc_oii); cxa_nrm2 replaces cuNorm_internal_d
- * (src/backend/linalg_internal_gpu/cuNorm_internal.cu:14-27, type Normfunc_oii).
+ * Vector reductions into device memory; workspace: device buffer of cxa_vec_workspace_bytes()
+ * bytes.  Deterministic (fixed reduction order); products are summed in FP64 for every dtype.
+ * cxa_dot replaces cuVectordot_internal_{d,f,cd,cf} (src/backend/linalg_internal_gpu/
+ * cuVectordot_internal.cu:10-180, type Vectordotfunc_oii, src/backend/linalg_internal_interface
+ * .hpp:141-144): result = sum_i x_i y_i, with is_conj sum_i conj(x_i) y_i, in the dtype (two
+ * values re, im for C128 / C64).  cxa_nrm2 replaces cuNorm_internal_{d,f,cd,cf}
+ * (src/backend/linalg_internal_gpu/cuNorm_internal.cu:14-60, type Normfunc_oii, :87):
+ * result[0] = sqrt(sum_i |x_i|^2) in the dtype's real type.  dtype: F64, F32, C128, C64.
  */
 int cxa_vec_workspace_bytes(void);
-int cxa_dot(int64_t n, const double *x, const double *y, double *result, void *workspace,
-            cxa_stream_t stream);
-int cxa_nrm2(int64_t n, const double *x, double *result, void *workspace, cxa_stream_t stream);
+int cxa_dot(int dtype, int64_t n, const void *x, const void *y, int is_conj, void *result,
+            void *workspace, cxa_stream_t stream);
+int cxa_nrm2(int dtype, int64_t n, const void *x, void *result, void *workspace,
+             cxa_stream_t stream);
 
 /*
  * Lowest eigenpair of the symmetric tridiagonal matrix with diagonal a[n] and off-diagonal

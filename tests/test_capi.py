@@ -61,3 +61,15 @@ def test_check_maps_status_to_exceptions():
         L = _lib.lib()
         _lib.check(L.cxa_permute(3, 1, _lib.u64_array([4]), _lib.u64_array([3]), None, None,
                                  None))
+
+
+def test_vector_reductions_validate_on_host():
+    L = _lib.lib()
+    ws = ctypes.c_void_p(16)  # never dereferenced: every call below fails validation first
+    assert L.cxa_vec_workspace_bytes() >= 2 * 512 * 8
+    rc = L.cxa_dot(5, 4, ctypes.c_void_p(8), ctypes.c_void_p(8), 0, ctypes.c_void_p(8), ws, None)
+    assert rc == _lib.ERR_UNSUPPORTED
+    rc = L.cxa_nrm2(3, -1, ctypes.c_void_p(8), ctypes.c_void_p(8), ws, None)
+    assert rc == _lib.ERR_INVALID
+    rc = L.cxa_dot(3, 4, None, None, 0, ctypes.c_void_p(8), ws, None)
+    assert rc == _lib.ERR_INVALID

@@ -51,22 +51,36 @@ def test_matmul(dtype):
         assert relerr(out.numpy(), ref) <= TOL[dtype], (M, K, N)
 
 
-def test_gemm_batch_groups_alpha_beta():
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.complex128, np.complex64])
+def test_gemm_batch_groups_alpha_beta(dtype):
+    """Two groups with their own alpha / beta (complex Scalars for complex dtypes, as
+    Gemm_Batchfunc_oii passes them), against alpha A.B + beta C from the oracle's matmul."""
     rng = np.random.default_rng(22)
-    shapes = [(3, 4, 5), (17, 33, 9), (40, 2, 70), (1, 1, 1)]
-    alpha, beta, group = [1.0, -0.5], [0.0, 2.0], [2, 2]
+    shapes = [(3, 4, 5), (17, 33, 9), (40, 2, 70), (1, 1, 1), (64, 80, 48)]
+    cplx = np.issubdtype(dtype, np.complexfloating)
+    alpha = [1.0 - 0.25j, -0.5 + 2.0j] if cplx else [1.0, -0.5]
+    beta = [0.0, 2.0 - 1.5j] if cplx else [0.0, 2.0]
+    group = [2, 3]
     a, b, c, ref = [], [], [], []
+    rd = ref_dtype(dtype)
     for i, (M, K, N) in enumerate(shapes):
         g = 0 if i < 2 else 1
-        A, B, C = rand(rng, (M, K), np.float64), rand(rng, (K, N), np.float64), \
-            rand(rng, (M, N), np.float64)
+        A, B, C = rand(rng, (M, K), dtype), rand(rng, (K, N), dtype), rand(rng, (M, N), dtype)
         a.append(Tensor.from_numpy(A, 0))
         b.append(Tensor.from_numpy(B, 0))
         c.append(Tensor.from_numpy(C, 0))
-        ref.append(alpha[g] * po.matmul(A, B) + beta[g] * C)
+        ref.append(alpha[g] * po.matmul(A.astype(rd), B.astype(rd)) + beta[g] * C.astype(rd))
     linalg.Gemm_Batch(alpha, a, b, beta, c, group)
-    for got, r in zip(c, ref):
-        assert relerr(got.numpy(), r) <= 1e-12
+    for i, (got, r) in enumerate(zip(c, ref)):
+        err = relerr(got.numpy(), r)
+        print("gemm_batch", np.dtype(dtype).name, shapes[i], "err %.2e" % err, flush=True)
+        assert err <= TOL[dtype]
+
+
+def test_gemm_batch_rejects_complex_scalars_for_real_operands():
+    A = Tensor.from_numpy(np.ones((2, 2)), 0)
+    with pytest.raises(ValueError):
+        linalg.Gemm_Batch([1j], [A], [A], [0.0], [Tensor.from_numpy(np.ones((2, 2)), 0)], [1])
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128])
