@@ -1,0 +1,19 @@
+// Grouped GEMM launches in FP32 with an FP64 A operand (rounded where consumed).
+#define CXA_GEMM_TU fp32_a64
+#include "gemm_kernels.hpp"
+
+namespace cxa {
+
+int launch_gemm_fp32_a64(const GemmLaunch &g, hipStream_t s) {
+  const int a = g.a_dt, b = g.b_dt, c = g.c_dt;
+#define CXA_F32_CASE(TA_, TB_, TC_, IA, IB, IC) \
+  if (a == IA && b == IB && c == IC) return launch_t<CXA_PREC_FP32, TA_, TB_, TC_, false>(g, s);
+  CXA_F32_CASE(double, float, float, CXA_F64, CXA_F32, CXA_F32)
+  CXA_F32_CASE(double, float, double, CXA_F64, CXA_F32, CXA_F64)
+  CXA_F32_CASE(double, double, float, CXA_F64, CXA_F64, CXA_F32)
+  CXA_F32_CASE(double, double, double, CXA_F64, CXA_F64, CXA_F64)
+#undef CXA_F32_CASE
+  return fail(CXA_ERR_UNSUPPORTED, "grouped GEMM: FP32 with dtypes (%d,%d,%d)", a, b, c);
+}
+
+}  // namespace cxa
