@@ -36,6 +36,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_threads() -> int:
+    """Threads the CPU legs use: the process's CPU share (OMP_NUM_THREADS when the host sets it
+    for one GPU's share of the machine, else every core this process may run on)."""
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(int(share), avail) if share.isdigit() and int(share) > 0 else avail or 1)
+
+
+def host_cpu() -> dict:
+    """The host the CPU baseline runs on: lscpu model, sockets, cores, and the thread share."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "threads_available": host_threads()}
+    try:
+        text = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        text = ""
+    for line in text.splitlines():
+        k, _, v = line.partition(":")
+        k = k.strip()
+        if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+            info[k.lower().replace("(s)", "s").replace(" ", "_")] = v.strip()
+    return info
+
+
 def time_matvec(H, x, y, steps, warmup, torch, graph=True):
     """Average matvec time over `steps` (HIP events on the launch stream).  With graph=True each
     matvec is one replay of the captured chain (one host launch per matvec); otherwise each
@@ -205,7 +229,7 @@ def bench_c1(torch, cpu=True, iters=100):
     if cpu:
         sys.path.insert(0, os.path.join(HERE, "oracle"))
         import pyoracle as po
-        blas = po.use_openblas(nthreads=min(16, os.cpu_count() or 1))
+        blas = po.use_openblas(nthreads=host_threads())
 
         def mv(x):
             A = x.reshape(chi, d, chi)
@@ -218,7 +242,7 @@ def bench_c1(torch, cpu=True, iters=100):
         tc = time.perf_counter() - t0
         po.use_reference_loops()
         out["cpu"] = {"seconds": tc, "iterations": int(ko), "tflops": ko * flops / tc / 1e12,
-                      "cores": min(16, os.cpu_count() or 1), "kind": "port",
+                      "cores": host_threads(), "kind": "port", "host": host_cpu(),
                       "blas": blas, "E": Eo}
         out["speedup_vs_cpu"] = tc / t * (k / max(ko, 1))
     return out
@@ -328,7 +352,7 @@ def bench_dmrg(torch, L=100, chi=2048, sweeps=1, maxit=30, cpu=True):
         M2r = last.M.relabel(["-6", "-7", "-3", "2"])
         f = Heff2(psi, Lr, M1r, M2r, Rr).ref_flops
         Lo, Ro, M1, M2, P = ob(Lr), ob(Rr), ob(M1r), ob(M2r), ob(psi)
-        ncpu = min(16, os.cpu_count() or 1)
+        ncpu = host_threads()
         blas = po.use_openblas(nthreads=ncpu)
         t0 = time.perf_counter()
         X = po.block_contract(Lo, P)
@@ -339,6 +363,7 @@ def bench_dmrg(torch, L=100, chi=2048, sweeps=1, maxit=30, cpu=True):
         po.use_reference_loops()
         est = tc / f * last.stats["flops"]
         out["cpu"] = {"matvec_s_at_centre": tc, "centre_flops": f, "cores": ncpu, "kind": "port",
+                      "host": host_cpu(),
                       "blas": blas, "lanczos_s_estimate": est,
                       "sample": "one centre-bond matvec of the oracle's restated "
                                 "BlockUniTensor::contract chain (FP64), scaled by the run's "
@@ -352,7 +377,7 @@ def cpu_baseline(problem, flops, seconds):
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import pyoracle as po
 
-    ncpu = min(16, os.cpu_count() or 1)
+    ncpu = host_threads()
     psi, L, M1, M2, R = problem
 
     def ob(ut):
@@ -393,6 +418,7 @@ def cpu_baseline(problem, flops, seconds):
     per = el / n
     po.use_reference_loops()
     return {"value": flops / per / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
+            "host": host_cpu(), "threads_tried": sorted({1, ncpu}),
             "sample": "%d matvecs (%.1f s) of the same chi=1024 two-site U(1) Heff: the oracle's "
                       "restated BlockUniTensor::contract chain (per block pair: permute copies, "
                       "dgemm via %s, add pass), FP64" % (n, el, blas or "reference loops"),

@@ -214,6 +214,7 @@ extern "C" int cxa_tridiag_ground(int n, const double *a, const double *b, doubl
 struct cxa_chain {
   int device = 0;
   int nsteps = 0;
+  int dtype = CXA_F64;  // every plan's dtype: F64, or C128 (complex chains: FP64, unpacked)
   std::vector<const cxa_bc_plan *> plans;
   std::vector<const void *> fixed;
   std::vector<int> fixed_left;
@@ -275,10 +276,19 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
   CXA_HIP(hipGetDevice(&c->device));
   c->nsteps = nsteps;
   int64_t wsmax = 0;
+  CXA_CHECK(plans[0], CXA_ERR_INVALID, "cxa_chain_create: null plan at step 0");
+  c->dtype = bc_dtype(plans[0]);
+  CXA_CHECK(c->dtype == CXA_F64 || c->dtype == CXA_C128, CXA_ERR_UNSUPPORTED,
+            "cxa_chain_create: chains run F64 or C128 plans");
+  const bool cplx = c->dtype == CXA_C128;
+  const size_t es = cplx ? 16 : 8;
   for (int k = 0; k < nsteps; ++k) {
     CXA_CHECK(plans[k], CXA_ERR_INVALID, "cxa_chain_create: null plan at step %d", k);
-    CXA_CHECK(bc_dtype(plans[k]) == CXA_F64, CXA_ERR_UNSUPPORTED,
-              "cxa_chain_create: chains run real FP64 plans (step %d)", k);
+    CXA_CHECK(bc_dtype(plans[k]) == c->dtype, CXA_ERR_UNSUPPORTED,
+              "cxa_chain_create: every step's plan must have the first step's dtype (step %d)",
+              k);
+    CXA_CHECK(!cplx || !bc_is_mix(plans[k]), CXA_ERR_UNSUPPORTED,
+              "cxa_chain_create: mix plans are real (step %d)", k);
     c->plans.push_back(plans[k]);
     c->fixed.push_back(fixed[k]);
     c->fixed_left.push_back(fixed_is_left[k] ? 1 : 0);
@@ -289,7 +299,7 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
   }
   c->mid64.assign(nsteps > 1 ? nsteps - 1 : 0, nullptr);
   for (int k = 0; k + 1 < nsteps; ++k)
-    CXA_HIP(hipMalloc(&c->mid64[k], std::max<int64_t>(1, out_elems[k]) * sizeof(double)));
+    CXA_HIP(hipMalloc(&c->mid64[k], std::max<int64_t>(1, out_elems[k]) * es));
   if (wsmax > 0) CXA_HIP(hipMalloc(&c->ws, wsmax * 16));
   // k4-packed copies of the fixed B operands (read once here: the chain keeps using these
   // copies, so a fixed operand changed after creation needs a new chain)
@@ -305,7 +315,7 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
   for (size_t k = 0; k < c->mid64.size(); ++k) c->mid_elems[k] = std::max<int64_t>(1, out_elems[k]);
   std::vector<std::vector<cxa::GSeg>> segs_host(nsteps);
   bool synced = false;
-  for (int k = 0; k < nsteps; ++k) {
+  for (int k = 0; k < nsteps && !cplx; ++k) {
     if (c->fixed_left[k]) continue;
     std::vector<cxa::GSeg> &segs = segs_host[k];
     int64_t elems = 0;
@@ -327,7 +337,7 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
     if (rc) return rc;
   }
   // a mix step whose output the next step reads as its (running) A operand: write it packed
-  for (int k = 0; k + 1 < nsteps; ++k) {
+  for (int k = 0; k + 1 < nsteps && !cplx; ++k) {
     if (!cxa::bc_is_mix(plans[k]) || !c->packed_segs[k] || c->fixed_left[k + 1] ||
         !c->packed_segs[k + 1])
       continue;
@@ -362,6 +372,8 @@ extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
   CXA_CHECK(c, CXA_ERR_INVALID, "cxa_chain_prepare: null chain");
   CXA_CHECK(prec >= CXA_PREC_FP64 && prec <= CXA_PREC_FP32X3, CXA_ERR_INVALID,
             "cxa_chain_prepare: unknown precision %d", prec);
+  CXA_CHECK(c->dtype == CXA_F64 || prec == CXA_PREC_FP64, CXA_ERR_UNSUPPORTED,
+            "cxa_chain_prepare: complex chains run in FP64 only");
   c->prec = prec;
   if (prec != CXA_PREC_FP32 || !c->fixed32.empty()) return CXA_OK;
   hipStream_t s = (hipStream_t)stream;
@@ -391,6 +403,14 @@ extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
 namespace {
 
 int chain_step(cxa_chain *c, int k, const void *x, void *y, hipStream_t s) {
+  if (c->dtype != CXA_F64) {  // complex chain: FP64, every operand as the plan lays it out
+    const void *in = k == 0 ? x : c->mid64[k - 1];
+    void *outp = k == c->nsteps - 1 ? y : c->mid64[k];
+    const int dt = c->dtype;
+    return c->fixed_left[k]
+               ? bc_exec_typed(c->plans[k], c->fixed[k], in, outp, dt, dt, dt, c->prec, c->ws, s)
+               : bc_exec_typed(c->plans[k], in, c->fixed[k], outp, dt, dt, dt, c->prec, c->ws, s);
+  }
   const bool f32 = c->prec == CXA_PREC_FP32;
   const bool last = k == c->nsteps - 1;
   const void *in = k == 0 ? x : (f32 ? c->mid32[k - 1] : c->mid64[k - 1]);
@@ -498,6 +518,8 @@ extern "C" int cxa_chain_tune(cxa_chain *c, const void *x, void *y, int reps, in
                               cand[ci][0] == TK_4x2 || cand[ci][0] == TK_ROWS ||
                               cand[ci][1] == 2))
         continue;
+      // the big-tile and row classes are real-only (complex plans fall back to the default)
+      if (c->dtype != CXA_F64 && (cand[ci][0] == TK_4x2 || cand[ci][0] == TK_ROWS)) continue;
       if ((rc = cxa_block_contract_set_config(plan, c->prec, cand[ci][0], cand[ci][1]))) break;
       for (int w = 0; w < 2 && rc == CXA_OK; ++w) rc = chain_step(c, k, x, y, s);
       if (rc) break;
@@ -607,10 +629,11 @@ struct cxa_lanczos {
     if (hbuf) (void)hipHostFree(hbuf);
     if (own) (void)hipStreamDestroy(own);
   }
+  // device scalars: alphas, betas, projection coefficients (complex: two doubles each), norm
   double *As() { return sc; }
   double *Bs() { return sc + (maxiter + 1); }
   double *h() { return sc + 2 * (maxiter + 1); }
-  double *nrm() { return sc + 3 * (maxiter + 1); }
+  double *nrm() { return sc + 4 * (maxiter + 1); }
 };
 
 extern "C" int cxa_lanczos_create(int64_t n, int maxiter, cxa_lanczos **out) {
@@ -625,11 +648,13 @@ extern "C" int cxa_lanczos_create(int64_t n, int maxiter, cxa_lanczos **out) {
   CXA_HIP(hipMalloc(&L->V, sizeof(double) * n * m1));
   CXA_HIP(hipMalloc(&L->w, sizeof(double) * n));
   CXA_HIP(hipMalloc(&L->cur, sizeof(double) * n));
-  CXA_HIP(hipMalloc(&L->part, sizeof(double) * kPartials * m1));
-  CXA_HIP(hipMalloc(&L->sc, sizeof(double) * (3 * m1 + 8)));
-  CXA_HIP(hipMalloc(&L->coef, sizeof(double) * m1));
+  // partial sums: two rows of m1 (the imaginary parts of complex projections in the second)
+  CXA_HIP(hipMalloc(&L->part, sizeof(double) * kPartials * 2 * m1));
+  CXA_HIP(hipMalloc(&L->sc, sizeof(double) * (4 * m1 + 8)));
+  CXA_HIP(hipMalloc(&L->coef, sizeof(double) * 2 * m1));
   CXA_HIP(hipMalloc(&L->counter, sizeof(int)));
-  CXA_HIP(hipHostMalloc((void **)&L->hbuf, sizeof(double) * 3 * m1, hipHostMallocDefault));
+  // pinned: alphas, betas, then up to 2 * m1 doubles of staged coefficients
+  CXA_HIP(hipHostMalloc((void **)&L->hbuf, sizeof(double) * 4 * m1, hipHostMallocDefault));
   *out = L.release();
   return CXA_OK;
 }
@@ -643,7 +668,7 @@ namespace {
 
 // One iteration.  it >= 0: the iteration index is passed to the kernels by value; it < 0: they
 // read the device counter, which the iteration then advances (the captured form).
-int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, int it,
+int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, bool cplx, int it,
                  hipStream_t s) {
   const IterIdx iter{it < 0 ? L->counter : nullptr, it < 0 ? 0 : it};
   double *part_a = L->part, *part_b = L->part + kPartials;
@@ -656,17 +681,18 @@ int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, int it
   if ((rc = vec_residual(L->w, L->cur, L->V, L->n, part_a, L->As(), L->Bs(), iter, part_b, s)))
     return rc;
   if (reorth && (rc = vec_reorth(L->cur, L->V, L->n, L->maxiter + 1, iter, L->part, L->h(),
-                                 part_b, s)))
+                                 part_b, cplx, s)))
     return rc;
   return it < 0 ? vec_counter_add(L->counter, 1, s) : CXA_OK;
 }
 
 // `iters` iterations (device-counter form) captured on stream ws as one executable graph
-int capture_iterations(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, int iters,
-                       hipStream_t ws, hipGraphExec_t *out) {
+int capture_iterations(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, bool cplx,
+                       int iters, hipStream_t ws, hipGraphExec_t *out) {
   CXA_HIP(hipStreamBeginCapture(ws, hipStreamCaptureModeThreadLocal));
   int rc = CXA_OK;
-  for (int i = 0; i < iters && rc == CXA_OK; ++i) rc = lanczos_body(L, fn, ctx, reorth, -1, ws);
+  for (int i = 0; i < iters && rc == CXA_OK; ++i)
+    rc = lanczos_body(L, fn, ctx, reorth, cplx, -1, ws);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(ws, &g);
   if (rc == CXA_OK && e == hipSuccess) {
@@ -718,16 +744,17 @@ int run_end(RunStream &r) {
 // behind (fn, ctx) may have changed since the last run (a chain's precision, its buffers).  One
 // iteration, and a whole batch of them so that a full batch is one host launch.
 int run_prologue(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, const double *Tin, int reorth,
-                 bool use_graph, int batch, int maxiter, hipStream_t ws) {
+                 bool cplx, bool use_graph, int batch, int maxiter, hipStream_t ws) {
   int rc;
   if ((rc = vec_counter_set(L->counter, 0, ws))) return rc;
   if ((rc = vec_dot(Tin, Tin, L->n, L->part, L->nrm(), nullptr, 0, 1, ws))) return rc;
   if ((rc = vec_normalize_init(L->cur, L->V, Tin, L->n, L->nrm(), ws))) return rc;
   if (use_graph) {
     L->drop_graphs();
-    if ((rc = capture_iterations(L, matvec, ctx, reorth, 1, ws, &L->gexec1))) return rc;
+    if ((rc = capture_iterations(L, matvec, ctx, reorth, cplx, 1, ws, &L->gexec1))) return rc;
     if (batch > 1 && maxiter >= batch) {
-      if ((rc = capture_iterations(L, matvec, ctx, reorth, batch, ws, &L->gexecB))) return rc;
+      if ((rc = capture_iterations(L, matvec, ctx, reorth, cplx, batch, ws, &L->gexecB)))
+        return rc;
       L->gbatch = batch;
     }
   }
@@ -736,8 +763,8 @@ int run_prologue(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, const double *
 
 // Iterations [it, it + nb), then alphas / betas [0, it + nb) copied into the pinned host buffers
 // hA / hB; returns after they have arrived.
-int run_batch(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, int reorth, bool use_graph, int it,
-              int nb, hipStream_t ws) {
+int run_batch(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, int reorth, bool cplx,
+              bool use_graph, int it, int nb, hipStream_t ws) {
   int rc;
   if (use_graph && nb == L->gbatch) {
     CXA_HIP(hipGraphLaunch(L->gexecB, ws));
@@ -745,7 +772,7 @@ int run_batch(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, int reorth, bool 
     for (int b = 0; b < nb; ++b) {
       if (use_graph) {
         CXA_HIP(hipGraphLaunch(L->gexec1, ws));
-      } else if ((rc = lanczos_body(L, matvec, ctx, reorth, it + b, ws))) {
+      } else if ((rc = lanczos_body(L, matvec, ctx, reorth, cplx, it + b, ws))) {
         return rc;
       }
     }
@@ -762,13 +789,16 @@ int run_batch(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, int reorth, bool 
   return CXA_OK;
 }
 
-// out = sum_{j < k} coef[j] V_j with coef given on the host (staged through the pinned buffer).
-int combine_host_coef(cxa_lanczos *L, const double *coef, int k, double *out, hipStream_t ws) {
+// out = sum_{j < k} coef[j] V_j with coef given on the host (staged through the pinned buffer);
+// mode as vec_combine (1, 2: k complex coefficients, 2k doubles).
+int combine_host_coef(cxa_lanczos *L, const double *coef, int k, double *out, int mode,
+                      hipStream_t ws) {
   double *hz = L->hbuf + 2 * (L->maxiter + 1);
+  const int nd = mode ? 2 * k : k;
   CXA_HIP(hipStreamSynchronize(ws));  // the staging buffer may still feed an earlier upload
-  std::memcpy(hz, coef, sizeof(double) * k);
-  CXA_HIP(hipMemcpyAsync(L->coef, hz, sizeof(double) * k, hipMemcpyHostToDevice, ws));
-  return vec_combine(out, L->V, L->coef, k, L->n, ws);
+  std::memcpy(hz, coef, sizeof(double) * nd);
+  CXA_HIP(hipMemcpyAsync(L->coef, hz, sizeof(double) * nd, hipMemcpyHostToDevice, ws));
+  return vec_combine(out, L->V, L->coef, k, L->n, mode, ws);
 }
 
 // Eigenvalues of the symmetric tridiagonal matrix (diagonal a[n], off-diagonal b[n-1]) by the
@@ -855,7 +885,10 @@ void tridiag_qr_full(int n, const double *a, const double *b, std::vector<double
 
 }  // namespace
 
-// flags: bit 0 = FP64 full re-orthogonalisation, bit 1 = replay captured iteration graphs
+// flags: bit 0 = FP64 full re-orthogonalisation, bit 1 = replay captured iteration graphs,
+// bit 2 = complex Hermitian operator on C128 vectors of n / 2 values (alpha and beta are real
+// dots of the n doubles: Re <v|Hv> and |u|, as Lanczos_Gnd_Ut.cpp:96-99 takes them; the
+// re-orthogonalisation projects with complex coefficients)
 extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, const double *Tin,
                                double CvgCrit, int maxiter, int batch, int flags, double *E_out,
                                double *evec, int *niter, double *alphas, double *betas,
@@ -868,11 +901,16 @@ extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
   if (batch < 1) batch = 1;
   const int reorth = flags & 1;
   const bool use_graph = (flags >> 1) & 1;
+  const bool cplx = (flags >> 2) & 1;
+  CXA_CHECK(!cplx || L->n % 2 == 0, CXA_ERR_INVALID,
+            "cxa_lanczos_run: complex vectors need an even number of doubles (got %lld)",
+            (long long)L->n);
   RunStream r;
   int rc;
   if ((rc = run_begin(L, use_graph, (hipStream_t)stream, r))) return rc;
   const hipStream_t ws = r.ws;
-  if ((rc = run_prologue(L, matvec, ctx, Tin, reorth, use_graph, batch, maxiter, ws))) return rc;
+  if ((rc = run_prologue(L, matvec, ctx, Tin, reorth, cplx, use_graph, batch, maxiter, ws)))
+    return rc;
 
   const int m1 = L->maxiter + 1;
   const double *hA = L->hbuf, *hB = L->hbuf + m1;
@@ -881,7 +919,7 @@ extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
   int done = -1, it = 0;
   while (done < 0 && it < maxiter) {
     const int nb = std::min(batch, maxiter - it);
-    if ((rc = run_batch(L, matvec, ctx, reorth, use_graph, it, nb, ws))) return rc;
+    if ((rc = run_batch(L, matvec, ctx, reorth, cplx, use_graph, it, nb, ws))) return rc;
     const int upto = it + nb;  // alphas/betas [0, upto) are valid after the batch
     for (int j = it; j < upto && done < 0; ++j) {
       if (j == 0) {
@@ -916,7 +954,8 @@ extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
   if (niter) *niter = k;
   if (alphas) std::memcpy(alphas, hA, sizeof(double) * k);
   if (betas) std::memcpy(betas, hB, sizeof(double) * k);
-  if (evec && (rc = combine_host_coef(L, z.data(), k, evec, ws))) return rc;
+  // real coefficients: the same combination of the doubles for real and complex vectors
+  if (evec && (rc = combine_host_coef(L, z.data(), k, evec, 0, ws))) return rc;
   return run_end(r);
 }
 
@@ -939,15 +978,24 @@ extern "C" int cxa_tridiag_eigh(int n, const double *a, const double *b, double 
 // Tin / |Tin|; at step i (i >= 1) the step stops before v_i when beta_i is at the round-off
 // level 100 eps max(1, |alpha|, |alpha| + beta_i + beta_{i-1}, ...) sqrt(i), otherwise after v_i
 // when |exp(tau T_{i+1})[i, 0]| < CvgCrit or i = min(Maxiter, n) - 1; then
-// out = V_k exp(tau T_k) (V_k^T Tin).  Real tau, FP64.
+// out = V_k exp(tau T_k) (V_k^H Tin).  tau = tau_re + i tau_im.  flags bit 2: complex Hermitian
+// operator on C128 vectors of n / 2 values (out: n doubles).  Real vectors with tau_im != 0:
+// out is complex, 2n doubles (n C128 values).  T is the real symmetric tridiagonal either way
+// (alpha = Re <v|Hv>), and exp(tau T) = Z diag(exp(tau w)) Z^T from its eigenpairs.
 extern "C" int cxa_lanczos_exp(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, const double *Tin,
-                               double tau, double CvgCrit, int Maxiter, double *out, int *niter,
-                               cxa_stream_t stream) {
+                               double tau_re, double tau_im, double CvgCrit, int Maxiter,
+                               int flags, double *out, int *niter, cxa_stream_t stream) {
   CXA_CHECK(L && matvec && Tin && out, CXA_ERR_INVALID, "cxa_lanczos_exp: bad arguments");
   CXA_CHECK(CvgCrit > 0, CXA_ERR_INVALID, "[ERROR][Lanczos_Exp] converge criteria must >0");
   CXA_CHECK(Maxiter >= 2, CXA_ERR_INVALID, "[ERROR][Lanczos_Exp] Maxiter must >1");
-  CXA_CHECK(std::isfinite(tau), CXA_ERR_INVALID, "[ERROR][Lanczos_Exp] tau must be finite");
-  const int imp = (int)std::min<int64_t>(Maxiter, L->n);  // Krylov vectors at most
+  CXA_CHECK(std::isfinite(tau_re) && std::isfinite(tau_im), CXA_ERR_INVALID,
+            "[ERROR][Lanczos_Exp] tau must be finite");
+  const bool cplx = (flags >> 2) & 1;
+  CXA_CHECK(!cplx || L->n % 2 == 0, CXA_ERR_INVALID,
+            "cxa_lanczos_exp: complex vectors need an even number of doubles (got %lld)",
+            (long long)L->n);
+  const int64_t ne = cplx ? L->n / 2 : L->n;  // vector length in elements
+  const int imp = (int)std::min<int64_t>(Maxiter, ne);  // Krylov vectors at most
   CXA_CHECK(imp <= L->maxiter, CXA_ERR_INVALID,
             "cxa_lanczos_exp: %d Krylov vectors above the context's %d", imp, L->maxiter);
   const double eps = 2.220446049250313e-16;
@@ -955,16 +1003,28 @@ extern "C" int cxa_lanczos_exp(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
   int rc;
   if ((rc = run_begin(L, false, (hipStream_t)stream, r))) return rc;
   const hipStream_t ws = r.ws;
-  if ((rc = run_prologue(L, matvec, ctx, Tin, 0, false, 1, imp, ws))) return rc;
+  if ((rc = run_prologue(L, matvec, ctx, Tin, 0, cplx, false, 1, imp, ws))) return rc;
   const int m1 = L->maxiter + 1;
   const double *hA = L->hbuf, *hB = L->hbuf + m1;
-  // exp(tau T_m)[m-1][0] from rows 0 and m-1 of the eigenvectors of the tridiagonal T_m
+  // exp(tau w) for the eigenvalues w of T (complex tau)
+  auto expw = [&](double w, double *re, double *im) {
+    const double mag = std::exp(tau_re * w);
+    *re = mag * std::cos(tau_im * w);
+    *im = tau_im == 0.0 ? 0.0 : mag * std::sin(tau_im * w);
+  };
+  // |exp(tau T_m)[m-1][0]| from rows 0 and m-1 of the eigenvectors of the tridiagonal T_m
   auto corner = [&](int m) {
     std::vector<double> w, zr;
     tridiag_qr(m, hA, hB, w, {0, m - 1}, zr);
-    double acc = 0.0;
-    for (int k = 0; k < m; ++k) acc += zr[(size_t)m + k] * std::exp(tau * w[k]) * zr[k];
-    return acc;
+    double ar = 0.0, ai = 0.0;
+    for (int k = 0; k < m; ++k) {
+      double er, ei;
+      expw(w[k], &er, &ei);
+      const double z = zr[(size_t)m + k] * zr[k];
+      ar += z * er;
+      ai += z * ei;
+    }
+    return std::hypot(ar, ai);
   };
   int size = -1, it = 0;
   double scale = 1.0;
@@ -973,7 +1033,7 @@ extern "C" int cxa_lanczos_exp(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
     // one more iteration than the host checks: alpha_i of step i comes from iteration i
     const int nb = std::min(batch, imp - it);
     if (nb <= 0) break;
-    if ((rc = run_batch(L, matvec, ctx, 0, false, it, nb, ws))) return rc;
+    if ((rc = run_batch(L, matvec, ctx, 0, cplx, false, it, nb, ws))) return rc;
     const int upto = it + nb;
     for (int i = std::max(it, 1); i < upto && size < 0; ++i) {
       if (i == 1) scale = std::max(std::fabs(hA[0]), 1.0);
@@ -984,26 +1044,52 @@ extern "C" int cxa_lanczos_exp(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
         size = i;  // breakdown: v_0 .. v_{i-1}
         break;
       }
-      if (std::fabs(corner(i + 1)) < CvgCrit || i == imp - 1) size = i + 1;
+      if (corner(i + 1) < CvgCrit || i == imp - 1) size = i + 1;
     }
     it = upto;
   }
   if (size < 0) size = std::max(1, std::min(it, imp));
-  // c = V^T Tin on the device, then y = B c on the host, out = V y
-  if ((rc = vec_project(L->V, Tin, L->n, size, L->part, L->h(), ws))) return rc;
+  // c = V^H Tin on the device (complex for complex vectors), then y = exp(tau T) c on the host,
+  // out = V y
+  if ((rc = vec_project(L->V, Tin, L->n, size, L->part, L->h(), cplx, ws))) return rc;
   double *hc = L->hbuf + 2 * m1;
-  CXA_HIP(hipMemcpyAsync(hc, L->h(), sizeof(double) * size, hipMemcpyDeviceToHost, ws));
+  CXA_HIP(hipMemcpyAsync(hc, L->h(), sizeof(double) * size * (cplx ? 2 : 1),
+                         hipMemcpyDeviceToHost, ws));
   CXA_HIP(hipStreamSynchronize(ws));
-  // y = exp(tau T) c = Z diag(exp(tau w)) Z^T c
-  std::vector<double> w, Z, t(size, 0.0), y(size, 0.0);
+  std::vector<double> w, Z;
   tridiag_qr_full(size, hA, hB, w, Z);
-  for (int k = 0; k < size; ++k) {
-    for (int j = 0; j < size; ++j) t[k] += Z[(size_t)j * size + k] * hc[j];
-    t[k] *= std::exp(tau * w[k]);
+  const bool cout = cplx || tau_im != 0.0;  // complex coefficients
+  std::vector<double> cr(size), ci(size, 0.0), tr(size, 0.0), ti(size, 0.0);
+  for (int j = 0; j < size; ++j) {
+    cr[j] = cplx ? hc[2 * j] : hc[j];
+    if (cplx) ci[j] = hc[2 * j + 1];
   }
-  for (int i = 0; i < size; ++i)
-    for (int k = 0; k < size; ++k) y[i] += Z[(size_t)i * size + k] * t[k];
-  if ((rc = combine_host_coef(L, y.data(), size, out, ws))) return rc;
+  for (int k = 0; k < size; ++k) {  // t = diag(exp(tau w)) Z^T c
+    double sr = 0.0, si = 0.0;
+    for (int j = 0; j < size; ++j) {
+      sr += Z[(size_t)j * size + k] * cr[j];
+      si += Z[(size_t)j * size + k] * ci[j];
+    }
+    double er, ei;
+    expw(w[k], &er, &ei);
+    tr[k] = er * sr - ei * si;
+    ti[k] = er * si + ei * sr;
+  }
+  std::vector<double> y(cout ? 2 * size : size, 0.0);  // y = Z t
+  for (int i = 0; i < size; ++i) {
+    double yr = 0.0, yi = 0.0;
+    for (int k = 0; k < size; ++k) {
+      yr += Z[(size_t)i * size + k] * tr[k];
+      yi += Z[(size_t)i * size + k] * ti[k];
+    }
+    if (cout) {
+      y[2 * i] = yr;
+      y[2 * i + 1] = yi;
+    } else {
+      y[i] = yr;
+    }
+  }
+  if ((rc = combine_host_coef(L, y.data(), size, out, cplx ? 1 : (cout ? 2 : 0), ws))) return rc;
   if (niter) *niter = size;
   return run_end(r);
 }
@@ -1019,16 +1105,21 @@ extern "C" int cxa_lanczos_exp(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
 // NULL) holds the k Ritz vectors as rows; nconv = pairs that met the bound.
 extern "C" int cxa_lanczos_eigsh(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx,
                                  const double *Tin, int which, int k, double tol, int Maxiter,
-                                 double *evals, double *evecs, int *niter, int *nconv,
-                                 cxa_stream_t stream) {
+                                 int flags, double *evals, double *evecs, int *niter,
+                                 int *nconv, cxa_stream_t stream) {
   CXA_CHECK(L && matvec && Tin && evals, CXA_ERR_INVALID, "cxa_lanczos_eigsh: bad arguments");
   CXA_CHECK(which >= 0 && which <= 2, CXA_ERR_INVALID,
             "[ERROR][Lanczos] 'which' should be 'LM', 'LA', 'SA'");
   CXA_CHECK(k >= 1, CXA_ERR_INVALID, "[ERROR][Lanczos] k should be >0");
-  CXA_CHECK(k <= L->n, CXA_ERR_INVALID,
+  const bool cplx = (flags >> 2) & 1;
+  CXA_CHECK(!cplx || L->n % 2 == 0, CXA_ERR_INVALID,
+            "cxa_lanczos_eigsh: complex vectors need an even number of doubles (got %lld)",
+            (long long)L->n);
+  const int64_t ne = cplx ? L->n / 2 : L->n;
+  CXA_CHECK(k <= ne, CXA_ERR_INVALID,
             "[ERROR][Lanczos] k can only be up to total dimension of input vector D");
   CXA_CHECK(Maxiter >= 1, CXA_ERR_INVALID, "cxa_lanczos_eigsh: Maxiter must be positive");
-  const int cap = (int)std::min<int64_t>({(int64_t)Maxiter, L->n, (int64_t)L->maxiter});
+  const int cap = (int)std::min<int64_t>({(int64_t)Maxiter, ne, (int64_t)L->maxiter});
   CXA_CHECK(cap >= k, CXA_ERR_INVALID,
             "cxa_lanczos_eigsh: %d Krylov vectors allowed, fewer than k = %d", cap, k);
   const double eps = 2.220446049250313e-16, eps23 = std::cbrt(eps * eps);
@@ -1037,7 +1128,7 @@ extern "C" int cxa_lanczos_eigsh(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx
   int rc;
   if ((rc = run_begin(L, false, (hipStream_t)stream, r))) return rc;
   const hipStream_t ws = r.ws;
-  if ((rc = run_prologue(L, matvec, ctx, Tin, 1, false, 1, cap, ws))) return rc;
+  if ((rc = run_prologue(L, matvec, ctx, Tin, 1, cplx, false, 1, cap, ws))) return rc;
   const int m1 = L->maxiter + 1;
   const double *hA = L->hbuf, *hB = L->hbuf + m1;
   std::vector<double> w, Z;
@@ -1058,7 +1149,7 @@ extern "C" int cxa_lanczos_eigsh(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx
   };
   for (;;) {
     const int nb = std::min(batch, cap - it);
-    if ((rc = run_batch(L, matvec, ctx, 1, false, it, nb, ws))) return rc;
+    if ((rc = run_batch(L, matvec, ctx, 1, cplx, false, it, nb, ws))) return rc;
     const int upto = it + nb;
     // an invariant subspace ends the basis early: beta_j at round-off level
     m = upto;
@@ -1092,7 +1183,7 @@ extern "C" int cxa_lanczos_eigsh(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx
     std::vector<double> y(m);
     for (int i = 0; i < k; ++i) {
       for (int j = 0; j < m; ++j) y[j] = Z[(size_t)j * m + pick[i]];
-      if ((rc = combine_host_coef(L, y.data(), m, evecs + (size_t)i * L->n, ws))) return rc;
+      if ((rc = combine_host_coef(L, y.data(), m, evecs + (size_t)i * L->n, 0, ws))) return rc;
     }
   }
   if (niter) *niter = m;

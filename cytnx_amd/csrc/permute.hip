@@ -156,10 +156,15 @@ struct FastDiv {
   }
 };
 
+// A tile is w (along A) x h (along B) x c: c > 1 batches consecutive indices of the innermost
+// remaining axis (outer axis no - 1, extent cext, strides c_istr / c_ostr) into one tile, for
+// permutations whose A and B runs are short and whole (a batch of small transposes, e.g. the last
+// two axes swapped): the tile then spans c * w * h elements, contiguous in the input and in the
+// output when that axis comes right before the A and B axes in both.
 struct TileArgs32 {
-  int na, nb, no, w, h;
-  uint32_t prodA, prodB, ntiles;
-  FastDiv dnca, dncb, dw, dh;
+  int na, nb, no, w, h, c;
+  uint32_t prodA, prodB, ntiles, cext, c_istr, c_ostr;
+  FastDiv dnca, dncb, dw, dh, dwh;
   FastDiv a_div[kMaxGroup];  // A axes (input-contiguous), outermost first
   uint32_t a_ostr[kMaxGroup];
   FastDiv b_div[kMaxGroup];  // B axes (output-contiguous), outermost first
@@ -181,7 +186,7 @@ __device__ __forceinline__ uint32_t decode32(uint32_t x, int n, const FastDiv *d
 }
 
 struct TileAt {
-  uint32_t oin, oout, a0, b0, tw, th;
+  uint32_t oin, oout, a0, b0, tw, th, tc;
 };
 
 __device__ __forceinline__ TileAt tile_at(const TileArgs32 &p, uint32_t t) {
@@ -194,12 +199,16 @@ __device__ __forceinline__ TileAt tile_at(const TileArgs32 &p, uint32_t t) {
   rem = q;
   x.oin = 0;
   x.oout = 0;
+  x.tc = 1;
   for (int i = p.no - 1; i >= 0; --i) {
     q = p.o_div[i].div(rem);
     const uint32_t o = rem - q * p.o_div[i].d;
     rem = q;
-    x.oin += o * p.o_istr[i];
-    x.oout += o * p.o_ostr[i];
+    // with batching, the innermost outer axis's digit counts chunks of c
+    const uint32_t o0 = (i == p.no - 1) ? o * (uint32_t)p.c : o;
+    if (i == p.no - 1) x.tc = min((uint32_t)p.c, p.cext - o0);
+    x.oin += o0 * p.o_istr[i];
+    x.oout += o0 * p.o_ostr[i];
   }
   x.a0 = ca * p.w;
   x.b0 = cb * p.h;
@@ -208,9 +217,38 @@ __device__ __forceinline__ TileAt tile_at(const TileArgs32 &p, uint32_t t) {
   return x;
 }
 
+// Element i of a tile in read order (a fastest, then b, then the batch index cc) and in write
+// order (b fastest, then a, then cc).
+struct TileIdx {
+  uint32_t cc, a, b;
+};
+// BATCH false (c = 1): no batch digit to split off.
+template <bool BATCH>
+__device__ __forceinline__ TileIdx read_idx(const TileArgs32 &p, uint32_t i) {
+  TileIdx r;
+  r.cc = BATCH ? p.dwh.div(i) : 0;
+  const uint32_t e = BATCH ? i - r.cc * p.dwh.d : i;
+  r.b = p.dw.div(e);
+  r.a = e - r.b * p.w;
+  return r;
+}
+template <bool BATCH>
+__device__ __forceinline__ TileIdx write_idx(const TileArgs32 &p, uint32_t i) {
+  TileIdx r;
+  r.cc = BATCH ? p.dwh.div(i) : 0;
+  const uint32_t e = BATCH ? i - r.cc * p.dwh.d : i;
+  r.a = p.dh.div(e);
+  r.b = e - r.a * p.h;
+  return r;
+}
+template <bool BATCH>
+__device__ __forceinline__ bool in_tile(const TileIdx &r, const TileAt &x) {
+  return r.a < x.tw && r.b < x.th && (!BATCH || r.cc < x.tc);
+}
+
 // One tile at a time: offsets, loads, LDS, stores (16-byte elements, where the pipelined form's
 // extra registers and offset tables cost more occupancy than the overlap gains).
-template <typename T, int MAXE>
+template <typename T, int MAXE, bool BATCH>
 __global__ __launch_bounds__(kThreads) void permute_tiled32_once(const T *__restrict__ src,
                                                                  T *__restrict__ dst,
                                                                  TileArgs32 p) {
@@ -219,7 +257,7 @@ __global__ __launch_bounds__(kThreads) void permute_tiled32_once(const T *__rest
   __shared__ uint32_t aoff[kMaxTile];
   __shared__ uint32_t boff[kMaxTile];
   const int w = p.w, h = p.h, ws = w + 1;
-  const uint32_t wh = (uint32_t)(w * h);
+  const uint32_t n = (uint32_t)(w * h * p.c);
   for (uint32_t t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
     const TileAt x = tile_at(p, t);
     __syncthreads();  // the previous tile is fully written out
@@ -235,28 +273,28 @@ __global__ __launch_bounds__(kThreads) void permute_tiled32_once(const T *__rest
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t i = threadIdx.x + k * kThreads;
-      const uint32_t b = p.dw.div(i), a = i - b * w;
-      if (i < wh && a < x.tw && b < x.th) r[k] = s[boff[b] + a];
+      const TileIdx e = read_idx<BATCH>(p, i);
+      if (i < n && in_tile<BATCH>(e, x)) r[k] = s[e.cc * p.c_istr + boff[e.b] + e.a];
     }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t i = threadIdx.x + k * kThreads;
-      const uint32_t b = p.dw.div(i), a = i - b * w;
-      if (i < wh && a < x.tw && b < x.th) tile[b * ws + a] = r[k];
+      const TileIdx e = read_idx<BATCH>(p, i);
+      if (i < n && in_tile<BATCH>(e, x)) tile[(e.cc * h + e.b) * ws + e.a] = r[k];
     }
     __syncthreads();
     T *d = dst + x.oout + x.b0;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t i = threadIdx.x + k * kThreads;
-      const uint32_t a = p.dh.div(i), b = i - a * h;
-      if (i < wh && a < x.tw && b < x.th) r[k] = tile[b * ws + a];
+      const TileIdx e = write_idx<BATCH>(p, i);
+      if (i < n && in_tile<BATCH>(e, x)) r[k] = tile[(e.cc * h + e.b) * ws + e.a];
     }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t i = threadIdx.x + k * kThreads;
-      const uint32_t a = p.dh.div(i), b = i - a * h;
-      if (i < wh && a < x.tw && b < x.th) d[aoff[a] + b] = r[k];
+      const TileIdx e = write_idx<BATCH>(p, i);
+      if (i < n && in_tile<BATCH>(e, x)) d[e.cc * p.c_ostr + aoff[e.a] + e.b] = r[k];
     }
   }
 }
@@ -265,7 +303,7 @@ __global__ __launch_bounds__(kThreads) void permute_tiled32_once(const T *__rest
 // are issued right after tile t is staged in LDS, so they are in flight while tile t is written
 // out -- reads and writes overlap within each workgroup instead of alternating in lockstep
 // across the whole launch.  The per-tile offset tables are double-buffered for that.
-template <typename T, int MAXE>
+template <typename T, int MAXE, bool BATCH>
 __global__ __launch_bounds__(kThreads) void permute_tiled32(const T *__restrict__ src,
                                                             T *__restrict__ dst, TileArgs32 p) {
   constexpr int PER = MAXE / kThreads;
@@ -273,7 +311,7 @@ __global__ __launch_bounds__(kThreads) void permute_tiled32(const T *__restrict_
   __shared__ uint32_t aoff[2][kMaxTile];
   __shared__ uint32_t boff[2][kMaxTile];
   const int w = p.w, h = p.h, ws = w + 1;
-  const uint32_t wh = (uint32_t)(w * h);
+  const uint32_t n = (uint32_t)(w * h * p.c);
   uint32_t t = blockIdx.x;
   if (t >= p.ntiles) return;  // uniform over the workgroup
   auto decode_offsets = [&](int buf, const TileAt &x) {
@@ -290,8 +328,8 @@ __global__ __launch_bounds__(kThreads) void permute_tiled32(const T *__restrict_
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t i = threadIdx.x + k * kThreads;
-      const uint32_t b = p.dw.div(i), a = i - b * w;
-      if (i < wh && a < x.tw && b < x.th) r[k] = s[boff[buf][b] + a];
+      const TileIdx e = read_idx<BATCH>(p, i);
+      if (i < n && in_tile<BATCH>(e, x)) r[k] = s[e.cc * p.c_istr + boff[buf][e.b] + e.a];
     }
   };
   TileAt cur = tile_at(p, t);
@@ -311,8 +349,8 @@ __global__ __launch_bounds__(kThreads) void permute_tiled32(const T *__restrict_
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t i = threadIdx.x + k * kThreads;
-      const uint32_t b = p.dw.div(i), a = i - b * w;
-      if (i < wh && a < cur.tw && b < cur.th) tile[b * ws + a] = r[k];
+      const TileIdx e = read_idx<BATCH>(p, i);
+      if (i < n && in_tile<BATCH>(e, cur)) tile[(e.cc * h + e.b) * ws + e.a] = r[k];
     }
     __syncthreads();  // tile staged; next offsets decoded
     if (more) load(buf ^ 1, nxt, r);  // next tile's loads in flight during the writes below
@@ -322,14 +360,14 @@ __global__ __launch_bounds__(kThreads) void permute_tiled32(const T *__restrict_
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t i = threadIdx.x + k * kThreads;
-      const uint32_t a = p.dh.div(i), b = i - a * h;
-      if (i < wh && a < cur.tw && b < cur.th) o[k] = tile[b * ws + a];
+      const TileIdx e = write_idx<BATCH>(p, i);
+      if (i < n && in_tile<BATCH>(e, cur)) o[k] = tile[(e.cc * h + e.b) * ws + e.a];
     }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const uint32_t i = threadIdx.x + k * kThreads;
-      const uint32_t a = p.dh.div(i), b = i - a * h;
-      if (i < wh && a < cur.tw && b < cur.th) d[aoff[buf][a] + b] = o[k];
+      const TileIdx e = write_idx<BATCH>(p, i);
+      if (i < n && in_tile<BATCH>(e, cur)) d[e.cc * p.c_ostr + aoff[buf][e.a] + e.b] = o[k];
     }
     __syncthreads();  // tile and this buffer's offsets consumed
     if (!more) break;
@@ -547,13 +585,30 @@ int launch_tiled(const Reduced &r, const void *src, void *dst, hipStream_t s) {
     q.no = p.no;
     q.w = p.w;
     q.h = p.h;
+    q.c = 1;
     q.prodA = (uint32_t)p.prodA;
     q.prodB = (uint32_t)p.prodB;
-    q.ntiles = (uint32_t)p.ntiles;
+    // Whole A and B runs in a tile still smaller than `small` (a batch of small transposes):
+    // batch c consecutive indices of the innermost outer axis into the tile when that axis is
+    // contiguous with the tile on one side (it directly precedes A and B in the input or in the
+    // output), so that a tile moves about kMinTile elements.
+    int64_t cext = no > 0 ? p.o_ext[no - 1] : 1;
+    if (no > 0 && p.w == prodA && p.h == prodB && (int64_t)p.w * p.h < small &&
+        (p.o_istr[no - 1] == prodA * prodB || p.o_ostr[no - 1] == prodA * prodB)) {
+      const int64_t cmax = std::min<int64_t>((int64_t)MAXE / (p.w * p.h),
+                                             (int64_t)(MAXE + 64) / (p.h * (p.w + 1)));
+      q.c = (int)std::max<int64_t>(1, std::min<int64_t>(cext, cmax));
+    }
+    q.cext = (uint32_t)cext;
+    q.c_istr = q.c > 1 ? (uint32_t)p.o_istr[no - 1] : 0;
+    q.c_ostr = q.c > 1 ? (uint32_t)p.o_ostr[no - 1] : 0;
+    const int64_t nchunks = (cext + q.c - 1) / q.c;
+    q.ntiles = (uint32_t)(p.ntiles / cext * nchunks);
     q.dnca = FastDiv::make((uint32_t)p.nca);
     q.dncb = FastDiv::make((uint32_t)p.ncb);
     q.dw = FastDiv::make((uint32_t)p.w);
     q.dh = FastDiv::make((uint32_t)p.h);
+    q.dwh = FastDiv::make((uint32_t)(p.w * p.h));
     for (int k = 0; k < p.na; ++k) {
       q.a_div[k] = FastDiv::make((uint32_t)p.a_ext[k]);
       q.a_ostr[k] = (uint32_t)p.a_ostr[k];
@@ -563,16 +618,29 @@ int launch_tiled(const Reduced &r, const void *src, void *dst, hipStream_t s) {
       q.b_istr[k] = (uint32_t)p.b_istr[k];
     }
     for (int k = 0; k < p.no; ++k) {
-      q.o_div[k] = FastDiv::make((uint32_t)p.o_ext[k]);
+      q.o_div[k] = FastDiv::make((uint32_t)(k == p.no - 1 ? nchunks : p.o_ext[k]));
       q.o_istr[k] = (uint32_t)p.o_istr[k];
       q.o_ostr[k] = (uint32_t)p.o_ostr[k];
     }
-    if (pipelined)
-      hipLaunchKernelGGL((permute_tiled32<T, MAXE>), dim3(grid), dim3(kThreads), 0, s,
+    const int grid32 =
+        (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)q.ntiles,
+                                                    pipelined ? std::min<int64_t>(
+                                                                    256 * 64,
+                                                                    std::max<int64_t>(
+                                                                        1024, q.ntiles / 2))
+                                                              : 4096));
+    if (pipelined && q.c > 1)
+      hipLaunchKernelGGL((permute_tiled32<T, MAXE, true>), dim3(grid32), dim3(kThreads), 0, s,
                          (const T *)src, (T *)dst, q);
+    else if (pipelined)
+      hipLaunchKernelGGL((permute_tiled32<T, MAXE, false>), dim3(grid32), dim3(kThreads), 0, s,
+                         (const T *)src, (T *)dst, q);
+    else if (q.c > 1)
+      hipLaunchKernelGGL((permute_tiled32_once<T, MAXE, true>), dim3(grid32), dim3(kThreads), 0,
+                         s, (const T *)src, (T *)dst, q);
     else
-      hipLaunchKernelGGL((permute_tiled32_once<T, MAXE>), dim3(grid), dim3(kThreads), 0, s,
-                         (const T *)src, (T *)dst, q);
+      hipLaunchKernelGGL((permute_tiled32_once<T, MAXE, false>), dim3(grid32), dim3(kThreads), 0,
+                         s, (const T *)src, (T *)dst, q);
     return 0;
   }
   hipLaunchKernelGGL((permute_tiled<T, MAXE>), dim3(grid), dim3(kThreads), 0, s,

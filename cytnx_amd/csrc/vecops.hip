@@ -127,52 +127,103 @@ __global__ __launch_bounds__(kT) void k_residual(const double *__restrict__ y,
 // iteration known at launch there is one row of blocks per group; a captured launch (iteration
 // read on the device) has a fixed number of rows that walk the groups.  Each partial sums the
 // same elements in the same order whatever the grid.
+// CPLX: the n doubles are n / 2 complex values (re, im) and h_j = conj(V_j) . w is complex: the
+// real parts are the real dot of the doubles, the imaginary parts sum vr wi - vi wr and go to
+// part_im[j * P + block].
+template <bool CPLX>
 __global__ __launch_bounds__(kT) void k_proj_partials(const double *__restrict__ V,
                                                       const double *__restrict__ w, int64_t n,
-                                                      IterIdx iter, double *__restrict__ part) {
+                                                      IterIdx iter, double *__restrict__ part,
+                                                      double *__restrict__ part_im) {
   const int it = iter.get();
   const int64_t P = gridDim.x;
+  const int64_t ne = CPLX ? n / 2 : n;
   for (int j0 = 4 * blockIdx.y; j0 <= it; j0 += 4 * gridDim.y) {
     const int nj = min(4, it + 1 - j0);
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += P * kT) {
-      const double wi = w[i];
+    double s[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < ne; i += P * kT) {
+      if constexpr (CPLX) {
+        const double wr = w[2 * i], wi = w[2 * i + 1];
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (q < nj) s[q] += V[(int64_t)(j0 + q) * n + i] * wi;
+        for (int q = 0; q < 4; ++q) {
+          if (q < nj) {
+            const double *v = V + (int64_t)(j0 + q) * n + 2 * i;
+            s[q] += v[0] * wr + v[1] * wi;
+            si[q] += v[0] * wi - v[1] * wr;
+          }
+        }
+      } else {
+        const double wi = w[i];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (q < nj) s[q] += V[(int64_t)(j0 + q) * n + i] * wi;
+      }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if (q < nj) {  // uniform over the block
         const double t = block_sum(s[q]);
         if (threadIdx.x == 0) part[(int64_t)(j0 + q) * P + blockIdx.x] = t;
+        if constexpr (CPLX) {
+          const double ti = block_sum(si[q]);
+          if (threadIdx.x == 0) part_im[(int64_t)(j0 + q) * P + blockIdx.x] = ti;
+        }
       }
     }
   }
 }
 
-__global__ __launch_bounds__(kT) void k_proj_finalize(const double *__restrict__ part, int P,
+// h[j] (CPLX: h[2j], h[2j + 1] = re, im) from the partials, j <= it
+template <bool CPLX>
+__global__ __launch_bounds__(kT) void k_proj_finalize(const double *__restrict__ part,
+                                                      const double *__restrict__ part_im, int P,
                                                       IterIdx iter, double *__restrict__ h) {
   const int j = blockIdx.x;
   if (j > iter.get()) return;
   double s = 0.0;
   for (int i = threadIdx.x; i < P; i += kT) s += part[(int64_t)j * P + i];
   s = block_sum(s);
-  if (threadIdx.x == 0) h[j] = s;
+  if constexpr (CPLX) {
+    double si = 0.0;
+    for (int i = threadIdx.x; i < P; i += kT) si += part_im[(int64_t)j * P + i];
+    si = block_sum(si);
+    if (threadIdx.x == 0) {
+      h[2 * j] = s;
+      h[2 * j + 1] = si;
+    }
+  } else {
+    if (threadIdx.x == 0) h[j] = s;
+  }
 }
 
-// w -= sum_{j <= it} h_j V_j, then partial sums of w^2
+// w -= sum_{j <= it} h_j V_j (CPLX: complex h_j times complex V_j), then partial sums of |w|^2
+template <bool CPLX>
 __global__ __launch_bounds__(kT) void k_proj_apply(double *__restrict__ w,
                                                    const double *__restrict__ V, int64_t n,
                                                    const double *__restrict__ h, IterIdx iter,
                                                    double *__restrict__ part) {
   const int it = iter.get();
+  const int64_t ne = CPLX ? n / 2 : n;
   double s = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
-    double v = w[i];
-    for (int j = 0; j <= it; ++j) v -= h[j] * V[(int64_t)j * n + i];
-    w[i] = v;
-    s += v * v;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < ne;
+       i += (int64_t)gridDim.x * kT) {
+    if constexpr (CPLX) {
+      double vr = w[2 * i], vi = w[2 * i + 1];
+      for (int j = 0; j <= it; ++j) {
+        const double *b = V + (int64_t)j * n + 2 * i;
+        const double hr = h[2 * j], hi = h[2 * j + 1];
+        vr -= hr * b[0] - hi * b[1];
+        vi -= hr * b[1] + hi * b[0];
+      }
+      w[2 * i] = vr;
+      w[2 * i + 1] = vi;
+      s += vr * vr + vi * vi;
+    } else {
+      double v = w[i];
+      for (int j = 0; j <= it; ++j) v -= h[j] * V[(int64_t)j * n + i];
+      w[i] = v;
+      s += v * v;
+    }
   }
   s = block_sum(s);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -194,15 +245,42 @@ __global__ __launch_bounds__(kT) void k_normalize_init(double *__restrict__ cur,
   }
 }
 
-// out = sum_j coef[j] V_j, accumulated in the order j = 0, 1, ...
+// out = sum_j coef[j] V_j, accumulated in the order j = 0, 1, ...  MODE 0: real coefficients
+// (any vector of doubles); MODE 1: complex coefficients (re, im pairs) and complex vectors of
+// n / 2 values; MODE 2: complex coefficients and real vectors of n values, out complex (2n
+// doubles: real parts from the coefficients' real parts, imaginary from their imaginary parts).
+template <int MODE>
 __global__ __launch_bounds__(kT) void k_combine(double *__restrict__ out,
                                                 const double *__restrict__ V,
                                                 const double *__restrict__ coef, int k,
                                                 int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
-    double v = coef[0] * V[i];
-    for (int j = 1; j < k; ++j) v += coef[j] * V[(int64_t)j * n + i];
-    out[i] = v;
+  const int64_t ne = MODE == 1 ? n / 2 : n;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < ne;
+       i += (int64_t)gridDim.x * kT) {
+    if constexpr (MODE == 0) {
+      double v = coef[0] * V[i];
+      for (int j = 1; j < k; ++j) v += coef[j] * V[(int64_t)j * n + i];
+      out[i] = v;
+    } else if constexpr (MODE == 1) {
+      double vr = 0.0, vi = 0.0;
+      for (int j = 0; j < k; ++j) {
+        const double *b = V + (int64_t)j * n + 2 * i;
+        const double cr = coef[2 * j], ci = coef[2 * j + 1];
+        vr += cr * b[0] - ci * b[1];
+        vi += cr * b[1] + ci * b[0];
+      }
+      out[2 * i] = vr;
+      out[2 * i + 1] = vi;
+    } else {
+      double vr = 0.0, vi = 0.0;
+      for (int j = 0; j < k; ++j) {
+        const double b = V[(int64_t)j * n + i];
+        vr += coef[2 * j] * b;
+        vi += coef[2 * j + 1] * b;
+      }
+      out[2 * i] = vr;
+      out[2 * i + 1] = vi;
+    }
   }
 }
 
@@ -341,24 +419,45 @@ int vec_residual(const double *y, double *u, const double *V, int64_t n, const d
 }
 
 int vec_reorth(double *w, const double *V, int64_t n, int max_j, IterIdx iter, double *part,
-               double *h, double *part_out, hipStream_t s) {
+               double *h, double *part_out, bool cplx, hipStream_t s) {
   // with the iteration known on the host only the live projections are launched
   const int nj = iter.p ? max_j : iter.v + 1;
   const int groups = iter.p ? std::min(8, (max_j + 3) / 4) : (nj + 3) / 4;
-  hipLaunchKernelGGL(k_proj_partials, dim3(kPartials, groups), dim3(kT), 0, s, V, w, n, iter,
-                     part);
-  hipLaunchKernelGGL(k_proj_finalize, dim3(nj), dim3(kT), 0, s, part, kPartials, iter, h);
-  hipLaunchKernelGGL(k_proj_apply, dim3(kPartials), dim3(kT), 0, s, w, V, n, h, iter, part_out);
+  double *part_im = part + (int64_t)kPartials * max_j;
+  if (cplx) {
+    hipLaunchKernelGGL(k_proj_partials<true>, dim3(kPartials, groups), dim3(kT), 0, s, V, w, n,
+                       iter, part, part_im);
+    hipLaunchKernelGGL(k_proj_finalize<true>, dim3(nj), dim3(kT), 0, s, part, part_im,
+                       kPartials, iter, h);
+    hipLaunchKernelGGL(k_proj_apply<true>, dim3(kPartials), dim3(kT), 0, s, w, V, n, h, iter,
+                       part_out);
+  } else {
+    hipLaunchKernelGGL(k_proj_partials<false>, dim3(kPartials, groups), dim3(kT), 0, s, V, w, n,
+                       iter, part, part_im);
+    hipLaunchKernelGGL(k_proj_finalize<false>, dim3(nj), dim3(kT), 0, s, part, part_im,
+                       kPartials, iter, h);
+    hipLaunchKernelGGL(k_proj_apply<false>, dim3(kPartials), dim3(kT), 0, s, w, V, n, h, iter,
+                       part_out);
+  }
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }
 
 int vec_project(const double *V, const double *x, int64_t n, int k, double *part, double *h,
-                hipStream_t s) {
+                bool cplx, hipStream_t s) {
   const IterIdx iter{nullptr, k - 1};
-  hipLaunchKernelGGL(k_proj_partials, dim3(kPartials, (k + 3) / 4), dim3(kT), 0, s, V, x, n, iter,
-                     part);
-  hipLaunchKernelGGL(k_proj_finalize, dim3(k), dim3(kT), 0, s, part, kPartials, iter, h);
+  double *part_im = part + (int64_t)kPartials * k;
+  if (cplx) {
+    hipLaunchKernelGGL(k_proj_partials<true>, dim3(kPartials, (k + 3) / 4), dim3(kT), 0, s, V, x,
+                       n, iter, part, part_im);
+    hipLaunchKernelGGL(k_proj_finalize<true>, dim3(k), dim3(kT), 0, s, part, part_im, kPartials,
+                       iter, h);
+  } else {
+    hipLaunchKernelGGL(k_proj_partials<false>, dim3(kPartials, (k + 3) / 4), dim3(kT), 0, s, V,
+                       x, n, iter, part, part_im);
+    hipLaunchKernelGGL(k_proj_finalize<false>, dim3(k), dim3(kT), 0, s, part, part_im, kPartials,
+                       iter, h);
+  }
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }
@@ -390,9 +489,15 @@ int vec_normalize_init(double *cur, double *V, const double *x, int64_t n, const
   return CXA_OK;
 }
 
-int vec_combine(double *out, const double *V, const double *coef, int k, int64_t n,
+int vec_combine(double *out, const double *V, const double *coef, int k, int64_t n, int mode,
                 hipStream_t s) {
-  hipLaunchKernelGGL(k_combine, dim3(grid_of(n)), dim3(kT), 0, s, out, V, coef, k, n);
+  const int grid = grid_of(mode == 1 ? n / 2 : n);
+  if (mode == 1)
+    hipLaunchKernelGGL(k_combine<1>, dim3(grid), dim3(kT), 0, s, out, V, coef, k, n);
+  else if (mode == 2)
+    hipLaunchKernelGGL(k_combine<2>, dim3(grid), dim3(kT), 0, s, out, V, coef, k, n);
+  else
+    hipLaunchKernelGGL(k_combine<0>, dim3(grid), dim3(kT), 0, s, out, V, coef, k, n);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }

@@ -32,19 +32,25 @@ int vec_dot_basis(const double *y, const double *u, double *V, int64_t n, const 
 int vec_residual(const double *y, double *u, const double *V, int64_t n, const double *part_a,
                  double *As, const double *Bs, IterIdx iter, double *part_b, hipStream_t s);
 // w -= sum_{j <= it} (V_j . w) V_j (classical Gram-Schmidt against the stored basis);
-// part_out = partials of the new |w|^2; part: kPartials * max_j scratch
+// part_out = partials of the new |w|^2; part: 2 * kPartials * max_j scratch.  cplx: the n
+// doubles are n / 2 complex values, the coefficients conj(V_j) . w are complex (h: 2 doubles
+// each).
 int vec_reorth(double *w, const double *V, int64_t n, int max_j, IterIdx iter, double *part,
-               double *h, double *part_out, hipStream_t s);
-// h[j] = V_j . x for j < k (no update of x); part: kPartials * k scratch
+               double *h, double *part_out, bool cplx, hipStream_t s);
+// h[j] = V_j . x (cplx: conj(V_j) . x, re / im pairs) for j < k (no update of x); part:
+// 2 * kPartials * k scratch
 int vec_project(const double *V, const double *x, int64_t n, int k, double *part, double *h,
-                hipStream_t s);
+                bool cplx, hipStream_t s);
 int vec_finalize(const double *part, double *out, const int *counter, int offset, int mode,
                  hipStream_t s);
 int vec_counter_add(int *counter, int d, hipStream_t s);
 int vec_counter_set(int *counter, int v, hipStream_t s);
 int vec_normalize_init(double *cur, double *V, const double *x, int64_t n, const double *nrm,
                        hipStream_t s);
-int vec_combine(double *out, const double *V, const double *coef, int k, int64_t n,
+// out = sum_{j < k} coef_j V_j; mode 0: real coefficients; 1: complex coefficients (re, im
+// pairs) and complex vectors (n doubles = n / 2 values); 2: complex coefficients and real
+// vectors of n doubles, out complex (2n doubles)
+int vec_combine(double *out, const double *V, const double *coef, int k, int64_t n, int mode,
                 hipStream_t s);
 int vec_cast_f64_f32(const double *x, float *y, int64_t n, hipStream_t s);
 

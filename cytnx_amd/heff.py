@@ -430,8 +430,14 @@ class Heff2(_HeffBase):
         for T in (psi, L, M1, M2, R):
             if not T.is_blockform():
                 raise TypeError("Heff2 takes U(1) block tensors")
-            if T.dtype() != 3:
-                raise TypeError("Heff2 runs real FP64 tensors")
+            if T.dtype() not in (1, 3):
+                raise TypeError("Heff2 runs FP64 or complex128 tensors")
+        # complex128 operands (a complex state or environments): every tensor complex, the
+        # three-launch chain in FP64 (the mix launch and the FP32 modes are real-only)
+        dt = 1 if any(T.dtype() == 1 for T in (psi, L, M1, M2, R)) else 3
+        if dt == 1:
+            psi, L, M1, M2, R = [T.astype(1) for T in (psi, L, M1, M2, R)]
+            mix = False
         dev = L.device()
         if dev < 0:
             raise ValueError("move the environments to the GPU first")
@@ -442,14 +448,14 @@ class Heff2(_HeffBase):
         Rp = R.permute([lab_r, lab_w2, lab_rp])  # (r, w'', r')
         psi_m = psi.clone_meta()
         ref_steps = []
-        X, c1, c2 = contract_meta(Lp, psi_m, 3, dev)
-        ref_steps.append((plan_contraction(Lp, psi_m, X, c1, c2, 3), Lp, True, X))
-        Y, c1, c2 = contract_meta(X, M1, 3, dev)
-        ref_steps.append((plan_contraction(X, M1, Y, c1, c2, 3), M1, False, Y))
-        Z, c1, c2 = contract_meta(Y, M2, 3, dev)
-        ref_steps.append((plan_contraction(Y, M2, Z, c1, c2, 3), M2, False, Z))
-        O, c1, c2 = contract_meta(Z, Rp, 3, dev)
-        ref_steps.append((plan_contraction(Z, Rp, O, c1, c2, 3), Rp, False, O))
+        X, c1, c2 = contract_meta(Lp, psi_m, dt, dev)
+        ref_steps.append((plan_contraction(Lp, psi_m, X, c1, c2, dt), Lp, True, X))
+        Y, c1, c2 = contract_meta(X, M1, dt, dev)
+        ref_steps.append((plan_contraction(X, M1, Y, c1, c2, dt), M1, False, Y))
+        Z, c1, c2 = contract_meta(Y, M2, dt, dev)
+        ref_steps.append((plan_contraction(Y, M2, Z, c1, c2, dt), M2, False, Z))
+        O, c1, c2 = contract_meta(Z, Rp, dt, dev)
+        ref_steps.append((plan_contraction(Z, Rp, O, c1, c2, dt), Rp, False, O))
         if O.itoi() != psi.itoi() or [b.dim() for b in O.bonds()] != psi.shape():
             raise ValueError("Heff2: output block layout differs from psi's")
         self.ref_flops = float(sum(st[0].flops for st in ref_steps))
@@ -459,10 +465,10 @@ class Heff2(_HeffBase):
         self.mode = "ref"
         if fuse_mpo:
             W12 = Contract(M1, M2)
-            Zf, c1, c2 = contract_meta(X, W12, 3, dev)
+            Zf, c1, c2 = contract_meta(X, W12, dt, dev)
             if Zf.labels() != Z.labels() or Zf.itoi() != Z.itoi():
                 raise ValueError("Heff2: fused MPO step changed the intermediate layout")
-            steps = [ref_steps[0], (plan_contraction(X, W12, Zf, c1, c2, 3), W12, False, Zf),
+            steps = [ref_steps[0], (plan_contraction(X, W12, Zf, c1, c2, dt), W12, False, Zf),
                      ref_steps[3]]
             self.step_names = ["L.psi", ".W12", ".R"]
             self.mode = "w12"
@@ -478,6 +484,7 @@ class Heff2(_HeffBase):
         self.copies = [st[0].ws_bytes for st in steps]
         self.chain = _Chain(steps, dev)
         super().__init__(meta_elems(psi), dev)
+        self._dtype = dt
 
     @staticmethod
     def _mix_steps(Lp, psi_m, M1, M2, W12, Zf, Rp, psi, dev):

@@ -103,20 +103,21 @@ def random_block(bonds: Sequence[Bond], labels: Sequence[str], rowrank: int, rng
     return T
 
 
-def two_site_problem(chi: int, seed: int = 1, hermitian: bool = True):
+def two_site_problem(chi: int, seed: int = 1, hermitian: bool = True, dtype=np.float64):
     """(psi, L, M1, M2, R) for the two-site U(1) Heff at bond dimension chi, on the host.
 
-    Blocks are iid N(0,1).  With hermitian=True the environments are symmetrised so that the
-    effective Hamiltonian is symmetric: L[w, l, l'] <- L[w, l, l'] + L[wbar, l', l] where wbar
-    maps the MPO state 1 <-> 2 (S+ <-> S-), the same for R; then Lanczos applies."""
+    Blocks are iid N(0,1) (complex128: real and imaginary parts iid N(0,1); the MPO stays
+    real).  With hermitian=True the environments are symmetrised so that the effective
+    Hamiltonian is Hermitian: L[w, l, l'] <- L[w, l, l'] + conj(L[wbar, l', l]) where wbar maps
+    the MPO state 1 <-> 2 (S+ <-> S-), the same for R; then Lanczos applies."""
     rng = np.random.default_rng(seed)
     vl, q, d = two_site_bonds(chi)
     bp = phys_bond()
     bw = mpo_bond()
     psi = random_block([vl, bp.redirect(), bp.redirect(), vl.redirect()],
-                       ["-1", "-2", "-3", "-4"], 1, rng)
-    L = random_block([bw.redirect(), vl.redirect(), vl], ["-5", "-1", "0"], 1, rng)
-    R = random_block([bw, vl, vl.redirect()], ["-7", "-4", "3"], 2, rng)
+                       ["-1", "-2", "-3", "-4"], 1, rng, dtype)
+    L = random_block([bw.redirect(), vl.redirect(), vl], ["-5", "-1", "0"], 1, rng, dtype)
+    R = random_block([bw, vl, vl.redirect()], ["-7", "-4", "3"], 2, rng, dtype)
     if hermitian:
         L = _hermitise_env(L)
         R = _hermitise_env(R)
@@ -128,13 +129,13 @@ def two_site_problem(chi: int, seed: int = 1, hermitian: bool = True):
 
 
 def _hermitise_env(E: UniTensor) -> UniTensor:
-    """E[w, a, b] + E[wbar, b, a] on the dense form, written back block by block."""
+    """E[w, a, b] + conj(E[wbar, b, a]) on the dense form, written back block by block."""
     D = E.to_dense()
     order = [s for _, states in MPO_GROUPS for s in states]   # grouped position -> state
     pos = {s: i for i, s in enumerate(order)}
     swap = {0: 0, 1: 2, 2: 1, 3: 3, 4: 4}
     perm = [pos[swap[order[i]]] for i in range(5)]           # grouped index of wbar
-    D2 = D + np.transpose(D[perm], (0, 2, 1))
+    D2 = D + np.conj(np.transpose(D[perm], (0, 2, 1)))
     out = E.clone_meta()
     import torch
     out._storage = torch.zeros_like(E._storage)

@@ -51,10 +51,11 @@ class LinOp:
 
 
 class _DevView:
-    """Exposes a raw HIP device pointer to torch (array interface, float64 vector)."""
+    """Exposes a raw HIP device pointer to torch (array interface, float64 or complex128
+    vector)."""
 
-    def __init__(self, ptr: int, n: int):
-        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<f8", "data": (ptr, False),
+    def __init__(self, ptr: int, n: int, typestr: str = "<f8"):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, False),
                                          "version": 2, "strides": None}
 
 
@@ -104,18 +105,31 @@ class KrylovCapError(RuntimeError):
     """A Lanczos_Exp run needs more Krylov vectors than the memory budget allows."""
 
 
+def _doubles(x: torch.Tensor, who: str) -> Tuple[torch.Tensor, bool]:
+    """(the flat vector as FP64 doubles, complex?) for the C driver: a complex128 vector is
+    handed over as its (re, im) pairs with the driver's complex flag."""
+    if x.device.type != "cuda" or x.dtype not in (torch.float64, torch.complex128):
+        raise TypeError("%s runs on FP64 or complex128 device vectors" % who)
+    if x.dtype == torch.complex128:
+        return torch.view_as_real(x.contiguous()).reshape(-1), True
+    return x.contiguous().reshape(-1), False
+
+
+CPLX_FLAG = 4  # the driver's complex-Hermitian flag (vectors of (re, im) pairs)
+
+
 def lanczos_gnd_native(op, psi0: torch.Tensor, CvgCrit: float = 1e-14, Maxiter: int = 10000,
                        batch: int = 8, reorth: bool = False, graph: bool = False,
                        fn_ctx=None) -> Tuple[float, torch.Tensor, int, np.ndarray, np.ndarray]:
-    """Run the C driver on a flat FP64 device vector.  Returns (E, evec, niter, alphas, betas).
-    ``op`` is a planned Heff (native chain) unless ``fn_ctx`` gives (fnptr, ctx)."""
-    if psi0.dtype != torch.float64 or psi0.device.type != "cuda":
-        raise TypeError("Lanczos runs on FP64 device vectors")
+    """Run the C driver on a flat FP64 or complex128 (Hermitian operator) device vector.
+    Returns (E, evec, niter, alphas, betas); evec has psi0's dtype.  ``op`` is a planned Heff
+    (native chain) unless ``fn_ctx`` gives (fnptr, ctx)."""
+    x, cplx = _doubles(psi0, "Lanczos")
     if CvgCrit <= 0:
         raise ValueError("[ERROR][Lanczos] converge criteria must >0")
     if Maxiter < 2:
         raise ValueError("[ERROR][Lanczos] Maxiter must >1")
-    n = psi0.numel()
+    n = x.numel()
     dev = psi0.device.index
     cap = _krylov_cap(n, Maxiter)
     ctx = _context(n, cap, dev)
@@ -124,10 +138,10 @@ def lanczos_gnd_native(op, psi0: torch.Tensor, CvgCrit: float = 1e-14, Maxiter: 
     niter = ctypes.c_int(0)
     alphas = np.zeros(cap + 1)
     betas = np.zeros(cap + 1)
-    evec = torch.empty_like(psi0)
-    flags = (1 if reorth else 0) | (2 if graph else 0)
+    evec = torch.empty_like(x)
+    flags = (1 if reorth else 0) | (2 if graph else 0) | (CPLX_FLAG if cplx else 0)
     _lib.check(_lib.lib().cxa_lanczos_run(
-        ctx, fn, mctx, psi0.data_ptr(), float(CvgCrit), int(min(Maxiter, cap)), int(batch), flags,
+        ctx, fn, mctx, x.data_ptr(), float(CvgCrit), int(min(Maxiter, cap)), int(batch), flags,
         ctypes.byref(E), evec.data_ptr(), ctypes.byref(niter),
         alphas.ctypes.data_as(_lib.dblp), betas.ctypes.data_as(_lib.dblp), stream_handle(dev)))
     k = niter.value
@@ -141,6 +155,8 @@ def lanczos_gnd_native(op, psi0: torch.Tensor, CvgCrit: float = 1e-14, Maxiter: 
                           "memory budget allows (%d vectors of %d elements), with |dE| = %.3e "
                           "above CvgCrit %.3e" % (k, cap, n, abs(E.value - prev), CvgCrit),
                           KrylovCapWarning, stacklevel=2)
+    if cplx:
+        evec = torch.view_as_complex(evec.view(-1, 2))
     return E.value, evec, k, alphas[:k].copy(), betas[:k].copy()
 
 
@@ -158,8 +174,8 @@ def _flat_vector(T) -> Tuple[torch.Tensor, object]:
         flat = t.storage().reshape(-1)
     else:
         raise TypeError("Tin must be a UniTensor or a Tensor")
-    if flat.dtype != torch.float64 or flat.device.type != "cuda":
-        raise TypeError("the GPU Lanczos runs real FP64 operators on device tensors")
+    if flat.dtype not in (torch.float64, torch.complex128) or flat.device.type != "cuda":
+        raise TypeError("the GPU Lanczos runs FP64 or complex128 operators on device tensors")
     return flat, t
 
 
@@ -186,12 +202,14 @@ def _flat_result(out) -> torch.Tensor:
 
 
 def _python_matvec(Hop, template):
-    n = int(template.storage().numel())  # the template is contiguous
+    st = template.storage()
+    n = int(st.numel())  # the template is contiguous
+    ts = "<c16" if st.dtype == torch.complex128 else "<f8"
 
     def cb(_ctx, xp, yp, _stream):
         try:
-            x = torch.as_tensor(_DevView(xp, n), device="cuda")
-            y = torch.as_tensor(_DevView(yp, n), device="cuda")
+            x = torch.as_tensor(_DevView(xp, n, ts), device="cuda")
+            y = torch.as_tensor(_DevView(yp, n, ts), device="cuda")
             out = Hop.matvec(_wrap_like(template, x))
             y.copy_(_flat_result(out))
             return 0
@@ -224,14 +242,16 @@ _WHICH = {"SA": 0, "LA": 1, "LM": 2}
 
 def eigsh_native(op, psi0: torch.Tensor, which: str = "SA", k: int = 1, tol: float = 0.0,
                  want_vectors: bool = True, fn_ctx=None, Maxiter: int = 10000):
-    """k extreme eigenpairs of a flat FP64 device operator (cxa_lanczos_eigsh).  Returns
-    (eigenvalues (numpy, reference order), eigenvectors (k x n device tensor or None), Krylov
-    vectors used, pairs within the tolerance).  ``op`` is a planned Heff unless ``fn_ctx``
-    gives (fnptr, ctx).  The basis grows to at most min(Maxiter, n) vectors (no restarts),
-    bounded by the device memory budget."""
-    n = psi0.numel()
+    """k extreme eigenpairs of a flat FP64 or complex128 (Hermitian) device operator
+    (cxa_lanczos_eigsh).  Returns (eigenvalues (numpy, reference order), eigenvectors (k x n
+    device tensor of psi0's dtype, or None), Krylov vectors used, pairs within the tolerance).
+    ``op`` is a planned Heff unless ``fn_ctx`` gives (fnptr, ctx).  The basis grows to at most
+    min(Maxiter, n) vectors (no restarts), bounded by the device memory budget."""
+    x, cplx = _doubles(psi0, "Lanczos")
+    n = x.numel()
+    ne = psi0.numel()
     dev = psi0.device.index
-    cap = max(2, _krylov_cap(n, min(int(Maxiter), n)))
+    cap = max(2, _krylov_cap(n, min(int(Maxiter), ne)))
     ctx = _context(n, cap, dev)
     fn, mctx = fn_ctx if fn_ctx is not None else op.native_matvec()
     evals = np.zeros(k)
@@ -239,33 +259,46 @@ def eigsh_native(op, psi0: torch.Tensor, which: str = "SA", k: int = 1, tol: flo
     niter = ctypes.c_int(0)
     nconv = ctypes.c_int(0)
     _lib.check(_lib.lib().cxa_lanczos_eigsh(
-        ctx, fn, mctx, psi0.data_ptr(), _WHICH[which], int(k), float(tol), int(min(cap, n)),
-        evals.ctypes.data_as(_lib.dblp), evecs.data_ptr() if evecs is not None else None,
-        ctypes.byref(niter), ctypes.byref(nconv), stream_handle(dev)))
+        ctx, fn, mctx, x.data_ptr(), _WHICH[which], int(k), float(tol), int(min(cap, ne)),
+        CPLX_FLAG if cplx else 0, evals.ctypes.data_as(_lib.dblp),
+        evecs.data_ptr() if evecs is not None else None, ctypes.byref(niter),
+        ctypes.byref(nconv), stream_handle(dev)))
+    if cplx and evecs is not None:
+        evecs = torch.view_as_complex(evecs.view(k, ne, 2))
     return evals, evecs, niter.value, nconv.value
 
 
-def exp_native(op, psi0: torch.Tensor, tau: float, CvgCrit: float = 1e-14,
+def exp_native(op, psi0: torch.Tensor, tau, CvgCrit: float = 1e-14,
                Maxiter: int = 10000, fn_ctx=None):
-    """exp(tau H) psi0 for a flat FP64 device operator (cxa_lanczos_exp): (out, Krylov vectors
-    used).  The basis holds at most min(Maxiter, n) vectors within the device memory budget; a
-    run that reaches that budget before the reference's stop test raises KrylovCapError."""
-    n = psi0.numel()
+    """exp(tau H) psi0 for a flat device operator (cxa_lanczos_exp): (out, Krylov vectors used).
+    psi0 FP64 (real operator) or complex128 (Hermitian operator); tau real or complex.  out is
+    complex128 when psi0 is, or when tau has an imaginary part.  The basis holds at most
+    min(Maxiter, n) vectors within the device memory budget; a run that reaches that budget
+    before the reference's stop test raises KrylovCapError."""
+    x, cplx = _doubles(psi0, "Lanczos_Exp")
+    tau = complex(tau)
+    n = x.numel()
+    ne = psi0.numel()
     dev = psi0.device.index
-    want = max(2, min(int(Maxiter), n))
+    want = max(2, min(int(Maxiter), ne))
     cap = _krylov_cap(n, want)
     ctx = _context(n, cap, dev)
     fn, mctx = fn_ctx if fn_ctx is not None else op.native_matvec()
-    out = torch.empty_like(psi0)
+    cout = cplx or tau.imag != 0.0
+    out = torch.empty(2 * ne if cout else ne, dtype=torch.float64, device=psi0.device)
     niter = ctypes.c_int(0)
-    _lib.check(_lib.lib().cxa_lanczos_exp(ctx, fn, mctx, psi0.data_ptr(), float(tau),
-                                          float(CvgCrit), int(min(Maxiter, cap)), out.data_ptr(),
-                                          ctypes.byref(niter), stream_handle(dev)))
+    _lib.check(_lib.lib().cxa_lanczos_exp(ctx, fn, mctx, x.data_ptr(), float(tau.real),
+                                          float(tau.imag), float(CvgCrit),
+                                          int(min(Maxiter, cap)), CPLX_FLAG if cplx else 0,
+                                          out.data_ptr(), ctypes.byref(niter),
+                                          stream_handle(dev)))
     if cap < want and niter.value >= cap:
         raise KrylovCapError("Lanczos_Exp: the Krylov basis reached %d vectors of %d elements "
                              "(the device memory budget, %d bytes) before the stop test; pass "
                              "a smaller Maxiter or a smaller |tau|"
                              % (cap, n, KRYLOV_BUDGET_BYTES))
+    if cout:
+        out = torch.view_as_complex(out.view(ne, 2))
     return out, niter.value
 
 
@@ -287,8 +320,9 @@ def Lanczos(Hop: LinOp, Tin, method: str = "Gnd", CvgCrit: Optional[float] = Non
                               is_V, ncv, verbose)
     if method != "Gnd":
         raise NotImplementedError("method='Gnd', or which='SA'/'LA'/'LM' for k eigenpairs")
-    if Tin.dtype() != 3:
-        raise TypeError("the GPU Lanczos runs real FP64 operators")
+    if Tin.dtype() not in (1, 3):
+        raise TypeError("the GPU Lanczos runs FP64 (Double) or complex128 (ComplexDouble) "
+                        "operators")
     cvg = 1.0e-14 if CvgCrit is None else float(CvgCrit)
     x0, tmpl = _flat_vector(Tin)
     fn_ctx, keep = _matvec_of(Hop, tmpl)
@@ -300,7 +334,7 @@ def Lanczos(Hop: LinOp, Tin, method: str = "Gnd", CvgCrit: Optional[float] = Non
     except _lib.CxaError as err:
         _reraise_matvec(err)
     from .unitensor import _scalar_block
-    e = _scalar_block(3, x0.device.index)
+    e = _scalar_block(3, x0.device.index)  # the energy is real for both dtypes
     e._storage.fill_(E)
     out = [e]
     if is_V:
@@ -340,6 +374,8 @@ def _lanczos_which(Hop, Tin, which, Maxiter, CvgCrit, k, is_V, ncv, verbose):
                       "Krylov vectors" % (nconv, k, niter))
     dev = x0.device.index
     vals = torch.tensor(evals, dtype=torch.float64, device=x0.device)
+    if evecs is not None:
+        evecs = evecs.reshape(k, -1)
     if isinstance(Tin, UniTensor):
         ev = UniTensor([Bond(k)], rowrank=1, dtype=3, device=dev)
         ev._block = Tensor.from_storage(vals, [k])
@@ -355,12 +391,12 @@ def _lanczos_which(Hop, Tin, which, Maxiter, CvgCrit, k, is_V, ncv, verbose):
 
 def Lanczos_Exp(Hop: LinOp, v, tau, CvgCrit: float = 1.0e-14, Maxiter: int = 10000,
                 verbose: bool = False):
-    """exp(tau H) v by Lanczos (src/linalg/Lanczos_Exp.cpp:266-450), real tau, FP64 on the
-    device (cxa_lanczos_exp).  Returns a tensor with v's metadata."""
-    if isinstance(tau, complex):
-        if tau.imag != 0:
-            raise NotImplementedError("Lanczos_Exp on this path takes a real tau (FP64 operators)")
-        tau = tau.real
+    """exp(tau H) v by Lanczos (src/linalg/Lanczos_Exp.cpp:266-450) on the device
+    (cxa_lanczos_exp): FP64 or complex128 (Hermitian) operators, real or complex tau (e.g.
+    tau = -i dt for a real-time step).  Returns a tensor with v's metadata; its dtype is complex
+    when v is complex or tau has an imaginary part (the reference's output dtype rule,
+    Lanczos_Exp.cpp:67-76, 383-384)."""
+    tau = complex(tau)
     if CvgCrit <= 0:
         raise ValueError("[ERROR][Lanczos_Exp] converge criteria must >0")
     if Maxiter < 2:
@@ -374,7 +410,7 @@ def Lanczos_Exp(Hop: LinOp, v, tau, CvgCrit: float = 1.0e-14, Maxiter: int = 100
     del keep
     if verbose:
         print("Lanczos_Exp: %d Krylov vectors" % niter)
-    return _wrap_like(tmpl, out)
+    return _wrap_like(tmpl, out)  # the dtype follows out's storage (complex for complex tau)
 
 
 def ground_state_mixed(op, psi0: torch.Tensor, CvgCrit: float = 1e-12, Maxiter: int = 1000,

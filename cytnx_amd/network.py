@@ -463,4 +463,47 @@ class Network:
     PrintNet = __repr__
 
 
-__all__ = ["Network", "search_order", "parse_order", "tokenize_order", "parse_tout"]
+def Contracts(TNs: Sequence[UniTensor], order: str = "", optimal: bool = True) -> UniTensor:
+    """Contract a list of UniTensors over their shared labels (Contracts and the list form of
+    Contract, src/UniTensor_base.cpp:711-756, pybind/unitensor_py.cpp:2661-2671).
+
+    A network over the tensors' own labels: named T0, T1, ... when the order is searched
+    (optimal=True, which overrides a given order with a warning) or when no order is given;
+    with optimal=False and an order, the tensors' names (UniTensor.name()) are the names the
+    order refers to.  The output's labels are the uncontracted labels in sorted order with
+    rowrank 1 (the reference builds the network with no TOUT and output rowrank -1, so the
+    filled-in TOUT keeps one row label); a full contraction gives a rank-0 tensor."""
+    TNs = list(TNs)
+    if len(TNs) < 2:
+        raise ValueError("[ERROR][Contracts] should have more than 2 TNs to contract.")
+    orderin = order
+    if optimal or not order:
+        if order:
+            warnings.warn("[WARNING][Contracts] Setting Optimal = true while specifying the "
+                          "order, will find the optimal order instead. To use the desired order "
+                          "please set Optimal = false.")
+            orderin = ""
+        names = ["T%d" % i for i in range(len(TNs))]
+    else:
+        names = []
+        for i, T in enumerate(TNs):
+            if T.name() == "":
+                raise ValueError("[ERROR][Contracts] Unable to contract with the specified "
+                                 "order, missing tensor name at index %d." % i)
+            names.append(T.name())
+    net = Network()
+    for name, T in zip(names, TNs):
+        net._register(name, T.labels())
+    order_tokens = None
+    if orderin:
+        net.order_line = orderin
+        order_tokens = tokenize_order(orderin)
+    net._finish(order_tokens, fill_tout=True)
+    net.tout_rowrank = 1 if net.tout else 0
+    net.PutUniTensors(names, TNs)
+    net.setOrder(optimal, orderin)
+    return net.Launch()
+
+
+__all__ = ["Network", "Contracts", "search_order", "parse_order", "tokenize_order",
+           "parse_tout"]

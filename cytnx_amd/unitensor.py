@@ -313,6 +313,9 @@ class UniTensor:
         self._name = name
         return self
 
+    def name(self) -> str:
+        return self._name
+
     def storage(self) -> torch.Tensor:
         return self._storage if self._is_block else self._block.storage()
 
@@ -692,9 +695,27 @@ def plan_contraction(A: UniTensor, B: UniTensor, C: UniTensor, c1, c2, dtype: in
     return _plan_for(dtype, A, B, list(c1), list(c2), C)
 
 
-def Contract(A: UniTensor, B: UniTensor) -> UniTensor:
-    """UniTensor::contract on the GPU (src/BlockUniTensor.cpp:777-1235,
-    src/DenseUniTensor.cpp:1022-1164)."""
+def Contract(A, B=None, *args, **kwargs) -> UniTensor:
+    """The two overloads of the Python binding (pybind/unitensor_py.cpp:2655-2666):
+
+    * ``Contract(Tl, Tr, cacheL=False, cacheR=False)``: UniTensor::contract on the GPU
+      (src/BlockUniTensor.cpp:777-1235, src/DenseUniTensor.cpp:1022-1164); the cache flags are
+      accepted and have no effect (plans are cached by metadata anyway);
+    * ``Contract(TNs, order="", optimal=True)``: a list of UniTensors, as ``Contracts``
+      (src/UniTensor_base.cpp:711-714)."""
+    if isinstance(A, (list, tuple)):
+        from .network import Contracts
+        return Contracts(A, *(([] if B is None else [B]) + list(args)), **kwargs)
+    for k in kwargs:
+        if k not in ("cacheL", "cacheR"):
+            raise TypeError("Contract() got an unexpected keyword argument '%s'" % k)
+    if len(args) > 2 or B is None:
+        raise TypeError("Contract(Tl, Tr, cacheL=False, cacheR=False)")
+    return _contract_pair(A, B)
+
+
+def _contract_pair(A: UniTensor, B: UniTensor) -> UniTensor:
+    """A.contract(B): common labels contracted, free legs of A then B."""
     if A.is_blockform() != B.is_blockform():
         raise TypeError("Cannot contract BlockUniTensor with other type of UniTensor.")
     comm, c1, c2 = _common(A.labels(), B.labels())

@@ -115,6 +115,7 @@ int cxa_tensordot(int dtype, void *out, const void *L, int rankL, const uint64_t
                   const uint64_t *idxl, const uint64_t *idxr, void *workspace,
                   cxa_stream_t stream);
 
+
 /*
  * Zero-flux block list of a U(1) BlockUniTensor, in the reference's odometer order (last bond
  * fastest).  Replaces the block loop of BlockUniTensor::Init (src/BlockUniTensor.cpp:190-230)
@@ -245,9 +246,10 @@ typedef int (*cxa_matvec_fn)(void *ctx, const void *x, void *y, cxa_stream_t str
  * out_elems give buffer sizes in elements.  This replaces the per-call Network::Launch of the
  * DMRG effective Hamiltonian (src/RegularNetwork.cpp:901-1007 driven by
  * example/DMRG/dmrg_two_sites_U1.py:12-41) with a planned program whose application allocates
- * nothing.  Plans must be real FP64 and must outlive the chain.  Precision (cxa_chain_prepare)
- * selects FP64 (drop-in), FP32 (FP32 copies of the fixed operands and intermediates; input and
- * output stay FP64) or FP32X3.  cxa_chain_matvec is a cxa_matvec_fn with ctx = the chain.
+ * nothing.  Plans must be all real FP64 or all C128, and must outlive the chain.  Precision
+ * (cxa_chain_prepare) selects FP64 (drop-in), FP32 (FP32 copies of the fixed operands and
+ * intermediates; input and output stay FP64) or FP32X3; C128 chains run FP64 only and their
+ * vectors are C128 arrays.  cxa_chain_matvec is a cxa_matvec_fn with ctx = the chain.
  * A fixed operand on the plan's right (B) is read at cxa_chain_create into a k-quad packed
  * copy (and at cxa_chain_prepare into its FP32 form); the chain uses those copies, so a fixed
  * operand changed after creation needs a new chain.  cxa_chain_create synchronises the device
@@ -297,8 +299,11 @@ int cxa_graph_destroy(cxa_graph *graph);
  * sum_k z_k psi_k.  A context holds the Krylov basis for up to `maxiter` iterations.
  * run: batch = iterations enqueued between host checks (results do not depend on it);
  * flags bit 0 = FP64 full re-orthogonalisation (not in the reference), bit 1 = capture one
- * iteration as a graph and replay it.  Outputs: E_out (host), evec (device, may be NULL),
- * niter (tridiagonal size at the stop), alphas/betas (host, may be NULL, niter entries).
+ * iteration as a graph and replay it, bit 2 = complex Hermitian operator: the vectors are C128
+ * arrays of n / 2 values (n counts doubles and must be even), alpha = Re <v|Hv> as
+ * Lanczos_Gnd_Ut takes it (:96-99), and re-orthogonalisation projects with complex
+ * coefficients.  Outputs: E_out (host), evec (device, may be NULL), niter (tridiagonal size at
+ * the stop), alphas/betas (host, may be NULL, niter entries).
  */
 typedef struct cxa_lanczos cxa_lanczos;
 int cxa_lanczos_create(int64_t n, int maxiter, cxa_lanczos **ctx);
@@ -308,19 +313,23 @@ int cxa_lanczos_run(cxa_lanczos *ctx, cxa_matvec_fn matvec, void *matvec_ctx, co
 int cxa_lanczos_destroy(cxa_lanczos *ctx);
 
 /*
- * Lanczos_Exp(Hop, Tin, tau, CvgCrit, Maxiter) for a real tau on FP64 device vectors
+ * Lanczos_Exp(Hop, Tin, tau, CvgCrit, Maxiter) with tau = tau_re + i tau_im on device vectors
  * (src/linalg/Lanczos_Exp.cpp:266-390, :404-450): the three-term recurrence from Tin / |Tin|,
  * stopping before v_i at a round-off beta_i (100 eps times the largest |alpha| + beta seen, times
- * sqrt(i)) or after v_i once |exp(tau T)[i, 0]| < CvgCrit or i = min(Maxiter, n) - 1; then
- * out = V exp(tau T) (V^T Tin) (device).  niter = Krylov vectors used.  The context must hold
- * min(Maxiter, n) vectors.
+ * sqrt(i)) or after v_i once |exp(tau T)[i, 0]| < CvgCrit or i = min(Maxiter, len) - 1; then
+ * out = V exp(tau T) (V^H Tin) (device).  flags bit 2: complex Hermitian operator on C128
+ * vectors (n / 2 values, as for cxa_lanczos_run); out holds n doubles.  Real vectors (flags bit
+ * 2 clear) with tau_im != 0: out is complex, 2n doubles (n C128 values: a real operator's
+ * exp(tau H) Tin for complex tau).  niter = Krylov vectors used.  The context must hold
+ * min(Maxiter, len) vectors.
  */
 int cxa_lanczos_exp(cxa_lanczos *ctx, cxa_matvec_fn matvec, void *matvec_ctx, const double *Tin,
-                    double tau, double CvgCrit, int Maxiter, double *out, int *niter,
-                    cxa_stream_t stream);
+                    double tau_re, double tau_im, double CvgCrit, int Maxiter, int flags,
+                    double *out, int *niter, cxa_stream_t stream);
 
 /*
- * k extreme eigenpairs of a real symmetric operator: linalg::Lanczos(Hop, Tin, which, maxiter,
+ * k extreme eigenpairs of a real symmetric (flags bit 2: complex Hermitian, C128 vectors of
+ * n / 2 values) operator: linalg::Lanczos(Hop, Tin, which, maxiter,
  * cvg_crit, k, is_V, ncv) (src/linalg/Lanczos.cpp:249-402, :453-583), which the reference runs
  * with ARPACK dsaupd/dseupd.  which: 0 = "SA", 1 = "LA", 2 = "LM".  Lanczos with full FP64
  * re-orthogonalisation (no restarts: up to min(Maxiter, n, context) basis vectors); converged
@@ -330,8 +339,8 @@ int cxa_lanczos_exp(cxa_lanczos *ctx, cxa_matvec_fn matvec, void *matvec_ctx, co
  * k x n) the Ritz vectors as rows; niter = basis size; nconv = pairs within the bound.
  */
 int cxa_lanczos_eigsh(cxa_lanczos *ctx, cxa_matvec_fn matvec, void *matvec_ctx, const double *Tin,
-                      int which, int k, double tol, int Maxiter, double *evals, double *evecs,
-                      int *niter, int *nconv, cxa_stream_t stream);
+                      int which, int k, double tol, int Maxiter, int flags, double *evals,
+                      double *evecs, int *niter, int *nconv, cxa_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,197 @@
+"""Complex Hermitian operators under the device Lanczos drivers (Lanczos_Gnd_Ut on complex
+operators, src/linalg/Lanczos_Gnd_Ut.cpp:19-27, 96-99; linalg::Lanczos(which=...) on complex
+operators, src/linalg/Lanczos.cpp:249-583; Lanczos_Exp with a complex tau,
+src/linalg/Lanczos_Exp.cpp:266-450).
+
+Pinned by numpy.linalg.eigh and scipy.linalg.expm of the same dense Hermitian matrices, and for
+the two-site U(1) Heff on complex128 block tensors by the oracle's chain of restated
+BlockUniTensor::contract calls (matvec) and by eigh of the dense matrix that chain builds column
+by column (ground state, real-time step).
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+import torch
+
+import pyoracle as po
+from cytnx_amd import heisenberg
+from cytnx_amd.heff import Heff2
+from cytnx_amd.lanczos import (LinOp, Lanczos, Lanczos_Exp, eigsh_native, exp_native,
+                               lanczos_gnd_native)
+from cytnx_amd.tensor import Tensor
+
+pytestmark = pytest.mark.gpu
+
+
+class DenseOp(LinOp):
+    """y = H x for a dense Hermitian (or real symmetric) H held on the device."""
+
+    def __init__(self, H):
+        cplx = np.iscomplexobj(H)
+        super().__init__("mv", H.shape[0], 1 if cplx else 3, 0)
+        self.H = torch.as_tensor(H, dtype=torch.complex128 if cplx else torch.float64,
+                                 device="cuda")
+
+    def matvec(self, v):
+        x = v.storage().reshape(-1)
+        return Tensor.from_storage(self.H @ x, [x.numel()])
+
+
+def herm(n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    return (A + A.conj().T) / 2
+
+
+def cvec(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+def dev(a):
+    t = torch.as_tensor(np.asarray(a), device="cuda")
+    return Tensor.from_storage(t, [t.numel()])
+
+
+@pytest.mark.parametrize("n", [3, 40, 120])
+def test_complex_ground_state_vs_eigh(n):
+    H = herm(n, 10 + n)
+    w, U = np.linalg.eigh(H)
+    E, v = Lanczos(DenseOp(H), dev(cvec(n, 3)), "Gnd", 1e-14, 10000)
+    E = E.item()
+    x = v.storage().cpu().numpy()
+    print("n", n, "E %.15f eigh %.15f" % (E, w[0]), flush=True)
+    assert x.dtype == np.complex128
+    assert abs(E - w[0]) <= 1e-10 * max(1.0, abs(w[0]))
+    # the Ritz vector: unit norm, residual small, overlap with the eigh vector of modulus one
+    assert abs(np.linalg.norm(x) - 1) <= 1e-10
+    assert np.linalg.norm(H @ x - E * x) <= 1e-6 * max(1.0, abs(w[0]))
+    assert abs(abs(np.vdot(U[:, 0], x)) - 1) <= 1e-8
+
+
+def test_complex_ground_state_with_reorthogonalisation():
+    """The complex re-orthogonalisation (complex projection coefficients) keeps the basis
+    orthonormal: the energy equals eigh's and the run with it matches the run without."""
+    n = 200
+    H = herm(n, 77)
+    w = np.linalg.eigvalsh(H)
+    op = DenseOp(H)
+    x0 = torch.as_tensor(cvec(n, 5), device="cuda")
+    from cytnx_amd.lanczos import _matvec_of
+    fn_ctx, keep = _matvec_of(op, dev(cvec(n, 5)))
+    E1, _, k1, _, _ = lanczos_gnd_native(None, x0, 1e-13, 400, batch=1, fn_ctx=fn_ctx,
+                                         reorth=True)
+    E0, _, k0, _, _ = lanczos_gnd_native(None, x0, 1e-13, 400, batch=1, fn_ctx=fn_ctx)
+    print("reorth: E %.15f (%d it), plain %.15f (%d it), eigh %.15f" % (E1, k1, E0, k0, w[0]),
+          flush=True)
+    assert abs(E1 - w[0]) <= 1e-10 * abs(w[0])
+    assert abs(E0 - w[0]) <= 1e-10 * abs(w[0])
+    del keep
+
+
+@pytest.mark.parametrize("which", ["SA", "LA", "LM"])
+def test_complex_which_eigenpairs_vs_eigh(which):
+    n, k = 50, 4
+    H = herm(n, 200)
+    w = np.linalg.eigvalsh(H)
+    want = {"SA": np.sort(w), "LA": np.sort(w)[::-1],
+            "LM": w[np.argsort(-np.abs(w), kind="stable")]}[which][:k]
+    out = Lanczos(DenseOp(H), dev(cvec(n, 9)), which=which, k=k, Maxiter=n)
+    got = out[0].storage().cpu().numpy()
+    vecs = out[1].storage().cpu().numpy().reshape(k, n)
+    print(which, got, want, flush=True)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-9)
+    for i in range(k):
+        assert np.linalg.norm(H @ vecs[i] - got[i] * vecs[i]) <= 1e-7
+    G = vecs.conj() @ vecs.T
+    assert np.abs(G - np.eye(k)).max() <= 1e-8
+
+
+@pytest.mark.parametrize("tau", [-0.3j, -0.1 + 0.25j, -0.2])
+def test_complex_lanczos_exp_vs_expm(tau):
+    n = 80
+    H = herm(n, 300) / 4
+    v = cvec(n, 11)
+    want = scipy.linalg.expm(tau * H) @ v
+    out = Lanczos_Exp(DenseOp(H), dev(v), tau, 1e-14, 200)
+    got = out.storage().cpu().numpy()
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print("tau", tau, "rel err %.3e" % err, flush=True)
+    assert got.dtype == np.complex128
+    assert err <= 1e-10
+
+
+def test_real_operator_complex_tau_gives_complex_output():
+    """A real symmetric operator with a complex tau: exp(tau H) v is complex (the reference's
+    output-dtype rule), from the real Krylov basis."""
+    n = 70
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((n, n))
+    H = (A + A.T) / 8
+    v = rng.standard_normal(n)
+    for tau in (-0.5j, 0.3 - 0.4j):
+        want = scipy.linalg.expm(tau * H) @ v
+        out = Lanczos_Exp(DenseOp(H), dev(v), tau, 1e-14, 200)
+        got = out.storage().cpu().numpy()
+        assert got.dtype == np.complex128
+        assert np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want)
+
+
+def _oblock(ut):
+    blocks = [ut.get_block_(i).detach().cpu().numpy() for i in range(ut.Nblocks())]
+    bonds = [po.OBond(b.type(), b.qnums(), b.getDegeneracies()) for b in ut.bonds()]
+    return po.OBlockTensor(bonds, ut.labels(), ut.rowrank(), itoi=ut.itoi(), blocks=blocks)
+
+
+@pytest.fixture(scope="module")
+def complex_heff():
+    host = heisenberg.two_site_problem(12, 3, hermitian=True, dtype=np.complex128)
+    psi, L, M1, M2, R = [t.to(0) for t in host]
+    H = Heff2(psi, L, M1, M2, R)
+    n = psi._storage.numel()
+    # the dense matrix of the chain, column by column
+    cols = []
+    for j in range(n):
+        e = torch.zeros(n, dtype=torch.complex128, device="cuda")
+        e[j] = 1
+        y = torch.empty_like(e)
+        H.apply_flat(e, y)
+        cols.append(y.cpu().numpy())
+    D = np.stack(cols, axis=1)
+    return host, psi, H, D
+
+
+def test_complex_heff2_matvec_matches_oracle(complex_heff):
+    host, psi, H, D = complex_heff
+    assert H.mode == "w12" and H.dtype() == 1
+    P, Lo, M1o, M2o, Ro = [_oblock(t) for t in host]
+    ref = po.block_contract(po.block_contract(po.block_contract(po.block_contract(Lo, P), M1o),
+                                              M2o), Ro).flat()
+    y = torch.empty_like(psi._storage)
+    H.apply_flat(psi._storage, y)
+    got = y.cpu().numpy()
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print("complex Heff2 n", psi._storage.numel(), "rel err %.3e" % err, flush=True)
+    assert err <= 1e-12
+    # the hermitised environments make the operator Hermitian
+    assert np.abs(D - D.conj().T).max() <= 1e-12 * np.abs(D).max()
+
+
+def test_complex_heff2_ground_state_and_real_time_step(complex_heff):
+    host, psi, H, D = complex_heff
+    w = np.linalg.eigvalsh(D)
+    x0 = psi._storage.clone()
+    E, v, k, _, _ = lanczos_gnd_native(H, x0, 1e-13, 2000, batch=8)
+    print("complex Heff2 ground state %.14f eigh %.14f (%d it)" % (E, w[0], k), flush=True)
+    assert abs(E - w[0]) <= 1e-9 * abs(w[0])
+    ev, _, m, nconv = eigsh_native(H, x0, "SA", 2, 1e-12, False)
+    np.testing.assert_allclose(ev, w[:2], rtol=0, atol=1e-8 * abs(w[0]))
+    # one real-time step exp(-i dt H) psi: against expm, and unitary
+    dt = 0.02
+    out, m = exp_native(H, x0, -1j * dt, 1e-13, 300)
+    want = scipy.linalg.expm(-1j * dt * D) @ x0.cpu().numpy()
+    got = out.cpu().numpy()
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print("real-time step: %d Krylov vectors, rel err %.3e" % (m, err), flush=True)
+    assert err <= 1e-10
+    assert abs(np.linalg.norm(got) / np.linalg.norm(x0.cpu().numpy()) - 1) <= 1e-11

@@ -160,3 +160,41 @@ def test_block_network_three_tensors_matches_direct_contract():
         out = net.Launch()
         assert out.labels() == ["c_", "d_", "f_", "g_"] and out.rowrank() == 2
         _same_blocks(_ut_blocks(out), _ut_blocks(ref), 1e-12)
+
+
+@pytest.mark.parametrize("how", ["optimal", "default", "named_order", "contract_list"])
+def test_contracts_dmrg_projector_matches_oracle(how):
+    """Contracts / Contract(list) (src/UniTensor_base.cpp:711-756) on the two-site projector
+    tensors labelled as the DMRG network labels them: the output is the uncontracted labels
+    sorted (0, 1, 2, 3) with rowrank 1, equal to the oracle's block-contraction chain."""
+    from cytnx_amd.network import Contracts
+    from cytnx_amd.unitensor import Contract
+    psi, L, M1, M2, R = problem(32, seed=33)
+    tn = [psi.relabel(["-1", "-2", "-3", "-4"]), L.relabel(["-5", "-1", "0"]),
+          R.relabel(["-7", "-4", "3"]), M1.relabel(["-5", "-6", "-2", "1"]),
+          M2.relabel(["-6", "-7", "-3", "2"])]
+    for T, name in zip(tn, ["psi", "L", "R", "M1", "M2"]):
+        T.set_name_(name)
+    if how == "optimal":
+        out = Contracts(tn)
+    elif how == "default":
+        out = Contracts(tn, "", False)
+    elif how == "named_order":
+        out = Contracts(tn, "(((L,psi),M1),(M2,R))", False)
+    else:
+        out = Contract(tn, "", True)
+    torch.cuda.synchronize()
+    assert out.labels() == ["0", "1", "2", "3"] and out.rowrank() == 1
+    ref = oracle_heff2(psi, L, M1, M2, R)
+    _same_blocks(_ut_blocks(out), _by_key(ref.itoi, ref.blocks), 1e-12)
+
+
+def test_contracts_full_contraction_is_a_scalar():
+    """<psi|psi> as Contracts([psi, psi.Dagger()]): every label contracted, rank 0."""
+    from cytnx_amd.network import Contracts
+    psi = problem(16, seed=5)[0]
+    bra = psi.Dagger()
+    out = Contracts([psi, bra])
+    assert out.rank() == 0
+    want = float(torch.sum(psi._storage * psi._storage).item())
+    assert abs(out.item() - want) <= 1e-12 * want

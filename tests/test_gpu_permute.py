@@ -56,6 +56,30 @@ def test_permute_sweep_bit_exact(shape, dtype):
         assert np.array_equal(_bits(got), _bits(ref)), perm
 
 
+# batches of small transposes (the last two axes swapped, or two short axes moved past each
+# other): tiles that batch several indices of the next outer axis, with batch counts that do not
+# divide that axis (ragged last tiles)
+BATCH_CASES = [((16, 8, 16, 8, 16, 8), [0, 1, 2, 3, 5, 4]), ((32, 4, 32, 4, 32, 4), [0, 1, 2, 3, 5, 4]),
+               ((3, 16, 8), [0, 2, 1]), ((7, 4, 32), [0, 2, 1]), ((11, 8, 16), [0, 2, 1]),
+               ((13, 3, 5), [0, 2, 1]), ((9, 5, 6, 3), [1, 0, 3, 2]), ((2, 3, 4, 5, 6), [0, 1, 2, 4, 3]),
+               ((4, 100, 3), [0, 2, 1]), ((5, 3, 7, 2), [0, 1, 3, 2])]
+
+
+@pytest.mark.parametrize("dtype", [np.bool_, np.int16, np.float32, np.float64, np.complex128])
+def test_permute_batched_small_transposes_bit_exact(dtype):
+    rng = np.random.default_rng(13)
+    for shape, perm in BATCH_CASES:
+        x = rng.uniform(-1, 1, size=shape)
+        if dtype == np.complex128:
+            x = x + 1j * rng.uniform(-1, 1, size=shape)
+        if dtype == np.bool_:
+            x = x > 0
+        x = (x * 100).astype(dtype) if np.dtype(dtype).kind == "i" else x.astype(dtype)
+        print("permute", shape, np.dtype(dtype).name, perm, flush=True)
+        got = _gpu_permute(x, perm)
+        assert np.array_equal(_bits(got), _bits(po.permute(x, perm)))
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.complex64, np.int32, np.int16, np.uint64,
                                    np.bool_, np.float64])
 def test_permute_all_element_sizes(dtype):

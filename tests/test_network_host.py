@@ -261,3 +261,56 @@ def test_optimal_order_of_dmrg_network_uses_declared_labels():
     _, default = _tree_cost(((((0, 1), 2), 3), 4), net.label_arr,
                             [t.shape() for t in net.tensors])
     assert cost <= default
+
+
+def _cpu_dense(shape, labels, name=""):
+    from cytnx_amd.unitensor import Bond, UniTensor
+    T = UniTensor([Bond(int(s)) for s in shape], labels, 1, 3, -1)
+    return T.set_name_(name)
+
+
+def test_contracts_validation():
+    """Contracts' argument checks (src/UniTensor_base.cpp:717-746), before any data moves."""
+    from cytnx_amd.network import Contracts
+    from cytnx_amd.unitensor import Contract
+    A = _cpu_dense([2, 3], ["a", "b"])
+    B = _cpu_dense([3, 4], ["b", "c"])
+    with pytest.raises(ValueError, match="more than 2 TNs"):
+        Contracts([A])
+    with pytest.raises(ValueError, match="more than 2 TNs"):
+        Contract([A])
+    # a given order with optimal=False names the tensors by their own names
+    with pytest.raises(ValueError, match="missing tensor name at index 0"):
+        Contracts([A, B], "(A,B)", False)
+    B.set_name_("B")
+    A.set_name_("A")
+    with pytest.raises(ValueError):
+        Contracts([A, B], "(A,X)", False)  # an order naming a tensor that is not in the list
+    with pytest.raises(TypeError):
+        Contract(A, B, False, False, True)
+    with pytest.raises(TypeError):
+        Contract(A, B, cache=True)
+
+
+def test_contracts_optimal_with_order_warns():
+    """optimal=True with an order: the order is dropped with the reference's warning."""
+    from cytnx_amd import network
+    A = _cpu_dense([2, 3], ["a", "b"])
+    B = _cpu_dense([3, 4], ["b", "c"])
+    seen = {}
+
+    def fake_launch(self):
+        seen["tout"], seen["rr"], seen["order"] = list(self.tout), self.tout_rowrank, \
+            self.order_line
+        return None
+
+    orig = network.Network.Launch
+    network.Network.Launch = fake_launch
+    try:
+        with pytest.warns(UserWarning, match="Setting Optimal = true"):
+            network.Contracts([B, A], "(T1,T0)", True)
+    finally:
+        network.Network.Launch = orig
+    # sorted uncontracted labels, rowrank 1, the searched order over T0, T1
+    assert seen["tout"] == ["a", "c"] and seen["rr"] == 1
+    assert seen["order"] == "(T0,T1)"
