@@ -51,6 +51,7 @@ def _declare(L):
                                 ctypes.POINTER(vp), i64p, i64, i64p, vp]),
         "cxa_tensordot_workspace": (ci, [ci, ci, u64p, ci, u64p, ci, u64p, u64p, i64p]),
         "cxa_tensordot": (ci, [ci, vp, vp, ci, u64p, vp, ci, u64p, ci, u64p, u64p, vp, vp]),
+        "cxa_tensordot_noalloc": (ci, [ci, vp, vp, ci, u64p, vp, ci, u64p, ci, u64p, u64p, vp]),
         "cxa_u1_enumerate_blocks": (ci, [ci, intp, i64p, i64p, i64, u64p, i64p]),
         "cxa_block_contract_plan": (ci, [ci, ci, ci, u64p, u64p, i64p, ci, ci, u64p, u64p, i64p,
                                          ci, u64p, u64p, ci, u64p, u64p, i64p,

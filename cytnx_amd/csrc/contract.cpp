@@ -6,6 +6,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "cxa_common.hpp"
@@ -937,4 +938,62 @@ extern "C" int cxa_tensordot(int dtype, void *out, const void *L, int rankL,
   g.blk0 = GBlk{0, N, 1.0, 0.0, (int32_t)M, (int32_t)N, 0, K > 0 ? 1 : 0};
   g.seg0 = GSeg{0, 0, sam, sak, sbk, sbn, (int32_t)K, 0};
   return launch_grouped_gemm(g, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------------------------------
+// Tensordot that allocates nothing: the operand copies that cxa_tensordot permutes into the
+// caller's workspace go to a fixed device array of this library instead (`scratch`, 32 MiB on
+// every GPU the library runs on, held until the process ends).  Every call on one GPU shares
+// that GPU's array: calls that overlap on one GPU (different streams, or threads) race.  The
+// copies overwrite every scratch element the GEMM reads, so nothing needs clearing.
+// ------------------------------------------------------------------------------------------
+namespace {
+
+constexpr int64_t kScratchBytes = CXA_TENSORDOT_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch`, per device ordinal
+
+// The address of `scratch` on the current device, looked up once per device (host code must
+// not pass the array's own name to a kernel: on the host it is not a device address).
+int scratch_on_current_device(void **out) {
+  int dev = -1;
+  CXA_HIP(hipGetDevice(&dev));
+  CXA_CHECK(dev >= 0 && dev < kMaxDevices, CXA_ERR_UNSUPPORTED,
+            "cxa_tensordot_noalloc: device ordinal %d above %d", dev, kMaxDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    CXA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    CXA_CHECK(p != nullptr, CXA_ERR_HIP, "cxa_tensordot_noalloc: no device address for scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return CXA_OK;
+}
+
+}  // namespace
+
+extern "C" int cxa_tensordot_noalloc(int dtype, void *out, const void *L, int rankL,
+                                     const uint64_t *shapeL, const void *R, int rankR,
+                                     const uint64_t *shapeR, int ncomm, const uint64_t *idxl,
+                                     const uint64_t *idxr, cxa_stream_t stream) {
+  CXA_CHECK(dtype == CXA_F64 || dtype == CXA_F32 || dtype == CXA_C128 || dtype == CXA_C64,
+            CXA_ERR_UNSUPPORTED, "cxa_tensordot_noalloc: unsupported dtype %d", dtype);
+  CXA_CHECK(rankL >= 0 && rankR >= 0 && ncomm >= 1 && idxl && idxr && shapeL && shapeR,
+            CXA_ERR_INVALID, "cxa_tensordot_noalloc: bad arguments");
+  int64_t need = 0;
+  int rc = cxa_tensordot_workspace(dtype, rankL, shapeL, rankR, shapeR, ncomm, idxl, idxr, &need);
+  if (rc) return rc;
+  CXA_CHECK(need <= kScratchBytes, CXA_ERR_INVALID,
+            "cxa_tensordot_noalloc: the permuted operands need %lld bytes, above the %lld-byte "
+            "static scratch; use cxa_tensordot with a workspace",
+            (long long)need, (long long)kScratchBytes);
+  void *ws = nullptr;
+  if (need > 0 && (rc = scratch_on_current_device(&ws))) return rc;
+  return cxa_tensordot(dtype, out, L, rankL, shapeL, R, rankR, shapeR, ncomm, idxl, idxr, ws,
+                       stream);
 }

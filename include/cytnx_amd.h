@@ -115,6 +115,21 @@ int cxa_tensordot(int dtype, void *out, const void *L, int rankL, const uint64_t
                   const uint64_t *idxl, const uint64_t *idxr, void *workspace,
                   cxa_stream_t stream);
 
+/*
+ * cxa_tensordot without a workspace argument, allocating nothing: no counterpart in the
+ * reference.  It is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  The operand copies that
+ * cxa_tensordot permutes into the workspace go to a static device array of the library,
+ * CXA_TENSORDOT_SCRATCH_BYTES on every GPU the library runs on (held until the process ends);
+ * a call whose cxa_tensordot_workspace size exceeds it returns CXA_ERR_INVALID before any
+ * device work.  Results equal cxa_tensordot's bit for bit.  All calls on one GPU share its
+ * array: calls that overlap on one GPU (other streams, other threads) race.  The stream must
+ * belong to the current device.
+ */
+#define CXA_TENSORDOT_SCRATCH_BYTES (32ll << 20)
+int cxa_tensordot_noalloc(int dtype, void *out, const void *L, int rankL, const uint64_t *shapeL,
+                          const void *R, int rankR, const uint64_t *shapeR, int ncomm,
+                          const uint64_t *idxl, const uint64_t *idxr, cxa_stream_t stream);
 
 /*
  * Zero-flux block list of a U(1) BlockUniTensor, in the reference's odometer order (last bond

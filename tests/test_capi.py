@@ -73,3 +73,20 @@ def test_vector_reductions_validate_on_host():
     assert rc == _lib.ERR_INVALID
     rc = L.cxa_dot(3, 4, None, None, 0, ctypes.c_void_p(8), ws, None)
     assert rc == _lib.ERR_INVALID
+
+
+def test_tensordot_noalloc_rejects_oversize_on_host():
+    """The no-allocation tensordot refuses operands whose permuted copies exceed its static
+    scratch, before any device work (so no pointers are needed)."""
+    L = _lib.lib()
+    # L (128, 3, 16384) contracted over axis 1: its free axes (0, 2) are not adjacent, so L is
+    # copied: 6,291,456 doubles = 48 MiB > 32 MiB
+    rc = L.cxa_tensordot_noalloc(3, None, None, 3, _lib.u64_array([128, 3, 16384]), None, 2,
+                                 _lib.u64_array([3, 2]), 1, _lib.u64_array([1]),
+                                 _lib.u64_array([0]), None)
+    assert rc == _lib.ERR_INVALID
+    assert b"static scratch" in L.cxa_last_error()
+    rc = L.cxa_tensordot_noalloc(5, None, None, 1, _lib.u64_array([2]), None, 1,
+                                 _lib.u64_array([2]), 1, _lib.u64_array([0]),
+                                 _lib.u64_array([0]), None)
+    assert rc == _lib.ERR_UNSUPPORTED
