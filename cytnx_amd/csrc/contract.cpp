@@ -545,6 +545,8 @@ int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, i
   g.splitk = v.splitk;
   g.tile_class = v.tile_class;
   g.conj = is_complex(p->dtype) ? conj : 0;
+  g.dot_x = lay->dot_x;
+  g.dot_part = lay->dot_part;
   if (p->mix) {
     g.outs = lay->outs ? lay->outs : (const MixOut *)p->d_outs.p;
     g.kind0 = v.launch_kind;
@@ -653,6 +655,13 @@ int bc_mix_pack_outs(const cxa_bc_plan *mix, const std::vector<PackRegion> &regi
 
 bool bc_is_mix(const cxa_bc_plan *p) { return p->mix; }
 int64_t bc_workspace_elems(const cxa_bc_plan *p) { return p->wsA + p->wsB; }
+
+int bc_dot_slots(const cxa_bc_plan *p, int prec) {
+  if (!p || p->mix || is_complex(p->dtype) || prec < 0 || prec > CXA_PREC_FP32X3) return 0;
+  const PlanVariant &v = p->var[prec];
+  if (v.tile_class == 1 || v.tiles.empty()) return 0;  // make_tiles' row-streaming class
+  return (int)v.tiles.size();
+}
 double bc_flops(const cxa_bc_plan *p) { return p->flops; }
 int bc_dtype(const cxa_bc_plan *p) { return p->dtype; }
 

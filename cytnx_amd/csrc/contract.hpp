@@ -17,6 +17,9 @@ struct ExecLayout {
   const GSeg *segs = nullptr;
   const MixOut *outs = nullptr;
   bool a_k4 = false, b_k4 = false, c_k4 = false;
+  // dot-product epilogue (GemmLaunch::dot_x / dot_part): dot_part gets bc_dot_slots entries
+  const double *dot_x = nullptr;
+  double *dot_part = nullptr;
 };
 
 // conj (complex plans): bit 0 conjugates L, bit 1 conjugates R as they are read.
@@ -50,6 +53,9 @@ int bc_pack_a(const cxa_bc_plan *p, int64_t elems, std::vector<GSeg> &segs,
 int bc_mix_pack_outs(const cxa_bc_plan *mix, const std::vector<PackRegion> &regions,
                      std::vector<MixOut> &outs);
 bool bc_is_mix(const cxa_bc_plan *p);
+// Tile slots a dot-product epilogue of an execution at precision prec writes (the launch's tile
+// list length), or 0 when that launch cannot carry one (mix, complex, row-streaming tiles).
+int bc_dot_slots(const cxa_bc_plan *p, int prec);
 int64_t bc_workspace_elems(const cxa_bc_plan *p);
 double bc_flops(const cxa_bc_plan *p);
 int bc_dtype(const cxa_bc_plan *p);

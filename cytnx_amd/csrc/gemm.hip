@@ -7,6 +7,8 @@
 namespace cxa {
 
 int launch_grouped_gemm(const GemmLaunch &g, hipStream_t s) {
+  if (g.outs && g.dot_part)
+    return fail(CXA_ERR_UNSUPPORTED, "grouped GEMM: no dot epilogue on mix launches");
   if (g.outs) return launch_mix(g, s);
   switch (g.prec) {
     case CXA_PREC_FP64: return launch_gemm_fp64(g, s);

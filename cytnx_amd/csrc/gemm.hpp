@@ -91,6 +91,13 @@ struct GemmLaunch {
   // and ldc the k of its first row: output (row, col) at c_off + (k >> 2) * 4N + col * 4 +
   // (k & 3) with k = ldc + row and N the item's columns
   bool c_k4 = false;
+  // Dot-product epilogue (real grouped launches with an FP64 C, MFMA tile classes): with dot_part
+  // set (2 * ntiles doubles), the tile in slot i of the tile list (i = x * per_xcd + t) also
+  // writes dot_part[i] = sum of C[e] * dot_x[e] and dot_part[ntiles + i] = sum of dot_x[e]^2
+  // over its stored elements e (element offsets in C; a padding slot writes zeros), in a fixed
+  // order: summed over all slots, C . dot_x and |dot_x|^2 where the tiles cover C.
+  const double *dot_x = nullptr;
+  double *dot_part = nullptr;
   // single-block mode: one block with one segment passed by value, no device task arrays
   bool single = false;
   GBlk blk0{};

@@ -29,6 +29,7 @@
 // CXA_GEMM_TU (a distinct token per translation unit) before including it: each unit carries
 // its own zero cell.
 #include <algorithm>
+#include <type_traits>
 
 #include "cxa_common.hpp"
 #include "gemm.hpp"
@@ -372,12 +373,33 @@ __device__ __forceinline__ void store_c(gptr<TC> dst, double val, const GBlk &bl
   *dst = (TC)val;
 }
 
+// Sum over the 64 lanes of a wave (butterfly; every lane gets the same value).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// The dot-product epilogue of one tile: lane partials of C[e] * x[e] and of x[e]^2 summed over
+// the wave into slot[0] and slot[nslots] (lane 0).  The C values come from the accumulators.
+__device__ __forceinline__ void dot_epilogue(double cx, double xx, double *slot, int64_t nslots,
+                                             int lane) {
+  cx = wave_sum(cx);
+  xx = wave_sum(xx);
+  if (lane == 0) {
+    slot[0] = cx;
+    slot[nslots] = xx;
+  }
+}
+
 template <int PREC, typename TA, typename TB, typename TC, int MI, int NI, int PK,
           typename SEGS>
 __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
                                                const char *__restrict__ Bbase,
                                                char *__restrict__ Cbase, const GBlk &blk,
-                                               SEGS segs, int m0, int n0, int lane) {
+                                               SEGS segs, int m0, int n0, int lane,
+                                               const double *__restrict__ dotx = nullptr,
+                                               double *dslot = nullptr, int64_t nslots = 0) {
   RealAcc<PREC, MI, NI> acc;
   acc.zero();
   mfma_accumulate<PREC, TA, TB, MI, NI, PK>(acc, Abase, Bbase, blk, segs, m0, n0, lane, 0, 1);
@@ -395,6 +417,24 @@ __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
         if (row < mt && col < nt)
           store_c(Cp + (int64_t)(m0 + row) * blk.ldc + (n0 + col), acc.get(i, jj, v), blk);
       }
+  if (dslot) {  // uniform over the wave; only launches with an FP64 C and alpha 1, beta 0
+    double cx = 0.0, xx = 0.0;
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+      for (int jj = 0; jj < NI; ++jj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * i + acc_row(RealAcc<PREC, MI, NI>::F64, g, v);
+          const int col = 16 * jj + r;
+          if (row < mt && col < nt) {
+            const double xe = dotx[blk.c_off + (int64_t)(m0 + row) * blk.ldc + (n0 + col)];
+            cx += acc.get(i, jj, v) * xe;
+            xx += xe * xe;
+          }
+        }
+    dot_epilogue(cx, xx, dslot, nslots, lane);
+  }
 }
 
 // Split-K tile: every wave of the workgroup (WAVES of them) calls this for the same tile; wave
@@ -405,7 +445,9 @@ __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
                                              const char *__restrict__ Bbase,
                                              char *__restrict__ Cbase, const GBlk &blk,
                                              SEGS segs, int m0, int n0, int lane, int wave,
-                                             double (*red)[16][64]) {
+                                             double (*red)[16][64],
+                                             const double *__restrict__ dotx = nullptr,
+                                             double *dslot = nullptr, int64_t nslots = 0) {
   RealAcc<PREC, MI, NI> acc;
   acc.zero();
   mfma_accumulate<PREC, TA, TB, MI, NI, PK>(acc, Abase, Bbase, blk, segs, m0, n0, lane, wave,
@@ -436,6 +478,28 @@ __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
           if (row < mt && col < nt)
             store_c(Cp + (int64_t)(m0 + row) * blk.ldc + (n0 + col), s, blk);
         }
+    if (dslot) {  // the same sums again (from LDS) times the dot vector
+      double cx = 0.0, xx = 0.0;
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int jj = 0; jj < NI; ++jj)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = 16 * i + acc_row(RealAcc<PREC, MI, NI>::F64, g, v);
+            const int col = 16 * jj + r;
+            const int idx = (i * NI + jj) * 4 + v;
+            double s = red[0][idx][lane];
+#pragma unroll
+            for (int q = 1; q < WAVES; ++q) s += red[q][idx][lane];
+            if (row < mt && col < nt) {
+              const double xe = dotx[blk.c_off + (int64_t)(m0 + row) * blk.ldc + (n0 + col)];
+              cx += s * xe;
+              xx += xe * xe;
+            }
+          }
+      dot_epilogue(cx, xx, dslot, nslots, lane);
+    }
   }
   __syncthreads();
 }
@@ -689,7 +753,8 @@ template <int PREC, typename TA, typename TB, typename TC, bool CPLX, int CLS, i
           typename SEGS>
 __device__ __forceinline__ void run_tile(int kind, const char *A, const char *B, char *C,
                                          const GBlk &blk, SEGS segs, int m0, int n0, int lane,
-                                         int conj) {
+                                         int conj, const double *dotx = nullptr,
+                                         double *dslot = nullptr, int64_t nslots = 0) {
   if constexpr (CPLX) {
     switch (kind) {
       case TK_1x1: wave_tile_cplx<TA, 1, 1>(A, B, C, blk, segs, m0, n0, lane, conj); break;
@@ -700,16 +765,25 @@ __device__ __forceinline__ void run_tile(int kind, const char *A, const char *B,
   } else if constexpr (CLS == TC_ROWS) {
     wave_rows<TA, TB, TC, PK>(A, B, C, blk, segs, m0, lane);
   } else if constexpr (CLS == TC_BIG) {
-    wave_tile_real<PREC, TA, TB, TC, 4, 2, PK>(A, B, C, blk, segs, m0, n0, lane);
+    wave_tile_real<PREC, TA, TB, TC, 4, 2, PK>(A, B, C, blk, segs, m0, n0, lane, dotx, dslot,
+                                               nslots);
   } else {
     switch (kind) {
-      case TK_1x1: wave_tile_real<PREC, TA, TB, TC, 1, 1, PK>(A, B, C, blk, segs, m0, n0, lane);
+      case TK_1x1:
+        wave_tile_real<PREC, TA, TB, TC, 1, 1, PK>(A, B, C, blk, segs, m0, n0, lane, dotx, dslot,
+                                                   nslots);
         break;
-      case TK_2x1: wave_tile_real<PREC, TA, TB, TC, 2, 1, PK>(A, B, C, blk, segs, m0, n0, lane);
+      case TK_2x1:
+        wave_tile_real<PREC, TA, TB, TC, 2, 1, PK>(A, B, C, blk, segs, m0, n0, lane, dotx, dslot,
+                                                   nslots);
         break;
-      case TK_1x2: wave_tile_real<PREC, TA, TB, TC, 1, 2, PK>(A, B, C, blk, segs, m0, n0, lane);
+      case TK_1x2:
+        wave_tile_real<PREC, TA, TB, TC, 1, 2, PK>(A, B, C, blk, segs, m0, n0, lane, dotx, dslot,
+                                                   nslots);
         break;
-      default: wave_tile_real<PREC, TA, TB, TC, 2, 2, PK>(A, B, C, blk, segs, m0, n0, lane);
+      default:
+        wave_tile_real<PREC, TA, TB, TC, 2, 2, PK>(A, B, C, blk, segs, m0, n0, lane, dotx, dslot,
+                                                   nslots);
         break;
     }
   }
@@ -722,18 +796,24 @@ __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict
                                                         int per_xcd, const GBlk *__restrict__ blks,
                                                         const GSeg *__restrict__ segs,
                                                         const char *A, const char *B, char *C,
-                                                        int conj) {
+                                                        int conj, const double *dotx,
+                                                        double *dotp) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const GTile *range = tiles + (int64_t)(blockIdx.x % kXcds) * per_xcd;
+  const int x = blockIdx.x % kXcds;
+  const GTile *range = tiles + (int64_t)x * per_xcd;
   const int stride = (gridDim.x / kXcds) * kWaves;
   for (int t = (blockIdx.x / kXcds) * kWaves + wave; t < per_xcd; t += stride) {
     const GTile tl = range[t];
-    if (tl.blk < 0) continue;  // padding
+    double *dslot = dotp ? dotp + (int64_t)x * per_xcd + t : nullptr;
+    if (tl.blk < 0) {  // padding
+      if (dslot && lane == 0) dslot[0] = dslot[(int64_t)kXcds * per_xcd] = 0.0;
+      continue;
+    }
     const GBlk blk = blks[tl.blk];
     run_tile<PREC, TA, TB, TC, CPLX, CLS, PK>(tl.kind, A, B, C, blk, SegArray{segs}, tl.m0,
-                                               tl.n0,
-                                          lane, conj);
+                                               tl.n0, lane, conj, dotx, dslot,
+                                               (int64_t)kXcds * per_xcd);
   }
 }
 
@@ -742,16 +822,22 @@ __global__ __launch_bounds__(64 * kWaves) void k_grouped(const GTile *__restrict
 template <int PREC, typename TA, typename TB, typename TC, int CLS, int WAVES, int PK>
 __global__ __launch_bounds__(64 * WAVES) void k_grouped_split(
     const GTile *__restrict__ tiles, int per_xcd, const GBlk *__restrict__ blks,
-    const GSeg *__restrict__ segs, const char *A, const char *B, char *C) {
+    const GSeg *__restrict__ segs, const char *A, const char *B, char *C,
+    const double *dotx, double *dotp) {
   static_assert(CLS != TC_ROWS || WAVES == kWaves, "row tiles split over four waves");
   __shared__ double red[WAVES][16][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const GTile *range = tiles + (int64_t)(blockIdx.x % kXcds) * per_xcd;
+  const int x = blockIdx.x % kXcds;
+  const GTile *range = tiles + (int64_t)x * per_xcd;
   const int stride = gridDim.x / kXcds;
   for (int t = blockIdx.x / kXcds; t < per_xcd; t += stride) {
     const GTile tl = range[t];
-    if (tl.blk < 0) continue;  // padding (uniform over the workgroup)
+    double *dslot = dotp ? dotp + (int64_t)x * per_xcd + t : nullptr;
+    if (tl.blk < 0) {  // padding (uniform over the workgroup)
+      if (dslot && threadIdx.x == 0) dslot[0] = dslot[(int64_t)kXcds * per_xcd] = 0.0;
+      continue;
+    }
     const GBlk blk = blks[tl.blk];
     const SegArray sa{segs};
     if constexpr (CLS == TC_ROWS) {
@@ -761,22 +847,26 @@ __global__ __launch_bounds__(64 * WAVES) void k_grouped_split(
         case TK_1x1:
           wg_tile_real<PREC, TA, TB, TC, 1, 1, WAVES, PK>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
                                                           wave,
-                                                     red);
+                                                     red, dotx, dslot,
+                                                     (int64_t)kXcds * per_xcd);
           break;
         case TK_2x1:
           wg_tile_real<PREC, TA, TB, TC, 2, 1, WAVES, PK>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
                                                           wave,
-                                                     red);
+                                                     red, dotx, dslot,
+                                                     (int64_t)kXcds * per_xcd);
           break;
         case TK_1x2:
           wg_tile_real<PREC, TA, TB, TC, 1, 2, WAVES, PK>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
                                                           wave,
-                                                     red);
+                                                     red, dotx, dslot,
+                                                     (int64_t)kXcds * per_xcd);
           break;
         default:
           wg_tile_real<PREC, TA, TB, TC, 2, 2, WAVES, PK>(A, B, C, blk, sa, tl.m0, tl.n0, lane,
                                                           wave,
-                                                     red);
+                                                     red, dotx, dslot,
+                                                     (int64_t)kXcds * per_xcd);
           break;
       }
     }
@@ -1052,28 +1142,40 @@ void launch_real(const GemmLaunch &g, int cls, int wg_grid, int wave_grid, hipSt
   const int P = g.per_xcd;
   const char *A = (const char *)g.A, *B = (const char *)g.B;
   char *C = (char *)g.C;
+  const double *dx = g.dot_x;
+  double *dp = g.dot_part;
   if (g.splitk && cls == TC_ROWS)
     hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_ROWS, kWaves, PK>), dim3(wg_grid),
-                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C);
+                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, nullptr,
+                       nullptr);
   else if (g.splitk == 8 && cls == TC_MIXED)
     hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED, 8, PK>), dim3(wg_grid),
-                       dim3(64 * 8), 0, s, g.tiles, P, g.blks, g.segs, A, B, C);
+                       dim3(64 * 8), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, dx, dp);
   else if (g.splitk && cls == TC_MIXED)
     hipLaunchKernelGGL((k_grouped_split<PREC, TA, TB, TC, TC_MIXED, kWaves, PK>), dim3(wg_grid),
-                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C);
+                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, dx, dp);
   else if (cls == TC_ROWS)
     hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_ROWS, PK>), dim3(wave_grid),
-                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, g.conj);
+                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, g.conj,
+                       nullptr, nullptr);
   else if (cls == TC_BIG)
     hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_BIG, PK>), dim3(wave_grid),
-                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, g.conj);
+                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, g.conj, dx,
+                       dp);
   else
     hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, false, TC_MIXED, PK>), dim3(wave_grid),
-                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, g.conj);
+                       dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs, A, B, C, g.conj, dx,
+                       dp);
 }
 
 template <int PREC, typename TA, typename TB, typename TC, bool CPLX>
 int launch_t(const GemmLaunch &g, hipStream_t s) {
+  if (g.dot_part) {  // the dot epilogue: real MFMA tile classes with an FP64 C only
+    bool ok = !CPLX && !g.single && g.tile_class != TC_ROWS && g.dot_x != nullptr;
+    if constexpr (!std::is_same<TC, double>::value) ok = false;
+    if (!ok)
+      return fail(CXA_ERR_UNSUPPORTED, "grouped GEMM: no dot epilogue for this launch");
+  }
   if (g.single) {
     if (g.a_k4 || g.b_k4 || g.c_k4)
       return fail(CXA_ERR_INVALID, "single-block GEMM: packed layouts are not supported");
@@ -1127,7 +1229,8 @@ int launch_t(const GemmLaunch &g, hipStream_t s) {
       return fail(CXA_ERR_UNSUPPORTED, "grouped GEMM: packed layouts are real-only");
     hipLaunchKernelGGL((k_grouped<PREC, TA, TB, TC, true, TC_MIXED, 0>), dim3(wave_grid),
                        dim3(64 * kWaves), 0, s, g.tiles, P, g.blks, g.segs,
-                       (const char *)g.A, (const char *)g.B, (char *)g.C, g.conj);
+                       (const char *)g.A, (const char *)g.B, (char *)g.C, g.conj, nullptr,
+                       nullptr);
   }
   CXA_HIP(hipGetLastError());
   return CXA_OK;

@@ -402,7 +402,10 @@ extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
 
 namespace {
 
-int chain_step(cxa_chain *c, int k, const void *x, void *y, hipStream_t s) {
+// dotx / dotp (last step of a real chain only): the launch's dot-product epilogue, with dotp
+// holding 2 * chain_dot_slots doubles.
+int chain_step(cxa_chain *c, int k, const void *x, void *y, hipStream_t s,
+               const double *dotx = nullptr, double *dotp = nullptr) {
   if (c->dtype != CXA_F64) {  // complex chain: FP64, every operand as the plan lays it out
     const void *in = k == 0 ? x : c->mid64[k - 1];
     void *outp = k == c->nsteps - 1 ? y : c->mid64[k];
@@ -418,11 +421,13 @@ int chain_step(cxa_chain *c, int k, const void *x, void *y, hipStream_t s) {
   void *outp = last ? y : (f32 ? c->mid32[k] : c->mid64[k]);
   const int out_dt = (last || !f32) ? CXA_F64 : CXA_F32;
   const int fx_dt = f32 ? CXA_F32 : CXA_F64;
+  cxa::ExecLayout lay;
+  lay.dot_x = dotx;
+  lay.dot_part = dotp;
   if (c->fixed_left[k])
     return bc_exec_typed(c->plans[k], f32 ? c->fixed32[k] : c->fixed[k], in, outp, fx_dt, in_dt,
-                         out_dt, c->prec, c->ws, s);
+                         out_dt, c->prec, c->ws, s, 0, &lay);
   if (c->packed_segs[k]) {
-    cxa::ExecLayout lay;
     lay.segs = (const cxa::GSeg *)c->packed_segs[k];
     lay.b_k4 = true;
     lay.a_k4 = c->a_packed[k] != 0;
@@ -434,12 +439,31 @@ int chain_step(cxa_chain *c, int k, const void *x, void *y, hipStream_t s) {
                          fx_dt, out_dt, c->prec, c->ws, s, 0, &lay);
   }
   return bc_exec_typed(c->plans[k], in, f32 ? c->fixed32[k] : c->fixed[k], outp, in_dt, fx_dt,
-                       out_dt, c->prec, c->ws, s);
+                       out_dt, c->prec, c->ws, s, 0, &lay);
 }
 
 int chain_apply(cxa_chain *c, const void *x, void *y, hipStream_t s) {
   for (int k = 0; k < c->nsteps; ++k) {
     int rc = chain_step(c, k, x, y, s);
+    if (rc) return rc;
+  }
+  return CXA_OK;
+}
+
+// Tile slots of the dot-product epilogue of the chain's last launch at its current precision,
+// or 0 when that launch cannot carry it (complex chain, mix or row-streaming launch).
+int chain_dot_slots(const cxa_chain *c) {
+  if (c->dtype != CXA_F64) return 0;
+  return bc_dot_slots(c->plans[c->nsteps - 1], c->prec);
+}
+
+// y = H x with the last launch also writing the per-slot partials of y . x and x . x into dotp
+// (x and y share the vector layout, as the Lanczos vectors do).
+int chain_apply_dot(cxa_chain *c, const void *x, void *y, double *dotp, hipStream_t s) {
+  for (int k = 0; k < c->nsteps; ++k) {
+    const bool last = k == c->nsteps - 1;
+    int rc = chain_step(c, k, x, y, s, last ? (const double *)x : nullptr,
+                        last ? dotp : nullptr);
     if (rc) return rc;
   }
   return CXA_OK;
@@ -611,6 +635,11 @@ struct cxa_lanczos {
          *coef = nullptr;
   int *counter = nullptr;
   double *hbuf = nullptr;  // pinned: alpha, beta history and Ritz coefficients
+  // A run whose matvec is a real chain with a dot-capable last launch takes the fused form:
+  // that launch writes the partials of Hu . u and u . u (2 * fused doubles in dotp) and one
+  // residual launch follows, instead of two vector launches.
+  double *dotp = nullptr;
+  int dot_cap = 0, fused = 0;
   hipStream_t own = nullptr;
   // captured iterations: one (gexec1) and `gbatch` back to back (gexecB)
   hipGraphExec_t gexec1 = nullptr, gexecB = nullptr;
@@ -623,7 +652,7 @@ struct cxa_lanczos {
   }
   ~cxa_lanczos() {
     drop_graphs();
-    for (double *p : {V, w, cur, part, sc, coef})
+    for (double *p : {V, w, cur, part, sc, coef, dotp})
       if (p) (void)hipFree(p);
     if (counter) (void)hipFree(counter);
     if (hbuf) (void)hipHostFree(hbuf);
@@ -672,9 +701,19 @@ int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, bool c
                  hipStream_t s) {
   const IterIdx iter{it < 0 ? L->counter : nullptr, it < 0 ? 0 : it};
   double *part_a = L->part, *part_b = L->part + kPartials;
+  int rc;
+  if (L->fused > 0) {  // the chain's last launch leaves the Hu . u and u . u partials
+    if ((rc = chain_apply_dot((cxa_chain *)ctx, L->cur, L->w, L->dotp, s))) return rc;
+    if ((rc = vec_residual_fused(L->w, L->cur, L->V, L->n, L->dotp, L->fused, L->As(), L->Bs(),
+                                 iter, part_b, s)))
+      return rc;
+    if (reorth && (rc = vec_reorth(L->cur, L->V, L->n, L->maxiter + 1, iter, L->part, L->h(),
+                                   part_b, cplx, s)))
+      return rc;
+    return it < 0 ? vec_counter_add(L->counter, 1, s) : CXA_OK;
+  }
   // cur holds the previous residual u (V[0] at it = 0); w = H u
-  int rc = fn(ctx, L->cur, L->w, (cxa_stream_t)s);
-  if (rc) return rc;
+  if ((rc = fn(ctx, L->cur, L->w, (cxa_stream_t)s))) return rc;
   // beta of u and V[it] = u / beta with the alpha partials; alpha with the residual (into cur);
   // (re-orthogonalisation of the residual against V[0..it])
   if ((rc = vec_dot_basis(L->w, L->cur, L->V, L->n, part_b, L->Bs(), iter, part_a, s))) return rc;
@@ -746,6 +785,17 @@ int run_end(RunStream &r) {
 int run_prologue(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, const double *Tin, int reorth,
                  bool cplx, bool use_graph, int batch, int maxiter, hipStream_t ws) {
   int rc;
+  // the fused iteration for real chains with a dot-capable last launch (decided per run: the
+  // chain's precision, hence its launch, may have changed since the last one)
+  L->fused = (!cplx && matvec == cxa_chain_matvec) ? chain_dot_slots((const cxa_chain *)ctx) : 0;
+  if (L->fused > L->dot_cap) {
+    CXA_HIP(hipDeviceSynchronize());  // an earlier run's launches may still read the old buffer
+    if (L->dotp) CXA_HIP(hipFree(L->dotp));
+    L->dotp = nullptr;
+    L->dot_cap = 0;
+    CXA_HIP(hipMalloc(&L->dotp, sizeof(double) * 2 * L->fused));
+    L->dot_cap = L->fused;
+  }
   if ((rc = vec_counter_set(L->counter, 0, ws))) return rc;
   if ((rc = vec_dot(Tin, Tin, L->n, L->part, L->nrm(), nullptr, 0, 1, ws))) return rc;
   if ((rc = vec_normalize_init(L->cur, L->V, Tin, L->n, L->nrm(), ws))) return rc;
@@ -916,9 +966,20 @@ extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
   const double *hA = L->hbuf, *hB = L->hbuf + m1;
   std::vector<double> z(m1);
   double E = 0.0, eig = 0.0, dE = 0.0;
+  std::vector<double> dEs;  // |E_j - E_{j-1}| per checked iteration j >= 1
   int done = -1, it = 0;
   while (done < 0 && it < maxiter) {
-    const int nb = std::min(batch, maxiter - it);
+    // The next batch stops near the predicted convergence point: |dE| has fallen geometrically
+    // over the last four iterations at rate r, so about log(CvgCrit / |dE|) / log(r) more are
+    // needed.  A batch that is too short costs one more host check; one that runs past the stop
+    // costs whole iterations.  Results do not depend on the batch sizes.
+    int nb = std::min(batch, maxiter - it);
+    const size_t nd = dEs.size();
+    if (nb > 2 && nd >= 5 && dEs[nd - 1] > 0 && dEs[nd - 5] > dEs[nd - 1]) {
+      const double r = std::pow(dEs[nd - 1] / dEs[nd - 5], 0.25);
+      const double need = std::log(CvgCrit / dEs[nd - 1]) / std::log(r);
+      if (need < nb - 1) nb = std::max(2, (int)std::ceil(need) + 1);
+    }
     if ((rc = run_batch(L, matvec, ctx, reorth, cplx, use_graph, it, nb, ws))) return rc;
     const int upto = it + nb;  // alphas/betas [0, upto) are valid after the batch
     for (int j = it; j < upto && done < 0; ++j) {
@@ -943,6 +1004,7 @@ extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
         break;
       }
       dE = std::fabs(E - eig);
+      dEs.push_back(dE);
       E = eig;
     }
     it = upto;

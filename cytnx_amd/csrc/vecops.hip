@@ -122,6 +122,43 @@ __global__ __launch_bounds__(kT) void k_residual(const double *__restrict__ y,
   if (threadIdx.x == 0) part_b[blockIdx.x] = s;
 }
 
+// The residual launch when the matvec's last launch has already produced, per tile slot, the
+// partials of (H u) . u and of u . u (the chain's dot-product epilogue, dotp[0..S) and
+// dotp[S..2S)): beta = |u| (1 at it = 0), alpha = (Hu . u) / beta^2 (recorded in As[it], beta
+// in Bs[it - 1] by block 0), V[it] = u / beta, and the residual u = Hu / beta - alpha V[it] -
+// beta V[it - 1] with its |u|^2 partials into part_b.  This replaces k_dot_basis + k_residual:
+// one launch and one read of u fewer per iteration.
+__global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict__ y,
+                                                       double *__restrict__ u,
+                                                       double *__restrict__ V, int64_t n,
+                                                       const double *__restrict__ dotp, int S,
+                                                       double *__restrict__ As,
+                                                       double *__restrict__ Bs, IterIdx iter,
+                                                       double *__restrict__ part_b) {
+  const int it = iter.get();
+  const double wu = all_sum_partials(dotp, S);
+  const double uu = all_sum_partials(dotp + S, S);
+  const double b = it > 0 ? sqrt(uu) : 1.0;
+  const double a = (wu / b) / b;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    As[it] = a;
+    if (it > 0) Bs[it - 1] = b;
+  }
+  double *cur = V + (int64_t)it * n;
+  const double *prev = it > 0 ? V + (int64_t)(it - 1) * n : cur;
+  const double bp = it > 0 ? b : 0.0;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    const double vi = u[i] / b;
+    if (it > 0) cur[i] = vi;
+    const double t = y[i] / b - (a * vi + bp * prev[i]);
+    u[i] = t;
+    s += t * t;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) part_b[blockIdx.x] = s;
+}
+
 // h_j partial sums: part[j * P + block] = sum over the block's slice of V_j * w, j <= it.
 // Block (x, y) takes the basis vectors in groups of four, groups y, y + gridDim.y, ...: with the
 // iteration known at launch there is one row of blocks per group; a captured launch (iteration
@@ -414,6 +451,15 @@ int vec_residual(const double *y, double *u, const double *V, int64_t n, const d
                  double *As, const double *Bs, IterIdx iter, double *part_b, hipStream_t s) {
   hipLaunchKernelGGL(k_residual, dim3(kPartials), dim3(kT), 0, s, y, u, V, n, part_a, As, Bs, iter,
                      part_b);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
+int vec_residual_fused(const double *y, double *u, double *V, int64_t n, const double *dotp,
+                       int S, double *As, double *Bs, IterIdx iter, double *part_b,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_residual_fused, dim3(kPartials), dim3(kT), 0, s, y, u, V, n, dotp, S, As,
+                     Bs, iter, part_b);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }

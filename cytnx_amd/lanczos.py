@@ -10,8 +10,8 @@ The iteration itself runs in the C++ driver (cxa_lanczos_run) on device vectors:
   PyLinOp trampoline (pybind/linop_py.cpp:22-41).
 
 ``ground_state_mixed`` is the FP32 + refinement scheme: Lanczos with the contraction bulk on
-FP32 MFMA (vectors, scalars and re-orthogonalisation in FP64) down to the FP32 criterion, then
-a short FP64 Lanczos restarted from that Ritz vector.
+FP32 MFMA (vectors and scalars in FP64) to three decades above the caller's criterion, then a
+short FP64 Lanczos restarted from that Ritz vector.
 """
 from __future__ import annotations
 
@@ -24,7 +24,6 @@ import torch
 from . import _lib
 from .device import stream_handle
 
-FLOAT_CVG_FLOOR = 1.0e-7  # Lanczos_Gnd_Ut.cpp:185-193 caps CvgCrit for float operators
 
 
 class LinOp:
@@ -414,21 +413,28 @@ def Lanczos_Exp(Hop: LinOp, v, tau, CvgCrit: float = 1.0e-14, Maxiter: int = 100
 
 
 def ground_state_mixed(op, psi0: torch.Tensor, CvgCrit: float = 1e-12, Maxiter: int = 1000,
-                       batch: int = 8, graph: bool = False, fp32_cvg: float = FLOAT_CVG_FLOOR):
+                       batch: int = 16, graph: bool = False, fp32_cvg: Optional[float] = None,
+                       reorth: bool = False):
     """FP32 + refinement: (E, evec, n_fp32_iter, n_fp64_iter).
 
-    Phase 1 runs the contraction bulk on FP32 MFMA with FP64 vectors and FP64 full
-    re-orthogonalisation until |dE| < fp32_cvg; phase 2 restarts an FP64 Lanczos from the phase-1
-    Ritz vector with the caller's CvgCrit.  The eigensolve, norms and orthogonalisation are FP64
-    throughout."""
+    Phase 1 is the Lanczos iteration with the contraction bulk on FP32 MFMA (the vectors, the
+    recurrence scalars, the tridiagonal eigensolve and the Ritz vector in FP64) until
+    |dE| < fp32_cvg, by default 1000 CvgCrit: the Ritz vector of an FP32 operator keeps
+    converging after its energy has stopped improving (that energy carries the FP32 operator's
+    ~1e-7 relative error), so the switch comes three decades above the target.  Phase 2 is the
+    plain FP64 Lanczos restarted from that Ritz vector, stopped by the reference's rule
+    |dE| < CvgCrit (Lanczos_Gnd_Ut.cpp:140-162); from a good start vector it needs a few
+    iterations.  ``reorth`` applies to both phases (the reference's driver has none)."""
     from .unitensor import PREC_FP32, PREC_FP64
+    if fp32_cvg is None:
+        fp32_cvg = 1000.0 * CvgCrit
     prev = op.precision
     try:
         op.set_precision(PREC_FP32)
         _, v1, n1, _, _ = lanczos_gnd_native(op, psi0, max(fp32_cvg, CvgCrit), Maxiter, batch,
-                                             reorth=True, graph=graph)
+                                             reorth=reorth, graph=graph)
         op.set_precision(PREC_FP64)
-        E, v2, n2, _, _ = lanczos_gnd_native(op, v1, CvgCrit, Maxiter, batch, reorth=True,
+        E, v2, n2, _, _ = lanczos_gnd_native(op, v1, CvgCrit, Maxiter, batch, reorth=reorth,
                                              graph=graph)
     finally:
         op.set_precision(prev)

@@ -251,6 +251,37 @@ def test_lanczos_batching_and_graph_do_not_change_results():
     assert torch.equal(r1[1], r3[1])
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fused_iteration_matches_two_launch_iteration(prec):
+    """A chain matvec runs the fused iteration (the .R launch's epilogue leaves the Hu . u and
+    u . u partials, one residual launch follows); the same operator behind a plain callback runs
+    the two vector launches.  Same iterations, alphas and betas to rounding, same energy."""
+    import ctypes
+    from cytnx_amd import _lib
+    from cytnx_amd.lanczos import _MATVEC_T
+    psi, L, M1, M2, R = problem(96, seed=29)
+    H = Heff2(psi, L, M1, M2, R)
+    H.set_precision(prec)
+    x0 = psi._storage.clone()
+    E1, v1, k1, a1, b1 = lanczos_gnd_native(H, x0, 1e-12, 400, batch=8)
+    n = x0.numel()
+
+    def cb(_ctx, xp, yp, _s):
+        _lib.lib().cxa_chain_matvec(H.chain.handle, xp, yp, _s)
+        return 0
+
+    f = _MATVEC_T(cb)
+    fn = ctypes.cast(f, ctypes.c_void_p).value
+    E2, v2, k2, a2, b2 = lanczos_gnd_native(None, x0, 1e-12, 400, batch=8, fn_ctx=(fn, None))
+    H.set_precision(0)
+    print("prec", prec, "fused", k1, E1, "two-launch", k2, E2, flush=True)
+    assert k1 == k2
+    np.testing.assert_allclose(a1, a2, rtol=1e-11, atol=1e-11 * np.abs(a2).max())
+    np.testing.assert_allclose(b1, b2, rtol=1e-9, atol=1e-11 * np.abs(b2).max())
+    assert abs(E1 - E2) <= 1e-11 * abs(E2)
+    assert torch.linalg.vector_norm(v1 - v2).item() <= 1e-6
+
+
 def test_lanczos_matches_oracle_recurrence_on_heff2():
     psi, L, M1, M2, R = problem(16, seed=17)
     H = Heff2(psi, L, M1, M2, R)
