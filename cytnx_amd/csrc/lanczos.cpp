@@ -707,9 +707,10 @@ int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, bool c
     if ((rc = vec_residual_fused(L->w, L->cur, L->V, L->n, L->dotp, L->fused, L->As(), L->Bs(),
                                  iter, part_b, s)))
       return rc;
-    if (reorth && (rc = vec_reorth(L->cur, L->V, L->n, L->maxiter + 1, iter, L->part, L->h(),
-                                   part_b, cplx, s)))
-      return rc;
+    for (int pass = 0; pass < reorth; ++pass)
+      if ((rc = vec_reorth(L->cur, L->V, L->n, L->maxiter + 1, iter, L->part, L->h(), part_b,
+                           cplx, s)))
+        return rc;
     return it < 0 ? vec_counter_add(L->counter, 1, s) : CXA_OK;
   }
   // cur holds the previous residual u (V[0] at it = 0); w = H u
@@ -719,9 +720,10 @@ int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, bool c
   if ((rc = vec_dot_basis(L->w, L->cur, L->V, L->n, part_b, L->Bs(), iter, part_a, s))) return rc;
   if ((rc = vec_residual(L->w, L->cur, L->V, L->n, part_a, L->As(), L->Bs(), iter, part_b, s)))
     return rc;
-  if (reorth && (rc = vec_reorth(L->cur, L->V, L->n, L->maxiter + 1, iter, L->part, L->h(),
-                                 part_b, cplx, s)))
-    return rc;
+  for (int pass = 0; pass < reorth; ++pass)  // classical Gram-Schmidt passes
+    if ((rc = vec_reorth(L->cur, L->V, L->n, L->maxiter + 1, iter, L->part, L->h(), part_b, cplx,
+                         s)))
+      return rc;
   return it < 0 ? vec_counter_add(L->counter, 1, s) : CXA_OK;
 }
 
@@ -1156,18 +1158,85 @@ extern "C" int cxa_lanczos_exp(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
   return run_end(r);
 }
 
-// k extreme eigenpairs of a symmetric operator: which 0 = "SA" (smallest algebraic), 1 = "LA"
-// (largest algebraic), 2 = "LM" (largest magnitude), the contract of linalg::Lanczos(Hop, Tin,
-// which, ...) (src/linalg/Lanczos.cpp:249-402).  Lanczos with full FP64 re-orthogonalisation
-// from Tin / |Tin| (no restarts: the basis grows to at most min(Maxiter, n) vectors); after each
-// batch the Ritz pairs of T_m are checked with the residual bound |beta_m z_i[m - 1]| <=
-// tol max(eps^(2/3), |theta_i|) (tol <= 0: machine epsilon), the test of ARPACK's dsconv.  The
-// run also ends at an invariant subspace holding k pairs.  Eigenvalues sorted as the reference
-// sorts them (SA ascending, LA descending, LM by descending magnitude); evecs (device, may be
-// NULL) holds the k Ritz vectors as rows; nconv = pairs that met the bound.
+namespace {
+
+// Eigenpairs of a dense symmetric m x m matrix A (row-major, overwritten) by the cyclic Jacobi
+// method (Golub & Van Loan, Matrix Computations, 4th ed., Algorithm 8.5.3: 2 x 2 symmetric Schur
+// rotations in row order, sweeps until the off-diagonal mass is below eps times the norm).
+// w ascending; Z row-major with Z[i * m + k] = component i of the eigenvector of w[k].  Used for
+// the projected matrix of a restarted Lanczos (a tridiagonal with an arrow), of order ncv.
+void sym_eig_jacobi(int m, std::vector<double> &A, std::vector<double> &w,
+                    std::vector<double> &Z) {
+  const double eps = 2.220446049250313e-16;
+  std::vector<double> V((size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i) V[(size_t)i * m + i] = 1.0;
+  double fro = 0.0;
+  for (double x : A) fro += x * x;
+  fro = std::sqrt(fro);
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < m; ++p)
+      for (int q = p + 1; q < m; ++q) off += 2.0 * A[(size_t)p * m + q] * A[(size_t)p * m + q];
+    if (std::sqrt(off) <= eps * fro) break;
+    for (int p = 0; p < m; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        const double apq = A[(size_t)p * m + q];
+        if (apq == 0.0) continue;
+        const double app = A[(size_t)p * m + p], aqq = A[(size_t)q * m + q];
+        const double tau = (aqq - app) / (2.0 * apq);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+        const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c;
+        for (int r = 0; r < m; ++r) {  // A <- A J (columns p, q)
+          const double arp = A[(size_t)r * m + p], arq = A[(size_t)r * m + q];
+          A[(size_t)r * m + p] = c * arp - sn * arq;
+          A[(size_t)r * m + q] = sn * arp + c * arq;
+        }
+        for (int r = 0; r < m; ++r) {  // A <- J^T A (rows p, q)
+          const double apr = A[(size_t)p * m + r], aqr = A[(size_t)q * m + r];
+          A[(size_t)p * m + r] = c * apr - sn * aqr;
+          A[(size_t)q * m + r] = sn * apr + c * aqr;
+        }
+        A[(size_t)p * m + q] = A[(size_t)q * m + p] = 0.0;
+        for (int r = 0; r < m; ++r) {  // V <- V J
+          const double vrp = V[(size_t)r * m + p], vrq = V[(size_t)r * m + q];
+          V[(size_t)r * m + p] = c * vrp - sn * vrq;
+          V[(size_t)r * m + q] = sn * vrp + c * vrq;
+        }
+      }
+  }
+  std::vector<int> ord(m);
+  for (int i = 0; i < m; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(),
+                   [&](int x, int y) { return A[(size_t)x * m + x] < A[(size_t)y * m + y]; });
+  w.resize(m);
+  Z.assign((size_t)m * m, 0.0);
+  for (int k = 0; k < m; ++k) {
+    w[k] = A[(size_t)ord[k] * m + ord[k]];
+    for (int i = 0; i < m; ++i) Z[(size_t)i * m + k] = V[(size_t)i * m + ord[k]];
+  }
+}
+
+}  // namespace
+
+// k extreme eigenpairs of a symmetric (complex Hermitian with flags bit 2) operator: which 0 =
+// "SA" (smallest algebraic), 1 = "LA" (largest algebraic), 2 = "LM" (largest magnitude), the
+// contract of linalg::Lanczos(Hop, Tin, which, maxiter, cvg_crit, k, is_V, ncv)
+// (src/linalg/Lanczos.cpp:249-402), which runs ARPACK's implicitly restarted Lanczos with ncv
+// basis vectors (default min(n, 2k + 10)) for at most maxiter restart cycles.  Here: a thick-
+// restart Lanczos (Wu & Simon, SIAM J. Matrix Anal. Appl. 22 (2000) 602-616) with full FP64
+// re-orthogonalisation.  A cycle extends the basis to ncv vectors; the projected matrix (a
+// tridiagonal, with an arrow after a restart) is diagonalised on the host, and the Ritz pairs
+// are accepted with ARPACK's dsconv test |beta_m y_i[m - 1]| <= tol max(eps^(2/3), |theta_i|)
+// (tol <= 0: machine epsilon).  Otherwise the basis restarts from the k + (ncv - k) / 2 most
+// wanted Ritz vectors (one GEMM over the basis) and the residual; their couplings to it,
+// beta_m y_i[m - 1], are the arrow.  A start vector in an invariant subspace of at least k
+// dimensions ends the run early.  Eigenvalues in the reference's order (SA ascending, LA
+// descending, LM by descending magnitude); evecs (device, may be NULL): the k Ritz vectors as
+// rows; niter = matvecs applied; nconv = pairs within the bound.  The context must hold
+// 2 * ncv basis vectors (ncv for the basis, the rest for the restart's product).
 extern "C" int cxa_lanczos_eigsh(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx,
                                  const double *Tin, int which, int k, double tol, int Maxiter,
-                                 int flags, double *evals, double *evecs, int *niter,
+                                 int ncv, int flags, double *evals, double *evecs, int *niter,
                                  int *nconv, cxa_stream_t stream) {
   CXA_CHECK(L && matvec && Tin && evals, CXA_ERR_INVALID, "cxa_lanczos_eigsh: bad arguments");
   CXA_CHECK(which >= 0 && which <= 2, CXA_ERR_INVALID,
@@ -1181,24 +1250,24 @@ extern "C" int cxa_lanczos_eigsh(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx
   CXA_CHECK(k <= ne, CXA_ERR_INVALID,
             "[ERROR][Lanczos] k can only be up to total dimension of input vector D");
   CXA_CHECK(Maxiter >= 1, CXA_ERR_INVALID, "cxa_lanczos_eigsh: Maxiter must be positive");
-  const int cap = (int)std::min<int64_t>({(int64_t)Maxiter, ne, (int64_t)L->maxiter});
-  CXA_CHECK(cap >= k, CXA_ERR_INVALID,
-            "cxa_lanczos_eigsh: %d Krylov vectors allowed, fewer than k = %d", cap, k);
+  CXA_CHECK(ncv == 0 || (ncv >= k + 2 && ncv <= ne) || (ncv == ne && ne < k + 2),
+            CXA_ERR_INVALID, "[ERROR][Lanczos] ncv should be 2+k<=ncv<=nx");
+  const int m = (int)std::min<int64_t>(ne, ncv > 0 ? ncv : 2 * (int64_t)k + 10);
+  const int full = m == ne;  // the basis spans the space: one cycle, no restart
+  CXA_CHECK(2 * m <= L->maxiter + 1, CXA_ERR_INVALID,
+            "cxa_lanczos_eigsh: the context holds %d vectors, a basis of %d needs %d",
+            L->maxiter + 1, m, 2 * m);
   const double eps = 2.220446049250313e-16, eps23 = std::cbrt(eps * eps);
   if (!(tol > 0)) tol = eps;
   RunStream r;
   int rc;
   if ((rc = run_begin(L, false, (hipStream_t)stream, r))) return rc;
   const hipStream_t ws = r.ws;
-  if ((rc = run_prologue(L, matvec, ctx, Tin, 1, cplx, false, 1, cap, ws))) return rc;
+  if ((rc = run_prologue(L, matvec, ctx, Tin, 1, cplx, false, 1, m, ws))) return rc;
   const int m1 = L->maxiter + 1;
   const double *hA = L->hbuf, *hB = L->hbuf + m1;
-  std::vector<double> w, Z;
-  std::vector<int> pick;
-  int m = 0, conv = 0, it = 0;
-  const int batch = std::max(8, k);
-  // the wanted k of the m Ritz values, in the reference's output order
-  auto select = [&](int mm) {
+  // the wanted Ritz values in the reference's output order (first `cnt` of the m)
+  auto select = [&](const std::vector<double> &w, int mm, int cnt) {
     std::vector<int> idx(mm);
     for (int i = 0; i < mm; ++i) idx[i] = i;
     std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
@@ -1206,49 +1275,122 @@ extern "C" int cxa_lanczos_eigsh(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx
       if (which == 1) return w[x] > w[y];
       return std::fabs(w[x]) > std::fabs(w[y]);
     });
-    idx.resize(std::min(k, mm));
+    idx.resize(std::min(cnt, mm));
     return idx;
   };
+  std::vector<double> theta, svec;  // kept Ritz values and their arrow couplings
+  std::vector<double> w, Z, T;
+  std::vector<int> pick;
+  double *dY = nullptr;  // the restart's coefficient matrix (kept x m) on the device
+  int start = 0, mm = m, conv = 0, matvecs = 0, cycles = 0;
+  const int batch = 16;
   for (;;) {
-    const int nb = std::min(batch, cap - it);
-    if ((rc = run_batch(L, matvec, ctx, 1, cplx, false, it, nb, ws))) return rc;
-    const int upto = it + nb;
-    // an invariant subspace ends the basis early: beta_j at round-off level
-    m = upto;
-    double tnorm = 0.0;
-    for (int j = 0; j < upto; ++j)
-      tnorm = std::max(tnorm, std::fabs(hA[j]) + std::fabs(hB[j]) + (j ? std::fabs(hB[j - 1]) : 0));
+    // extend the basis from `start` to m vectors (the first vector after a restart gets a
+    // second Gram-Schmidt pass: its residual sheds the arrow's components)
     bool invariant = false;
-    for (int j = 0; j < upto; ++j)
-      if (!(hB[j] > 100.0 * eps * std::max(tnorm, 1.0) * std::sqrt((double)(j + 1)))) {
-        m = j + 1;
-        invariant = true;
+    mm = m;
+    for (int it = start; it < m;) {
+      const int nb = std::min(batch, m - it);
+      for (int b = 0; b < nb && rc == CXA_OK; ++b)
+        rc = lanczos_body(L, matvec, ctx, (it + b == start && start > 0) ? 2 : 1, cplx, it + b,
+                          ws);
+      if (rc) break;
+      matvecs += nb;
+      it += nb;
+      rc = vec_finalize(L->part + kPartials, L->Bs(), nullptr, it - 1, 1, ws);
+      if (rc) break;
+      CXA_HIP(hipMemcpyAsync(L->hbuf, L->As(), sizeof(double) * it, hipMemcpyDeviceToHost, ws));
+      CXA_HIP(hipMemcpyAsync(L->hbuf + m1, L->Bs(), sizeof(double) * it, hipMemcpyDeviceToHost,
+                             ws));
+      CXA_HIP(hipStreamSynchronize(ws));
+      // an invariant subspace ends the basis early: beta_j at round-off level
+      double tnorm = 0.0;
+      for (int j = 0; j < (int)theta.size(); ++j)
+        tnorm = std::max(tnorm, std::fabs(theta[j]) + std::fabs(svec[j]));
+      for (int j = start; j < it; ++j)
+        tnorm = std::max(tnorm, std::fabs(hA[j]) + std::fabs(hB[j]) +
+                                    (j > start ? std::fabs(hB[j - 1]) : 0.0));
+      for (int j = start; j < it; ++j)
+        if (!(hB[j] > 100.0 * eps * std::max(tnorm, 1.0) * std::sqrt((double)(j + 1)))) {
+          mm = j + 1;
+          invariant = true;
+          break;
+        }
+      if (invariant) break;
+    }
+    if (rc) break;
+    // the projected matrix: kept Ritz values with their arrow, then the new tridiagonal part
+    const int ks = start;
+    T.assign((size_t)mm * mm, 0.0);
+    for (int i = 0; i < ks; ++i) {
+      T[(size_t)i * mm + i] = theta[i];
+      if (ks < mm) T[(size_t)i * mm + ks] = T[(size_t)ks * mm + i] = svec[i];
+    }
+    for (int j = ks; j < mm; ++j) {
+      T[(size_t)j * mm + j] = hA[j];
+      if (j + 1 < mm) T[(size_t)j * mm + j + 1] = T[(size_t)(j + 1) * mm + j] = hB[j];
+    }
+    sym_eig_jacobi(mm, T, w, Z);
+    pick = select(w, mm, k);
+    const double bm = invariant ? 0.0 : hB[mm - 1];
+    conv = 0;
+    for (int i : pick)
+      if (std::fabs(bm * Z[(size_t)(mm - 1) * mm + i]) <= tol * std::max(eps23, std::fabs(w[i])))
+        ++conv;
+    ++cycles;
+    if ((int)pick.size() == k && conv == k) break;
+    if (invariant) {
+      if (mm < k) rc = fail(CXA_ERR_NUMERIC, "[ERROR][Lanczos] the start vector spans an "
+                                             "invariant subspace of dimension %d < k = %d",
+                            mm, k);
+      break;
+    }
+    if (full || cycles >= Maxiter) break;
+    // thick restart: U = V Y for the `kept` most wanted Ritz vectors, written over V[0..kept);
+    // the residual (cur, its norm partials) continues as the next basis vector
+    const int kept = std::min(m - 1, k + (m - k) / 2);
+    std::vector<int> keep = select(w, mm, kept);
+    std::vector<double> Yt((size_t)kept * mm);
+    theta.assign(kept, 0.0);
+    svec.assign(kept, 0.0);
+    for (int i = 0; i < kept; ++i) {
+      theta[i] = w[keep[i]];
+      svec[i] = bm * Z[(size_t)(mm - 1) * mm + keep[i]];
+      for (int j = 0; j < mm; ++j) Yt[(size_t)i * mm + j] = Z[(size_t)j * mm + keep[i]];
+    }
+    if (!dY) {
+      if (hipMalloc(&dY, sizeof(double) * (size_t)m * m) != hipSuccess) {
+        rc = fail(CXA_ERR_NOMEM, "cxa_lanczos_eigsh: restart coefficients");
         break;
       }
-    tridiag_qr(m, hA, hB, w, {m - 1}, Z);  // eigenvalues and the last row of Z
-    pick = select(m);
-    conv = 0;
-    const double bm = invariant ? 0.0 : hB[m - 1];
-    for (int i : pick)
-      if (std::fabs(bm * Z[i]) <= tol * std::max(eps23, std::fabs(w[i]))) ++conv;
-    it = upto;
-    if ((int)pick.size() == k && conv == k) break;
-    CXA_CHECK(!(invariant && m < k), CXA_ERR_NUMERIC,
-              "[ERROR][Lanczos] the start vector spans an invariant subspace of dimension %d < k "
-              "= %d",
-              m, k);
-    if (invariant || it >= cap) break;
+    }
+    double *tmp = L->V + (size_t)m * L->n;  // rows m .. m + kept - 1 of the context
+    if (hipMemcpyAsync(dY, Yt.data(), sizeof(double) * Yt.size(), hipMemcpyHostToDevice, ws) !=
+        hipSuccess) {
+      rc = fail(CXA_ERR_HIP, "cxa_lanczos_eigsh: coefficient upload failed");
+      break;
+    }
+    // tmp (kept x n) = Yt (kept x mm) . V (mm x n), every basis vector a row of n doubles
+    if ((rc = cxa_matmul(CXA_F64, tmp, dY, L->V, kept, mm, L->n, (cxa_stream_t)ws))) break;
+    if (hipMemcpyAsync(L->V, tmp, sizeof(double) * (size_t)kept * L->n, hipMemcpyDeviceToDevice,
+                       ws) != hipSuccess ||
+        hipStreamSynchronize(ws) != hipSuccess) {  // Yt is read by the upload above
+      rc = fail(CXA_ERR_HIP, "cxa_lanczos_eigsh: restart copy failed");
+      break;
+    }
+    start = kept;
   }
-  for (int i = 0; i < k; ++i) evals[i] = w[pick[i]];
+  if (dY) (void)hipFree(dY);
+  if (rc) return rc;
+  for (int i = 0; i < k && i < (int)pick.size(); ++i) evals[i] = w[pick[i]];
   if (evecs) {
-    tridiag_qr_full(m, hA, hB, w, Z);  // the same eigenvalue order as the checks
-    std::vector<double> y(m);
-    for (int i = 0; i < k; ++i) {
-      for (int j = 0; j < m; ++j) y[j] = Z[(size_t)j * m + pick[i]];
-      if ((rc = combine_host_coef(L, y.data(), m, evecs + (size_t)i * L->n, 0, ws))) return rc;
+    std::vector<double> y(mm);
+    for (int i = 0; i < k && i < (int)pick.size(); ++i) {
+      for (int j = 0; j < mm; ++j) y[j] = Z[(size_t)j * mm + pick[i]];
+      if ((rc = combine_host_coef(L, y.data(), mm, evecs + (size_t)i * L->n, 0, ws))) return rc;
     }
   }
-  if (niter) *niter = m;
+  if (niter) *niter = matvecs;
   if (nconv) *nconv = conv;
   return run_end(r);
 }

@@ -240,25 +240,25 @@ _WHICH = {"SA": 0, "LA": 1, "LM": 2}
 
 
 def eigsh_native(op, psi0: torch.Tensor, which: str = "SA", k: int = 1, tol: float = 0.0,
-                 want_vectors: bool = True, fn_ctx=None, Maxiter: int = 10000):
+                 want_vectors: bool = True, fn_ctx=None, Maxiter: int = 10000, ncv: int = 0):
     """k extreme eigenpairs of a flat FP64 or complex128 (Hermitian) device operator
-    (cxa_lanczos_eigsh).  Returns (eigenvalues (numpy, reference order), eigenvectors (k x n
-    device tensor of psi0's dtype, or None), Krylov vectors used, pairs within the tolerance).
-    ``op`` is a planned Heff unless ``fn_ctx`` gives (fnptr, ctx).  The basis grows to at most
-    min(Maxiter, n) vectors (no restarts), bounded by the device memory budget."""
+    (cxa_lanczos_eigsh: thick-restart Lanczos with ncv basis vectors, 0 = min(n, 2k + 10), for at
+    most Maxiter restart cycles).  Returns (eigenvalues (numpy, reference order), eigenvectors
+    (k x n device tensor of psi0's dtype, or None), matvecs applied, pairs within the
+    tolerance).  ``op`` is a planned Heff unless ``fn_ctx`` gives (fnptr, ctx)."""
     x, cplx = _doubles(psi0, "Lanczos")
     n = x.numel()
     ne = psi0.numel()
     dev = psi0.device.index
-    cap = max(2, _krylov_cap(n, min(int(Maxiter), ne)))
-    ctx = _context(n, cap, dev)
+    m = min(ne, int(ncv) if ncv else 2 * int(k) + 10)
+    ctx = _context(n, max(2, 2 * m), dev)
     fn, mctx = fn_ctx if fn_ctx is not None else op.native_matvec()
     evals = np.zeros(k)
     evecs = torch.empty((k, n), dtype=torch.float64, device=psi0.device) if want_vectors else None
     niter = ctypes.c_int(0)
     nconv = ctypes.c_int(0)
     _lib.check(_lib.lib().cxa_lanczos_eigsh(
-        ctx, fn, mctx, x.data_ptr(), _WHICH[which], int(k), float(tol), int(min(cap, ne)),
+        ctx, fn, mctx, x.data_ptr(), _WHICH[which], int(k), float(tol), int(Maxiter), int(ncv),
         CPLX_FLAG if cplx else 0, evals.ctypes.data_as(_lib.dblp),
         evecs.data_ptr() if evecs is not None else None, ctypes.byref(niter),
         ctypes.byref(nconv), stream_handle(dev)))
@@ -364,7 +364,7 @@ def _lanczos_which(Hop, Tin, which, Maxiter, CvgCrit, k, is_V, ncv, verbose):
     fn_ctx, keep = _matvec_of(Hop, tmpl)
     try:
         evals, evecs, niter, nconv = eigsh_native(None, x0, which, k, CvgCrit, is_V,
-                                                  fn_ctx=fn_ctx, Maxiter=Maxiter)
+                                                  fn_ctx=fn_ctx, Maxiter=Maxiter, ncv=ncv)
     except _lib.CxaError as err:
         _reraise_matvec(err)
     del keep

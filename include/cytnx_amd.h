@@ -344,18 +344,22 @@ int cxa_lanczos_exp(cxa_lanczos *ctx, cxa_matvec_fn matvec, void *matvec_ctx, co
 
 /*
  * k extreme eigenpairs of a real symmetric (flags bit 2: complex Hermitian, C128 vectors of
- * n / 2 values) operator: linalg::Lanczos(Hop, Tin, which, maxiter,
- * cvg_crit, k, is_V, ncv) (src/linalg/Lanczos.cpp:249-402, :453-583), which the reference runs
- * with ARPACK dsaupd/dseupd.  which: 0 = "SA", 1 = "LA", 2 = "LM".  Lanczos with full FP64
- * re-orthogonalisation (no restarts: up to min(Maxiter, n, context) basis vectors); converged
- * when every wanted Ritz pair has |beta_m z_i[m-1]| <= tol max(eps^(2/3), |theta_i|) (ARPACK's
- * dsconv test; tol <= 0 means machine epsilon).  evals[k] (host) ordered as the reference orders
- * them (SA ascending, LA descending, LM by descending magnitude); evecs (device, may be NULL,
- * k x n) the Ritz vectors as rows; niter = basis size; nconv = pairs within the bound.
+ * n / 2 values) operator: linalg::Lanczos(Hop, Tin, which, maxiter, cvg_crit, k, is_V, ncv)
+ * (src/linalg/Lanczos.cpp:249-402, :453-583), which the reference runs with ARPACK's implicitly
+ * restarted Lanczos (dsaupd/dseupd).  which: 0 = "SA", 1 = "LA", 2 = "LM".  A thick-restart
+ * Lanczos with full FP64 re-orthogonalisation: cycles extend the basis to ncv vectors (0: the
+ * reference's default min(n, 2k + 10); otherwise k + 2 <= ncv <= n), restart from the
+ * k + (ncv - k) / 2 most wanted Ritz vectors, and stop when every wanted Ritz pair has
+ * |beta_m y_i[m-1]| <= tol max(eps^(2/3), |theta_i|) (ARPACK's dsconv test; tol <= 0 means
+ * machine epsilon) or after Maxiter cycles (the reference's maxiter counts ARPACK's restart
+ * iterations).  evals[k] (host) ordered as the reference orders them (SA ascending, LA
+ * descending, LM by descending magnitude); evecs (device, may be NULL, k x n) the Ritz vectors
+ * as rows; niter = matvecs applied; nconv = pairs within the bound.  The context must hold
+ * 2 * ncv vectors (cxa_lanczos_create with maxiter >= 2 * ncv - 1).
  */
 int cxa_lanczos_eigsh(cxa_lanczos *ctx, cxa_matvec_fn matvec, void *matvec_ctx, const double *Tin,
-                      int which, int k, double tol, int Maxiter, int flags, double *evals,
-                      double *evecs, int *niter, int *nconv, cxa_stream_t stream);
+                      int which, int k, double tol, int Maxiter, int ncv, int flags,
+                      double *evals, double *evecs, int *niter, int *nconv, cxa_stream_t stream);
 
 #ifdef __cplusplus
 }

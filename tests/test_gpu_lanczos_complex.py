@@ -107,6 +107,26 @@ def test_complex_which_eigenpairs_vs_eigh(which):
     assert np.abs(G - np.eye(k)).max() <= 1e-8
 
 
+def test_complex_thick_restart_vs_eigh():
+    """The restarted k-pair Lanczos on a complex Hermitian operator with a 12-vector basis."""
+    from cytnx_amd.lanczos import _matvec_of, eigsh_native
+    n, k = 150, 3
+    H = herm(n, 41)
+    w = np.linalg.eigvalsh(H)
+    x0 = dev(cvec(n, 2))
+    fn_ctx, keep = _matvec_of(DenseOp(H), x0)
+    vals, vecs, matvecs, nconv = eigsh_native(None, x0.storage(), "SA", k, 0.0, True,
+                                              fn_ctx=fn_ctx, Maxiter=5000, ncv=12)
+    print("complex restart: matvecs", matvecs, "nconv", nconv, flush=True)
+    assert nconv == k and matvecs > 12
+    np.testing.assert_allclose(vals, w[:k], rtol=0, atol=1e-9)
+    V = vecs.cpu().numpy()
+    for i in range(k):
+        assert np.linalg.norm(H @ V[i] - vals[i] * V[i]) <= 1e-7
+    assert np.abs(V.conj() @ V.T - np.eye(k)).max() <= 1e-8
+    del keep
+
+
 @pytest.mark.parametrize("tau", [-0.3j, -0.1 + 0.25j, -0.2])
 def test_complex_lanczos_exp_vs_expm(tau):
     n = 80

@@ -71,6 +71,33 @@ def test_which_eigenpairs_vs_eigh(which, n, k):
     assert list(out[1].shape()) == ([n] if k == 1 else [k, n])
 
 
+@pytest.mark.parametrize("which", ["SA", "LA", "LM"])
+@pytest.mark.parametrize("ncv", [0, 9, 14])
+def test_thick_restart_with_a_small_basis_vs_eigh(which, ncv):
+    """ncv basis vectors (0: the reference's default 2k + 10) far below n: the run restarts
+    from the wanted Ritz vectors many times and still returns eigh's eigenpairs."""
+    from cytnx_amd.lanczos import _matvec_of, eigsh_native
+    n, k = 300, 4
+    H = sym(n, 500 + ncv)
+    w = np.linalg.eigvalsh(H)
+    want = order(w, which)[:k]
+    x0 = dev_vec(np.random.default_rng(8).standard_normal(n))
+    fn_ctx, keep = _matvec_of(DenseOp(H), x0)
+    vals, vecs, matvecs, nconv = eigsh_native(None, x0.storage(), which, k, 0.0, True,
+                                              fn_ctx=fn_ctx, Maxiter=5000, ncv=ncv)
+    m = ncv or 2 * k + 10
+    print(which, "ncv", m, "matvecs", matvecs, "nconv", nconv, flush=True)
+    assert nconv == k and matvecs > m  # converged, after at least one restart
+    scale = max(1.0, np.abs(w).max())
+    assert np.allclose(vals, want, atol=1e-10 * scale, rtol=0)
+    V = vecs.cpu().numpy()
+    for i in range(k):
+        assert abs(np.linalg.norm(V[i]) - 1) < 1e-10
+        assert np.linalg.norm(H @ V[i] - vals[i] * V[i]) < 1e-8 * scale
+    assert np.abs(V @ V.T - np.eye(k)).max() < 1e-9
+    del keep
+
+
 def test_which_sa_on_a_native_two_site_chain_vs_eigh():
     from cytnx_amd import heisenberg
     from cytnx_amd.heff import Heff2
