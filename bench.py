@@ -192,11 +192,32 @@ def bench_permute(torch, nperm=32):
             torch.cuda.synchronize()
             copy_gbs = 2.0 * n * es / (a.elapsed_time(b) * 1e-3 / (3 * nb)) / 1e9
             del g
+            # this library's own streaming copy (the identity permute), timed the same way: the
+            # HBM rate a plain read-once / write-once kernel reaches at this size
+            ident = _lib.u64_array(list(range(6)))
+            g = torch.cuda.CUDAGraph()
+            cs = torch.cuda.Stream()
+            with torch.cuda.graph(g, stream=cs):
+                hs = stream_handle(0)
+                for i in range(nb):
+                    _lib.check(L.cxa_permute(dt_id, 6, shp, ident, src.data_ptr() + i * n * es,
+                                             dst.data_ptr() + i * n * es, hs))
+            g.replay()
+            torch.cuda.synchronize()
+            a.record()
+            for _ in range(3):
+                g.replay()
+            b.record()
+            torch.cuda.synchronize()
+            stream_gbs = 2.0 * n * es / (a.elapsed_time(b) * 1e-3 / (3 * nb)) / 1e9
+            del g
             key = "%s_%s" % ("x".join(map(str, shape)), name)
             res[key] = {"mean_GBs": float(np.mean(gbs)), "min_GBs": float(np.min(gbs)),
                         "max_GBs": float(np.max(gbs)),
                         "frac_of_8TBs": float(np.mean(gbs)) / PEAK_HBM_GBS,
                         "copy_GBs": copy_gbs, "frac_of_copy": float(np.mean(gbs)) / copy_gbs,
+                        "stream_copy_GBs": stream_gbs,
+                        "frac_of_stream_copy": float(np.mean(gbs)) / stream_gbs,
                         "buffers_rotated": nb}
             del src, dst
     allg = [v["mean_GBs"] for v in res.values()]

@@ -66,6 +66,9 @@ def _declare(L):
         "cxa_block_contract_destroy": (ci, [vp]),
         "cxa_mix_contract_plan": (ci, [ci, ci, i64p, i64, i64p, i64, i64p, dblp,
                                        ctypes.POINTER(vp)]),
+        "cxa_heff2_mix_plan": (ci, [ci, i64p, i64p, i64p, intp, ci, i64p, i64p, i64p, intp, ci,
+                                    i64p, i64p, i64p, intp, i64p, i64p, dblp, i64p, vp, i64, i64,
+                                    i64, i64p, i64p, i64p, dblp, vp, i64p]),
         "cxa_vec_workspace_bytes": (ci, []),
         "cxa_dot": (ci, [ci, i64, vp, vp, ci, vp, vp, vp]),
         "cxa_nrm2": (ci, [ci, i64, vp, vp, vp, vp]),

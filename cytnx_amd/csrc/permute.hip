@@ -418,6 +418,30 @@ __global__ __launch_bounds__(kThreads) void permute_rows32(const T *__restrict__
   }
 }
 
+// Identity permute: a streaming copy in 16-byte vectors.  U vectors per thread, the U loads
+// issued back to back ahead of the (guarded) stores, non-temporal on both sides (the data is
+// touched once).
+template <int U>
+__global__ __launch_bounds__(kThreads) void copy16(const U128 *__restrict__ src,
+                                                   U128 *__restrict__ dst, uint64_t nvec) {
+  const uint64_t G = (uint64_t)gridDim.x * kThreads;
+  for (uint64_t v0 = (uint64_t)blockIdx.x * kThreads + threadIdx.x; v0 < nvec; v0 += U * G) {
+    U128 r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t v = v0 + u * G < nvec ? v0 + u * G : v0;
+      r[u] = __builtin_nontemporal_load(src + v);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) __asm__ volatile("" ::"v"(r[u].x), "v"(r[u].y));
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t v = v0 + u * G;
+      if (v < nvec) __builtin_nontemporal_store(r[u], dst + v);
+    }
+  }
+}
+
 struct Reduced {
   int rank = 0;
   int64_t shape[16];  // old (input) extents, row-major
@@ -694,7 +718,17 @@ int launch_gather(const Reduced &r, const void *src, void *dst, hipStream_t s) {
 template <typename T>
 int run_permute(const Reduced &r, const void *src, void *dst, hipStream_t s) {
   if (r.rank <= 1) {
-    CXA_HIP(hipMemcpyAsync(dst, src, (size_t)r.total * sizeof(T), hipMemcpyDeviceToDevice, s));
+    const uint64_t bytes = (uint64_t)r.total * sizeof(T);
+    if (bytes % 16 == 0 && (uintptr_t)src % 16 == 0 && (uintptr_t)dst % 16 == 0) {
+      constexpr int U = 4;
+      const uint64_t nvec = bytes / 16;
+      const int grid = grid_for((int64_t)((nvec + U - 1) / U), kThreads);
+      hipLaunchKernelGGL((copy16<U>), dim3(grid), dim3(kThreads), 0, s, (const U128 *)src,
+                         (U128 *)dst, nvec);
+      CXA_HIP(hipGetLastError());
+    } else {
+      CXA_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, s));
+    }
     return CXA_OK;
   }
   int rc;

@@ -136,8 +136,21 @@ __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict_
                                                        double *__restrict__ Bs, IterIdx iter,
                                                        double *__restrict__ part_b) {
   const int it = iter.get();
-  const double wu = all_sum_partials(dotp, S);
-  const double uu = all_sum_partials(dotp + S, S);
+  // both sums in one pass over the partials (one round of loads, then two block reductions)
+  __shared__ double bc[2];
+  double a0 = 0.0, a1 = 0.0;
+  for (int i = threadIdx.x; i < S; i += kT) {
+    a0 += dotp[i];
+    a1 += dotp[S + i];
+  }
+  a0 = block_sum(a0);
+  a1 = block_sum(a1);
+  if (threadIdx.x == 0) {
+    bc[0] = a0;
+    bc[1] = a1;
+  }
+  __syncthreads();
+  const double wu = bc[0], uu = bc[1];
   const double b = it > 0 ? sqrt(uu) : 1.0;
   const double a = (wu / b) / b;
   if (blockIdx.x == 0 && threadIdx.x == 0) {

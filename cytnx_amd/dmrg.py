@@ -261,7 +261,10 @@ class DMRG:
         if self.prec == "fp64":
             E, v, n, _, _ = lanczos_gnd_native(H, x0, self.cvg, self.maxit)
         else:
-            E, v, n1, n2 = ground_state_mixed(H, x0, self.cvg, self.maxit)
+            # the bond's iteration budget goes to the FP32 phase; the FP64 refinement polishes
+            # the Ritz vector with a sixth of it (at least four iterations)
+            E, v, n1, n2 = ground_state_mixed(H, x0, self.cvg, self.maxit,
+                                              refine_maxiter=max(4, self.maxit // 6))
             n = n1 + n2
         torch.cuda.synchronize(self.device)
         t2 = time.perf_counter()

@@ -295,6 +295,60 @@ def mix_plan_arrays(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict, W12: 
             np.asarray(coefs, dtype=np.float64).reshape(-1, MIX_SLICES))
 
 
+def mix_plan_arrays_native(Lp: UniTensor, labL: dict, psi: UniTensor, labP: dict,
+                           W12: np.ndarray, Y: UniTensor, labY: dict,
+                           live: Optional[np.ndarray] = None, read: Optional[np.ndarray] = None):
+    """mix_plan_arrays computed by the library's C++ planner (cxa_heff2_mix_plan): the same
+    arguments and the same arrays, element for element."""
+    pL = [Lp.labels().index(labL[k]) for k in ("l", "w", "lp")]
+    pP = [psi.labels().index(labP[k]) for k in ("l", "s1", "s2", "r")]
+    pY = [Y.labels().index(labY[k]) for k in ("lp", "s1p", "s2p", "w2", "r")]
+    bonds = [Lp.bonds()[pL[1]], psi.bonds()[pP[1]], psi.bonds()[pP[2]], Y.bonds()[pY[1]],
+             Y.bonds()[pY[2]], Y.bonds()[pY[3]]]
+    secs = [_sector_offsets(b) for b in bonds]
+    sec_off = np.ascontiguousarray(np.concatenate(secs).astype(np.int64))
+    sec_n = np.asarray([len(x) for x in secs], dtype=np.int64)
+    W = np.ascontiguousarray(W12, dtype=np.float64)
+    wd = np.asarray(W.shape, dtype=np.int64)
+
+    def meta(T, rank):
+        it = np.ascontiguousarray(np.asarray(T._itoi, dtype=np.int64).reshape(-1, rank))
+        sh = np.ascontiguousarray(np.asarray(T._shapes, dtype=np.int64).reshape(-1, rank))
+        of = np.ascontiguousarray(np.asarray(T._offsets, dtype=np.int64))
+        return it, sh, of
+
+    iL, sL, oL = meta(Lp, 3)
+    iP, sP, oP = meta(psi, 4)
+    iY, sY, oY = meta(Y, 5)
+    nbY = Y.Nblocks()
+    cap_items = max(1, nbY)
+    cap_slices = MIX_SLICES * cap_items
+    cap_outs = max(1, int(np.sum(np.prod(sY[:, [pY[1], pY[2], pY[3]]], axis=1)))) if nbY else 1
+    items = np.zeros((cap_items, 4), dtype=np.int64)
+    slices = np.zeros((cap_slices, 7), dtype=np.int64)
+    outs = np.zeros((cap_outs, 2), dtype=np.int64)
+    coef = np.zeros((cap_outs, MIX_SLICES), dtype=np.float64)
+    rd = None if read is None else np.ascontiguousarray(read, dtype=np.uint8)
+    lv = np.zeros(max(1, nbY), dtype=np.uint8)
+    counts = np.zeros(3, dtype=np.int64)
+    ip = lambda a: a.ctypes.data_as(_lib.i64p)
+    rc = _lib.lib().cxa_heff2_mix_plan(
+        Lp.Nblocks(), ip(iL), ip(sL), ip(oL), _lib.int_array(pL),
+        psi.Nblocks(), ip(iP), ip(sP), ip(oP), _lib.int_array(pP),
+        nbY, ip(iY), ip(sY), ip(oY), _lib.int_array(pY),
+        ip(sec_off), ip(sec_n), W.ctypes.data_as(_lib.dblp), ip(wd),
+        rd.ctypes.data if rd is not None else None, cap_items, cap_slices, cap_outs,
+        ip(items), ip(slices), ip(outs), coef.ctypes.data_as(_lib.dblp), lv.ctypes.data,
+        ip(counts))
+    if rc == _lib.ERR_UNSUPPORTED:
+        raise MixUnsupported(_lib.lib().cxa_last_error().decode(errors="replace"))
+    _lib.check(rc)
+    if live is not None:
+        live[:] = lv[:nbY].astype(bool)
+    ni, ns, no = (int(c) for c in counts)
+    return items[:ni].copy(), slices[:ns].copy(), outs[:no].copy(), coef[:no].copy()
+
+
 def subset_blocks(T: UniTensor, keep: np.ndarray) -> UniTensor:
     """Metadata of T restricted to the blocks with keep[i] (same order, packed layout)."""
     out = T.clone_meta()
@@ -526,7 +580,7 @@ class Heff2(_HeffBase):
         pw, pr = Y.labels().index(kw), Y.labels().index(kr)
         read = np.array([(int(t[pw]), int(t[pr])) in rk for t in Y._itoi], dtype=bool)
         keep = np.zeros(Y.Nblocks(), dtype=bool)
-        arrays = mix_plan_arrays(*args, Y, ylab, keep, read)
+        arrays = mix_plan_arrays_native(*args, Y, ylab, keep, read)
         Y = subset_blocks(Y, keep)
         psi_left = psi_is_left(Y, ylab)
         na, nb = Lp._storage.numel(), meta_elems(psi_m)

@@ -414,7 +414,7 @@ def Lanczos_Exp(Hop: LinOp, v, tau, CvgCrit: float = 1.0e-14, Maxiter: int = 100
 
 def ground_state_mixed(op, psi0: torch.Tensor, CvgCrit: float = 1e-12, Maxiter: int = 1000,
                        batch: int = 16, graph: bool = False, fp32_cvg: Optional[float] = None,
-                       reorth: bool = False):
+                       reorth: bool = False, refine_maxiter: Optional[int] = None):
     """FP32 + refinement: (E, evec, n_fp32_iter, n_fp64_iter).
 
     Phase 1 is the Lanczos iteration with the contraction bulk on FP32 MFMA (the vectors, the
@@ -424,7 +424,9 @@ def ground_state_mixed(op, psi0: torch.Tensor, CvgCrit: float = 1e-12, Maxiter: 
     ~1e-7 relative error), so the switch comes three decades above the target.  Phase 2 is the
     plain FP64 Lanczos restarted from that Ritz vector, stopped by the reference's rule
     |dE| < CvgCrit (Lanczos_Gnd_Ut.cpp:140-162); from a good start vector it needs a few
-    iterations.  ``reorth`` applies to both phases (the reference's driver has none)."""
+    iterations.  ``reorth`` applies to both phases (the reference's driver has none);
+    ``refine_maxiter`` caps phase 2 (default Maxiter), for callers with a fixed iteration
+    budget such as a DMRG sweep's bond updates."""
     from .unitensor import PREC_FP32, PREC_FP64
     if fp32_cvg is None:
         fp32_cvg = 1000.0 * CvgCrit
@@ -434,8 +436,10 @@ def ground_state_mixed(op, psi0: torch.Tensor, CvgCrit: float = 1e-12, Maxiter: 
         _, v1, n1, _, _ = lanczos_gnd_native(op, psi0, max(fp32_cvg, CvgCrit), Maxiter, batch,
                                              reorth=reorth, graph=graph)
         op.set_precision(PREC_FP64)
-        E, v2, n2, _, _ = lanczos_gnd_native(op, v1, CvgCrit, Maxiter, batch, reorth=reorth,
-                                             graph=graph)
+        E, v2, n2, _, _ = lanczos_gnd_native(op, v1, CvgCrit,
+                                             Maxiter if refine_maxiter is None
+                                             else max(2, int(refine_maxiter)),
+                                             batch, reorth=reorth, graph=graph)
     finally:
         op.set_precision(prev)
     return E, v2, n1, n2

@@ -216,6 +216,34 @@ int cxa_mix_contract_plan(int dtype, int nitems, const int64_t *items, int64_t n
                           const double *coef, cxa_bc_plan **plan);
 
 /*
+ * The task arrays of cxa_mix_contract_plan for the two-site effective Hamiltonian (host only; no
+ * counterpart in the reference, which contracts the network of example/DMRG/
+ * dmrg_two_sites_U1.py:18-37 pairwise): L permuted to (l, w, l') (nbL blocks of rank 3), psi
+ * (l, s1, s2, r) (nbP blocks of rank 4), the output layout Y with legs (l', s1', s2', w'', r) in
+ * any order (nbY blocks of rank 5; r, or l' when it comes after r, contiguous).  pos* give each
+ * role's leg index (L: l, w, l'; psi: l, s1, s2, r; Y: l', s1', s2', w'', r); itoi / shape
+ * row-major per block, off element offsets.  sec_off: the sector offsets of the bonds w (L),
+ * s1, s2 (psi), s1', s2', w'' (Y), concatenated, sec_n their counts.  W12: the dense two-site
+ * MPO (w, s1, s1', w'', s2, s2') of extents wdims[6].  One item per (l', r) sector pair of Y;
+ * its slices are the L.psi products of every (w, s1, s2) element reaching one of its outputs,
+ * its outputs the (s1', s2', w'') sub-matrices with their W12 weights.  read (nbY, may be NULL):
+ * the Y blocks the next launch reads; with it the plan addresses the pruned layout (blocks
+ * reached by a slice and read, in order, packed; offY unused) and items follow their first
+ * block; live (nbY, may be NULL) receives the kept (or, without read, the reached) blocks.
+ * Outputs: items (4 each), slices (7), outs (2), coef (CXA_MIX_SLICES each), within the given
+ * capacities; counts = (items, slices, outs).  CXA_ERR_UNSUPPORTED if an item needs more than
+ * CXA_MIX_SLICES slices.
+ */
+int cxa_heff2_mix_plan(int nbL, const int64_t *itoiL, const int64_t *shapeL, const int64_t *offL,
+                       const int *posL, int nbP, const int64_t *itoiP, const int64_t *shapeP,
+                       const int64_t *offP, const int *posP, int nbY, const int64_t *itoiY,
+                       const int64_t *shapeY, const int64_t *offY, const int *posY,
+                       const int64_t *sec_off, const int64_t *sec_n, const double *W12,
+                       const int64_t *wdims, const unsigned char *read, int64_t cap_items,
+                       int64_t cap_slices, int64_t cap_outs, int64_t *items, int64_t *slices,
+                       int64_t *outs, double *coef, unsigned char *live, int64_t *counts);
+
+/*
  * Vector reductions into device memory; workspace: device buffer of cxa_vec_workspace_bytes()
  * bytes.  Deterministic (fixed reduction order); products are summed in FP64 for every dtype.
  * cxa_dot replaces cuVectordot_internal_{d,f,cd,cf} (src/backend/linalg_internal_gpu/

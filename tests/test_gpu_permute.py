@@ -91,6 +91,22 @@ def test_permute_all_element_sizes(dtype):
             assert np.array_equal(_gpu_permute(x, perm), np.transpose(x, perm))
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32, np.int16, np.bool_])
+def test_identity_permute_streaming_copy(dtype):
+    """Permutations that reduce to the identity (unit axes, axes that stay in order) are a
+    streaming copy in 16-byte vectors, or a device copy when the bytes are not 16-byte whole."""
+    rng = np.random.default_rng(17)
+    for shape, perm in [((1 << 20,), [0]), ((3, 1, 1000), [1, 0, 2]), ((7, 5, 3), [0, 1, 2]),
+                        ((1, 33, 1), [2, 1, 0]), ((5,), [0]), ((64, 64, 1, 3), [0, 2, 1, 3])]:
+        x = rng.uniform(-1, 1, size=shape)
+        if dtype == np.complex128:
+            x = x + 1j * rng.uniform(-1, 1, size=shape)
+        x = (x > 0) if dtype == np.bool_ else (x * 100).astype(dtype)
+        print("identity permute", shape, np.dtype(dtype).name, perm, flush=True)
+        got = _gpu_permute(x, perm)
+        assert np.array_equal(_bits(got), _bits(np.ascontiguousarray(np.transpose(x, perm))))
+
+
 def test_permute_empty_and_scalar_like():
     x = np.zeros((0, 4, 3))
     assert _gpu_permute(x, [2, 0, 1]).shape == (3, 0, 4)

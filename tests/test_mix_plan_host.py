@@ -202,3 +202,63 @@ def test_mix_plan_single_pass_equals_two_passes(chi, transposed):
         assert np.array_equal(keep, live & read)
         for a, b in zip(one, two):
             assert a.shape == b.shape and np.array_equal(a, b)
+
+
+def _mix_args(chi, seed, transposed):
+    psi, L, M1, M2, R = heisenberg.two_site_problem(chi, seed=seed, hermitian=True)
+    Lb = L.bonds()
+    Lp = _meta([Lb[1], Lb[0], Lb[2]], ["-1", "-5", "0"])
+    W = np.einsum("awsx,wvty->asxvty", M1.to_dense(), M2.to_dense())
+    yb = [Lb[2], M1.bonds()[3], M2.bonds()[3], M2.bonds()[1], psi.bonds()[3]]
+    yl = ["0", "1", "2", "k0", "k1"]
+    if transposed:  # (w'', r, l', s1', s2'): l' contiguous
+        yb = [yb[3], yb[4], yb[0], yb[1], yb[2]]
+        yl = [yl[3], yl[4], yl[0], yl[1], yl[2]]
+    Y = _meta(yb, yl)
+    labs = ({"l": "-1", "w": "-5", "lp": "0"}, {"l": "-1", "s1": "-2", "s2": "-3", "r": "-4"},
+            {"lp": "0", "s1p": "1", "s2p": "2", "w2": "k0", "r": "k1"})
+    return Lp, psi, W, Y, labs
+
+
+@pytest.mark.parametrize("transposed", [False, True])
+@pytest.mark.parametrize("chi,seed", [(8, 3), (20, 4), (64, 5), (160, 6)])
+def test_native_mix_plan_equals_python_plan(chi, seed, transposed):
+    """The library's C++ planner (cxa_heff2_mix_plan, what Heff2 uses) gives the same task
+    arrays as the Python restatement above, element for element, with and without the read
+    mask (the pruned layout) and for both Y leg orders."""
+    from cytnx_amd.heff import mix_plan_arrays_native
+    Lp, psi, W, Y, labs = _mix_args(chi, seed, transposed)
+    live_py = np.zeros(Y.Nblocks(), dtype=bool)
+    live_nt = np.zeros(Y.Nblocks(), dtype=bool)
+    a_py = mix_plan_arrays(Lp, labs[0], psi, labs[1], W, Y, labs[2], live_py)
+    a_nt = mix_plan_arrays_native(Lp, labs[0], psi, labs[1], W, Y, labs[2], live_nt)
+    for x, y in zip(a_py, a_nt):
+        assert x.shape == y.shape and np.array_equal(x, y)
+    assert np.array_equal(live_py, live_nt)
+    # with a read mask: every other reached block read
+    read = live_py.copy()
+    read[np.nonzero(read)[0][::2]] = False
+    k_py = np.zeros(Y.Nblocks(), dtype=bool)
+    k_nt = np.zeros(Y.Nblocks(), dtype=bool)
+    b_py = mix_plan_arrays(Lp, labs[0], psi, labs[1], W, Y, labs[2], k_py, read)
+    b_nt = mix_plan_arrays_native(Lp, labs[0], psi, labs[1], W, Y, labs[2], k_nt, read)
+    for x, y in zip(b_py, b_nt):
+        assert x.shape == y.shape and np.array_equal(x, y)
+    assert np.array_equal(k_py, k_nt)
+
+
+def test_native_mix_plan_rejects_more_than_eight_slices():
+    """An environment with a wider MPO bond (charge-0 group of 6 states) and an all-nonzero
+    two-site MPO gives items of more than CXA_MIX_SLICES slices: both planners refuse (Heff2
+    then falls back to the three-launch chain)."""
+    from cytnx_amd.heff import MixUnsupported, mix_plan_arrays_native
+    from cytnx_amd.unitensor import BD_BRA, Bond
+    Lp, psi, W, Y, labs = _mix_args(20, 9, False)
+    wide = Bond(BD_BRA, [[0], [2], [-2]], [6, 2, 2])
+    Lb = Lp.bonds()
+    Lw = _meta([Lb[0], wide, Lb[2]], ["-1", "-5", "0"])
+    W10 = np.ones((10,) + W.shape[1:])
+    with pytest.raises(MixUnsupported):
+        mix_plan_arrays(Lw, labs[0], psi, labs[1], W10, Y, labs[2])
+    with pytest.raises(MixUnsupported):
+        mix_plan_arrays_native(Lw, labs[0], psi, labs[1], W10, Y, labs[2])
