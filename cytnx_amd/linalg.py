@@ -256,11 +256,16 @@ def Svd(T, is_UvT: bool = True):
     return _svd(T, is_UvT)
 
 
-def Svd_truncate(T, keepdim: int, err: float = 0.0, is_UvT: bool = True, return_err: int = 0,
-                 mindim: int = 1):
-    """linalg::Svd_truncate on a UniTensor (cytnx_amd.svd)."""
+def Svd_truncate(T, keepdim: int, *args, **kwargs):
+    """linalg::Svd_truncate on a UniTensor, with or without min_blockdim (cytnx_amd.svd)."""
     from .svd import Svd_truncate as _svdt
-    return _svdt(T, keepdim, err, is_UvT, return_err, mindim)
+    return _svdt(T, keepdim, *args, **kwargs)
+
+
+def Gesvd_truncate(T, keepdim: int, *args, **kwargs):
+    """linalg::Gesvd_truncate on a UniTensor, with or without min_blockdim (cytnx_amd.svd)."""
+    from .svd import Gesvd_truncate as _gsvdt
+    return _gsvdt(T, keepdim, *args, **kwargs)
 
 
 def Lanczos(*args, **kwargs):
@@ -270,10 +275,10 @@ def Lanczos(*args, **kwargs):
 
 
 def Lanczos_Exp(*args, **kwargs):
-    """linalg::Lanczos_Exp for a real tau (cytnx_amd.lanczos.Lanczos_Exp)."""
+    """linalg::Lanczos_Exp for a real or complex tau (cytnx_amd.lanczos.Lanczos_Exp)."""
     from .lanczos import Lanczos_Exp as _lanczos_exp
     return _lanczos_exp(*args, **kwargs)
 
 
-__all__ = ["Matmul", "Tensordot", "Gemm_Batch", "Vectordot", "norm", "Norm", "Gesvd", "Svd", "Svd_truncate", "Lanczos",
-           "Lanczos_Exp"]
+__all__ = ["Matmul", "Tensordot", "Gemm_Batch", "Vectordot", "norm", "Norm", "Gesvd", "Svd", "Svd_truncate",
+           "Gesvd_truncate", "Lanczos", "Lanczos_Exp"]

@@ -434,36 +434,165 @@ def _err_tensor(Sall: np.ndarray, dropped: int, return_err: int, dtype: int,
     return T
 
 
-def Svd_truncate(T: UniTensor, keepdim: int, err: float = 0.0, is_UvT: bool = True,
-                 return_err: int = 0, mindim: int = 1) -> List[UniTensor]:
-    """[S, U, vT] (+ [err]) keeping at most keepdim singular values (see module docstring)."""
-    _check_input(T, "Svd_truncate")
+def _cut_blockdim(blocks: Sequence[np.ndarray], keepdim: int, min_blockdim: Sequence[int],
+                  err: float, mindim: int) -> Tuple[List[int], np.ndarray, int]:
+    """Kept count per block, the candidate values (ascending) and how many of them are dropped,
+    for the block rule with a per-block floor (Svd_truncate_Block_UTs_internal with
+    min_blockdim, src/linalg/Svd_truncate.cpp:357-520; the same steps in
+    src/linalg/Gesvd_truncate.cpp:431-590):
+
+    1. block b keeps its min_blockdim[b] largest values outright (all of them when it has no
+       more), and keepdim and mindim are reduced by what it keeps so; its other values are
+       candidates;
+    2. over the candidates the rule of Svd_truncate runs with the reduced keepdim / mindim (the
+       floor is max(mindim, 1) if no block has a floor, else max(mindim, 0); the err loop may
+       end at zero kept candidates), and a block also keeps every candidate >= the cut value;
+    3. with no candidates at all every block is kept whole, and with the reduced keepdim <= 0
+       every candidate is dropped.
+    min_blockdim with one element applies to every block."""
+    nb = len(blocks)
+    mbd = [int(x) for x in min_blockdim]
+    if not mbd:
+        raise ValueError("min_blockdim must not be empty; use the overload without min_blockdim "
+                         "if no per-block floor is needed.")
+    if len(mbd) == 1:
+        mbd = mbd * nb
+    if len(mbd) != nb:
+        raise ValueError("min_blockdim must have the same number of elements as blocks in the "
+                         "singular value UniTensor")
+    keep_dim, min_dim = int(keepdim), max(int(mindim), 1)
+    cand, any_floor = [], False
+    for s, m in zip(blocks, mbd):
+        if m < 1:
+            cand.append(s)
+            continue
+        any_floor = True
+        if s.size <= m:
+            keep_dim -= s.size
+            min_dim -= s.size
+            continue
+        cand.append(s[m:])
+        keep_dim -= m
+        min_dim -= m
+    if not cand:
+        return [int(s.size) for s in blocks], np.zeros(0), 0
+    Sall = np.sort(np.concatenate(cand))
+    n = Sall.size
+    Smin = 0.0
+    if keep_dim > 0:
+        min_dim = max(min_dim, 0 if any_floor else 1)
+        if keep_dim < n:
+            smidx = n - keep_dim
+        else:
+            keep_dim, smidx = n, 0
+        Smin = Sall[smidx]
+        while Smin < err and keep_dim > min_dim:
+            keep_dim -= 1
+            if keep_dim == 0:
+                break
+            smidx += 1
+            Smin = Sall[smidx]
+        if keep_dim == 0:
+            dropped = n
+        else:
+            while smidx > 0 and not (Sall[smidx - 1] < Smin):
+                smidx -= 1
+            dropped = smidx
+    else:
+        dropped = n
+    keep = []
+    for s, m in zip(blocks, mbd):
+        k = min(m, int(s.size))
+        if keep_dim > 0:
+            for i in range(int(s.size), m, -1):
+                if s[i - 1] >= Smin:
+                    k = i
+                    break
+        keep.append(max(k, 0))
+    return keep, Sall, dropped
+
+
+def _truncate(T: UniTensor, keepdim: int, min_blockdim, err: float, is_U: bool, is_vT: bool,
+              return_err: int, mindim: int, who: str) -> List[UniTensor]:
+    """The block / dense truncated SVD shared by Svd_truncate and Gesvd_truncate."""
+    _check_input(T, who)
     if keepdim < 1:
-        raise ValueError("[ERROR][Svd_truncate] keepdim must be >=1")
+        raise ValueError("[ERROR][%s] keepdim must be >=1" % who)
+    if return_err < 0:
+        raise ValueError("[ERROR][%s] return_err cannot be negative" % who)
     rdt = _REAL_OF[T.dtype()]
     if T.is_blockform():
-        sectors = _block_gesvd(T, is_UvT, is_UvT)
-        svals = np.concatenate([sec.s.detach().cpu().numpy() for sec in sectors]) \
-            if sectors else np.zeros(0)
-        Smin, dropped, Sall = _cut(svals, keepdim, err, mindim)
-        keep = []
-        for sec in sectors:
-            s = sec.s.detach().cpu().numpy()
-            keep.append(int(np.count_nonzero(s >= Smin)))
-        out = _assemble_block(T, sectors, keep, is_UvT, is_UvT)
+        sectors = _block_gesvd(T, is_U, is_vT)
+        svals = [sec.s.detach().cpu().numpy() for sec in sectors]
+        if min_blockdim is None:
+            Smin, dropped, Sall = _cut(np.concatenate(svals) if svals else np.zeros(0),
+                                       keepdim, err, mindim)
+            keep = [int(np.count_nonzero(s >= Smin)) for s in svals]
+        else:
+            keep, Sall, dropped = _cut_blockdim(svals, keepdim, min_blockdim, err, mindim)
+        out = _assemble_block(T, sectors, keep, is_U, is_vT)
     else:
+        if min_blockdim is not None:
+            if len(min_blockdim) != 1:
+                raise ValueError("[ERROR][%s] min_blockdim must have one element for dense "
+                                 "UniTensor" % who)
+            mindim = max(int(mindim), int(min_blockdim[0]))
         M, _ = _dense_matrix(T)
-        if is_UvT:
+        if is_U or is_vT:
             U, s, Vh = torch.linalg.svd(M, full_matrices=False)
         else:
             U, s, Vh = None, torch.linalg.svdvals(M), None
         sh = s.detach().cpu().numpy()
         k = _cut_dense(sh, keepdim, err, mindim)
         dropped, Sall = sh.size - k, np.sort(sh)
-        out = _assemble_dense(T, U, s, Vh, k, is_UvT, is_UvT)
+        out = _assemble_dense(T, U, s, Vh, k, is_U, is_vT)
     if return_err:
         out.append(_err_tensor(Sall, dropped, int(return_err), rdt, T.device()))
     return out
 
 
-__all__ = ["Gesvd", "Svd", "Svd_truncate"]
+def _split_overload(args: tuple, kwargs: dict) -> tuple:
+    """(min_blockdim or None, remaining positional arguments): the overloads with a per-block
+    floor take a sequence of integers as their third argument."""
+    if "min_blockdim" in kwargs:
+        return list(kwargs.pop("min_blockdim")), args
+    if args and isinstance(args[0], (list, tuple, np.ndarray)):
+        return list(args[0]), args[1:]
+    return None, args
+
+
+def Svd_truncate(T: UniTensor, keepdim: int, *args, **kwargs) -> List[UniTensor]:
+    """[S, U, vT] (+ [err]) keeping at most keepdim singular values (see module docstring).
+
+    Svd_truncate(T, keepdim, err=0, is_UvT=True, return_err=0, mindim=1) or, with a per-block
+    floor, Svd_truncate(T, keepdim, min_blockdim, err=0, is_UvT=True, return_err=0, mindim=1)
+    (include/linalg.hpp:753-810)."""
+    mbd, args = _split_overload(args, kwargs)
+    names = ("err", "is_UvT", "return_err", "mindim")
+    vals = dict(zip(names, (0.0, True, 0, 1)))
+    vals.update(zip(names, args))
+    vals.update(kwargs)
+    return _truncate(T, keepdim, mbd, float(vals["err"]), bool(vals["is_UvT"]),
+                     bool(vals["is_UvT"]), int(vals["return_err"]), int(vals["mindim"]),
+                     "Svd_truncate")
+
+
+def Gesvd_truncate(T: UniTensor, keepdim: int, *args, **kwargs) -> List[UniTensor]:
+    """[S, U (if is_U), vT (if is_vT)] (+ [err]): Svd_truncate with separate U / vT flags
+    (linalg::Gesvd_truncate, include/linalg.hpp:824-845; src/linalg/Gesvd_truncate.cpp:222-430:
+    the same cut rule on the singular values of Gesvd).
+
+    Gesvd_truncate(T, keepdim, err=0, is_U=True, is_vT=True, return_err=0, mindim=1) or
+    Gesvd_truncate(T, keepdim, min_blockdim, err=0, is_U=True, is_vT=True, return_err=0,
+    mindim=1)."""
+    mbd, args = _split_overload(args, kwargs)
+    names = ("err", "is_U", "is_vT", "return_err", "mindim")
+    vals = dict(zip(names, (0.0, True, True, 0, 1)))
+    vals.update(zip(names, args))
+    vals.update(kwargs)
+    return _truncate(T, keepdim, mbd, float(vals["err"]), bool(vals["is_U"]),
+                     bool(vals["is_vT"]), int(vals["return_err"]), int(vals["mindim"]),
+                     "Gesvd_truncate")
+
+
+__all__ = ["Gesvd", "Svd", "Svd_truncate", "Gesvd_truncate"]

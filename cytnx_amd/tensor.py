@@ -45,6 +45,9 @@ class Tensor:
     @classmethod
     def from_storage(cls, flat: torch.Tensor, shape: Sequence[int]) -> "Tensor":
         t = cls.__new__(cls)
+        # the kernels read raw memory: a lazily conjugated or negated torch view (torch's SVD
+        # returns Vh as one) is materialised here
+        flat = flat.resolve_conj().resolve_neg()
         t._storage = flat.reshape(-1)
         t._shape = [int(s) for s in shape]
         t._mapper = list(range(len(t._shape)))

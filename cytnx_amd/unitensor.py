@@ -420,14 +420,15 @@ class UniTensor:
     def from_saved(cls, ut: fileio.SavedUniTensor, device: int = CPU) -> "UniTensor":
         if ut.uten_type == fileio.UTEN_DENSE:
             bonds = [Bond(b.dim) for b in ut.bonds]
-            out = cls(bonds, ut.labels, ut.rowrank, ut.blocks[0].dtype, device, name=ut.name)
+            out = cls(bonds, ut.labels, ut.rowrank, ut.blocks[0].dtype, device,
+                      is_diag=ut.is_diag, name=ut.name)
             out._block = Tensor.from_numpy(ut.blocks[0].logical(), device)
             return out
         if ut.uten_type != fileio.UTEN_BLOCK:
             raise NotImplementedError("only dense and block UniTensors are supported")
         bonds = [Bond(b.btype, [q[0] for q in b.qnums], b.degs) for b in ut.bonds]
         dtype = ut.blocks[0].dtype if ut.blocks else Type_F64
-        out = cls(bonds, ut.labels, ut.rowrank, dtype, device, name=ut.name)
+        out = cls(bonds, ut.labels, ut.rowrank, dtype, device, is_diag=ut.is_diag, name=ut.name)
         want = {tuple(int(x) for x in t): i for i, t in enumerate(out._itoi)}
         for t, blk in zip(ut.itoi, ut.blocks):
             out.put_block(blk.logical(), want[tuple(t)])

@@ -21,7 +21,8 @@ from cytnx_amd import fileio
 from cytnx_amd.tensor import Tensor
 from cytnx_amd.unitensor import BD_KET, Bond, UniTensor
 
-UTENS = sorted(glob.glob(os.path.join(REF_FIXTURES, "*.cytnx")))
+UTENS = sorted(glob.glob(os.path.join(REF_FIXTURES, "*.cytnx")) +
+               glob.glob(os.path.join(REF_FIXTURES, "truncate", "*", "*.cytnx")))
 TENS = sorted(glob.glob(os.path.join(REF_FIXTURES, "*.cytn")))
 
 

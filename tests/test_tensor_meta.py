@@ -52,3 +52,15 @@ def test_cpu_move_is_refused():
     t.permute_(1, 0)
     with pytest.raises(NotImplementedError):
         t.contiguous_()
+
+
+def test_from_storage_materialises_lazy_conjugates():
+    """The kernels read raw memory, so a torch conj / neg view must not reach them: torch's SVD
+    returns Vh as a conjugate view, which .contiguous() keeps."""
+    import torch
+    x = torch.randn(3, 4, dtype=torch.complex128)
+    t = Tensor.from_storage(x.conj().contiguous(), [3, 4])
+    assert not t.storage().is_conj()
+    assert torch.equal(t.storage().reshape(3, 4), torch.conj_physical(x))
+    n = Tensor.from_storage(torch._neg_view(x), [3, 4])
+    assert not n.storage().is_neg() and torch.equal(n.storage().reshape(3, 4), -x)
