@@ -320,7 +320,7 @@ def measured_traffic(chi, chain, prec, kernel):
     return t.get("hbm_bytes_per_launch", {}).get(kernel)
 
 
-def bench_dmrg(torch, L=100, chi=2048, sweeps=1, maxit=30, cpu=True):
+def bench_dmrg(torch, L=100, chi=2048, sweeps=4, maxit=30, cpu=True):
     """Two-site U(1) DMRG of the open Heisenberg chain: L sites, bond dimension chi, `sweeps`
     full sweeps (left to right and back) from a random right-canonical MPS, `maxit` Lanczos
     iterations per bond.  Run twice from the same seed: contractions in FP32 + FP64 refinement
@@ -478,7 +478,7 @@ def main() -> int:
                     help="apply tilings from a --save-tiles file instead of tuning")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the permute sweep and the dense one-site Lanczos lines")
-    ap.add_argument("--dmrg-sweeps", type=int, default=1,
+    ap.add_argument("--dmrg-sweeps", type=int, default=4,
                     help="full sweeps of the L=100, chi=2048 two-site DMRG line (0: skip)")
     ap.add_argument("--dmrg-chi", type=int, default=2048)
     ap.add_argument("--dmrg-L", type=int, default=100)

@@ -69,6 +69,7 @@ struct CopyJob {
 // keeps one per precision mode, so switching precision never re-uploads (launches stay
 // capturable) and the tuner can pick different shapes for FP64 and FP32.
 struct PlanVariant {
+  bool built = false;   // tile list made and uploaded (done on first use, see bc_ensure)
   int kind = -1;        // tile kind override, -1 = per-block default
   int split_mode = -1;  // -1 automatic, 0 off, 1 on (four waves per tile), 2 eight waves
   std::vector<cxa::GTile> tiles;  // kXcds ranges of per_xcd tiles
@@ -85,8 +86,12 @@ struct cxa_bc_plan {
   int device = 0;
   std::vector<cxa::GBlk> blks;
   std::vector<cxa::GSeg> segs;
-  cxa::DevBuf d_blks, d_segs;
+  cxa::DevBuf d_meta;  // blks, segs and (mix plans) outs in one device allocation
+  const cxa::GBlk *dev_blks = nullptr;
+  const cxa::GSeg *dev_segs = nullptr;
+  const cxa::MixOut *dev_outs = nullptr;
   PlanVariant var[3];  // indexed by CXA_PREC_*
+  std::mutex mu;       // guards building a variant on first use
   bool copyA = false, copyB = false;
   std::vector<cxa::CopyJob> jobsA, jobsB;
   int64_t wsA = 0, wsB = 0;  // workspace elements for A and B copies
@@ -95,7 +100,6 @@ struct cxa_bc_plan {
   // mix plans (cxa_mix_contract_plan): blks are items, segs their slices, outs the outputs
   bool mix = false;
   std::vector<cxa::MixOut> outs;
-  cxa::DevBuf d_outs;
 };
 
 namespace cxa {
@@ -107,6 +111,7 @@ bool mix_kind_ok(int kind) { return kind == TK_1x1 || kind == TK_2x1 || kind == 
 int build_variant(cxa_bc_plan *p, int prec) {
   const bool real = p->dtype == CXA_F64 || p->dtype == CXA_F32;
   PlanVariant &v = p->var[prec];
+  v.built = false;
   if (p->mix) {
     v.launch_kind = mix_kind_ok(v.kind) ? v.kind : TK_1x1;
     v.tile_class = make_tiles(p->blks, p->segs, v.tiles, &v.per_xcd, v.launch_kind, false);
@@ -115,7 +120,9 @@ int build_variant(cxa_bc_plan *p, int prec) {
       CXA_HIP(hipFree(v.d_tiles.p));
       v.d_tiles.p = nullptr;
     }
-    return upload(v.d_tiles, v.tiles);
+    const int rc = upload(v.d_tiles, v.tiles);
+    v.built = rc == CXA_OK;
+    return rc;
   }
   v.tile_class = make_tiles(p->blks, p->segs, v.tiles, &v.per_xcd, v.kind, real);
   const bool split = real && v.tile_class != 2 &&
@@ -128,19 +135,66 @@ int build_variant(cxa_bc_plan *p, int prec) {
     CXA_HIP(hipFree(v.d_tiles.p));
     v.d_tiles.p = nullptr;
   }
-  return upload(v.d_tiles, v.tiles);
+  const int rc = upload(v.d_tiles, v.tiles);
+  v.built = rc == CXA_OK;
+  return rc;
 }
 
+// The precision an exec of a plan of this dtype runs when asked for FP64 (FP32 data runs FP32).
+int native_prec(int dtype) {
+  return dtype == CXA_F32 || dtype == CXA_C64 ? CXA_PREC_FP32 : CXA_PREC_FP64;
+}
+
+// The task metadata in one allocation and one copy; only the tile list of the plan's own
+// precision is made now, the others on first use (a tile list of a large contraction takes
+// about a millisecond of host time, and most plans only ever run one precision).
 int upload_plan(cxa_bc_plan *p) {
-  int rc;
-  if ((rc = upload(p->d_blks, p->blks)) || (rc = upload(p->d_segs, p->segs))) return rc;
-  if (p->mix && (rc = upload(p->d_outs, p->outs))) return rc;
-  for (int prec = 0; prec < 3; ++prec)
-    if ((rc = build_variant(p, prec))) return rc;
-  return CXA_OK;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t nb = up(p->blks.size() * sizeof(GBlk)), ns = up(p->segs.size() * sizeof(GSeg));
+  const size_t no = p->mix ? up(p->outs.size() * sizeof(MixOut)) : 0;
+  if (nb + ns + no) {
+    std::vector<char> host(nb + ns + no, 0);
+    if (!p->blks.empty()) std::memcpy(host.data(), p->blks.data(), p->blks.size() * sizeof(GBlk));
+    if (!p->segs.empty())
+      std::memcpy(host.data() + nb, p->segs.data(), p->segs.size() * sizeof(GSeg));
+    if (no) std::memcpy(host.data() + nb + ns, p->outs.data(), p->outs.size() * sizeof(MixOut));
+    CXA_HIP(hipMalloc(&p->d_meta.p, host.size()));
+    CXA_HIP(hipMemcpy(p->d_meta.p, host.data(), host.size(), hipMemcpyHostToDevice));
+    char *base = (char *)p->d_meta.p;
+    p->dev_blks = p->blks.empty() ? nullptr : (const GBlk *)base;
+    p->dev_segs = p->segs.empty() ? nullptr : (const GSeg *)(base + nb);
+    p->dev_outs = no ? (const MixOut *)(base + nb + ns) : nullptr;
+  }
+  return build_variant(p, native_prec(p->dtype));
 }
 
 }  // namespace
+
+// Make (once) the tile list of one precision.  A host-side copy runs, so an exec inside a graph
+// capture refuses a precision that was never prepared.
+int bc_ensure(const cxa_bc_plan *cp, int prec, hipStream_t s) {
+  cxa_bc_plan *p = const_cast<cxa_bc_plan *>(cp);
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (p->var[prec].built) return CXA_OK;
+  if (s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    CXA_HIP(hipStreamIsCapturing(s, &st));
+    CXA_CHECK(st == hipStreamCaptureStatusNone, CXA_ERR_INVALID,
+              "block contract: the precision %d tile list of this plan is made on first use, "
+              "which cannot happen during a graph capture (run or configure it once first)",
+              prec);
+  }
+  return build_variant(p, prec);
+}
+
+int bc_prepare(const cxa_bc_plan *p) {
+  for (int q = 0; q < 3; ++q) {
+    const int rc = bc_ensure(p, q, nullptr);
+    if (rc) return rc;
+  }
+  return CXA_OK;
+}
+
 }  // namespace cxa
 
 using namespace cxa;
@@ -451,11 +505,14 @@ extern "C" int cxa_block_contract_set_config(cxa_bc_plan *p, int prec, int kind,
             "mix plans run 16x16, 32x16 or 16x32 tiles (kind %d)", kind);
   CXA_CHECK(!p->mix || split <= 1, CXA_ERR_UNSUPPORTED,
             "mix plans run eight- or four-wave workgroups (split 0 or 1)");
+  std::lock_guard<std::mutex> lk(p->mu);
   for (int q = 0; q < 3; ++q) {
     if (prec >= 0 && q != prec) continue;
-    if (p->var[q].kind == kind && p->var[q].split_mode == split && p->var[q].d_tiles.p) continue;
-    p->var[q].kind = kind;
-    p->var[q].split_mode = split;
+    PlanVariant &v = p->var[q];
+    if (v.kind == kind && v.split_mode == split && v.built) continue;
+    v.kind = kind;
+    v.split_mode = split;
+    if (!v.built && q != prec) continue;  // made with this configuration on first use
     int rc = build_variant(p, q);
     if (rc) return rc;
   }
@@ -466,6 +523,8 @@ extern "C" int cxa_block_contract_get_config(const cxa_bc_plan *p, int prec, int
                                              int *split, int *ntiles) {
   CXA_CHECK(p && prec >= 0 && prec <= CXA_PREC_FP32X3, CXA_ERR_INVALID,
             "cxa_block_contract_get_config: bad arguments");
+  int rc = bc_ensure(p, prec, nullptr);
+  if (rc) return rc;
   const PlanVariant &v = p->var[prec];
   if (kind) *kind = v.kind;
   if (split) *split = v.splitk == 8 ? 2 : (v.splitk ? 1 : 0);
@@ -480,8 +539,10 @@ extern "C" int cxa_block_contract_get_config(const cxa_bc_plan *p, int prec, int
 extern "C" int cxa_block_contract_set_tile(cxa_bc_plan *p, int kind) {
   CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_set_tile: null plan");
   CXA_CHECK(kind >= -1 && kind < TK_COUNT, CXA_ERR_INVALID, "unknown tile kind %d", kind);
+  std::lock_guard<std::mutex> lk(p->mu);
   for (int q = 0; q < 3; ++q) {
     p->var[q].kind = kind;
+    if (!p->var[q].built) continue;  // made with this kind on first use
     int rc = build_variant(p, q);
     if (rc) return rc;
   }
@@ -533,10 +594,12 @@ int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, i
   g.A = p->copyA ? (const void *)ws : L;
   g.B = p->copyB ? (const void *)wsB : R;
   g.C = C;
+  int rc = bc_ensure(p, prec_eff, s);
+  if (rc) return rc;
   const PlanVariant &v = p->var[prec_eff];
   g.tiles = (const GTile *)v.d_tiles.p;
-  g.blks = (const GBlk *)p->d_blks.p;
-  g.segs = lay->segs ? lay->segs : (const GSeg *)p->d_segs.p;
+  g.blks = p->dev_blks;
+  g.segs = lay->segs ? lay->segs : p->dev_segs;
   g.a_k4 = lay->a_k4;
   g.b_k4 = lay->b_k4;
   g.c_k4 = lay->c_k4;
@@ -548,7 +611,7 @@ int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, i
   g.dot_x = lay->dot_x;
   g.dot_part = lay->dot_part;
   if (p->mix) {
-    g.outs = lay->outs ? lay->outs : (const MixOut *)p->d_outs.p;
+    g.outs = lay->outs ? lay->outs : p->dev_outs;
     g.kind0 = v.launch_kind;
   }
   return launch_grouped_gemm(g, s);
@@ -658,6 +721,7 @@ int64_t bc_workspace_elems(const cxa_bc_plan *p) { return p->wsA + p->wsB; }
 
 int bc_dot_slots(const cxa_bc_plan *p, int prec) {
   if (!p || p->mix || is_complex(p->dtype) || prec < 0 || prec > CXA_PREC_FP32X3) return 0;
+  if (bc_ensure(p, prec, nullptr)) return 0;
   const PlanVariant &v = p->var[prec];
   if (v.tile_class == 1 || v.tiles.empty()) return 0;  // make_tiles' row-streaming class
   return (int)v.tiles.size();
@@ -670,8 +734,10 @@ int bc_dtype(const cxa_bc_plan *p) { return p->dtype; }
 extern "C" int cxa_block_contract_set_split(cxa_bc_plan *p, int mode) {
   CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_set_split: null plan");
   CXA_CHECK(mode >= -1 && mode <= 2, CXA_ERR_INVALID, "split mode must be -1, 0, 1 or 2");
+  std::lock_guard<std::mutex> lk(p->mu);
   for (int q = 0; q < 3; ++q) {
     p->var[q].split_mode = mode;
+    if (!p->var[q].built) continue;  // made with this mode on first use
     int rc = build_variant(p, q);
     if (rc) return rc;
   }

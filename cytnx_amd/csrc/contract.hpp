@@ -53,6 +53,10 @@ int bc_pack_a(const cxa_bc_plan *p, int64_t elems, std::vector<GSeg> &segs,
 int bc_mix_pack_outs(const cxa_bc_plan *mix, const std::vector<PackRegion> &regions,
                      std::vector<MixOut> &outs);
 bool bc_is_mix(const cxa_bc_plan *p);
+// Make a plan's tile list for one precision if it is not made yet (the plan makes only its own
+// precision at creation); refused while `s` is capturing a graph.  bc_prepare makes all three.
+int bc_ensure(const cxa_bc_plan *p, int prec, hipStream_t s);
+int bc_prepare(const cxa_bc_plan *p);
 // Tile slots a dot-product epilogue of an execution at precision prec writes (the launch's tile
 // list length), or 0 when that launch cannot carry one (mix, complex, row-streaming tiles).
 int bc_dot_slots(const cxa_bc_plan *p, int prec);

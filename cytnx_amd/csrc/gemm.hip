@@ -51,6 +51,7 @@ int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
   };
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
   std::vector<Item> items;
+  items.reserve(order.size() * 4);
   for (int bi : order) {
     const GBlk &b = blks[bi];
     int kind;
@@ -72,19 +73,31 @@ int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
   // kXcds contiguous ranges of equal tile count (at these sizes a tile's time is mostly its
   // fixed chain of dependent loads, so equal counts balance better than equal flops, and the
   // ranges need no padding beyond the last few tiles)
-  std::vector<std::vector<Item>> ranges(kXcds);
+  // Within a range the tiles go in decreasing cost, ties in list order.  A block's tiles share
+  // one cost and sit together, so the range is sorted as runs of equal cost (one per block
+  // piece): the same order as sorting the tiles, for hundreds of runs instead of thousands of
+  // tiles.
   const size_t per = (items.size() + kXcds - 1) / kXcds;
-  for (size_t i = 0; i < items.size(); ++i) ranges[per ? i / per : 0].push_back(items[i]);
-  size_t P = 0;
-  for (auto &r : ranges) {
-    std::stable_sort(r.begin(), r.end(),
-                     [](const Item &x, const Item &y) { return x.cost > y.cost; });
-    P = std::max(P, r.size());
-  }
-  P = (P + kWaves - 1) / kWaves * kWaves;
+  const size_t P = per ? (per + kWaves - 1) / kWaves * kWaves : 0;
   tiles.assign(kXcds * P, GTile{-1, 0, 0, 0});
-  for (int x = 0; x < kXcds; ++x)
-    for (size_t i = 0; i < ranges[x].size(); ++i) tiles[x * P + i] = ranges[x][i].t;
+  struct Run {
+    size_t at, len;
+    int64_t cost;
+  };
+  std::vector<Run> runs;
+  for (int x = 0; x < kXcds; ++x) {
+    const size_t lo = std::min(items.size(), x * per), hi = std::min(items.size(), lo + per);
+    runs.clear();
+    for (size_t i = lo; i < hi; ++i) {
+      if (!runs.empty() && runs.back().cost == items[i].cost) ++runs.back().len;
+      else runs.push_back({i, 1, items[i].cost});
+    }
+    std::stable_sort(runs.begin(), runs.end(),
+                     [](const Run &a, const Run &b) { return a.cost > b.cost; });
+    size_t o = x * P;
+    for (const Run &r : runs)
+      for (size_t i = 0; i < r.len; ++i) tiles[o++] = items[r.at + i].t;
+  }
   *per_xcd = (int)P;
   return cls;
 }

@@ -375,6 +375,10 @@ extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
   CXA_CHECK(c->dtype == CXA_F64 || prec == CXA_PREC_FP64, CXA_ERR_UNSUPPORTED,
             "cxa_chain_prepare: complex chains run in FP64 only");
   c->prec = prec;
+  for (int k = 0; k < c->nsteps; ++k) {  // tile lists of this precision, before any capture
+    const int rc = bc_ensure(c->plans[k], prec, nullptr);
+    if (rc) return rc;
+  }
   if (prec != CXA_PREC_FP32 || !c->fixed32.empty()) return CXA_OK;
   hipStream_t s = (hipStream_t)stream;
   c->fixed32.assign(c->nsteps, nullptr);

@@ -161,7 +161,10 @@ int cxa_u1_enumerate_blocks(int rank, const int *btype, const int64_t *nq, const
  * by cxa_block_contract_info (NULL allowed when 0).  prec: CXA_PREC_* for real dtypes
  * (complex dtypes always use their native precision).  Plans are bound to the device that was
  * current at creation and may be executed concurrently on different streams with different
- * workspaces.
+ * workspaces.  A plan makes the tile list of its dtype's own precision at creation and those of
+ * the other real precisions on their first exec, configuration or chain preparation (a
+ * synchronous upload): run or configure a precision once before capturing its exec in a graph
+ * (an exec inside a capture that would need the upload returns CXA_ERR_INVALID).
  */
 typedef struct cxa_bc_plan cxa_bc_plan;
 int cxa_block_contract_plan(int dtype, int rankL, int nbL, const uint64_t *itoiL,
