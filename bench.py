@@ -346,6 +346,7 @@ def bench_dmrg(torch, L=100, chi=2048, sweeps=4, maxit=30, cpu=True):
                       "lanczos_s": st["lanczos"], "lanczos_tflops":
                           st["flops"] / st["lanczos"] / 1e12 if st["lanczos"] else None,
                       "heff_plan_s": st["heff"], "split_s": st["svd"], "env_s": st["env"],
+                      "fp64_updates": st["fp64_updates"],
                       "max_bond": max(dm.bond_dims())}
         log("dmrg", prec, json.dumps(runs[prec]))
         last = dm

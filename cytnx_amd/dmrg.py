@@ -10,7 +10,10 @@ Index conventions (those of the example and of ``heff.Heff2``):
 
 One bond update (p, p+1): psi = A_p . A_{p+1}; ground state of the two-site effective
 Hamiltonian (Heff2: the L.psi.W12 mix launch and the .R grouped GEMM) by the native Lanczos
-driver, FP64 or FP32 + FP64 refinement; Svd_truncate(psi, min(d dim_l, d dim_r, chi)) with the
+driver, FP64 or FP32 + FP64 refinement (in mixed mode a bond whose predecessor converged in
+fewer than MIXED_MIN_ITERS iterations runs FP64 directly; operators of 2^17 elements or more
+run tilings tuned once per size class and precision); Svd_truncate(psi, min(d dim_l, d dim_r,
+chi)) with the
 singular values normalised; the singular values absorbed towards the sweep direction; and the
 environment of the site left behind updated by three block contractions
 (L' = ((L.A).M).A^dagger, as the example's network "L: -2,-1,-3 / A: -1,-4,1 / M: -2,0,-4,-5 /
@@ -35,7 +38,7 @@ from .heisenberg import MPO_GROUPS, mpo_bond, phys_bond, xxz_mpo_block
 from .lanczos import ground_state_mixed, lanczos_gnd_native
 from .mps import bond_sectors
 from .svd import Gesvd, Svd_truncate, orthogonal_split, truncated_split
-from .unitensor import BD_BRA, BD_KET, Bond, Contract, UniTensor
+from .unitensor import BD_BRA, BD_KET, PREC_FP32, PREC_FP64, Bond, Contract, UniTensor
 
 
 def _bra(A: UniTensor) -> UniTensor:
@@ -223,9 +226,17 @@ def right_canonicalize(As: List[UniTensor], svd: str = "gram") -> List[UniTensor
 class DMRG:
     """Two-site DMRG for the open XXZ chain (see module docstring)."""
 
+    # operators with fewer elements run the default tilings (tuning would cost more than it
+    # saves); larger ones are tuned once per precision and power-of-two size class
+    TUNE_MIN_ELEMS = 1 << 17
+    # in mixed mode a bond runs FP32 + refinement when the previous bond update took at least
+    # this many iterations, plain FP64 otherwise: the refinement's few FP64 iterations only pay
+    # off against a long FP32 phase (a converged sweep needs a handful of iterations per bond)
+    MIXED_MIN_ITERS = 16
+
     def __init__(self, L: int, chi: int, J: float = 1.0, Jz: float = 1.0, seed: int = 1,
                  device: int = 0, maxit: int = 30, cvg: float = 1e-14, prec: str = "fp64",
-                 mix: bool = True, Q: int = 0, svd: str = "gram"):
+                 mix: bool = True, Q: int = 0, svd: str = "gram", tune: bool = True):
         if prec not in ("fp64", "mixed"):
             raise ValueError("prec is 'fp64' or 'mixed'")
         if svd not in ("gram", "gesvd"):
@@ -234,6 +245,9 @@ class DMRG:
             raise ValueError("DMRG needs at least 2 sites")
         self.L, self.chi, self.device = L, chi, device
         self.maxit, self.cvg, self.prec, self.mix, self.svd = maxit, cvg, prec, mix, svd
+        self.tune = tune
+        self._tilings: Dict[tuple, list] = {}
+        self._last_iters: Optional[int] = None
         self.M = xxz_mpo_block(J, Jz, device=device)
         self.As = random_right_canonical_mps(L, chi, seed, device, Q)
         self.LR: List[Optional[UniTensor]] = [None] * (L + 1)
@@ -243,7 +257,22 @@ class DMRG:
             self.LR[k] = right_env_step(self.LR[k + 1], self.As[k], self.M)
         self.energies: List[float] = []
         self.stats: Dict[str, float] = {"heff": 0.0, "lanczos": 0.0, "svd": 0.0, "env": 0.0,
-                                        "matvecs": 0, "flops": 0.0, "updates": 0}
+                                        "matvecs": 0, "flops": 0.0, "updates": 0,
+                                        "fp64_updates": 0}
+
+    def _tile(self, H: Heff2, precs) -> None:
+        """Tilings of the operator's launches: tuned on the first operator of each size class
+        and precision (Heff2.tune), reused for the later ones."""
+        if not self.tune or not H.chain.steps or H.nx() < self.TUNE_MIN_ELEMS:
+            return
+        size_class = int(np.log2(H.nx()))
+        for prec in precs:
+            H.set_precision(prec)
+            choice = self._tilings.get((prec, size_class))
+            if choice is None:
+                self._tilings[(prec, size_class)] = H.tune(5)
+            else:
+                H.set_tiles(choice)
 
     def _optimize(self, p: int) -> UniTensor:
         t0 = time.perf_counter()
@@ -255,17 +284,24 @@ class DMRG:
                   self.M.relabel(["-5", "-6", "-2", "1"]),
                   self.M.relabel(["-6", "-7", "-3", "2"]),
                   self.LR[p + 2].relabel(["-7", "-4", "3"]), mix=self.mix)
+        mixed = self.prec == "mixed" and (self._last_iters is None or
+                                          self._last_iters >= self.MIXED_MIN_ITERS)
+        self._tile(H, (PREC_FP32, PREC_FP64) if mixed else (PREC_FP64,))
+        H.set_precision(PREC_FP64)
         torch.cuda.synchronize(self.device)
         t1 = time.perf_counter()
         x0 = psi._storage / torch.linalg.vector_norm(psi._storage)
-        if self.prec == "fp64":
+        if not mixed:
             E, v, n, _, _ = lanczos_gnd_native(H, x0, self.cvg, self.maxit)
+            self._last_iters = n
+            self.stats["fp64_updates"] += 1
         else:
             # the bond's iteration budget goes to the FP32 phase; the FP64 refinement polishes
             # the Ritz vector with a sixth of it (at least four iterations)
             E, v, n1, n2 = ground_state_mixed(H, x0, self.cvg, self.maxit,
                                               refine_maxiter=max(4, self.maxit // 6))
             n = n1 + n2
+            self._last_iters = n1
         torch.cuda.synchronize(self.device)
         t2 = time.perf_counter()
         out = psi.clone_meta()
