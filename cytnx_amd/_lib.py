@@ -10,7 +10,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcytnx_amd.so")
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_NOMEM, ERR_NUMERIC = 0, 1, 2, 3, 4, 5
 
@@ -45,6 +45,9 @@ def _declare(L):
         "cxa_last_error": (ctypes.c_char_p, []),
         "cxa_abi_version": (ci, []),
         "cxa_permute": (ci, [ci, ci, u64p, u64p, vp, vp, vp]),
+        "cxa_block_permute_plan": (ci, [ci, ci, i64, u64p, i64p, i64p, u64p, ctypes.c_void_p]),
+        "cxa_block_permute_exec": (ci, [vp, vp, vp, vp]),
+        "cxa_block_permute_destroy": (ci, [vp]),
         "cxa_matmul": (ci, [ci, vp, vp, vp, i64, i64, i64, vp]),
         "cxa_gemm_batch": (ci, [ci, ctypes.c_char_p, ctypes.c_char_p, i64p, i64p, i64p, dblp,
                                 ctypes.POINTER(vp), i64p, ctypes.POINTER(vp), i64p, dblp,

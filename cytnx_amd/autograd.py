@@ -29,7 +29,7 @@ import torch
 from . import _lib
 from .device import stream_handle, torch_device
 from .unitensor import (CPU, PREC_FP64, TORCH_OF, UniTensor, _common, _plan_for, _Plan,
-                        _promote_pair, _scalar_block)
+                        _promote_pair, _scalar_block, block_permute_plan)
 
 
 def _permuted_meta(T: UniTensor, axes: Sequence[int]) -> UniTensor:
@@ -46,15 +46,10 @@ def _permuted_meta(T: UniTensor, axes: Sequence[int]) -> UniTensor:
 def _block_permute(src: torch.Tensor, meta: UniTensor, mapper: Sequence[int],
                    dst: torch.Tensor) -> None:
     """Every block of `src` (layout `meta`) permuted by `mapper` into the same offsets of dst."""
-    L = _lib.lib()
-    s = stream_handle(src.device.index)
-    es = src.element_size()
-    mpa = _lib.u64_array(mapper)
     dt = {torch.float64: 3, torch.float32: 4, torch.complex128: 1, torch.complex64: 2}[src.dtype]
-    for bi in range(meta.Nblocks()):
-        o = int(meta._offsets[bi]) * es
-        _lib.check(L.cxa_permute(dt, meta.rank(), meta._shapes[bi].ctypes.data_as(_lib.u64p),
-                                 mpa, src.data_ptr() + o, dst.data_ptr() + o, s))
+    plan = block_permute_plan(dt, src.device.index, meta._shapes, meta._offsets, mapper)
+    _lib.check(_lib.lib().cxa_block_permute_exec(plan.handle, src.data_ptr(), dst.data_ptr(),
+                                                 stream_handle(src.device.index)))
 
 
 class _Spec:
@@ -172,15 +167,33 @@ def contract_ad(A: UniTensor, B: UniTensor) -> UniTensor:
     out_labels = [A._labels[i] for i in nc1] + [B._labels[i] for i in nc2]
     rr = A.rowrank() + B.rowrank() - sum(1 for i in c1 if i < A.rowrank()) \
         - sum(1 for i in c2 if i < B.rowrank())
+    # the output's block list follows from the operands' layouts and the free bonds: reuse the
+    # one made for the same structure (a sweep contracts the same structures again and again)
+    key = (dt, dev, A._itoi.tobytes(), A._shapes.tobytes(), B._itoi.tobytes(),
+           B._shapes.tobytes(), tuple(c1), tuple(c2),
+           tuple((b._type, tuple(b._qnums), tuple(b._degs), b._dim) for b in out_bonds))
+    hit = _OUTS.get(key)
+    if hit is None:
+        if out_bonds:
+            tmpl = UniTensor(out_bonds, out_labels, max(0, min(rr, len(out_bonds))), dt, dev,
+                             _alloc=False)
+        else:
+            tmpl = _scalar_block(dt, dev)
+            tmpl._storage = tmpl._storage[:0]
+        hit = (tmpl, _spec_for(A, B, c1, c2, tmpl, dt))
+        if len(_OUTS) > 2048:
+            _OUTS.clear()
+        _OUTS[key] = hit
+    tmpl, sp = hit
+    out = tmpl.clone_meta()
+    out._labels = out_labels
     if out_bonds:
-        out = UniTensor(out_bonds, out_labels, max(0, min(rr, len(out_bonds))), dt, dev,
-                        _alloc=False)
-    else:
-        out = _scalar_block(dt, dev)
-        out._storage = out._storage[:0]
-    sp = _spec_for(A, B, c1, c2, out, dt)
+        out._rowrank = max(0, min(rr, len(out_bonds)))
     out._storage = BlockContractFn.apply(A._storage, B._storage, sp)
     return out
+
+
+_OUTS: Dict[tuple, Tuple[UniTensor, _Spec]] = {}
 
 
 def dagger_ad(A: UniTensor) -> UniTensor:

@@ -14,4 +14,4 @@ const std::string &get_error() { return g_last_error; }
 
 extern "C" const char *cxa_last_error(void) { return cxa::get_error().c_str(); }
 
-extern "C" int cxa_abi_version(void) { return 4; }
+extern "C" int cxa_abi_version(void) { return 5; }

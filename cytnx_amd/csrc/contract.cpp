@@ -93,7 +93,12 @@ struct cxa_bc_plan {
   PlanVariant var[3];  // indexed by CXA_PREC_*
   std::mutex mu;       // guards building a variant on first use
   bool copyA = false, copyB = false;
-  std::vector<cxa::CopyJob> jobsA, jobsB;
+  std::vector<cxa::CopyJob> jobsA, jobsB;  // blocks too large for the batched copy
+  // the other operand blocks, permuted into the workspace by one batched launch per operand
+  std::vector<cxa::BatchCopyJob> bjobsA, bjobsB;
+  std::vector<cxa::BatchCopyTask> btasksA, btasksB;
+  const cxa::BatchCopyJob *dev_bjobsA = nullptr, *dev_bjobsB = nullptr;
+  const cxa::BatchCopyTask *dev_btasksA = nullptr, *dev_btasksB = nullptr;
   int64_t wsA = 0, wsB = 0;  // workspace elements for A and B copies
   double flops = 0;
   int64_t npairs = 0;
@@ -149,23 +154,55 @@ int native_prec(int dtype) {
 // precision is made now, the others on first use (a tile list of a large contraction takes
 // about a millisecond of host time, and most plans only ever run one precision).
 int upload_plan(cxa_bc_plan *p) {
-  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-  const size_t nb = up(p->blks.size() * sizeof(GBlk)), ns = up(p->segs.size() * sizeof(GSeg));
-  const size_t no = p->mix ? up(p->outs.size() * sizeof(MixOut)) : 0;
-  if (nb + ns + no) {
-    std::vector<char> host(nb + ns + no, 0);
-    if (!p->blks.empty()) std::memcpy(host.data(), p->blks.data(), p->blks.size() * sizeof(GBlk));
-    if (!p->segs.empty())
-      std::memcpy(host.data() + nb, p->segs.data(), p->segs.size() * sizeof(GSeg));
-    if (no) std::memcpy(host.data() + nb + ns, p->outs.data(), p->outs.size() * sizeof(MixOut));
+  // blks, segs, outs and the batched copy jobs / tasks of both operands, each 256-byte aligned
+  struct Part {
+    const void *src;
+    size_t bytes, at;
+  };
+  Part parts[7] = {{p->blks.data(), p->blks.size() * sizeof(GBlk), 0},
+                   {p->segs.data(), p->segs.size() * sizeof(GSeg), 0},
+                   {p->outs.data(), p->mix ? p->outs.size() * sizeof(MixOut) : 0, 0},
+                   {p->bjobsA.data(), p->bjobsA.size() * sizeof(BatchCopyJob), 0},
+                   {p->btasksA.data(), p->btasksA.size() * sizeof(BatchCopyTask), 0},
+                   {p->bjobsB.data(), p->bjobsB.size() * sizeof(BatchCopyJob), 0},
+                   {p->btasksB.data(), p->btasksB.size() * sizeof(BatchCopyTask), 0}};
+  size_t total = 0;
+  for (Part &q : parts) {
+    q.at = total;
+    total += (q.bytes + 255) / 256 * 256;
+  }
+  if (total) {
+    std::vector<char> host(total, 0);
+    for (const Part &q : parts)
+      if (q.bytes) std::memcpy(host.data() + q.at, q.src, q.bytes);
     CXA_HIP(hipMalloc(&p->d_meta.p, host.size()));
     CXA_HIP(hipMemcpy(p->d_meta.p, host.data(), host.size(), hipMemcpyHostToDevice));
     char *base = (char *)p->d_meta.p;
-    p->dev_blks = p->blks.empty() ? nullptr : (const GBlk *)base;
-    p->dev_segs = p->segs.empty() ? nullptr : (const GSeg *)(base + nb);
-    p->dev_outs = no ? (const MixOut *)(base + nb + ns) : nullptr;
+    auto at = [&](int i) -> const void * { return parts[i].bytes ? base + parts[i].at : nullptr; };
+    p->dev_blks = (const GBlk *)at(0);
+    p->dev_segs = (const GSeg *)at(1);
+    p->dev_outs = (const MixOut *)at(2);
+    p->dev_bjobsA = (const BatchCopyJob *)at(3);
+    p->dev_btasksA = (const BatchCopyTask *)at(4);
+    p->dev_bjobsB = (const BatchCopyJob *)at(5);
+    p->dev_btasksB = (const BatchCopyTask *)at(6);
   }
   return build_variant(p, native_prec(p->dtype));
+}
+
+// Route a block's permute-to-workspace either into the batched launch (job + tasks) or, when
+// it is too large, onto the one-by-one list.
+void add_copy(const CopyJob &j, std::vector<CopyJob> &big, std::vector<BatchCopyJob> &bj,
+              std::vector<BatchCopyTask> &bt) {
+  BatchCopyJob b;
+  if (!batch_copy_job(j.rank, j.shape, j.mapper, j.src_off, j.dst_off, &b)) {
+    big.push_back(j);
+    return;
+  }
+  const int id = (int)bj.size();
+  bj.push_back(b);
+  for (int32_t at = 0; at < b.n; at += kBatchCopyTaskElems)
+    bt.push_back({id, at, std::min(kBatchCopyTaskElems, b.n - at), 0});
 }
 
 }  // namespace
@@ -338,7 +375,7 @@ extern "C" int cxa_block_contract_plan(int dtype, int rankL, int nbL, const uint
       sam[a] = K;
       sak[a] = 1;
       p->wsA += (n + 15) / 16 * 16;
-      p->jobsA.push_back(j);
+      add_copy(j, p->jobsA, p->bjobsA, p->btasksA);
       (void)M;
     }
   }
@@ -361,7 +398,7 @@ extern "C" int cxa_block_contract_plan(int dtype, int rankL, int nbL, const uint
       sbk[b] = N;
       sbn[b] = 1;
       p->wsB += (n + 15) / 16 * 16;
-      p->jobsB.push_back(j);
+      add_copy(j, p->jobsB, p->bjobsB, p->btasksB);
     }
   }
 
@@ -577,10 +614,20 @@ int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, i
                          ws + j.dst_off * esA, s);
     if (rc) return rc;
   }
+  if (!p->btasksA.empty()) {
+    int rc = launch_batch_copy(esA, p->dev_bjobsA, p->dev_btasksA, (int)p->btasksA.size(), L, ws,
+                               s);
+    if (rc) return rc;
+  }
   char *wsB = ws ? ws + p->wsA * esA : nullptr;
   for (const CopyJob &j : p->jobsB) {
     int rc = cxa_permute(b_dt, j.rank, j.shape, j.mapper, (const char *)R + j.src_off * esB,
                          wsB + j.dst_off * esB, s);
+    if (rc) return rc;
+  }
+  if (!p->btasksB.empty()) {
+    int rc = launch_batch_copy(esB, p->dev_bjobsB, p->dev_btasksB, (int)p->btasksB.size(), R,
+                               wsB, s);
     if (rc) return rc;
   }
   int prec_eff = prec;

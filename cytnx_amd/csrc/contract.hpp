@@ -22,6 +22,29 @@ struct ExecLayout {
   double *dot_part = nullptr;
 };
 
+// Batched permute of many small blocks in one launch (permute.hip).  Job j writes its
+// n elements contiguously: dst[dst_off + e] = src[src_off + sum_d idx_d(e) * sstr[d]], idx(e)
+// the row-major coordinates of e over ext[0 .. rank); a task is a contiguous element range of
+// one job, one workgroup each.
+constexpr int kBatchCopyMaxElems = 1 << 18;  // larger blocks go through cxa_permute one by one
+constexpr int kBatchCopyTaskElems = 4096;
+struct BatchCopyJob {
+  int64_t src_off, dst_off;  // elements
+  int32_t n, rank;
+  int32_t ext[16];
+  int32_t sstr[16];
+};
+struct BatchCopyTask {
+  int32_t job, start, count, pad;
+};
+// The job that permutes one row-major block of extents shape[rank] by mapper (new axis d = old
+// axis mapper[d]); false when the block is too large for the batched launch.
+bool batch_copy_job(int rank, const uint64_t *shape, const uint64_t *mapper, int64_t src_off,
+                    int64_t dst_off, BatchCopyJob *job);
+// One launch over ntasks tasks (device arrays) for elements of elem_size bytes (4, 8 or 16).
+int launch_batch_copy(int elem_size, const BatchCopyJob *jobs, const BatchCopyTask *tasks,
+                      int ntasks, const void *src, void *dst, hipStream_t s);
+
 // conj (complex plans): bit 0 conjugates L, bit 1 conjugates R as they are read.
 int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, int a_dt,
                   int b_dt, int c_dt, int prec, void *workspace, hipStream_t s, int conj = 0,

@@ -16,8 +16,11 @@
 // (src/backend/utils_internal_cpu/Movemem_cpu.cpp:18-211,
 //  src/backend/utils_internal_gpu/cuMovemem_gpu.cu:60-304): out = transpose(in, mapper).
 #include <algorithm>
+#include <cstring>
+#include <memory>
 #include <vector>
 
+#include "contract.hpp"
 #include "cxa_common.hpp"
 
 namespace cxa {
@@ -741,7 +744,92 @@ int run_permute(const Reduced &r, const void *src, void *dst, hipStream_t s) {
   return CXA_OK;
 }
 
+// Batched small-block permute: one workgroup per task; the job's reduced extents and source
+// strides sit in LDS, every thread decodes its elements' coordinates (32-bit: a job holds at
+// most kBatchCopyMaxElems elements) and gathers them; writes are contiguous.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_batch_copy(const BatchCopyJob *__restrict__ jobs,
+                                                         const BatchCopyTask *__restrict__ tasks,
+                                                         const T *__restrict__ src,
+                                                         T *__restrict__ dst) {
+  __shared__ int32_t ext[16], sstr[16];
+  const BatchCopyTask tk = tasks[blockIdx.x];
+  const BatchCopyJob *j = jobs + tk.job;
+  const int rank = j->rank;
+  if (threadIdx.x < 16) {
+    ext[threadIdx.x] = threadIdx.x < (unsigned)rank ? j->ext[threadIdx.x] : 1;
+    sstr[threadIdx.x] = threadIdx.x < (unsigned)rank ? j->sstr[threadIdx.x] : 0;
+  }
+  __syncthreads();
+  const T *s = src + j->src_off;
+  T *d = dst + j->dst_off;
+  const int end = tk.start + tk.count;
+  for (int e = tk.start + (int)threadIdx.x; e < end; e += kThreads) {
+    int rem = e, off = 0;
+    for (int a = rank - 1; a > 0; --a) {
+      const int q = rem / ext[a];
+      off += (rem - q * ext[a]) * sstr[a];
+      rem = q;
+    }
+    off += rem * sstr[0];
+    d[e] = s[off];
+  }
+}
+
 }  // namespace
+
+bool batch_copy_job(int rank, const uint64_t *shape, const uint64_t *mapper, int64_t src_off,
+                    int64_t dst_off, BatchCopyJob *job) {
+  if (rank < 0 || rank > 16) return false;
+  const Reduced r = reduce_perm(rank, shape, mapper);
+  if (r.total > kBatchCopyMaxElems) return false;
+  *job = BatchCopyJob{};
+  job->src_off = src_off;
+  job->dst_off = dst_off;
+  job->n = (int32_t)r.total;
+  job->rank = std::max(1, r.rank);
+  // row-major strides of the reduced old axes, then the new axes' extents and source strides
+  int64_t ostr[16];
+  int64_t acc = 1;
+  for (int i = r.rank - 1; i >= 0; --i) {
+    ostr[i] = acc;
+    acc *= r.shape[i];
+  }
+  if (r.rank == 0) {
+    job->ext[0] = (int32_t)r.total;
+    job->sstr[0] = 1;
+    return true;
+  }
+  for (int d = 0; d < r.rank; ++d) {
+    job->ext[d] = (int32_t)r.shape[r.perm[d]];
+    job->sstr[d] = (int32_t)ostr[r.perm[d]];
+  }
+  return true;
+}
+
+int launch_batch_copy(int elem_size, const BatchCopyJob *jobs, const BatchCopyTask *tasks,
+                      int ntasks, const void *src, void *dst, hipStream_t s) {
+  if (ntasks <= 0) return CXA_OK;
+  switch (elem_size) {
+    case 4:
+      hipLaunchKernelGGL((k_batch_copy<uint32_t>), dim3(ntasks), dim3(kThreads), 0, s, jobs,
+                         tasks, (const uint32_t *)src, (uint32_t *)dst);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_batch_copy<uint64_t>), dim3(ntasks), dim3(kThreads), 0, s, jobs,
+                         tasks, (const uint64_t *)src, (uint64_t *)dst);
+      break;
+    case 16:
+      hipLaunchKernelGGL((k_batch_copy<U128>), dim3(ntasks), dim3(kThreads), 0, s, jobs, tasks,
+                         (const U128 *)src, (U128 *)dst);
+      break;
+    default:
+      return fail(CXA_ERR_UNSUPPORTED, "batched copy: element size %d", elem_size);
+  }
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
 }  // namespace cxa
 
 extern "C" int cxa_permute(int dtype, int rank, const uint64_t *old_shape,
@@ -771,4 +859,108 @@ extern "C" int cxa_permute(int dtype, int rank, const uint64_t *old_shape,
     case 8: return run_permute<uint64_t>(r, src, dst, s);
     default: return run_permute<U128>(r, src, dst, s);
   }
+}
+
+// ------------------------------------------------------------------------------------------
+// Block permute plans (cxa_block_permute_*).
+// ------------------------------------------------------------------------------------------
+struct cxa_bp_plan {
+  int dtype = 0, es = 0, rank = 0;
+  uint64_t mapper[16] = {};
+  struct Big {
+    uint64_t shape[16];
+    int64_t src_off, dst_off;
+  };
+  std::vector<Big> big;  // blocks moved by cxa_permute one by one
+  int ntasks = 0;
+  void *dev = nullptr;  // batched jobs, then tasks (256-byte aligned)
+  const cxa::BatchCopyJob *jobs = nullptr;
+  const cxa::BatchCopyTask *tasks = nullptr;
+  ~cxa_bp_plan() {
+    if (dev) (void)hipFree(dev);
+  }
+};
+
+extern "C" int cxa_block_permute_plan(int dtype, int rank, int64_t nblocks,
+                                      const uint64_t *shapes, const int64_t *src_offsets,
+                                      const int64_t *dst_offsets, const uint64_t *mapper,
+                                      cxa_bp_plan **out) {
+  using namespace cxa;
+  CXA_CHECK(out, CXA_ERR_INVALID, "cxa_block_permute_plan: null output handle");
+  *out = nullptr;
+  const int es = elem_size(dtype);
+  CXA_CHECK(es > 0, CXA_ERR_UNSUPPORTED, "cxa_block_permute_plan: unknown dtype %d", dtype);
+  CXA_CHECK(rank >= 0 && rank <= 16, CXA_ERR_UNSUPPORTED, "cxa_block_permute_plan: rank %d > 16",
+            rank);
+  CXA_CHECK(nblocks >= 0 && (nblocks == 0 || (src_offsets && dst_offsets)) &&
+                (rank == 0 || nblocks == 0 || (shapes && mapper)),
+            CXA_ERR_INVALID, "cxa_block_permute_plan: null metadata");
+  bool seen[16] = {false};
+  for (int i = 0; i < rank; ++i) {
+    CXA_CHECK(mapper[i] < (uint64_t)rank && !seen[mapper[i]], CXA_ERR_INVALID,
+              "cxa_block_permute_plan: mapper is not a permutation");
+    seen[mapper[i]] = true;
+  }
+  std::unique_ptr<cxa_bp_plan> p(new cxa_bp_plan());
+  p->dtype = dtype;
+  p->es = es;
+  p->rank = rank;
+  for (int i = 0; i < rank; ++i) p->mapper[i] = mapper[i];
+  std::vector<BatchCopyJob> jobs;
+  std::vector<BatchCopyTask> tasks;
+  const bool batched = es == 4 || es == 8 || es == 16;
+  for (int64_t b = 0; b < nblocks; ++b) {
+    const uint64_t *sh = shapes + b * rank;
+    CXA_CHECK(src_offsets[b] >= 0 && dst_offsets[b] >= 0, CXA_ERR_INVALID,
+              "cxa_block_permute_plan: negative offset of block %lld", (long long)b);
+    BatchCopyJob j;
+    if (batched && batch_copy_job(rank, sh, mapper, src_offsets[b], dst_offsets[b], &j)) {
+      const int id = (int)jobs.size();
+      jobs.push_back(j);
+      for (int32_t at = 0; at < j.n; at += kBatchCopyTaskElems)
+        tasks.push_back({id, at, std::min(kBatchCopyTaskElems, j.n - at), 0});
+      continue;
+    }
+    cxa_bp_plan::Big g{};
+    for (int i = 0; i < rank; ++i) g.shape[i] = sh[i];
+    g.src_off = src_offsets[b];
+    g.dst_off = dst_offsets[b];
+    p->big.push_back(g);
+  }
+  if (!tasks.empty()) {
+    const size_t nj = (jobs.size() * sizeof(BatchCopyJob) + 255) / 256 * 256;
+    const size_t nt = tasks.size() * sizeof(BatchCopyTask);
+    std::vector<char> host(nj + nt, 0);
+    std::memcpy(host.data(), jobs.data(), jobs.size() * sizeof(BatchCopyJob));
+    std::memcpy(host.data() + nj, tasks.data(), nt);
+    CXA_HIP(hipMalloc(&p->dev, host.size()));
+    CXA_HIP(hipMemcpy(p->dev, host.data(), host.size(), hipMemcpyHostToDevice));
+    p->jobs = (const BatchCopyJob *)p->dev;
+    p->tasks = (const BatchCopyTask *)((char *)p->dev + nj);
+    p->ntasks = (int)tasks.size();
+  }
+  *out = p.release();
+  return CXA_OK;
+}
+
+extern "C" int cxa_block_permute_exec(const cxa_bp_plan *p, const void *src, void *dst,
+                                      cxa_stream_t stream) {
+  using namespace cxa;
+  CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_permute_exec: null plan");
+  if (p->ntasks == 0 && p->big.empty()) return CXA_OK;
+  CXA_CHECK(src && dst, CXA_ERR_INVALID, "cxa_block_permute_exec: null data pointer");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = launch_batch_copy(p->es, p->jobs, p->tasks, p->ntasks, src, dst, s);
+  if (rc) return rc;
+  for (const cxa_bp_plan::Big &g : p->big) {
+    rc = cxa_permute(p->dtype, p->rank, g.shape, p->mapper, (const char *)src + g.src_off * p->es,
+                     (char *)dst + g.dst_off * p->es, stream);
+    if (rc) return rc;
+  }
+  return CXA_OK;
+}
+
+extern "C" int cxa_block_permute_destroy(cxa_bp_plan *p) {
+  delete p;
+  return CXA_OK;
 }

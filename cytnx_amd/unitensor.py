@@ -169,6 +169,46 @@ def _plan_for(dtype: int, A: "UniTensor", B: "UniTensor", c1: List[int], c2: Lis
     return p
 
 
+class _BPPlan:
+    """A block permute plan (cxa_block_permute_plan) and its destruction."""
+
+    def __init__(self, handle: int):
+        self.handle = handle
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.lib().cxa_block_permute_destroy(self.handle)
+        except Exception:
+            pass
+
+
+_BP_CACHE: Dict[tuple, _BPPlan] = {}
+
+
+def block_permute_plan(dtype: int, device: int, shapes: np.ndarray, offsets: np.ndarray,
+                       mapper: Sequence[int]) -> _BPPlan:
+    """The (cached) plan that permutes every block of a layout (row-major blocks of extents
+    shapes[b] at element offsets offsets[b]) by mapper into the same offsets."""
+    shp = np.ascontiguousarray(shapes, dtype=np.uint64)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    rank = shp.shape[1] if shp.ndim == 2 else 0
+    key = (dtype, device, rank, shp.tobytes(), off.tobytes(), tuple(int(m) for m in mapper))
+    p = _BP_CACHE.get(key)
+    if p is not None:
+        return p
+    h = ctypes.c_void_p(0)
+    torch.cuda.set_device(device)
+    _lib.check(_lib.lib().cxa_block_permute_plan(
+        dtype, rank, off.size, shp.ctypes.data_as(_lib.u64p), off.ctypes.data_as(_lib.i64p),
+        off.ctypes.data_as(_lib.i64p), _lib.u64_array(list(mapper)), ctypes.byref(h)))
+    p = _BPPlan(h.value)
+    if len(_BP_CACHE) > 4096:
+        _BP_CACHE.clear()
+    _BP_CACHE[key] = p
+    return p
+
+
 def _block_layout(bonds: Sequence[Bond], itoi: np.ndarray) -> Tuple[np.ndarray, np.ndarray, int]:
     rank = len(bonds)
     degs = [np.asarray(b.getDegeneracies(), dtype=np.uint64) for b in bonds]
@@ -510,13 +550,10 @@ class UniTensor:
         if self._storage.numel():
             if self._storage.device.type == "cpu":
                 raise NotImplementedError("cytnx_amd implements the Device.cuda branch")
-            s = stream_handle(self.device())
-            mpa = _lib.u64_array(mp)
-            for bi in range(self.Nblocks()):
-                o = int(self._offsets[bi]) * self._storage.element_size()
-                _lib.check(L.cxa_permute(self.dtype(), self.rank(),
-                                         self._shapes[bi].ctypes.data_as(_lib.u64p), mpa,
-                                         self._storage.data_ptr() + o, dst.data_ptr() + o, s))
+            plan = block_permute_plan(self.dtype(), self.device(), self._shapes, self._offsets,
+                                      mp)
+            _lib.check(L.cxa_block_permute_exec(plan.handle, self._storage.data_ptr(),
+                                                dst.data_ptr(), stream_handle(self.device())))
         out._storage = dst
         return out
 

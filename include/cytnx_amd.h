@@ -70,6 +70,23 @@ int cxa_permute(int dtype, int rank, const uint64_t *old_shape, const uint64_t *
                 const void *src, void *dst, cxa_stream_t stream);
 
 /*
+ * Block permute: every block of a block (U(1)) tensor permuted by one mapper, the GPU part of
+ * BlockUniTensor::permute followed by contiguous() (src/BlockUniTensor.cpp:565-633, which
+ * permutes block by block).  Block b is the row-major array of extents shapes[b*rank ..] at
+ * element offset src_offsets[b] of src; its permuted copy (axis j = old axis mapper[j]) goes to
+ * element offset dst_offsets[b] of dst.  The plan is built once from the metadata (host): blocks
+ * of up to 2^18 elements of 4-, 8- or 16-byte dtypes are moved by one batched launch, larger
+ * ones by cxa_permute each.  exec enqueues on `stream`; dst must not overlap src.
+ */
+typedef struct cxa_bp_plan cxa_bp_plan;
+int cxa_block_permute_plan(int dtype, int rank, int64_t nblocks, const uint64_t *shapes,
+                           const int64_t *src_offsets, const int64_t *dst_offsets,
+                           const uint64_t *mapper, cxa_bp_plan **plan);
+int cxa_block_permute_exec(const cxa_bp_plan *plan, const void *src, void *dst,
+                           cxa_stream_t stream);
+int cxa_block_permute_destroy(cxa_bp_plan *plan);
+
+/*
  * Dense row-major matrix product out(Ml x Nr) = inl(Ml x Comm) . inr(Comm x Nr), beta = 0.
  * Replaces cuMatmul_internal_{d,f,cd,cf} (src/backend/linalg_internal_gpu/cuMatmul_internal.cu
  * :25-103; type Matmulfunc_oii, src/backend/linalg_internal_interface.hpp:128-131), called

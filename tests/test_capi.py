@@ -56,6 +56,26 @@ def test_permute_rejects_bad_mapper_on_host():
     assert rc == _lib.OK
 
 
+def test_block_permute_plan_validates_on_host():
+    L = _lib.lib()
+    h = ctypes.c_void_p(0)
+    rc = L.cxa_block_permute_plan(3, 2, 0, None, None, None, _lib.u64_array([0, 0]),
+                                  ctypes.byref(h))
+    assert rc == _lib.ERR_INVALID and b"permutation" in L.cxa_last_error()
+    rc = L.cxa_block_permute_plan(99, 2, 0, None, None, None, _lib.u64_array([1, 0]),
+                                  ctypes.byref(h))
+    assert rc == _lib.ERR_UNSUPPORTED
+    rc = L.cxa_block_permute_plan(3, 2, 1, None, None, None, _lib.u64_array([1, 0]),
+                                  ctypes.byref(h))
+    assert rc == _lib.ERR_INVALID
+    # no blocks: an empty plan, built and executed without touching the device
+    rc = L.cxa_block_permute_plan(3, 2, 0, None, None, None, _lib.u64_array([1, 0]),
+                                  ctypes.byref(h))
+    assert rc == _lib.OK and h.value
+    assert L.cxa_block_permute_exec(h, None, None, None) == _lib.OK
+    assert L.cxa_block_permute_destroy(h) == _lib.OK
+
+
 def test_check_maps_status_to_exceptions():
     with pytest.raises(_lib.CxaInvalid):
         L = _lib.lib()

@@ -145,3 +145,46 @@ def test_block_permute_fixtures_per_block(case, perm):
         key = tuple(t[m] for m in perm)
         got = _gpu_permute(blk, perm)
         assert np.array_equal(_bits(got), _bits(lookup[key]))
+
+
+@pytest.mark.parametrize("case,perm", [(1, [1, 2, 0]), (2, [1, 0]), (3, [1, 0])])
+def test_unitensor_block_permute_fixtures(case, perm):
+    """UniTensor.permute of the reference's block tensors (one batched block-permute plan per
+    layout) against the reference's answers, bit for bit."""
+    from cytnx_amd.unitensor import UniTensor
+    T = UniTensor.Load(os.path.join(REF_FIXTURES, "permute_T%d.cytnx" % case), 0)
+    ans = po.from_saved(fileio.read_unitensor(os.path.join(REF_FIXTURES,
+                                                           "permute_ans%d.cytnx" % case)))
+    P = T.permute(perm)
+    lookup = {tuple(t): b for t, b in zip(ans.itoi, ans.blocks)}
+    assert len(P.itoi()) == len(ans.itoi)
+    for i, t in enumerate(P.itoi()):
+        got = P.get_block_(i).cpu().numpy()
+        assert np.array_equal(_bits(got), _bits(lookup[tuple(t)]))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.complex64, np.complex128])
+def test_block_permute_plan_small_and_large_blocks(dtype):
+    """A block tensor whose blocks go through both routes of a block-permute plan: the small
+    ones in one batched launch, the 72 x 64 x 64 one (> 2^18 elements) through cxa_permute;
+    every block bit-exact against numpy.transpose."""
+    from cytnx_amd.unitensor import BD_BRA, BD_KET, Bond, UniTensor
+    rng = np.random.default_rng(5)
+    bl = Bond(BD_KET, [[0], [1]], [72, 3])
+    bs = Bond(BD_KET, [[0], [1]], [64, 2])
+    br = Bond(BD_BRA, [[0], [1], [2]], [64, 5, 2])
+    dt = {np.float32: 4, np.float64: 3, np.complex64: 2, np.complex128: 1}[dtype]
+    T = UniTensor([bl, bs, br], ["l", "s", "r"], 2, dt, device=0)
+    x = rng.standard_normal(T.storage().numel())
+    if np.issubdtype(dtype, np.complexfloating):
+        x = x + 1j * rng.standard_normal(x.size)
+    T._storage = torch.from_numpy(x.astype(dtype)).to("cuda:0")
+    sizes = [int(np.prod(T._shapes[i].astype(np.int64))) for i in range(T.Nblocks())]
+    assert max(sizes) > 1 << 18 and min(sizes) < 1 << 18
+    for perm in ([2, 0, 1], [1, 2, 0], [0, 2, 1], [0, 1, 2]):
+        P = T.permute(perm)
+        for i in range(T.Nblocks()):
+            src = T.get_block_(i).cpu().numpy()
+            j = P.itoi().index([T.itoi()[i][m] for m in perm])
+            got = P.get_block_(j).cpu().numpy()
+            assert np.array_equal(_bits(got), _bits(np.transpose(src, perm)))
