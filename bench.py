@@ -228,7 +228,8 @@ def bench_permute(torch, nperm=32):
 
 def bench_c1(torch, cpu=True, iters=100):
     """Dense one-site Heff at chi=128 (L, A, R iid N(0,1), exact XXZ MPO D=5, d=2), order
-    ((L.A).W).R, `iters` FP64 Lanczos iterations (CvgCrit below any reachable |dE|)."""
+    ((L.A).W).R with autotuned tilings, `iters` FP64 Lanczos iterations (CvgCrit below any
+    reachable |dE|)."""
     import numpy as np
     from cytnx_amd import heisenberg
     from cytnx_amd.heff import Heff1Dense
@@ -243,6 +244,7 @@ def bench_c1(torch, cpu=True, iters=100):
                    torch.from_numpy(Rh).cuda(), Ah.shape)
     x0 = torch.from_numpy(Ah.ravel().copy()).cuda()
     cvg = 1e-300
+    tiles = H.tune(reps=10)  # the library's autotuner, on this operator's own buffers
     lanczos_gnd_native(H, x0, cvg, iters, batch=iters)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -251,7 +253,8 @@ def bench_c1(torch, cpu=True, iters=100):
     t = time.perf_counter() - t0
     flops = 4 * D * d * chi ** 3 + 2 * D * D * d * d * chi * chi
     out = {"iterations": int(k), "seconds": t, "flops_per_matvec": flops,
-           "tflops": k * flops / t / 1e12, "ms_per_iteration": t / k * 1e3, "E": E}
+           "tflops": k * flops / t / 1e12, "ms_per_iteration": t / k * 1e3, "E": E,
+           "tuned_tiles": tiles}
     if cpu:
         sys.path.insert(0, os.path.join(HERE, "oracle"))
         import pyoracle as po

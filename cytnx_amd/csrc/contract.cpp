@@ -190,19 +190,32 @@ int upload_plan(cxa_bc_plan *p) {
   return build_variant(p, native_prec(p->dtype));
 }
 
-// Route a block's permute-to-workspace either into the batched launch (job + tasks) or, when
-// it is too large, onto the one-by-one list.
-void add_copy(const CopyJob &j, std::vector<CopyJob> &big, std::vector<BatchCopyJob> &bj,
-              std::vector<BatchCopyTask> &bt) {
-  BatchCopyJob b;
-  if (!batch_copy_job(j.rank, j.shape, j.mapper, j.src_off, j.dst_off, &b)) {
-    big.push_back(j);
+// Split an operand's permute-to-workspace jobs: small blocks into one batched launch (jobs +
+// tasks), the rest (large blocks, or a lone small block, which the tiled permute moves faster)
+// onto the one-by-one list.
+void split_copies(const std::vector<CopyJob> &all, std::vector<CopyJob> &big,
+                  std::vector<BatchCopyJob> &bj, std::vector<BatchCopyTask> &bt) {
+  std::vector<BatchCopyJob> small;
+  std::vector<const CopyJob *> from;
+  for (const CopyJob &j : all) {
+    BatchCopyJob b;
+    if (batch_copy_job(j.rank, j.shape, j.mapper, j.src_off, j.dst_off, &b)) {
+      small.push_back(b);
+      from.push_back(&j);
+    } else {
+      big.push_back(j);
+    }
+  }
+  if (small.size() < 2) {
+    for (const CopyJob *j : from) big.push_back(*j);
     return;
   }
-  const int id = (int)bj.size();
-  bj.push_back(b);
-  for (int32_t at = 0; at < b.n; at += kBatchCopyTaskElems)
-    bt.push_back({id, at, std::min(kBatchCopyTaskElems, b.n - at), 0});
+  for (const BatchCopyJob &b : small) {
+    const int id = (int)bj.size();
+    bj.push_back(b);
+    for (int32_t at = 0; at < b.n; at += kBatchCopyTaskElems)
+      bt.push_back({id, at, std::min(kBatchCopyTaskElems, b.n - at), 0});
+  }
 }
 
 }  // namespace
@@ -355,6 +368,7 @@ extern "C" int cxa_block_contract_plan(int dtype, int rankL, int nbL, const uint
       p->copyB = true;
   }
   std::vector<int64_t> wsoffA(nbL, 0), wsoffB(nbR, 0);
+  std::vector<CopyJob> copiesA, copiesB;
   if (p->copyA) {
     for (int a = 0; a < nbL; ++a) {
       if (!usedA[a]) continue;
@@ -375,7 +389,7 @@ extern "C" int cxa_block_contract_plan(int dtype, int rankL, int nbL, const uint
       sam[a] = K;
       sak[a] = 1;
       p->wsA += (n + 15) / 16 * 16;
-      add_copy(j, p->jobsA, p->bjobsA, p->btasksA);
+      copiesA.push_back(j);
       (void)M;
     }
   }
@@ -398,9 +412,12 @@ extern "C" int cxa_block_contract_plan(int dtype, int rankL, int nbL, const uint
       sbk[b] = N;
       sbn[b] = 1;
       p->wsB += (n + 15) / 16 * 16;
-      add_copy(j, p->jobsB, p->bjobsB, p->btasksB);
+      copiesB.push_back(j);
     }
   }
+
+  split_copies(copiesA, p->jobsA, p->bjobsA, p->btasksA);
+  split_copies(copiesB, p->jobsB, p->bjobsB, p->btasksB);
 
   // one GBlk per output block, segments grouped by target in pair order
   std::vector<std::vector<int>> by_target(nbC);

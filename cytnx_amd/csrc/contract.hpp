@@ -26,7 +26,7 @@ struct ExecLayout {
 // n elements contiguously: dst[dst_off + e] = src[src_off + sum_d idx_d(e) * sstr[d]], idx(e)
 // the row-major coordinates of e over ext[0 .. rank); a task is a contiguous element range of
 // one job, one workgroup each.
-constexpr int kBatchCopyMaxElems = 1 << 18;  // larger blocks go through cxa_permute one by one
+constexpr int kBatchCopyMaxElems = 1 << 15;  // larger blocks go through cxa_permute one by one
 constexpr int kBatchCopyTaskElems = 4096;
 struct BatchCopyJob {
   int64_t src_off, dst_off;  // elements

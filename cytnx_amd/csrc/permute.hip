@@ -908,25 +908,37 @@ extern "C" int cxa_block_permute_plan(int dtype, int rank, int64_t nblocks,
   for (int i = 0; i < rank; ++i) p->mapper[i] = mapper[i];
   std::vector<BatchCopyJob> jobs;
   std::vector<BatchCopyTask> tasks;
+  std::vector<int64_t> small_ids;  // blocks in the batched launch (two or more of them)
   const bool batched = es == 4 || es == 8 || es == 16;
   for (int64_t b = 0; b < nblocks; ++b) {
-    const uint64_t *sh = shapes + b * rank;
     CXA_CHECK(src_offsets[b] >= 0 && dst_offsets[b] >= 0, CXA_ERR_INVALID,
               "cxa_block_permute_plan: negative offset of block %lld", (long long)b);
     BatchCopyJob j;
-    if (batched && batch_copy_job(rank, sh, mapper, src_offsets[b], dst_offsets[b], &j)) {
-      const int id = (int)jobs.size();
+    if (batched &&
+        batch_copy_job(rank, shapes + b * rank, mapper, src_offsets[b], dst_offsets[b], &j)) {
+      small_ids.push_back(b);
       jobs.push_back(j);
-      for (int32_t at = 0; at < j.n; at += kBatchCopyTaskElems)
-        tasks.push_back({id, at, std::min(kBatchCopyTaskElems, j.n - at), 0});
       continue;
     }
     cxa_bp_plan::Big g{};
-    for (int i = 0; i < rank; ++i) g.shape[i] = sh[i];
+    for (int i = 0; i < rank; ++i) g.shape[i] = shapes[b * rank + i];
     g.src_off = src_offsets[b];
     g.dst_off = dst_offsets[b];
     p->big.push_back(g);
   }
+  if (jobs.size() < 2) {  // a lone small block: the tiled permute moves it faster
+    for (int64_t b : small_ids) {
+      cxa_bp_plan::Big g{};
+      for (int i = 0; i < rank; ++i) g.shape[i] = shapes[b * rank + i];
+      g.src_off = src_offsets[b];
+      g.dst_off = dst_offsets[b];
+      p->big.push_back(g);
+    }
+    jobs.clear();
+  }
+  for (size_t id = 0; id < jobs.size(); ++id)
+    for (int32_t at = 0; at < jobs[id].n; at += kBatchCopyTaskElems)
+      tasks.push_back({(int32_t)id, at, std::min(kBatchCopyTaskElems, jobs[id].n - at), 0});
   if (!tasks.empty()) {
     const size_t nj = (jobs.size() * sizeof(BatchCopyJob) + 255) / 256 * 256;
     const size_t nt = tasks.size() * sizeof(BatchCopyTask);

@@ -75,8 +75,9 @@ int cxa_permute(int dtype, int rank, const uint64_t *old_shape, const uint64_t *
  * permutes block by block).  Block b is the row-major array of extents shapes[b*rank ..] at
  * element offset src_offsets[b] of src; its permuted copy (axis j = old axis mapper[j]) goes to
  * element offset dst_offsets[b] of dst.  The plan is built once from the metadata (host): blocks
- * of up to 2^18 elements of 4-, 8- or 16-byte dtypes are moved by one batched launch, larger
- * ones by cxa_permute each.  exec enqueues on `stream`; dst must not overlap src.
+ * of up to 2^15 elements of 4-, 8- or 16-byte dtypes are moved by one batched launch when there
+ * are two or more of them, the others by cxa_permute each.  exec enqueues on `stream`; dst must
+ * not overlap src.
  */
 typedef struct cxa_bp_plan cxa_bp_plan;
 int cxa_block_permute_plan(int dtype, int rank, int64_t nblocks, const uint64_t *shapes,

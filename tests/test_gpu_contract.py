@@ -221,7 +221,7 @@ def test_block_contract_forced_kernel_variants(kind, split):
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128])
 def test_block_contract_batched_and_large_operand_copies(dtype):
     """Operands whose contracted legs are not adjacent are permuted into the workspace: blocks
-    up to 2^18 elements by one batched launch per operand, larger ones one by one (here A's
+    up to 2^15 elements by one batched launch per operand, larger ones one by one (here A's
     80 x 64 x 64 block); both routes against the oracle."""
     rng = np.random.default_rng(77)
     bl = Bond(BD_KET, [[0], [1]], [80, 3])
@@ -230,7 +230,7 @@ def test_block_contract_batched_and_large_operand_copies(dtype):
     bx = Bond(BD_BRA, [[-1], [0], [1], [2]], [2, 3, 4, 1])
     A = _rand_ut([bl, bs, br], ["l", "s", "r"], 2, rng, dtype)
     B = _rand_ut([br.redirect(), bx, bl.redirect()], ["r", "x", "l"], 1, rng, dtype)
-    assert max(int(np.prod(A._shapes[i].astype(np.int64))) for i in range(A.Nblocks())) > 1 << 18
+    assert max(int(np.prod(A._shapes[i].astype(np.int64))) for i in range(A.Nblocks())) > 1 << 15
     C = Contract(A, B)
     ref = po.block_contract(_oracle_of(A), _oracle_of(B))
     assert C.itoi() == ref.itoi

@@ -166,7 +166,7 @@ def test_unitensor_block_permute_fixtures(case, perm):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64, np.complex64, np.complex128])
 def test_block_permute_plan_small_and_large_blocks(dtype):
     """A block tensor whose blocks go through both routes of a block-permute plan: the small
-    ones in one batched launch, the 72 x 64 x 64 one (> 2^18 elements) through cxa_permute;
+    ones in one batched launch, the 72 x 64 x 64 one (> 2^15 elements) through cxa_permute;
     every block bit-exact against numpy.transpose."""
     from cytnx_amd.unitensor import BD_BRA, BD_KET, Bond, UniTensor
     rng = np.random.default_rng(5)
@@ -180,7 +180,7 @@ def test_block_permute_plan_small_and_large_blocks(dtype):
         x = x + 1j * rng.standard_normal(x.size)
     T._storage = torch.from_numpy(x.astype(dtype)).to("cuda:0")
     sizes = [int(np.prod(T._shapes[i].astype(np.int64))) for i in range(T.Nblocks())]
-    assert max(sizes) > 1 << 18 and min(sizes) < 1 << 18
+    assert max(sizes) > 1 << 15 and min(sizes) < 1 << 15
     for perm in ([2, 0, 1], [1, 2, 0], [0, 2, 1], [0, 1, 2]):
         P = T.permute(perm)
         for i in range(T.Nblocks()):
