@@ -46,6 +46,7 @@ def _declare(L):
         "cxa_abi_version": (ci, []),
         "cxa_permute": (ci, [ci, ci, u64p, u64p, vp, vp, vp]),
         "cxa_block_permute_plan": (ci, [ci, ci, i64, u64p, i64p, i64p, u64p, ctypes.c_void_p]),
+        "cxa_strided_batch_plan": (ci, [ci] + [i64] * 12 + [ctypes.c_void_p]),
         "cxa_block_permute_exec": (ci, [vp, vp, vp, vp]),
         "cxa_block_permute_destroy": (ci, [vp]),
         "cxa_matmul": (ci, [ci, vp, vp, vp, i64, i64, i64, vp]),

@@ -541,6 +541,59 @@ extern "C" int cxa_mix_contract_plan(int dtype, int nitems, const int64_t *items
   return CXA_OK;
 }
 
+// ------------------------------------------------------------------------------------------
+// Strided-batch plan: nbatch GEMMs of one shape whose operands are strided matrix views of
+// three buffers (see cytnx_amd.h).  One GBlk with one GSeg per batch entry; no copies.
+// ------------------------------------------------------------------------------------------
+extern "C" int cxa_strided_batch_plan(int dtype, int64_t nbatch, int64_t M, int64_t N, int64_t K,
+                                      int64_t a_bstride, int64_t sam, int64_t sak,
+                                      int64_t b_bstride, int64_t sbk, int64_t sbn,
+                                      int64_t c_bstride, int64_t ldc, cxa_bc_plan **out) {
+  CXA_CHECK(out, CXA_ERR_INVALID, "cxa_strided_batch_plan: null output handle");
+  *out = nullptr;
+  CXA_CHECK(dtype == CXA_F64 || dtype == CXA_F32 || dtype == CXA_C128 || dtype == CXA_C64,
+            CXA_ERR_UNSUPPORTED, "strided batch plan: unsupported dtype %d", dtype);
+  CXA_CHECK(nbatch >= 0 && nbatch <= INT32_MAX && M >= 0 && N >= 0 && K >= 0 &&
+                M <= INT32_MAX && N <= INT32_MAX && K <= INT32_MAX,
+            CXA_ERR_INVALID, "strided batch plan: bad sizes (%lld x %lld x %lld, %lld)",
+            (long long)M, (long long)N, (long long)K, (long long)nbatch);
+  CXA_CHECK(a_bstride >= 0 && sam >= 0 && sak >= 0 && b_bstride >= 0 && sbk >= 0 && sbn >= 0 &&
+                c_bstride >= 0 && ldc >= N,
+            CXA_ERR_INVALID, "strided batch plan: negative stride or ldc below N");
+  std::unique_ptr<cxa_bc_plan> p(new cxa_bc_plan());
+  p->dtype = dtype;
+  p->es = elem_size(dtype);
+  CXA_HIP(hipGetDevice(&p->device));
+  const double cplx = is_complex(dtype) ? 4.0 : 1.0;
+  for (int64_t b = 0; b < nbatch; ++b) {
+    GBlk g{};
+    g.c_off = b * c_bstride;
+    g.ldc = ldc;
+    g.alpha = 1.0;
+    g.beta = 0.0;
+    g.M = (int32_t)M;
+    g.N = (int32_t)N;
+    g.seg0 = (int32_t)b;
+    g.nseg = 1;
+    p->blks.push_back(g);
+    GSeg sg{};
+    sg.a_off = b * a_bstride;
+    sg.b_off = b * b_bstride;
+    sg.sam = sam;
+    sg.sak = sak;
+    sg.sbk = sbk;
+    sg.sbn = sbn;
+    sg.K = (int32_t)K;
+    p->segs.push_back(sg);
+    p->flops += 2.0 * cplx * (double)M * (double)N * (double)K;
+  }
+  p->npairs = nbatch;
+  int rc = upload_plan(p.get());
+  if (rc) return rc;
+  *out = p.release();
+  return CXA_OK;
+}
+
 extern "C" int cxa_block_contract_info(const cxa_bc_plan *p, double *flops,
                                        int64_t *workspace_bytes, int64_t *npairs) {
   CXA_CHECK(p, CXA_ERR_INVALID, "cxa_block_contract_info: null plan");

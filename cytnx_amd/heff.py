@@ -13,7 +13,8 @@ the grouped GEMMs read them in place -- the permutes of the reference's per-pair
 ``permute().reshape()`` (src/BlockUniTensor.cpp:1121-1133) disappear from the matvec.
 
 ``Heff1Dense`` is the dense one-site operator L.A.W.R of the C1 workload with the order
-((L.A).W).R; its tensors are single blocks.
+((L.A).W).R; its three steps are strided-batch GEMM plans that read and write every tensor in
+place.
 
 Both expose ``matvec`` (a cytnx_amd.lanczos.LinOp) and ``flops`` -- the algorithmic flop count
 of one matvec summed over the block pairs each step pairs, as the reference's contract does.
@@ -35,7 +36,8 @@ from .unitensor import (BD_BRA, BD_KET, PREC_FP32, PREC_FP64, Bond, Contract, Un
 
 class _Chain:
     def __init__(self, steps, device: int):
-        """steps: list of (plan, fixed UniTensor, fixed_is_left, out meta UniTensor)."""
+        """steps: list of (plan, fixed operand (a UniTensor or anything with a flat
+        ``_storage``), fixed_is_left, output meta UniTensor or output element count)."""
         self.device = device
         self.steps = steps
         self._plans = [s[0] for s in steps]  # keep the plans alive
@@ -48,7 +50,8 @@ class _Chain:
             VP(*[f._storage.data_ptr() for f in self._fixed]),
             _lib.int_array([1 if s[2] else 0 for s in steps]),
             _lib.i64_array([f._storage.numel() for f in self._fixed]),
-            _lib.i64_array([meta_elems(s[3]) for s in steps]), ctypes.byref(h)))
+            _lib.i64_array([s[3] if isinstance(s[3], int) else meta_elems(s[3])
+                            for s in steps]), ctypes.byref(h)))
         self.handle = h.value
         f = ctypes.c_double(0)
         _lib.check(_lib.lib().cxa_chain_flops(self.handle, ctypes.byref(f)))
@@ -610,31 +613,71 @@ def dense_as_block(a: torch.Tensor, labels: Sequence[str], types: Sequence[int])
     return T
 
 
+class _Flat:
+    """A fixed chain operand given as a flat device tensor."""
+
+    def __init__(self, t: torch.Tensor):
+        self._storage = t.reshape(-1).contiguous()
+
+
+def strided_batch_plan(nbatch: int, M: int, N: int, K: int, a: Sequence[int],
+                       b: Sequence[int], c: Sequence[int], dtype: int = 3) -> _Plan:
+    """cxa_strided_batch_plan: C_b = A_b . B_b for b < nbatch with A_b[m, k] at
+    a[0]*b + a[1]*m + a[2]*k, B_b[k, n] at b[0]*b + b[1]*k + b[2]*n and C_b[m, n] at
+    c[0]*b + c[1]*m + n (element offsets)."""
+    h = ctypes.c_void_p(0)
+    _lib.check(_lib.lib().cxa_strided_batch_plan(
+        dtype, nbatch, M, N, K, a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1],
+        ctypes.byref(h)))
+    return _Plan(h.value, torch.cuda.current_device())
+
+
 class Heff1Dense(_HeffBase):
-    """Dense one-site L(l,w,l') A(l,s,r) W(w,w',s,s') R(r,w',r') with order ((L.A).W).R."""
+    """Dense one-site L(l,w,l') A(l,s,r) W(w,w',s,s') R(r,w',r') with the order ((L.A).W).R,
+    every step reading and writing its tensors in place (three launches, no permute copies):
+
+        X (l', w, s, r)   = L'(l', w, l) . A(l, s, r)        over l, one GEMM
+        Y (l', r, w', s') = X . W'(w, s, w', s')              over (w, s), a batch over r
+        O (l', s', r')    = Y . R(r, w', r')                  over (r, w'), a batch over s'
+
+    with L' and W' permuted once at construction (src/linalg/Tensordot.cpp:29-110 permutes
+    both operands of every step into GEMM order instead).  Flops are the reference order's."""
 
     def __init__(self, L: torch.Tensor, W: torch.Tensor, R: torch.Tensor, A_shape):
         dev = L.device.index
-        Lt = dense_as_block(L, ["l", "w", "lp"], [BD_BRA, BD_BRA, BD_KET])
-        Wt = dense_as_block(W, ["w", "wp", "s", "sp"], [BD_KET, BD_BRA, BD_KET, BD_BRA])
-        Rt = dense_as_block(R, ["r", "wp", "rp"], [BD_KET, BD_KET, BD_BRA])
-        At = UniTensor([_trivial_bond(int(d), t) for d, t in zip(A_shape, [BD_KET, BD_BRA,
-                                                                          BD_BRA])],
-                       ["l", "s", "r"], 1, 3, dev, _alloc=False)
-        self.psi_template = At
-        steps = []
-        X, c1, c2 = contract_meta(Lt, At, 3, dev)
-        steps.append((plan_contraction(Lt, At, X, c1, c2, 3), Lt, True, X))
-        Y, c1, c2 = contract_meta(X, Wt, 3, dev)
-        steps.append((plan_contraction(X, Wt, Y, c1, c2, 3), Wt, False, Y))
-        O, c1, c2 = contract_meta(Y, Rt, 3, dev)
-        steps.append((plan_contraction(Y, Rt, O, c1, c2, 3), Rt, False, O))
-        if O.labels() != ["lp", "sp", "rp"]:
-            raise ValueError("Heff1Dense: unexpected output order %s" % O.labels())
-        self.pairs = [s[0].npairs for s in steps]
-        self.copies = [s[0].ws_bytes for s in steps]
+        torch.cuda.set_device(dev)
+        nl, D, nlp = (int(x) for x in L.shape)
+        al, d, ar = (int(x) for x in A_shape)
+        Dw, Dwp, ds, dsp = (int(x) for x in W.shape)
+        rr, Dr, nrp = (int(x) for x in R.shape)
+        if al != nl or Dw != D or ds != d or rr != ar or Dr != Dwp:
+            raise ValueError("Heff1Dense: L %s, A %s, W %s and R %s do not match"
+                             % (tuple(L.shape), tuple(A_shape), tuple(W.shape), tuple(R.shape)))
+        self._Lp = _Flat(L.to(torch.float64).permute(2, 1, 0))      # (l', w, l)
+        self._Wp = _Flat(W.to(torch.float64).permute(0, 2, 1, 3))   # (w, s, w', s')
+        self._R = _Flat(R.to(torch.float64))                        # (r, w', r')
+        p1 = strided_batch_plan(1, nlp * D, d * ar, nl, (0, nl, 1), (0, d * ar, 1),
+                                (0, d * ar))
+        p2 = strided_batch_plan(ar, nlp, Dwp * dsp, D * d, (1, D * d * ar, ar),
+                                (0, Dwp * dsp, 1), (Dwp * dsp, ar * Dwp * dsp))
+        p3 = strided_batch_plan(dsp, nlp, nrp, ar * Dwp, (1, ar * Dwp * dsp, dsp),
+                                (0, nrp, 1), (nrp, dsp * nrp))
+        steps = [(p1, self._Lp, True, nlp * D * d * ar),
+                 (p2, self._Wp, False, nlp * ar * Dwp * dsp),
+                 (p3, self._R, False, nlp * dsp * nrp)]
+        self.psi_template = UniTensor([_trivial_bond(int(x), t) for x, t in
+                                       zip(A_shape, [BD_KET, BD_BRA, BD_BRA])],
+                                      ["l", "s", "r"], 1, 3, dev, _alloc=False)
+        self.pairs = [st[0].npairs for st in steps]
+        self.copies = [st[0].ws_bytes for st in steps]
         self.chain = _Chain(steps, dev)
-        super().__init__(int(np.prod(A_shape)), dev)
+        self.ref_flops = (2.0 * nlp * D * d * ar * nl + 2.0 * nlp * ar * Dwp * dsp * D * d
+                          + 2.0 * nlp * dsp * nrp * ar * Dwp)
+        super().__init__(nlp * dsp * nrp, dev)
+
+    @property
+    def flops(self) -> float:
+        return self.ref_flops
 
 
 __all__ = ["Heff2", "Heff1Dense", "dense_as_block"]

@@ -216,6 +216,22 @@ int cxa_block_contract_exec_ex(const cxa_bc_plan *plan, const void *L, const voi
 int cxa_block_contract_destroy(cxa_bc_plan *plan);
 
 /*
+ * Strided-batch plan: for b in [0, nbatch), C_b = A_b . B_b (M x N, contraction over K) with
+ * every operand a strided matrix view of one buffer (element offsets):
+ *   A_b[m, k] at b*a_bstride + m*sam + k*sak,  B_b[k, n] at b*b_bstride + k*sbk + n*sbn,
+ *   C_b[m, n] at b*c_bstride + m*ldc + n.
+ * No counterpart in the reference, whose dense contractions permute every operand into GEMM
+ * order first (src/linalg/Tensordot.cpp:29-110); for contractions whose free or contracted
+ * legs are interleaved with a batch leg (a leg carried through unchanged), a batch over that
+ * leg reads and writes every tensor in place.  Runs through the block-contract exec / config
+ * / destroy entry points and in chains.
+ */
+int cxa_strided_batch_plan(int dtype, int64_t nbatch, int64_t M, int64_t N, int64_t K,
+                           int64_t a_bstride, int64_t sam, int64_t sak, int64_t b_bstride,
+                           int64_t sbk, int64_t sbn, int64_t c_bstride, int64_t ldc,
+                           cxa_bc_plan **plan);
+
+/*
  * Mix plan: a block contraction followed by a small linear map, in one launch.  Item i (of
  * nitems) has shape M x N, ns slices and no outputs (items[4*i..] = M, N, ns, no); its slices
  * are the next ns rows of `slices` (7 int64 each: A offset, B offset, A strides along m and k,

@@ -76,6 +76,17 @@ def test_block_permute_plan_validates_on_host():
     assert L.cxa_block_permute_destroy(h) == _lib.OK
 
 
+def test_strided_batch_plan_validates_on_host():
+    L = _lib.lib()
+    h = ctypes.c_void_p(0)
+    ok = (3, 4, 5, 6, 7, 0, 7, 1, 0, 5, 1, 30, 5)  # dtype, nbatch, M, N, K, a, b, c strides
+    bad = [(99,) + ok[1:], ok[:2] + (-1,) + ok[3:], ok[:5] + (-1,) + ok[6:],
+           ok[:12] + (4,)]  # unknown dtype, negative M, negative stride, ldc < N
+    for args in bad:
+        rc = L.cxa_strided_batch_plan(*args, ctypes.byref(h))
+        assert rc in (_lib.ERR_INVALID, _lib.ERR_UNSUPPORTED) and not h.value
+
+
 def test_check_maps_status_to_exceptions():
     with pytest.raises(_lib.CxaInvalid):
         L = _lib.lib()

@@ -235,3 +235,26 @@ def test_block_contract_batched_and_large_operand_copies(dtype):
     ref = po.block_contract(_oracle_of(A), _oracle_of(B))
     assert C.itoi() == ref.itoi
     assert relerr(C.to_dense(), ref.to_dense()) <= 1e-12
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+def test_strided_batch_plan_vs_numpy(dtype):
+    """cxa_strided_batch_plan through the block-contract exec: a batch over a leg carried
+    through unchanged, operands read and written in place with interleaved strides."""
+    from cytnx_amd import _lib
+    from cytnx_amd.device import stream_handle
+    from cytnx_amd.heff import strided_batch_plan
+    rng = np.random.default_rng(9)
+    lp, w, s, r, wp, sp = 7, 5, 2, 6, 4, 3
+    X = rand(rng, (lp, w, s, r), dtype)                 # batch over r, K = (w, s)
+    W = rand(rng, (w, s, wp, sp), dtype)
+    dt = 3 if dtype == np.float64 else 1
+    plan = strided_batch_plan(r, lp, wp * sp, w * s, (1, w * s * r, r), (0, wp * sp, 1),
+                              (wp * sp, r * wp * sp), dtype=dt)
+    Xd, Wd = torch.from_numpy(X.ravel()).cuda(), torch.from_numpy(W.ravel()).cuda()
+    Y = torch.full((lp * r * wp * sp,), float("nan"), dtype=Xd.dtype, device="cuda")
+    _lib.check(_lib.lib().cxa_block_contract_exec(plan.handle, Xd.data_ptr(), Wd.data_ptr(),
+                                                  Y.data_ptr(), None, 0, stream_handle(0)))
+    want = np.einsum("awsr,wsvt->arvt", X, W)           # (lp, r, wp, sp)
+    assert relerr(Y.cpu().numpy().reshape(want.shape), want) <= 1e-12
+    assert plan.flops == 2.0 * r * lp * wp * sp * w * s * (4 if dt == 1 else 1)
