@@ -77,7 +77,8 @@ int cxa_permute(int dtype, int rank, const uint64_t *old_shape, const uint64_t *
  * element offset dst_offsets[b] of dst.  The plan is built once from the metadata (host): blocks
  * of up to 2^15 elements of 4-, 8- or 16-byte dtypes are moved by one batched launch when there
  * are two or more of them, the others by cxa_permute each.  exec enqueues on `stream`; dst must
- * not overlap src.
+ * not overlap src.  A plan with batched blocks holds their task list in the memory of the
+ * device current at its creation and runs there only.
  */
 typedef struct cxa_bp_plan cxa_bp_plan;
 int cxa_block_permute_plan(int dtype, int rank, int64_t nblocks, const uint64_t *shapes,
