@@ -402,9 +402,28 @@ __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
                                                double *dslot = nullptr, int64_t nslots = 0) {
   RealAcc<PREC, MI, NI> acc;
   acc.zero();
-  mfma_accumulate<PREC, TA, TB, MI, NI, PK>(acc, Abase, Bbase, blk, segs, m0, n0, lane, 0, 1);
   const int r = lane & 15, g = lane >> 4;
   const int mt = min(16 * MI, blk.M - m0), nt = min(16 * NI, blk.N - n0);
+  // FP32 launches load the dot vector's values of this tile before the K loop, so that their
+  // latency overlaps it instead of following it (FP64 launches load them after it: holding them
+  // across the loop would cost FP64 tiles a wave per SIMD of occupancy)
+  constexpr bool kPre = PREC == CXA_PREC_FP32;
+  double xpre[MI][NI][4];
+  if (kPre && dslot) {
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+      for (int jj = 0; jj < NI; ++jj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * i + acc_row(RealAcc<PREC, MI, NI>::F64, g, v);
+          const int col = 16 * jj + r;
+          xpre[i][jj][v] = row < mt && col < nt
+                               ? dotx[blk.c_off + (int64_t)(m0 + row) * blk.ldc + (n0 + col)]
+                               : 0.0;
+        }
+  }
+  mfma_accumulate<PREC, TA, TB, MI, NI, PK>(acc, Abase, Bbase, blk, segs, m0, n0, lane, 0, 1);
   gptr<TC> Cp = at<TC>(Cbase, blk.c_off);
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -428,7 +447,9 @@ __device__ __forceinline__ void wave_tile_real(const char *__restrict__ Abase,
           const int row = 16 * i + acc_row(RealAcc<PREC, MI, NI>::F64, g, v);
           const int col = 16 * jj + r;
           if (row < mt && col < nt) {
-            const double xe = dotx[blk.c_off + (int64_t)(m0 + row) * blk.ldc + (n0 + col)];
+            const double xe =
+                kPre ? xpre[i][jj][v]
+                     : dotx[blk.c_off + (int64_t)(m0 + row) * blk.ldc + (n0 + col)];
             cx += acc.get(i, jj, v) * xe;
             xx += xe * xe;
           }
@@ -450,6 +471,27 @@ __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
                                              double *dslot = nullptr, int64_t nslots = 0) {
   RealAcc<PREC, MI, NI> acc;
   acc.zero();
+  // wave 0 writes the tile: in FP32 launches with a dot vector it loads the tile's values of
+  // it before the K loop, so that their latency overlaps the loop instead of following it
+  // (FP64 launches load them after it: see wave_tile_real)
+  constexpr bool kPre = PREC == CXA_PREC_FP32;
+  double xpre[MI][NI][4];
+  if (kPre && dslot && wave == 0) {
+    const int r = lane & 15, g = lane >> 4;
+    const int mt = min(16 * MI, blk.M - m0), nt = min(16 * NI, blk.N - n0);
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+      for (int jj = 0; jj < NI; ++jj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * i + acc_row(RealAcc<PREC, MI, NI>::F64, g, v);
+          const int col = 16 * jj + r;
+          xpre[i][jj][v] = row < mt && col < nt
+                               ? dotx[blk.c_off + (int64_t)(m0 + row) * blk.ldc + (n0 + col)]
+                               : 0.0;
+        }
+  }
   mfma_accumulate<PREC, TA, TB, MI, NI, PK>(acc, Abase, Bbase, blk, segs, m0, n0, lane, wave,
                                         WAVES);
 #pragma unroll
@@ -493,7 +535,9 @@ __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
 #pragma unroll
             for (int q = 1; q < WAVES; ++q) s += red[q][idx][lane];
             if (row < mt && col < nt) {
-              const double xe = dotx[blk.c_off + (int64_t)(m0 + row) * blk.ldc + (n0 + col)];
+              const double xe =
+                  kPre ? xpre[i][jj][v]
+                       : dotx[blk.c_off + (int64_t)(m0 + row) * blk.ldc + (n0 + col)];
               cx += s * xe;
               xx += xe * xe;
             }

@@ -136,6 +136,21 @@ __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict_
                                                        double *__restrict__ Bs, IterIdx iter,
                                                        double *__restrict__ part_b) {
   const int it = iter.get();
+  double *cur = V + (int64_t)it * n;
+  const double *prev = it > 0 ? V + (int64_t)(it - 1) * n : cur;
+  // the thread's first elements are loaded before the partial sums, so that both rounds of
+  // loads are in flight together (the vector does not depend on alpha and beta)
+  constexpr int R = 2;
+  const int64_t i0 = (int64_t)blockIdx.x * kT + threadIdx.x, step = (int64_t)gridDim.x * kT;
+  double yr[R], ur[R], pr[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int64_t i = i0 + q * step;
+    const bool in = i < n;
+    yr[q] = in ? y[i] : 0.0;
+    ur[q] = in ? u[i] : 0.0;
+    pr[q] = in && it > 0 ? prev[i] : 0.0;
+  }
   // both sums in one pass over the partials (one round of loads, then two block reductions)
   __shared__ double bc[2];
   double a0 = 0.0, a1 = 0.0;
@@ -157,11 +172,20 @@ __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict_
     As[it] = a;
     if (it > 0) Bs[it - 1] = b;
   }
-  double *cur = V + (int64_t)it * n;
-  const double *prev = it > 0 ? V + (int64_t)(it - 1) * n : cur;
   const double bp = it > 0 ? b : 0.0;
   double s = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int64_t i = i0 + q * step;
+    if (i < n) {
+      const double vi = ur[q] / b;
+      if (it > 0) cur[i] = vi;
+      const double t = yr[q] / b - (a * vi + bp * pr[q]);
+      u[i] = t;
+      s += t * t;
+    }
+  }
+  for (int64_t i = i0 + R * step; i < n; i += step) {
     const double vi = u[i] / b;
     if (it > 0) cur[i] = vi;
     const double t = y[i] / b - (a * vi + bp * prev[i]);
