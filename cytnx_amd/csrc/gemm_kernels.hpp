@@ -471,26 +471,31 @@ __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
                                              double *dslot = nullptr, int64_t nslots = 0) {
   RealAcc<PREC, MI, NI> acc;
   acc.zero();
-  // wave 0 writes the tile: in FP32 launches with a dot vector it loads the tile's values of
-  // it before the K loop, so that their latency overlaps the loop instead of following it
-  // (FP64 launches load them after it: see wave_tile_real)
-  constexpr bool kPre = PREC == CXA_PREC_FP32;
-  double xpre[MI][NI][4];
-  if (kPre && dslot && wave == 0) {
-    const int r = lane & 15, g = lane >> 4;
-    const int mt = min(16 * MI, blk.M - m0), nt = min(16 * NI, blk.N - n0);
+  // The waves share the epilogue: wave w sums, stores and (with a dot vector) multiplies the
+  // tile values idx = w, w + WAVES, ... of every lane; the dot-vector entries of those values
+  // are loaded before the K loop, so that their latency overlaps it instead of following it.
+  constexpr int E = MI * NI * 4;
+  constexpr int PER = (E + WAVES - 1) / WAVES;
+  constexpr bool F64L = RealAcc<PREC, MI, NI>::F64;
+  const int r = lane & 15, g = lane >> 4;
+  const int mt = min(16 * MI, blk.M - m0), nt = min(16 * NI, blk.N - n0);
+  auto where = [&](int idx, int *row, int *col) {  // position of value idx of this lane
+    const int ij = idx >> 2;
+    *row = 16 * (ij / NI) + acc_row(F64L, g, idx & 3);
+    *col = 16 * (ij % NI) + r;
+  };
+  double xpre[PER];
 #pragma unroll
-    for (int i = 0; i < MI; ++i)
+  for (int t = 0; t < PER; ++t) xpre[t] = 0.0;
+  if (dslot) {
 #pragma unroll
-      for (int jj = 0; jj < NI; ++jj)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = 16 * i + acc_row(RealAcc<PREC, MI, NI>::F64, g, v);
-          const int col = 16 * jj + r;
-          xpre[i][jj][v] = row < mt && col < nt
-                               ? dotx[blk.c_off + (int64_t)(m0 + row) * blk.ldc + (n0 + col)]
-                               : 0.0;
-        }
+    for (int t = 0; t < PER; ++t) {
+      const int idx = wave + t * WAVES;
+      int row, col;
+      where(idx, &row, &col);
+      if (idx < E && row < mt && col < nt)
+        xpre[t] = dotx[blk.c_off + (int64_t)(m0 + row) * blk.ldc + (n0 + col)];
+    }
   }
   mfma_accumulate<PREC, TA, TB, MI, NI, PK>(acc, Abase, Bbase, blk, segs, m0, n0, lane, wave,
                                         WAVES);
@@ -501,48 +506,40 @@ __device__ __forceinline__ void wg_tile_real(const char *__restrict__ Abase,
 #pragma unroll
       for (int v = 0; v < 4; ++v) red[wave][(i * NI + jj) * 4 + v][lane] = acc.get(i, jj, v);
   __syncthreads();
-  if (wave == 0) {
-    const int r = lane & 15, g = lane >> 4;
-    const int mt = min(16 * MI, blk.M - m0), nt = min(16 * NI, blk.N - n0);
-    gptr<TC> Cp = at<TC>(Cbase, blk.c_off);
+  gptr<TC> Cp = at<TC>(Cbase, blk.c_off);
+  double cx = 0.0, xx = 0.0;
 #pragma unroll
-    for (int i = 0; i < MI; ++i)
+  for (int t = 0; t < PER; ++t) {
+    const int idx = wave + t * WAVES;
+    int row, col;
+    where(idx, &row, &col);
+    if (idx < E && row < mt && col < nt) {
+      double s = red[0][idx][lane];
 #pragma unroll
-      for (int jj = 0; jj < NI; ++jj)
+      for (int q = 1; q < WAVES; ++q) s += red[q][idx][lane];
+      store_c(Cp + (int64_t)(m0 + row) * blk.ldc + (n0 + col), s, blk);
+      cx += s * xpre[t];
+      xx += xpre[t] * xpre[t];
+    }
+  }
+  if (dslot) {  // uniform over the workgroup: wave partials, then their sum in wave order
+    __shared__ double dred[2][WAVES];
+    cx = wave_sum(cx);
+    xx = wave_sum(xx);
+    if (lane == 0) {
+      dred[0][wave] = cx;
+      dred[1][wave] = xx;
+    }
+    __syncthreads();
+    if (wave == 0 && lane == 0) {
+      double a = dred[0][0], b = dred[1][0];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = 16 * i + acc_row(RealAcc<PREC, MI, NI>::F64, g, v);
-          const int col = 16 * jj + r;
-          const int idx = (i * NI + jj) * 4 + v;
-          double s = red[0][idx][lane];
-#pragma unroll
-          for (int q = 1; q < WAVES; ++q) s += red[q][idx][lane];
-          if (row < mt && col < nt)
-            store_c(Cp + (int64_t)(m0 + row) * blk.ldc + (n0 + col), s, blk);
-        }
-    if (dslot) {  // the same sums again (from LDS) times the dot vector
-      double cx = 0.0, xx = 0.0;
-#pragma unroll
-      for (int i = 0; i < MI; ++i)
-#pragma unroll
-        for (int jj = 0; jj < NI; ++jj)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int row = 16 * i + acc_row(RealAcc<PREC, MI, NI>::F64, g, v);
-            const int col = 16 * jj + r;
-            const int idx = (i * NI + jj) * 4 + v;
-            double s = red[0][idx][lane];
-#pragma unroll
-            for (int q = 1; q < WAVES; ++q) s += red[q][idx][lane];
-            if (row < mt && col < nt) {
-              const double xe =
-                  kPre ? xpre[i][jj][v]
-                       : dotx[blk.c_off + (int64_t)(m0 + row) * blk.ldc + (n0 + col)];
-              cx += s * xe;
-              xx += xe * xe;
-            }
-          }
-      dot_epilogue(cx, xx, dslot, nslots, lane);
+      for (int q = 1; q < WAVES; ++q) {
+        a += dred[0][q];
+        b += dred[1][q];
+      }
+      dslot[0] = a;
+      dslot[nslots] = b;
     }
   }
   __syncthreads();
