@@ -1122,9 +1122,23 @@ void launch_mix_kind_t(const GemmLaunch &g, int grid, hipStream_t s) {
                        A, B, C);
 }
 
-// mix layouts: plain, packed B, packed B with packed outputs (launch_mix_t checks the flags)
+// A packed (k4) A operand of a mix launch is the FP32 running vector of an FP32 chain, read
+// with packed B and written to packed outputs; only that variant exists.
+template <int PREC, typename TA, typename TB, typename TC>
+constexpr bool mix_a4_ok() {
+  return PREC == CXA_PREC_FP32 && std::is_same<TA, float>::value && k4_ok<PREC, TB>();
+}
+
+// mix layouts: plain, packed B, packed B with packed outputs, packed A and B with packed
+// outputs (launch_mix_t checks the flags)
 template <int PREC, typename TA, typename TB, typename TC, int MI, int NI>
 void launch_mix_kind(const GemmLaunch &g, int grid, hipStream_t s) {
+  if constexpr (mix_a4_ok<PREC, TA, TB, TC>()) {
+    if (g.a_k4) {
+      launch_mix_kind_t<PREC, TA, TB, TC, MI, NI, kPackA | kPackB | kPackC>(g, grid, s);
+      return;
+    }
+  }
   if constexpr (k4_ok<PREC, TB>()) {
     if (g.b_k4 && g.c_k4) {
       launch_mix_kind_t<PREC, TA, TB, TC, MI, NI, kPackB | kPackC>(g, grid, s);
@@ -1144,7 +1158,8 @@ int launch_mix_t(const GemmLaunch &g, hipStream_t s) {
   if (g.ntiles != kXcds * g.per_xcd)
     return fail(CXA_ERR_INVALID, "mix GEMM: tile list of %d is not %d ranges of %d", g.ntiles,
                 kXcds, g.per_xcd);
-  if ((g.b_k4 && !k4_ok<PREC, TB>()) || (g.c_k4 && !g.b_k4) || g.a_k4)
+  if ((g.b_k4 && !k4_ok<PREC, TB>()) || (g.c_k4 && !g.b_k4) ||
+      (g.a_k4 && !(mix_a4_ok<PREC, TA, TB, TC>() && g.b_k4 && g.c_k4)))
     return fail(CXA_ERR_UNSUPPORTED, "mix GEMM: no variant for these packed layouts / dtypes");
   const int grid = kXcds * std::min(g.per_xcd, 2048);
   switch (g.kind0) {

@@ -231,6 +231,18 @@ struct cxa_chain {
   std::vector<void *> packed_outs;
   std::vector<int> a_packed, c_packed;
   std::vector<int64_t> mid_elems;  // elements of each intermediate buffer (packed: padded)
+  // The running vector as a k4-packed FP32 A operand of a first mix step (FP32 ground-state
+  // Lanczos): pk_segs0 holds that step's segments with B (and here A) remapped, pk_regions its
+  // A regions; cxa_chain_prepare(FP32) uploads them and builds pk_map (vector element ->
+  // packed position, -1 where no view reads it) and the zero-padded packed copy pk_psi32,
+  // which the fused residual launch rewrites every iteration.  Empty / null where the first
+  // step is not such a launch or a vector element has two packed positions.
+  std::vector<cxa::GSeg> pk_segs0;
+  std::vector<cxa::PackRegion> pk_regions;
+  int64_t pk_n = 0, pk_elems = 0;
+  void *pk_segs = nullptr, *pk_reg_dev = nullptr;
+  int *pk_map = nullptr;
+  float *pk_psi32 = nullptr;
   void *ws = nullptr;  // permute workspace (max over steps), 16 bytes per element
   int prec = CXA_PREC_FP64;
   double flops = 0;
@@ -238,6 +250,8 @@ struct cxa_chain {
     for (auto *v : {&mid64, &mid32, &fixed32, &packed_segs, &packed64, &packed32, &packed_outs})
       for (void *p : *v)
         if (p) (void)hipFree(p);
+    for (void *p : {pk_segs, pk_reg_dev, (void *)pk_map, (void *)pk_psi32})
+      if (p) (void)hipFree(p);
     if (ws) (void)hipFree(ws);
   }
 };
@@ -363,9 +377,56 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
     c->a_packed[k + 1] = 1;
     c->c_packed[k] = 1;
   }
+  // the running vector packed for a first mix step that writes packed outputs (the vector
+  // maps to itself: its length is the last step's output)
+  if (!cplx && nsteps > 1 && c->c_packed[0] && !c->fixed_left[0]) {
+    std::vector<cxa::GSeg> segs = segs_host[0];
+    int64_t elems = 0;
+    if (cxa::bc_pack_a(plans[0], out_elems[nsteps - 1], segs, c->pk_regions, &elems) == CXA_OK &&
+        elems <= INT32_MAX) {
+      c->pk_segs0 = std::move(segs);
+      c->pk_n = out_elems[nsteps - 1];
+      c->pk_elems = elems;
+    } else {
+      c->pk_regions.clear();
+    }
+  }
   *out = c.release();
   return CXA_OK;
 }
+
+namespace {
+
+// The packed running vector's map and buffers (FP32 set-up): every vector element that a view
+// reads gets exactly one packed position, or the chain keeps reading the FP64 vector.
+int build_packed_vector(cxa_chain *c) {
+  if (c->pk_regions.empty() || c->pk_n <= 0) return CXA_OK;
+  std::vector<int32_t> map((size_t)c->pk_n, -1);
+  for (const cxa::PackRegion &R : c->pk_regions) {
+    for (int64_t k = 0; k < R.K; ++k)
+      for (int64_t x = 0; x < R.X; ++x) {
+        const int64_t src = R.src_off + k * R.sk + x * R.sx;
+        const int64_t dst = R.dst_off + (k >> 2) * 4 * R.X + 4 * x + (k & 3);
+        if (src < 0 || src >= c->pk_n || dst < 0 || dst >= c->pk_elems) return CXA_OK;
+        if (map[src] != -1 && map[src] != (int32_t)dst) return CXA_OK;  // two positions
+        map[src] = (int32_t)dst;
+      }
+  }
+  CXA_HIP(hipMalloc(&c->pk_map, sizeof(int32_t) * c->pk_n));
+  CXA_HIP(hipMemcpy(c->pk_map, map.data(), sizeof(int32_t) * c->pk_n, hipMemcpyHostToDevice));
+  CXA_HIP(hipMalloc(&c->pk_segs, std::max<size_t>(1, c->pk_segs0.size()) * sizeof(cxa::GSeg)));
+  if (!c->pk_segs0.empty())
+    CXA_HIP(hipMemcpy(c->pk_segs, c->pk_segs0.data(), c->pk_segs0.size() * sizeof(cxa::GSeg),
+                      hipMemcpyHostToDevice));
+  CXA_HIP(hipMalloc(&c->pk_reg_dev, c->pk_regions.size() * sizeof(cxa::PackRegion)));
+  CXA_HIP(hipMemcpy(c->pk_reg_dev, c->pk_regions.data(),
+                    c->pk_regions.size() * sizeof(cxa::PackRegion), hipMemcpyHostToDevice));
+  CXA_HIP(hipMalloc(&c->pk_psi32, sizeof(float) * std::max<int64_t>(1, c->pk_elems)));
+  CXA_HIP(hipMemset(c->pk_psi32, 0, sizeof(float) * std::max<int64_t>(1, c->pk_elems)));
+  return CXA_OK;
+}
+
+}  // namespace
 
 // Allocate and fill the FP32 copies a precision mode needs (call before graph capture).
 extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
@@ -401,15 +462,17 @@ extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
     if (c->c_packed[k])  // the padding of a packed intermediate must read as zero
       CXA_HIP(hipMemsetAsync(c->mid32[k], 0, c->mid_elems[k] * sizeof(float), s));
   }
-  return CXA_OK;
+  return build_packed_vector(c);
 }
 
 namespace {
 
 // dotx / dotp (last step of a real chain only): the launch's dot-product epilogue, with dotp
 // holding 2 * chain_dot_slots doubles.
+// pk: step 0 reads the packed FP32 running vector (pk_psi32, kept equal to x by the caller)
+// instead of x.
 int chain_step(cxa_chain *c, int k, const void *x, void *y, hipStream_t s,
-               const double *dotx = nullptr, double *dotp = nullptr) {
+               const double *dotx = nullptr, double *dotp = nullptr, bool pk = false) {
   if (c->dtype != CXA_F64) {  // complex chain: FP64, every operand as the plan lays it out
     const void *in = k == 0 ? x : c->mid64[k - 1];
     void *outp = k == c->nsteps - 1 ? y : c->mid64[k];
@@ -439,6 +502,12 @@ int chain_step(cxa_chain *c, int k, const void *x, void *y, hipStream_t s,
       lay.outs = (const cxa::MixOut *)c->packed_outs[k];
       lay.c_k4 = true;
     }
+    if (pk && k == 0 && f32) {
+      lay.segs = (const cxa::GSeg *)c->pk_segs;
+      lay.a_k4 = true;
+      return bc_exec_typed(c->plans[0], c->pk_psi32, c->packed32[0], outp, CXA_F32, fx_dt,
+                           out_dt, c->prec, c->ws, s, 0, &lay);
+    }
     return bc_exec_typed(c->plans[k], in, f32 ? c->packed32[k] : c->packed64[k], outp, in_dt,
                          fx_dt, out_dt, c->prec, c->ws, s, 0, &lay);
   }
@@ -462,12 +531,13 @@ int chain_dot_slots(const cxa_chain *c) {
 }
 
 // y = H x with the last launch also writing the per-slot partials of y . x and x . x into dotp
-// (x and y share the vector layout, as the Lanczos vectors do).
-int chain_apply_dot(cxa_chain *c, const void *x, void *y, double *dotp, hipStream_t s) {
+// (x and y share the vector layout, as the Lanczos vectors do); pk as in chain_step.
+int chain_apply_dot(cxa_chain *c, const void *x, void *y, double *dotp, hipStream_t s,
+                    bool pk = false) {
   for (int k = 0; k < c->nsteps; ++k) {
     const bool last = k == c->nsteps - 1;
     int rc = chain_step(c, k, x, y, s, last ? (const double *)x : nullptr,
-                        last ? dotp : nullptr);
+                        last ? dotp : nullptr, pk);
     if (rc) return rc;
   }
   return CXA_OK;
@@ -500,7 +570,7 @@ extern "C" int cxa_chain_layout(const cxa_chain *c, int step, int *flags) {
   CXA_CHECK(c && flags && step >= 0 && step < c->nsteps, CXA_ERR_INVALID,
             "cxa_chain_layout: bad arguments");
   *flags = (c->packed_segs[step] ? 1 : 0) | (c->a_packed[step] ? 2 : 0) |
-           (c->c_packed[step] ? 4 : 0);
+           (c->c_packed[step] ? 4 : 0) | (step == 0 && c->pk_psi32 ? 8 : 0);
   return CXA_OK;
 }
 
@@ -644,6 +714,9 @@ struct cxa_lanczos {
   // residual launch follows, instead of two vector launches.
   double *dotp = nullptr;
   int dot_cap = 0, fused = 0;
+  // FP32 ground-state runs of a chain with a packed running vector: the residual launch also
+  // writes the next matvec input, rounded to FP32, into the chain's packed copy
+  bool pk = false;
   hipStream_t own = nullptr;
   // captured iterations: one (gexec1) and `gbatch` back to back (gexecB)
   hipGraphExec_t gexec1 = nullptr, gexecB = nullptr;
@@ -707,9 +780,11 @@ int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, bool c
   double *part_a = L->part, *part_b = L->part + kPartials;
   int rc;
   if (L->fused > 0) {  // the chain's last launch leaves the Hu . u and u . u partials
-    if ((rc = chain_apply_dot((cxa_chain *)ctx, L->cur, L->w, L->dotp, s))) return rc;
+    cxa_chain *c = (cxa_chain *)ctx;
+    if ((rc = chain_apply_dot(c, L->cur, L->w, L->dotp, s, L->pk))) return rc;
     if ((rc = vec_residual_fused(L->w, L->cur, L->V, L->n, L->dotp, L->fused, L->As(), L->Bs(),
-                                 iter, part_b, s)))
+                                 iter, part_b, L->pk ? c->pk_map : nullptr,
+                                 L->pk ? c->pk_psi32 : nullptr, s)))
       return rc;
     for (int pass = 0; pass < reorth; ++pass)
       if ((rc = vec_reorth(L->cur, L->V, L->n, L->maxiter + 1, iter, L->part, L->h(), part_b,
@@ -789,11 +864,17 @@ int run_end(RunStream &r) {
 // behind (fn, ctx) may have changed since the last run (a chain's precision, its buffers).  One
 // iteration, and a whole batch of them so that a full batch is one host launch.
 int run_prologue(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, const double *Tin, int reorth,
-                 bool cplx, bool use_graph, int batch, int maxiter, hipStream_t ws) {
+                 bool cplx, bool use_graph, int batch, int maxiter, hipStream_t ws,
+                 bool allow_pk = false) {
   int rc;
   // the fused iteration for real chains with a dot-capable last launch (decided per run: the
   // chain's precision, hence its launch, may have changed since the last one)
   L->fused = (!cplx && matvec == cxa_chain_matvec) ? chain_dot_slots((const cxa_chain *)ctx) : 0;
+  // the packed FP32 running vector: only where nothing but the residual launch changes the
+  // vector between matvecs (no re-orthogonalisation)
+  const cxa_chain *pc = L->fused > 0 ? (const cxa_chain *)ctx : nullptr;
+  L->pk = allow_pk && pc && reorth == 0 && pc->prec == CXA_PREC_FP32 && pc->pk_psi32 &&
+          pc->pk_n == L->n;
   if (L->fused > L->dot_cap) {
     CXA_HIP(hipDeviceSynchronize());  // an earlier run's launches may still read the old buffer
     if (L->dotp) CXA_HIP(hipFree(L->dotp));
@@ -805,6 +886,10 @@ int run_prologue(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, const double *
   if ((rc = vec_counter_set(L->counter, 0, ws))) return rc;
   if ((rc = vec_dot(Tin, Tin, L->n, L->part, L->nrm(), nullptr, 0, 1, ws))) return rc;
   if ((rc = vec_normalize_init(L->cur, L->V, Tin, L->n, L->nrm(), ws))) return rc;
+  if (L->pk && (rc = vec_pack_b(L->cur, pc->pk_psi32, true,
+                                (const cxa::PackRegion *)pc->pk_reg_dev,
+                                (int)pc->pk_regions.size(), ws)))
+    return rc;
   if (use_graph) {
     L->drop_graphs();
     if ((rc = capture_iterations(L, matvec, ctx, reorth, cplx, 1, ws, &L->gexec1))) return rc;
@@ -965,7 +1050,8 @@ extern "C" int cxa_lanczos_run(cxa_lanczos *L, cxa_matvec_fn matvec, void *ctx, 
   int rc;
   if ((rc = run_begin(L, use_graph, (hipStream_t)stream, r))) return rc;
   const hipStream_t ws = r.ws;
-  if ((rc = run_prologue(L, matvec, ctx, Tin, reorth, cplx, use_graph, batch, maxiter, ws)))
+  if ((rc = run_prologue(L, matvec, ctx, Tin, reorth, cplx, use_graph, batch, maxiter, ws,
+                         true)))
     return rc;
 
   const int m1 = L->maxiter + 1;

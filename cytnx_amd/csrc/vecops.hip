@@ -127,14 +127,17 @@ __global__ __launch_bounds__(kT) void k_residual(const double *__restrict__ y,
 // dotp[S..2S)): beta = |u| (1 at it = 0), alpha = (Hu . u) / beta^2 (recorded in As[it], beta
 // in Bs[it - 1] by block 0), V[it] = u / beta, and the residual u = Hu / beta - alpha V[it] -
 // beta V[it - 1] with its |u|^2 partials into part_b.  This replaces k_dot_basis + k_residual:
-// one launch and one read of u fewer per iteration.
+// one launch and one read of u fewer per iteration.  With p32 non-null the residual also goes,
+// rounded to FP32, to its packed position pmap[i] (skipped where pmap[i] < 0).
 __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict__ y,
                                                        double *__restrict__ u,
                                                        double *__restrict__ V, int64_t n,
                                                        const double *__restrict__ dotp, int S,
                                                        double *__restrict__ As,
                                                        double *__restrict__ Bs, IterIdx iter,
-                                                       double *__restrict__ part_b) {
+                                                       double *__restrict__ part_b,
+                                                       const int *__restrict__ pmap,
+                                                       float *__restrict__ p32) {
   const int it = iter.get();
   double *cur = V + (int64_t)it * n;
   const double *prev = it > 0 ? V + (int64_t)(it - 1) * n : cur;
@@ -143,6 +146,7 @@ __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict_
   constexpr int R = 2;
   const int64_t i0 = (int64_t)blockIdx.x * kT + threadIdx.x, step = (int64_t)gridDim.x * kT;
   double yr[R], ur[R], pr[R];
+  int dr[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int64_t i = i0 + q * step;
@@ -150,6 +154,7 @@ __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict_
     yr[q] = in ? y[i] : 0.0;
     ur[q] = in ? u[i] : 0.0;
     pr[q] = in && it > 0 ? prev[i] : 0.0;
+    dr[q] = in && p32 ? pmap[i] : -1;
   }
   // both sums in one pass over the partials (one round of loads, then two block reductions)
   __shared__ double bc[2];
@@ -182,6 +187,7 @@ __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict_
       if (it > 0) cur[i] = vi;
       const double t = yr[q] / b - (a * vi + bp * pr[q]);
       u[i] = t;
+      if (dr[q] >= 0) p32[dr[q]] = (float)t;
       s += t * t;
     }
   }
@@ -190,6 +196,10 @@ __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict_
     if (it > 0) cur[i] = vi;
     const double t = y[i] / b - (a * vi + bp * prev[i]);
     u[i] = t;
+    if (p32) {
+      const int d = pmap[i];
+      if (d >= 0) p32[d] = (float)t;
+    }
     s += t * t;
   }
   s = block_sum(s);
@@ -494,9 +504,9 @@ int vec_residual(const double *y, double *u, const double *V, int64_t n, const d
 
 int vec_residual_fused(const double *y, double *u, double *V, int64_t n, const double *dotp,
                        int S, double *As, double *Bs, IterIdx iter, double *part_b,
-                       hipStream_t s) {
+                       const int *pmap, float *p32, hipStream_t s) {
   hipLaunchKernelGGL(k_residual_fused, dim3(kPartials), dim3(kT), 0, s, y, u, V, n, dotp, S, As,
-                     Bs, iter, part_b);
+                     Bs, iter, part_b, pmap, p32);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }

@@ -33,9 +33,11 @@ int vec_residual(const double *y, double *u, const double *V, int64_t n, const d
                  double *As, const double *Bs, IterIdx iter, double *part_b, hipStream_t s);
 // The two launches above in one, when the matvec produced per-slot partials of Hu . u and
 // u . u (dotp[0..S), dotp[S..2S)): beta, alpha, V[it] and the residual with its |u|^2 partials.
+// With p32 non-null the residual is also written, rounded to FP32, to p32[pmap[i]] (elements
+// with pmap[i] < 0 are skipped): the next matvec's packed input.
 int vec_residual_fused(const double *y, double *u, double *V, int64_t n, const double *dotp,
                        int S, double *As, double *Bs, IterIdx iter, double *part_b,
-                       hipStream_t s);
+                       const int *pmap, float *p32, hipStream_t s);
 // w -= sum_{j <= it} (V_j . w) V_j (classical Gram-Schmidt against the stored basis);
 // part_out = partials of the new |w|^2; part: 2 * kPartials * max_j scratch.  cplx: the n
 // doubles are n / 2 complex values, the coefficients conj(V_j) . w are complex (h: 2 doubles

@@ -60,7 +60,8 @@ class _Chain:
 
     def layout(self, step: int) -> int:
         """Operand layouts of a step (cxa_chain_layout): bit 0 packed fixed B, bit 1 packed
-        running A, bit 2 outputs written packed for the next step."""
+        running A, bit 2 outputs written packed for the next step, bit 3 (step 0, FP32 prepared)
+        FP32 ground-state Lanczos runs feed it a packed FP32 copy of the Lanczos vector."""
         f = ctypes.c_int(0)
         _lib.check(_lib.lib().cxa_chain_layout(self.handle, int(step), ctypes.byref(f)))
         return f.value

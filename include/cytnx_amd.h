@@ -355,7 +355,9 @@ int cxa_chain_flops(const cxa_chain *chain, double *flops);
  * Operand layouts step `step` runs with (no counterpart in the reference: a diagnostic of this
  * library's packing): bit 0 its fixed B operand is read k4-packed, bit 1 its running A
  * operand (the previous step's output) is read k4-packed, bit 2 its outputs are written
- * k4-packed for the next step.
+ * k4-packed for the next step, bit 3 (step 0 only, after cxa_chain_prepare for FP32) FP32
+ * ground-state runs of cxa_lanczos_run without re-orthogonalisation feed this step a k4-packed
+ * FP32 copy of the Lanczos vector, which the residual launch writes.
  */
 int cxa_chain_layout(const cxa_chain *chain, int step, int *flags);
 int cxa_chain_destroy(cxa_chain *chain);

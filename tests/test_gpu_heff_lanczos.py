@@ -239,9 +239,11 @@ def test_heff2_lanczos_vs_eigh():
     assert np.linalg.norm(Hm @ vv - E * vv) <= 1e-5 * abs(E)
 
 
-def test_lanczos_batching_and_graph_do_not_change_results():
+@pytest.mark.parametrize("prec", [0, 1])
+def test_lanczos_batching_and_graph_do_not_change_results(prec):
     psi, L, M1, M2, R = problem(64, seed=13)
     H = Heff2(psi, L, M1, M2, R)
+    H.set_precision(prec)
     r1 = lanczos_gnd_native(H, psi._storage, 1e-12, 60, batch=1)
     r2 = lanczos_gnd_native(H, psi._storage, 1e-12, 60, batch=7)
     r3 = lanczos_gnd_native(H, psi._storage, 1e-12, 60, batch=5, graph=True)
@@ -262,6 +264,9 @@ def test_fused_iteration_matches_two_launch_iteration(prec):
     psi, L, M1, M2, R = problem(96, seed=29)
     H = Heff2(psi, L, M1, M2, R)
     H.set_precision(prec)
+    # FP32 mix chains: the fused run reads the packed FP32 copy of the Lanczos vector that its
+    # residual launch writes, the callback run the FP64 vector
+    assert H.mode == "mix" and bool(H.chain.layout(0) & 8) == (prec == 1)
     x0 = psi._storage.clone()
     E1, v1, k1, a1, b1 = lanczos_gnd_native(H, x0, 1e-12, 400, batch=8)
     n = x0.numel()
