@@ -111,7 +111,9 @@ namespace cxa {
 namespace {
 
 // tile kinds a mix launch can run (every slice of an item stays in the wave's registers)
-bool mix_kind_ok(int kind) { return kind == TK_1x1 || kind == TK_2x1 || kind == TK_1x2; }
+bool mix_kind_ok(int kind) {
+  return kind == TK_1x1 || kind == TK_2x1 || kind == TK_1x2 || kind == TK_2x2;
+}
 
 int build_variant(cxa_bc_plan *p, int prec) {
   const bool real = p->dtype == CXA_F64 || p->dtype == CXA_F32;
@@ -609,7 +611,7 @@ extern "C" int cxa_block_contract_set_config(cxa_bc_plan *p, int prec, int kind,
   CXA_CHECK(kind >= -1 && kind < TK_COUNT, CXA_ERR_INVALID, "unknown tile kind %d", kind);
   CXA_CHECK(split >= -1 && split <= 2, CXA_ERR_INVALID, "split mode must be -1, 0, 1 or 2");
   CXA_CHECK(!p->mix || kind == -1 || mix_kind_ok(kind), CXA_ERR_UNSUPPORTED,
-            "mix plans run 16x16, 32x16 or 16x32 tiles (kind %d)", kind);
+            "mix plans run 16x16, 32x16, 16x32 or 32x32 tiles (kind %d)", kind);
   CXA_CHECK(!p->mix || split <= 1, CXA_ERR_UNSUPPORTED,
             "mix plans run eight- or four-wave workgroups (split 0 or 1)");
   std::lock_guard<std::mutex> lk(p->mu);

@@ -24,8 +24,9 @@
 // FP32 MFMA, the two cross terms in their own accumulator, combined in FP64 at the end).
 //
 // This header holds the kernel templates; the translation units gemm_fp64.hip, gemm_fp32.hip,
-// gemm_fp32_a64.hip, gemm_fp32x3.hip and gemm_mix.hip each instantiate one precision family (so they compile in
-// parallel), and gemm.hip holds the dispatch and the host-side tile planning.  Define
+// gemm_fp32_a64.hip, gemm_fp32x3.hip and gemm_mix.hip each instantiate one precision family
+// (so they compile in parallel), and gemm.hip holds the dispatch and the host-side tile
+// planning.  Define
 // CXA_GEMM_TU (a distinct token per translation unit) before including it: each unit carries
 // its own zero cell.
 #include <algorithm>
@@ -1166,8 +1167,9 @@ int launch_mix_t(const GemmLaunch &g, hipStream_t s) {
     case TK_1x1: launch_mix_kind<PREC, TA, TB, TC, 1, 1>(g, grid, s); break;
     case TK_2x1: launch_mix_kind<PREC, TA, TB, TC, 2, 1>(g, grid, s); break;
     case TK_1x2: launch_mix_kind<PREC, TA, TB, TC, 1, 2>(g, grid, s); break;
+    case TK_2x2: launch_mix_kind<PREC, TA, TB, TC, 2, 2>(g, grid, s); break;
     default:
-      return fail(CXA_ERR_INVALID, "mix GEMM: tile kind %d is not 16x16, 32x16 or 16x32",
+      return fail(CXA_ERR_INVALID, "mix GEMM: tile kind %d is not 16x16, 32x16, 16x32 or 32x32",
                   g.kind0);
   }
   CXA_HIP(hipGetLastError());

@@ -611,10 +611,10 @@ extern "C" int cxa_chain_tune(cxa_chain *c, const void *x, void *y, int reps, in
     double best = 1e30;
     int bk = -1, bs = -1;
     for (int ci = 0; ci < ncand && rc == CXA_OK; ++ci) {
-      // mix plans: explicit 16x16, 32x16 or 16x32 tiles; split 0 / 1 = eight / four waves
-      if (bc_is_mix(plan) && (cand[ci][0] < 0 || cand[ci][0] == TK_2x2 ||
-                              cand[ci][0] == TK_4x2 || cand[ci][0] == TK_ROWS ||
-                              cand[ci][1] == 2))
+      // mix plans: explicit 16x16, 32x16, 16x32 or 32x32 tiles; split 0 / 1 = eight / four
+      // waves
+      if (bc_is_mix(plan) && (cand[ci][0] < 0 || cand[ci][0] == TK_4x2 ||
+                              cand[ci][0] == TK_ROWS || cand[ci][1] == 2))
         continue;
       // the big-tile and row classes are real-only (complex plans fall back to the default)
       if (c->dtype != CXA_F64 && (cand[ci][0] == TK_4x2 || cand[ci][0] == TK_ROWS)) continue;

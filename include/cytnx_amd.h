@@ -245,8 +245,9 @@ int cxa_strided_batch_plan(int dtype, int64_t nbatch, int64_t M, int64_t N, int6
  * (MPO index, physical indices) slices of L.psi and the weights are the elements of the
  * two-site MPO M1.M2.  FP64 operands only
  * (FP32 copies through a chain); exec / info / set_config / get_config / destroy are the
- * block-contract entry points (tile kinds 16x16, 32x16, 16x32; split mode 1 selects four-wave
- * workgroups, two slices per wave, instead of one wave per slice; flops count the products).
+ * block-contract entry points (tile kinds 16x16, 32x16, 16x32, 32x32; split mode 1 selects
+ * four-wave workgroups, two slices per wave, instead of one wave per slice; flops count the
+ * products).
  */
 #define CXA_MIX_SLICES 8
 int cxa_mix_contract_plan(int dtype, int nitems, const int64_t *items, int64_t nslices,

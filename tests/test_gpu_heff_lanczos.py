@@ -114,11 +114,11 @@ def test_heff2_tuned_tilings_match_oracle_at_chi512(mix):
 
 
 @pytest.mark.parametrize("split", [0, 1])
-@pytest.mark.parametrize("kind", [0, 2, 3])
+@pytest.mark.parametrize("kind", [0, 2, 3, 1])
 def test_heff2_mix_every_tile_kind_matches_oracle(kind, split):
-    """Each tile shape of the mix launch (16x16, 32x16, 16x32) with eight-wave (one slice per
-    wave) and four-wave (two slices per wave) workgroups, at chi=256, where sector widths (up
-    to 21) leave partial tiles in both directions."""
+    """Each tile shape of the mix launch (16x16, 32x16, 16x32, 32x32) with eight-wave (one
+    slice per wave) and four-wave (two slices per wave) workgroups, at chi=256, where sector
+    widths (up to 21) leave partial tiles in both directions."""
     psi, L, M1, M2, R = problem(256, seed=23)
     H = Heff2(psi, L, M1, M2, R)
     assert H.mode == "mix"
