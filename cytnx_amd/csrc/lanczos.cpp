@@ -608,6 +608,9 @@ extern "C" int cxa_chain_tune(cxa_chain *c, const void *x, void *y, int reps, in
   int rc = chain_apply(c, x, y, s);  // every intermediate in place
   for (int k = 0; k < c->nsteps && rc == CXA_OK; ++k) {
     cxa_bc_plan *plan = const_cast<cxa_bc_plan *>(c->plans[k]);
+    // FP32 chains with a packed running vector: step 0 is timed in the form FP32 ground-state
+    // Lanczos runs launch it (reading the packed FP32 copy)
+    const bool pk = k == 0 && c->prec == CXA_PREC_FP32 && c->pk_psi32;
     double best = 1e30;
     int bk = -1, bs = -1;
     for (int ci = 0; ci < ncand && rc == CXA_OK; ++ci) {
@@ -619,13 +622,15 @@ extern "C" int cxa_chain_tune(cxa_chain *c, const void *x, void *y, int reps, in
       // the big-tile and row classes are real-only (complex plans fall back to the default)
       if (c->dtype != CXA_F64 && (cand[ci][0] == TK_4x2 || cand[ci][0] == TK_ROWS)) continue;
       if ((rc = cxa_block_contract_set_config(plan, c->prec, cand[ci][0], cand[ci][1]))) break;
-      for (int w = 0; w < 2 && rc == CXA_OK; ++w) rc = chain_step(c, k, x, y, s);
+      for (int w = 0; w < 2 && rc == CXA_OK; ++w)
+        rc = chain_step(c, k, x, y, s, nullptr, nullptr, pk);
       if (rc) break;
       if (hipEventRecord(e0, s) != hipSuccess) {
         rc = fail(CXA_ERR_HIP, "cxa_chain_tune: event record failed");
         break;
       }
-      for (int r = 0; r < reps && rc == CXA_OK; ++r) rc = chain_step(c, k, x, y, s);
+      for (int r = 0; r < reps && rc == CXA_OK; ++r)
+        rc = chain_step(c, k, x, y, s, nullptr, nullptr, pk);
       if (rc) break;
       float t = 0;
       if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
