@@ -397,32 +397,39 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
 
 namespace {
 
-// The packed running vector's map and buffers (FP32 set-up): every vector element that a view
-// reads gets exactly one packed position, or the chain keeps reading the FP64 vector.
-int build_packed_vector(cxa_chain *c) {
+// The packed running vector's map and buffers (FP32 set-up), built on the device: every
+// vector element that a view reads must get exactly one packed position, or the chain keeps
+// reading the FP64 vector (the map and buffers are released again).
+int build_packed_vector(cxa_chain *c, hipStream_t s) {
   if (c->pk_regions.empty() || c->pk_n <= 0) return CXA_OK;
-  std::vector<int32_t> map((size_t)c->pk_n, -1);
-  for (const cxa::PackRegion &R : c->pk_regions) {
-    for (int64_t k = 0; k < R.K; ++k)
-      for (int64_t x = 0; x < R.X; ++x) {
-        const int64_t src = R.src_off + k * R.sk + x * R.sx;
-        const int64_t dst = R.dst_off + (k >> 2) * 4 * R.X + 4 * x + (k & 3);
-        if (src < 0 || src >= c->pk_n || dst < 0 || dst >= c->pk_elems) return CXA_OK;
-        if (map[src] != -1 && map[src] != (int32_t)dst) return CXA_OK;  // two positions
-        map[src] = (int32_t)dst;
-      }
+  const size_t nreg = c->pk_regions.size();
+  CXA_HIP(hipMalloc(&c->pk_reg_dev, nreg * sizeof(cxa::PackRegion)));
+  CXA_HIP(hipMemcpyAsync(c->pk_reg_dev, c->pk_regions.data(), nreg * sizeof(cxa::PackRegion),
+                         hipMemcpyHostToDevice, s));
+  CXA_HIP(hipMalloc(&c->pk_map, sizeof(int32_t) * (c->pk_n + 1)));  // + the conflict flag
+  int *flag = c->pk_map + c->pk_n;
+  CXA_HIP(hipMemsetAsync(c->pk_map, 0xff, sizeof(int32_t) * c->pk_n, s));
+  CXA_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), s));
+  int rc = vec_pack_map((const cxa::PackRegion *)c->pk_reg_dev, (int)nreg, c->pk_n, c->pk_map,
+                        flag, s);
+  if (rc) return rc;
+  int conflict = 0;
+  CXA_HIP(hipMemcpyAsync(&conflict, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  CXA_HIP(hipStreamSynchronize(s));
+  if (conflict) {  // no packed vector for this chain
+    CXA_HIP(hipFree(c->pk_map));
+    c->pk_map = nullptr;
+    CXA_HIP(hipFree(c->pk_reg_dev));
+    c->pk_reg_dev = nullptr;
+    return CXA_OK;
   }
-  CXA_HIP(hipMalloc(&c->pk_map, sizeof(int32_t) * c->pk_n));
-  CXA_HIP(hipMemcpy(c->pk_map, map.data(), sizeof(int32_t) * c->pk_n, hipMemcpyHostToDevice));
   CXA_HIP(hipMalloc(&c->pk_segs, std::max<size_t>(1, c->pk_segs0.size()) * sizeof(cxa::GSeg)));
   if (!c->pk_segs0.empty())
-    CXA_HIP(hipMemcpy(c->pk_segs, c->pk_segs0.data(), c->pk_segs0.size() * sizeof(cxa::GSeg),
-                      hipMemcpyHostToDevice));
-  CXA_HIP(hipMalloc(&c->pk_reg_dev, c->pk_regions.size() * sizeof(cxa::PackRegion)));
-  CXA_HIP(hipMemcpy(c->pk_reg_dev, c->pk_regions.data(),
-                    c->pk_regions.size() * sizeof(cxa::PackRegion), hipMemcpyHostToDevice));
+    CXA_HIP(hipMemcpyAsync(c->pk_segs, c->pk_segs0.data(),
+                           c->pk_segs0.size() * sizeof(cxa::GSeg), hipMemcpyHostToDevice, s));
   CXA_HIP(hipMalloc(&c->pk_psi32, sizeof(float) * std::max<int64_t>(1, c->pk_elems)));
-  CXA_HIP(hipMemset(c->pk_psi32, 0, sizeof(float) * std::max<int64_t>(1, c->pk_elems)));
+  CXA_HIP(hipMemsetAsync(c->pk_psi32, 0, sizeof(float) * std::max<int64_t>(1, c->pk_elems), s));
+  CXA_HIP(hipStreamSynchronize(s));  // ready for runs on any stream
   return CXA_OK;
 }
 
@@ -462,7 +469,7 @@ extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
     if (c->c_packed[k])  // the padding of a packed intermediate must read as zero
       CXA_HIP(hipMemsetAsync(c->mid32[k], 0, c->mid_elems[k] * sizeof(float), s));
   }
-  return build_packed_vector(c);
+  return build_packed_vector(c, s);
 }
 
 namespace {

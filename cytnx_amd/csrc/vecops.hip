@@ -622,6 +622,40 @@ __global__ __launch_bounds__(kT) void k_pack_b(const double *__restrict__ src, T
   }
 }
 
+__global__ __launch_bounds__(kT) void k_pack_map(const PackRegion *__restrict__ regions,
+                                                int nregions, int64_t n, int *__restrict__ pmap,
+                                                int *__restrict__ conflict) {
+  for (int r = blockIdx.x; r < nregions; r += gridDim.x) {
+    const PackRegion R = regions[r];
+    const int64_t row = 4 * (int64_t)R.X;
+    const int64_t nel = (int64_t)((R.K + 3) / 4) * row;
+    for (int64_t e = threadIdx.x; e < nel; e += kT) {
+      const int64_t q = e / row;
+      const int64_t rem = e - q * row;
+      const int64_t k = 4 * q + (rem & 3), x = rem >> 2;
+      if (k >= R.K) continue;  // padding: no source element
+      const int64_t src = R.src_off + k * R.sk + x * R.sx;
+      const int dst = (int)(R.dst_off + e);
+      if (src < 0 || src >= n) {
+        atomicOr(conflict, 1);
+        continue;
+      }
+      const int prev = atomicCAS(pmap + src, -1, dst);
+      if (prev != -1 && prev != dst) atomicOr(conflict, 1);
+    }
+  }
+}
+
+int vec_pack_map(const PackRegion *regions, int nregions, int64_t n, int *pmap, int *conflict,
+                 hipStream_t s) {
+  if (nregions <= 0) return CXA_OK;
+  const int grid = nregions < 4096 ? nregions : 4096;
+  hipLaunchKernelGGL(k_pack_map, dim3(grid), dim3(kT), 0, s, regions, nregions, n, pmap,
+                     conflict);
+  CXA_HIP(hipGetLastError());
+  return CXA_OK;
+}
+
 int vec_pack_b(const double *src, void *dst, bool dst_f32, const PackRegion *regions,
                int nregions, hipStream_t s) {
   if (nregions <= 0) return CXA_OK;

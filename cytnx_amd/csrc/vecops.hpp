@@ -66,5 +66,11 @@ int vec_cast_f64_f32(const double *x, float *y, int64_t n, hipStream_t s);
 struct PackRegion;
 int vec_pack_b(const double *src, void *dst, bool dst_f32, const PackRegion *regions,
                int nregions, hipStream_t s);
+// The map from source elements to packed positions of the same regions: pmap[src] = dst for
+// every element a region reads (pmap must hold -1 in all n entries beforehand); *conflict
+// (device, 0 beforehand) becomes nonzero if an element has two packed positions or a region
+// reads outside [0, n).
+int vec_pack_map(const PackRegion *regions, int nregions, int64_t n, int *pmap, int *conflict,
+                 hipStream_t s);
 
 }  // namespace cxa
