@@ -10,7 +10,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcytnx_amd.so")
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_NOMEM, ERR_NUMERIC = 0, 1, 2, 3, 4, 5
 
@@ -93,6 +93,7 @@ def _declare(L):
         "cxa_lanczos_run": (ci, [vp, vp, vp, vp, ctypes.c_double, ci, ci, ci, dblp, vp, intp,
                                  dblp, dblp, vp]),
         "cxa_lanczos_destroy": (ci, [vp]),
+        "cxa_lanczos_reshape": (ci, [vp, i64, ci]),
         "cxa_lanczos_exp": (ci, [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, ci, ci, vp, intp, vp]),
         "cxa_lanczos_eigsh": (ci, [vp, vp, vp, vp, ci, ci, ctypes.c_double, ci, ci, ci, dblp,

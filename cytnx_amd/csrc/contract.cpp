@@ -50,8 +50,7 @@ template <typename T>
 int upload(DevBuf &d, const std::vector<T> &v) {
   if (v.empty()) return CXA_OK;
   CXA_HIP(hipMalloc(&d.p, v.size() * sizeof(T)));
-  CXA_HIP(hipMemcpy(d.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-  return CXA_OK;
+  return upload_now(d.p, v.data(), v.size() * sizeof(T));
 }
 
 struct CopyJob {
@@ -178,7 +177,8 @@ int upload_plan(cxa_bc_plan *p) {
     for (const Part &q : parts)
       if (q.bytes) std::memcpy(host.data() + q.at, q.src, q.bytes);
     CXA_HIP(hipMalloc(&p->d_meta.p, host.size()));
-    CXA_HIP(hipMemcpy(p->d_meta.p, host.data(), host.size(), hipMemcpyHostToDevice));
+    const int rc = upload_now(p->d_meta.p, host.data(), host.size());
+    if (rc) return rc;
     char *base = (char *)p->d_meta.p;
     auto at = [&](int i) -> const void * { return parts[i].bytes ? base + parts[i].at : nullptr; };
     p->dev_blks = (const GBlk *)at(0);

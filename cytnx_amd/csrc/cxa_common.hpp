@@ -45,6 +45,12 @@ inline int fail(int code, const char *fmt, ...) {
   return code;
 }
 
+// A host-to-device copy of set-up data (plan metadata) that waits only for itself: issued on
+// a non-blocking stream of the library's own (one per device, made on first use) and waited
+// for there, so it neither waits for nor is ordered after work queued on the caller's streams.
+// The data is on the device when it returns.
+int upload_now(void *dst, const void *src, size_t bytes);
+
 }  // namespace cxa
 
 #define CXA_HIP(call)                                                                   \

@@ -717,6 +717,9 @@ struct cxa_lanczos {
   int device = 0;
   int64_t n = 0;
   int maxiter = 0;
+  // what the buffers were allocated for (cxa_lanczos_reshape stays within it)
+  int64_t cap_n = 0, cap_basis = 0;
+  int cap_maxiter = 0;
   double *V = nullptr, *w = nullptr, *cur = nullptr, *part = nullptr, *sc = nullptr,
          *coef = nullptr;
   int *counter = nullptr;
@@ -760,9 +763,10 @@ extern "C" int cxa_lanczos_create(int64_t n, int maxiter, cxa_lanczos **out) {
   *out = nullptr;
   std::unique_ptr<cxa_lanczos> L(new cxa_lanczos());
   CXA_HIP(hipGetDevice(&L->device));
-  L->n = n;
-  L->maxiter = maxiter;
+  L->n = L->cap_n = n;
+  L->maxiter = L->cap_maxiter = maxiter;
   const int64_t m1 = maxiter + 1;
+  L->cap_basis = n * m1;
   CXA_HIP(hipMalloc(&L->V, sizeof(double) * n * m1));
   CXA_HIP(hipMalloc(&L->w, sizeof(double) * n));
   CXA_HIP(hipMalloc(&L->cur, sizeof(double) * n));
@@ -779,6 +783,23 @@ extern "C" int cxa_lanczos_create(int64_t n, int maxiter, cxa_lanczos **out) {
 
 extern "C" int cxa_lanczos_destroy(cxa_lanczos *L) {
   delete L;
+  return CXA_OK;
+}
+
+// The context's buffers reused for vectors of n doubles and Maxiter iterations, within what
+// they were allocated for: n <= the creation n, Maxiter <= the creation Maxiter and
+// n * (Maxiter + 1) basis doubles <= the creation count.  Captured iteration graphs (made for
+// the old sizes) are dropped.
+extern "C" int cxa_lanczos_reshape(cxa_lanczos *L, int64_t n, int maxiter) {
+  CXA_CHECK(L && n > 0 && maxiter >= 2, CXA_ERR_INVALID,
+            "cxa_lanczos_reshape: need a context, n > 0 and Maxiter > 1");
+  CXA_CHECK(n <= L->cap_n && maxiter <= L->cap_maxiter && n * (maxiter + 1) <= L->cap_basis,
+            CXA_ERR_INVALID,
+            "cxa_lanczos_reshape: %lld x %d exceeds the context's %lld x %d", (long long)n,
+            maxiter, (long long)L->cap_n, L->cap_maxiter);
+  L->drop_graphs();
+  L->n = n;
+  L->maxiter = maxiter;
   return CXA_OK;
 }
 

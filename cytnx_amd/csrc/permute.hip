@@ -946,7 +946,8 @@ extern "C" int cxa_block_permute_plan(int dtype, int rank, int64_t nblocks,
     std::memcpy(host.data(), jobs.data(), jobs.size() * sizeof(BatchCopyJob));
     std::memcpy(host.data() + nj, tasks.data(), nt);
     CXA_HIP(hipMalloc(&p->dev, host.size()));
-    CXA_HIP(hipMemcpy(p->dev, host.data(), host.size(), hipMemcpyHostToDevice));
+    const int rc = upload_now(p->dev, host.data(), host.size());
+    if (rc) return rc;
     p->jobs = (const BatchCopyJob *)p->dev;
     p->tasks = (const BatchCopyTask *)((char *)p->dev + nj);
     p->ntasks = (int)tasks.size();

@@ -58,33 +58,53 @@ class _DevView:
                                          "version": 2, "strides": None}
 
 
-_CTX_CACHE = {}
+_CTX_CACHE = {}  # (n, maxiter, device) -> context handle, as last shaped
+_CTX_CAP = {}  # context handle -> (n, maxiter, device) it was created for
 _MATVEC_T = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                              ctypes.c_void_p)
 
 
 CTX_CACHE_BYTES = 16 << 30  # Krylov bases kept alive between runs (all devices)
+CTX_HEADROOM = 1.25  # a new context's vectors hold this many times the requested length
 
 
 def _context(n: int, maxiter: int, device: int):
     """A Lanczos context of n-element vectors holding maxiter + 1 basis vectors, reused across
-    runs; cached contexts are released (all of them) once their bases would pass
-    CTX_CACHE_BYTES."""
+    runs: a cached context of the device whose buffers hold the request is reshaped to it
+    (cxa_lanczos_reshape; the smallest such), otherwise a new one is made with CTX_HEADROOM
+    times the vector length, so that runs of slowly varying size (DMRG bond updates) share
+    contexts instead of freeing and allocating per run.  Cached contexts are released (all of
+    them) once their bases would pass CTX_CACHE_BYTES."""
     key = (n, maxiter, device)
     ctx = _CTX_CACHE.get(key)
-    if ctx is None:
-        need = 8 * n * (maxiter + 1)
-        held = sum(8 * k[0] * (k[1] + 1) for k in _CTX_CACHE)
-        if _CTX_CACHE and held + need > CTX_CACHE_BYTES:
-            torch.cuda.synchronize()
-            for v in _CTX_CACHE.values():
-                _lib.lib().cxa_lanczos_destroy(v)
-            _CTX_CACHE.clear()
-        h = ctypes.c_void_p(0)
-        torch.cuda.set_device(device)
-        _lib.check(_lib.lib().cxa_lanczos_create(n, maxiter, ctypes.byref(h)))
-        ctx = h.value
+    if ctx is not None:
+        return ctx
+    fits = [(cn * (cm + 1), h) for h, (cn, cm, cd) in _CTX_CAP.items()
+            if cd == device and cn >= n and cm >= maxiter]
+    if fits:
+        ctx = min(fits)[1]
+        _lib.check(_lib.lib().cxa_lanczos_reshape(ctx, n, maxiter))
+        for k in [k for k, v in _CTX_CACHE.items() if v == ctx]:
+            del _CTX_CACHE[k]
         _CTX_CACHE[key] = ctx
+        return ctx
+    cap_n = max(n, int(n * CTX_HEADROOM))
+    need = 8 * cap_n * (maxiter + 1)
+    held = sum(8 * cn * (cm + 1) for cn, cm, _ in _CTX_CAP.values())
+    if _CTX_CAP and held + need > CTX_CACHE_BYTES:
+        torch.cuda.synchronize()
+        for h in _CTX_CAP:
+            _lib.lib().cxa_lanczos_destroy(h)
+        _CTX_CACHE.clear()
+        _CTX_CAP.clear()
+    h = ctypes.c_void_p(0)
+    torch.cuda.set_device(device)
+    _lib.check(_lib.lib().cxa_lanczos_create(cap_n, maxiter, ctypes.byref(h)))
+    ctx = h.value
+    _CTX_CAP[ctx] = (cap_n, maxiter, device)
+    if cap_n != n:
+        _lib.check(_lib.lib().cxa_lanczos_reshape(ctx, n, maxiter))
+    _CTX_CACHE[key] = ctx
     return ctx
 
 

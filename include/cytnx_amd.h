@@ -396,6 +396,12 @@ int cxa_lanczos_run(cxa_lanczos *ctx, cxa_matvec_fn matvec, void *matvec_ctx, co
                     double CvgCrit, int Maxiter, int batch, int flags, double *E_out, double *evec,
                     int *niter, double *alphas, double *betas, cxa_stream_t stream);
 int cxa_lanczos_destroy(cxa_lanczos *ctx);
+/* No counterpart in the reference (its drivers allocate their Krylov vectors per call): reuse
+ * a context's device buffers for vectors of n doubles and Maxiter iterations within what it
+ * was created for (n and Maxiter at most the creation values, n * (Maxiter + 1) at most the
+ * creation basis), for callers that run many solves of varying size (DMRG bond updates)
+ * without freeing and allocating per solve.  Drops captured iteration graphs. */
+int cxa_lanczos_reshape(cxa_lanczos *ctx, int64_t n, int maxiter);
 
 /*
  * Lanczos_Exp(Hop, Tin, tau, CvgCrit, Maxiter) with tau = tau_re + i tau_im on device vectors

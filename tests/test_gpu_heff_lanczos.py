@@ -287,6 +287,48 @@ def test_fused_iteration_matches_two_launch_iteration(prec):
     assert torch.linalg.vector_norm(v1 - v2).item() <= 1e-6
 
 
+def test_lanczos_context_reuse_across_sizes():
+    """Runs of different vector lengths share one context (cxa_lanczos_reshape within its
+    allocation): a smaller operator after a larger one reuses it, and every run equals a run
+    on a freshly made context bit for bit, graph replay included; a reshape past the
+    allocation is refused."""
+    import ctypes
+    from cytnx_amd import _lib
+    from cytnx_amd import lanczos as lz
+
+    def fresh():
+        torch.cuda.synchronize()
+        for h in list(lz._CTX_CAP):
+            _lib.lib().cxa_lanczos_destroy(h)
+        lz._CTX_CACHE.clear()
+        lz._CTX_CAP.clear()
+
+    big = problem(64, seed=31)
+    small = problem(56, seed=37)
+    HA, HB = Heff2(*big), Heff2(*small)
+    xa, xb = big[0]._storage, small[0]._storage
+    assert xb.numel() < xa.numel()
+    fresh()
+    ref_b = lanczos_gnd_native(HB, xb, 1e-12, 120, batch=8)
+    fresh()
+    ref_a = lanczos_gnd_native(HA, xa, 1e-12, 120, batch=8)
+    assert len(lz._CTX_CAP) == 1
+    got_b = lanczos_gnd_native(HB, xb, 1e-12, 120, batch=8)
+    assert len(lz._CTX_CAP) == 1  # the larger context, reshaped
+    got_a = lanczos_gnd_native(HA, xa, 1e-12, 120, batch=8, graph=True)
+    got_b2 = lanczos_gnd_native(HB, xb, 1e-12, 120, batch=8, graph=True)
+    assert len(lz._CTX_CAP) == 1
+    for got, ref in ((got_b, ref_b), (got_a, ref_a), (got_b2, ref_b)):
+        assert got[0] == ref[0] and got[2] == ref[2]
+        assert np.array_equal(got[3], ref[3]) and np.array_equal(got[4], ref[4])
+        assert torch.equal(got[1], ref[1])
+    ctx = next(iter(lz._CTX_CAP))
+    cap_n, cap_m, _ = lz._CTX_CAP[ctx]
+    assert _lib.lib().cxa_lanczos_reshape(ctx, cap_n + 1, cap_m) == _lib.ERR_INVALID
+    assert _lib.lib().cxa_lanczos_reshape(ctx, cap_n, cap_m + 1) == _lib.ERR_INVALID
+    fresh()
+
+
 def test_lanczos_matches_oracle_recurrence_on_heff2():
     psi, L, M1, M2, R = problem(16, seed=17)
     H = Heff2(psi, L, M1, M2, R)
