@@ -71,8 +71,8 @@ struct PlanVariant {
   bool built = false;   // tile list made and uploaded (done on first use, see bc_ensure)
   int kind = -1;        // tile kind override, -1 = per-block default
   int split_mode = -1;  // -1 automatic, 0 off, 1 on (four waves per tile), 2 eight waves
-  std::vector<cxa::GTile> tiles;  // kXcds ranges of per_xcd tiles
-  cxa::DevBuf d_tiles;
+  cxa::DevBuf d_tiles;  // kXcds ranges of per_xcd tiles (made in a reused host buffer)
+  int ntiles = 0, nlive = 0;  // tiles in the list, and those that are not padding
   int per_xcd = 0;
   int splitk = 0;  // waves per tile of the split-K launch, 0 = none (mix plans: 1 = four-wave)
   int tile_class = 0;
@@ -118,30 +118,29 @@ int build_variant(cxa_bc_plan *p, int prec) {
   const bool real = p->dtype == CXA_F64 || p->dtype == CXA_F32;
   PlanVariant &v = p->var[prec];
   v.built = false;
+  // the host tile list lives only until it is uploaded: one buffer per thread keeps its pages
+  thread_local std::vector<GTile> tiles;
   if (p->mix) {
     v.launch_kind = mix_kind_ok(v.kind) ? v.kind : TK_1x1;
-    v.tile_class = make_tiles(p->blks, p->segs, v.tiles, &v.per_xcd, v.launch_kind, false);
+    v.tile_class = make_tiles(p->blks, p->segs, tiles, &v.per_xcd, v.launch_kind, false);
     v.splitk = v.split_mode == 1 ? 1 : 0;  // mix plans: four-wave workgroups (two slices each)
-    if (v.d_tiles.p) {
-      CXA_HIP(hipFree(v.d_tiles.p));
-      v.d_tiles.p = nullptr;
-    }
-    const int rc = upload(v.d_tiles, v.tiles);
-    v.built = rc == CXA_OK;
-    return rc;
+  } else {
+    v.tile_class = make_tiles(p->blks, p->segs, tiles, &v.per_xcd, v.kind, real);
+    const bool split = real && v.tile_class != 2 &&
+                       (v.split_mode >= 1 ||
+                        (v.split_mode < 0 && prefer_splitk(p->blks, p->segs, tiles,
+                                                           v.tile_class)));
+    // eight-wave split-K only for MFMA tiles (row tiles always split over four waves)
+    v.splitk = !split ? 0 : (v.split_mode == 2 && v.tile_class == 0 ? 8 : 4);
   }
-  v.tile_class = make_tiles(p->blks, p->segs, v.tiles, &v.per_xcd, v.kind, real);
-  const bool split = real && v.tile_class != 2 &&
-                     (v.split_mode >= 1 || (v.split_mode < 0 && prefer_splitk(p->blks, p->segs,
-                                                                              v.tiles,
-                                                                              v.tile_class)));
-  // eight-wave split-K only for MFMA tiles (row tiles always split over four waves)
-  v.splitk = !split ? 0 : (v.split_mode == 2 && v.tile_class == 0 ? 8 : 4);
+  v.ntiles = (int)tiles.size();
+  v.nlive = 0;
+  for (const GTile &t : tiles) v.nlive += t.blk >= 0;
   if (v.d_tiles.p) {
     CXA_HIP(hipFree(v.d_tiles.p));
     v.d_tiles.p = nullptr;
   }
-  const int rc = upload(v.d_tiles, v.tiles);
+  const int rc = upload(v.d_tiles, tiles);
   v.built = rc == CXA_OK;
   return rc;
 }
@@ -173,7 +172,8 @@ int upload_plan(cxa_bc_plan *p) {
     total += (q.bytes + 255) / 256 * 256;
   }
   if (total) {
-    std::vector<char> host(total, 0);
+    thread_local std::vector<char> host;  // keeps its pages between plans
+    host.assign(total, 0);
     for (const Part &q : parts)
       if (q.bytes) std::memcpy(host.data() + q.at, q.src, q.bytes);
     CXA_HIP(hipMalloc(&p->d_meta.p, host.size()));
@@ -637,11 +637,7 @@ extern "C" int cxa_block_contract_get_config(const cxa_bc_plan *p, int prec, int
   const PlanVariant &v = p->var[prec];
   if (kind) *kind = v.kind;
   if (split) *split = v.splitk == 8 ? 2 : (v.splitk ? 1 : 0);
-  if (ntiles) {
-    int n = 0;
-    for (const GTile &t : v.tiles) n += t.blk >= 0;
-    *ntiles = n;
-  }
+  if (ntiles) *ntiles = v.nlive;
   return CXA_OK;
 }
 
@@ -722,7 +718,7 @@ int bc_exec_typed(const cxa_bc_plan *p, const void *L, const void *R, void *C, i
   g.a_k4 = lay->a_k4;
   g.b_k4 = lay->b_k4;
   g.c_k4 = lay->c_k4;
-  g.ntiles = (int)v.tiles.size();
+  g.ntiles = v.ntiles;
   g.per_xcd = v.per_xcd;
   g.splitk = v.splitk;
   g.tile_class = v.tile_class;
@@ -842,8 +838,8 @@ int bc_dot_slots(const cxa_bc_plan *p, int prec) {
   if (!p || p->mix || is_complex(p->dtype) || prec < 0 || prec > CXA_PREC_FP32X3) return 0;
   if (bc_ensure(p, prec, nullptr)) return 0;
   const PlanVariant &v = p->var[prec];
-  if (v.tile_class == 1 || v.tiles.empty()) return 0;  // make_tiles' row-streaming class
-  return (int)v.tiles.size();
+  if (v.tile_class == 1 || v.ntiles == 0) return 0;  // make_tiles' row-streaming class
+  return v.ntiles;
 }
 double bc_flops(const cxa_bc_plan *p) { return p->flops; }
 int bc_dtype(const cxa_bc_plan *p) { return p->dtype; }

@@ -40,8 +40,11 @@ int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
   if (kind_override == TK_4x2 && allow_rows) cls = TC_BIG;  // allow_rows: real dtypes only
   else if (kind_override == TK_ROWS ? all_narrow : (kind_override < 0 && all_narrow))
     cls = TC_ROWS;
-  // blocks in operand order: first segment's A offset, then its B offset
-  std::vector<int> order;
+  // blocks in operand order: first segment's A offset, then its B offset (the scratch
+  // vectors keep their capacity between calls: plans of tens of thousands of tiles are made
+  // per DMRG bond, and fresh pages cost more than the planning itself)
+  thread_local std::vector<int> order;
+  order.clear();
   for (size_t bi = 0; bi < blks.size(); ++bi)
     if (blks[bi].M > 0 && blks[bi].N > 0) order.push_back((int)bi);
   auto key = [&](int bi) {
@@ -50,8 +53,8 @@ int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
                       : std::make_pair(INT64_MAX, INT64_MAX);
   };
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
-  std::vector<Item> items;
-  items.reserve(order.size() * 4);
+  thread_local std::vector<Item> items;
+  items.clear();
   for (int bi : order) {
     const GBlk &b = blks[bi];
     int kind;
@@ -84,7 +87,7 @@ int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
     size_t at, len;
     int64_t cost;
   };
-  std::vector<Run> runs;
+  thread_local std::vector<Run> runs;
   for (int x = 0; x < kXcds; ++x) {
     const size_t lo = std::min(items.size(), x * per), hi = std::min(items.size(), lo + per);
     runs.clear();
@@ -105,13 +108,21 @@ int make_tiles(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
 bool prefer_splitk(const std::vector<GBlk> &blks, const std::vector<GSeg> &segs,
                    const std::vector<GTile> &tiles, int tile_class) {
   if (tiles.empty() || tile_class == TC_BIG) return false;
+  // a tile's work is its block's: summed once per block, then once per tile
+  thread_local std::vector<int64_t> bwork;
+  bwork.assign(blks.size(), -1);
   int64_t work = 0, n = 0;
   for (const GTile &t : tiles) {
     if (t.blk < 0) continue;
-    const GBlk &b = blks[t.blk];
+    int64_t &w = bwork[t.blk];
+    if (w < 0) {
+      const GBlk &b = blks[t.blk];
+      w = 0;
+      for (int s = 0; s < b.nseg; ++s)
+        w += tile_class == TC_ROWS ? segs[b.seg0 + s].K : (segs[b.seg0 + s].K + 15) / 16;
+    }
     ++n;
-    for (int s = 0; s < b.nseg; ++s)
-      work += tile_class == TC_ROWS ? segs[b.seg0 + s].K : (segs[b.seg0 + s].K + 15) / 16;
+    work += w;
   }
   if (n == 0) return false;
   // few tiles (under two per SIMD of a 256-CU device) that each walk a long K range
