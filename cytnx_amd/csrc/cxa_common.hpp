@@ -50,6 +50,11 @@ inline int fail(int code, const char *fmt, ...) {
 // for there, so it neither waits for nor is ordered after work queued on the caller's streams.
 // The data is on the device when it returns.
 int upload_now(void *dst, const void *src, size_t bytes);
+// The same for clearing device memory (bytes set to zero).
+int zero_now(void *dst, size_t bytes);
+// That stream, for other set-up work that must not wait for the caller's queued work (the
+// caller synchronises it before the results are used).
+int upload_stream(hipStream_t *s);
 
 }  // namespace cxa
 

@@ -226,6 +226,7 @@ struct cxa_chain {
   std::vector<void *> packed_segs, packed64, packed32;
   std::vector<int64_t> packed_elems;
   std::vector<std::vector<cxa::PackRegion>> pack_regions;
+  std::vector<void *> pack_regions_dev;  // the same on the device (uploaded at the first fill)
   // A mix step followed by a step that reads its output as A: the output is written k4-packed
   // (packed_outs: the mix plan's outputs redirected, device) and read as a packed A.
   std::vector<void *> packed_outs;
@@ -247,7 +248,8 @@ struct cxa_chain {
   int prec = CXA_PREC_FP64;
   double flops = 0;
   ~cxa_chain() {
-    for (auto *v : {&mid64, &mid32, &fixed32, &packed_segs, &packed64, &packed32, &packed_outs})
+    for (auto *v : {&mid64, &mid32, &fixed32, &packed_segs, &packed64, &packed32, &packed_outs,
+                    &pack_regions_dev})
       for (void *p : *v)
         if (p) (void)hipFree(p);
     for (void *p : {pk_segs, pk_reg_dev, (void *)pk_map, (void *)pk_psi32})
@@ -258,24 +260,26 @@ struct cxa_chain {
 
 namespace {
 
-// Fill a packed copy of step k's fixed operand (FP64 source) on stream s and wait for it (the
-// region table is a temporary device buffer).
-int fill_packed(const cxa_chain *c, int k, void *dst, bool f32, hipStream_t s) {
+// Fill a packed copy of step k's fixed operand (FP64 source) and wait for it: on the library's
+// upload stream (the fixed operands were synchronised when the chain was created), with the
+// region table uploaded once and kept with the chain.
+int fill_packed(cxa_chain *c, int k, void *dst, bool f32) {
   const std::vector<cxa::PackRegion> &reg = c->pack_regions[k];
   if (reg.empty()) return CXA_OK;
-  void *d_reg = nullptr;
-  CXA_HIP(hipMalloc(&d_reg, reg.size() * sizeof(cxa::PackRegion)));
-  int rc = CXA_OK;
-  if (hipMemcpyAsync(d_reg, reg.data(), reg.size() * sizeof(cxa::PackRegion),
-                     hipMemcpyHostToDevice, s) != hipSuccess)
-    rc = cxa::fail(CXA_ERR_HIP, "packed copy: region upload failed");
-  if (rc == CXA_OK)
-    rc = cxa::vec_pack_b((const double *)c->fixed[k], dst, f32,
-                         (const cxa::PackRegion *)d_reg, (int)reg.size(), s);
-  if (hipStreamSynchronize(s) != hipSuccess && rc == CXA_OK)
-    rc = cxa::fail(CXA_ERR_HIP, "packed copy: stream synchronisation failed");
-  (void)hipFree(d_reg);
-  return rc;
+  hipStream_t s;
+  if (const int rc = cxa::upload_stream(&s)) return rc;
+  if (!c->pack_regions_dev[k]) {
+    CXA_HIP(hipMalloc(&c->pack_regions_dev[k], reg.size() * sizeof(cxa::PackRegion)));
+    const int rc = cxa::upload_now(c->pack_regions_dev[k], reg.data(),
+                                   reg.size() * sizeof(cxa::PackRegion));
+    if (rc) return rc;
+  }
+  const int rc = cxa::vec_pack_b((const double *)c->fixed[k], dst, f32,
+                                 (const cxa::PackRegion *)c->pack_regions_dev[k],
+                                 (int)reg.size(), s);
+  if (rc) return rc;
+  CXA_HIP(hipStreamSynchronize(s));
+  return CXA_OK;
 }
 
 }  // namespace
@@ -322,6 +326,7 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
   c->packed32.assign(nsteps, nullptr);
   c->packed_elems.assign(nsteps, 0);
   c->pack_regions.assign(nsteps, {});
+  c->pack_regions_dev.assign(nsteps, nullptr);
   c->packed_outs.assign(nsteps, nullptr);
   c->a_packed.assign(nsteps, 0);
   c->c_packed.assign(nsteps, 0);
@@ -343,11 +348,10 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
     }
     c->packed_elems[k] = elems;
     CXA_HIP(hipMalloc(&c->packed_segs[k], std::max<size_t>(1, segs.size()) * sizeof(cxa::GSeg)));
-    if (!segs.empty())
-      CXA_HIP(hipMemcpy(c->packed_segs[k], segs.data(), segs.size() * sizeof(cxa::GSeg),
-                        hipMemcpyHostToDevice));
+    int rc = cxa::upload_now(c->packed_segs[k], segs.data(), segs.size() * sizeof(cxa::GSeg));
+    if (rc) return rc;
     CXA_HIP(hipMalloc(&c->packed64[k], std::max<int64_t>(1, elems) * sizeof(double)));
-    int rc = fill_packed(c.get(), k, c->packed64[k], false, nullptr);
+    rc = fill_packed(c.get(), k, c->packed64[k], false);
     if (rc) return rc;
   }
   // a mix step whose output the next step reads as its (running) A operand: write it packed
@@ -362,18 +366,19 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
     if (cxa::bc_pack_a(plans[k + 1], out_elems[k], segs, regions, &elems) != CXA_OK ||
         cxa::bc_mix_pack_outs(plans[k], regions, outs) != CXA_OK)
       continue;
-    CXA_HIP(hipMemcpy(c->packed_segs[k + 1], segs.data(), segs.size() * sizeof(cxa::GSeg),
-                      hipMemcpyHostToDevice));
+    int rc = cxa::upload_now(c->packed_segs[k + 1], segs.data(),
+                             segs.size() * sizeof(cxa::GSeg));
+    if (rc) return rc;
     CXA_HIP(hipMalloc(&c->packed_outs[k], std::max<size_t>(1, outs.size()) * sizeof(cxa::MixOut)));
-    if (!outs.empty())
-      CXA_HIP(hipMemcpy(c->packed_outs[k], outs.data(), outs.size() * sizeof(cxa::MixOut),
-                        hipMemcpyHostToDevice));
+    rc = cxa::upload_now(c->packed_outs[k], outs.data(), outs.size() * sizeof(cxa::MixOut));
+    if (rc) return rc;
     // the packed intermediate: quads past K are never written and stay zero
     c->mid_elems[k] = std::max<int64_t>(1, elems);
     CXA_HIP(hipFree(c->mid64[k]));
     c->mid64[k] = nullptr;
     CXA_HIP(hipMalloc(&c->mid64[k], c->mid_elems[k] * sizeof(double)));
-    CXA_HIP(hipMemset(c->mid64[k], 0, c->mid_elems[k] * sizeof(double)));
+    rc = cxa::zero_now(c->mid64[k], c->mid_elems[k] * sizeof(double));
+    if (rc) return rc;
     c->a_packed[k + 1] = 1;
     c->c_packed[k] = 1;
   }
@@ -400,8 +405,12 @@ namespace {
 // The packed running vector's map and buffers (FP32 set-up), built on the device: every
 // vector element that a view reads must get exactly one packed position, or the chain keeps
 // reading the FP64 vector (the map and buffers are released again).
-int build_packed_vector(cxa_chain *c, hipStream_t s) {
+int build_packed_vector(cxa_chain *c) {
   if (c->pk_regions.empty() || c->pk_n <= 0) return CXA_OK;
+  // set-up work with no dependence on the caller's queued work: the library's upload stream
+  hipStream_t s;
+  int rc = upload_stream(&s);
+  if (rc) return rc;
   const size_t nreg = c->pk_regions.size();
   CXA_HIP(hipMalloc(&c->pk_reg_dev, nreg * sizeof(cxa::PackRegion)));
   CXA_HIP(hipMemcpyAsync(c->pk_reg_dev, c->pk_regions.data(), nreg * sizeof(cxa::PackRegion),
@@ -410,8 +419,8 @@ int build_packed_vector(cxa_chain *c, hipStream_t s) {
   int *flag = c->pk_map + c->pk_n;
   CXA_HIP(hipMemsetAsync(c->pk_map, 0xff, sizeof(int32_t) * c->pk_n, s));
   CXA_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), s));
-  int rc = vec_pack_map((const cxa::PackRegion *)c->pk_reg_dev, (int)nreg, c->pk_n, c->pk_map,
-                        flag, s);
+  rc = vec_pack_map((const cxa::PackRegion *)c->pk_reg_dev, (int)nreg, c->pk_n, c->pk_map, flag,
+                    s);
   if (rc) return rc;
   int conflict = 0;
   CXA_HIP(hipMemcpyAsync(&conflict, flag, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -455,7 +464,7 @@ extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
     if (c->packed_segs[k]) {  // packed steps keep only the packed FP32 copy
       CXA_HIP(hipMalloc(&c->packed32[k],
                         std::max<int64_t>(1, c->packed_elems[k]) * sizeof(float)));
-      int rc = fill_packed(c, k, c->packed32[k], true, s);
+      int rc = fill_packed(c, k, c->packed32[k], true);
       if (rc) return rc;
       continue;
     }
@@ -466,10 +475,12 @@ extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
   }
   for (size_t k = 0; k < c->mid32.size(); ++k) {
     CXA_HIP(hipMalloc(&c->mid32[k], c->mid_elems[k] * sizeof(float)));
-    if (c->c_packed[k])  // the padding of a packed intermediate must read as zero
-      CXA_HIP(hipMemsetAsync(c->mid32[k], 0, c->mid_elems[k] * sizeof(float), s));
+    if (c->c_packed[k]) {  // the padding of a packed intermediate must read as zero
+      const int rc = zero_now(c->mid32[k], c->mid_elems[k] * sizeof(float));
+      if (rc) return rc;
+    }
   }
-  return build_packed_vector(c, s);
+  return build_packed_vector(c);
 }
 
 namespace {

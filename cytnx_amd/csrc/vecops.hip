@@ -605,52 +605,68 @@ int vec_cast_f64_f32(const double *x, float *y, int64_t n, hipStream_t s) {
 // One workgroup per region (grid-stride over regions): packed element e of a region holds
 // source element (k, x) with k = 4 * (e / 4X) + e % 4, x = (e % 4X) / 4, or zero for k >= K.
 // Consecutive threads write consecutive packed elements.
-template <typename T>
-__global__ __launch_bounds__(kT) void k_pack_b(const double *__restrict__ src, T *__restrict__ dst,
-                                               const PackRegion *__restrict__ regions,
-                                               int nregions) {
+// The region loops of the packing launches: block (x, y) takes regions x, x + gridDim.x, ...
+// and, within each, the packed elements y * kT + threadIdx.x, stepping gridDim.y * kT (a few
+// large regions still spread over the whole grid).  Element e of a region is (k, x) with
+// k = 4 (e / 4X) + (e & 3), x = (e mod 4X) / 4.
+template <typename F>
+__device__ __forceinline__ void for_region_elems(const PackRegion *__restrict__ regions,
+                                                 int nregions, F f) {
   for (int r = blockIdx.x; r < nregions; r += gridDim.x) {
     const PackRegion R = regions[r];
     const int64_t row = 4 * (int64_t)R.X;
     const int64_t nel = (int64_t)((R.K + 3) / 4) * row;
-    for (int64_t e = threadIdx.x; e < nel; e += kT) {
-      const int64_t q = e / row;
-      const int64_t rem = e - q * row;
-      const int64_t k = 4 * q + (rem & 3), x = rem >> 2;
-      dst[R.dst_off + e] = k < R.K ? (T)src[R.src_off + k * R.sk + x * R.sx] : (T)0;
+    const int64_t step = (int64_t)gridDim.y * kT;
+    if (nel < (int64_t)1 << 31) {  // 32-bit index arithmetic
+      const uint32_t row32 = (uint32_t)row;
+      for (int64_t e = (int64_t)blockIdx.y * kT + threadIdx.x; e < nel; e += step) {
+        const uint32_t e32 = (uint32_t)e;
+        const uint32_t q = e32 / row32;
+        const uint32_t rem = e32 - q * row32;
+        f(R, e, (int64_t)(4 * q + (rem & 3)), (int64_t)(rem >> 2));
+      }
+    } else {
+      for (int64_t e = (int64_t)blockIdx.y * kT + threadIdx.x; e < nel; e += step) {
+        const int64_t q = e / row;
+        const int64_t rem = e - q * row;
+        f(R, e, 4 * q + (rem & 3), rem >> 2);
+      }
     }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kT) void k_pack_b(const double *__restrict__ src, T *__restrict__ dst,
+                                               const PackRegion *__restrict__ regions,
+                                               int nregions) {
+  for_region_elems(regions, nregions, [&](const PackRegion &R, int64_t e, int64_t k, int64_t x) {
+    dst[R.dst_off + e] = k < R.K ? (T)src[R.src_off + k * R.sk + x * R.sx] : (T)0;
+  });
 }
 
 __global__ __launch_bounds__(kT) void k_pack_map(const PackRegion *__restrict__ regions,
                                                 int nregions, int64_t n, int *__restrict__ pmap,
                                                 int *__restrict__ conflict) {
-  for (int r = blockIdx.x; r < nregions; r += gridDim.x) {
-    const PackRegion R = regions[r];
-    const int64_t row = 4 * (int64_t)R.X;
-    const int64_t nel = (int64_t)((R.K + 3) / 4) * row;
-    for (int64_t e = threadIdx.x; e < nel; e += kT) {
-      const int64_t q = e / row;
-      const int64_t rem = e - q * row;
-      const int64_t k = 4 * q + (rem & 3), x = rem >> 2;
-      if (k >= R.K) continue;  // padding: no source element
-      const int64_t src = R.src_off + k * R.sk + x * R.sx;
-      const int dst = (int)(R.dst_off + e);
-      if (src < 0 || src >= n) {
-        atomicOr(conflict, 1);
-        continue;
-      }
-      const int prev = atomicCAS(pmap + src, -1, dst);
-      if (prev != -1 && prev != dst) atomicOr(conflict, 1);
+  for_region_elems(regions, nregions, [&](const PackRegion &R, int64_t e, int64_t k, int64_t x) {
+    if (k >= R.K) return;  // padding: no source element
+    const int64_t src = R.src_off + k * R.sk + x * R.sx;
+    const int dst = (int)(R.dst_off + e);
+    if (src < 0 || src >= n) {
+      atomicOr(conflict, 1);
+      return;
     }
-  }
+    const int prev = atomicCAS(pmap + src, -1, dst);
+    if (prev != -1 && prev != dst) atomicOr(conflict, 1);
+  });
 }
+
+// grid of the packing launches: regions over x, 16 element slices over y
+static dim3 pack_grid(int nregions) { return dim3(nregions < 4096 ? nregions : 4096, 16); }
 
 int vec_pack_map(const PackRegion *regions, int nregions, int64_t n, int *pmap, int *conflict,
                  hipStream_t s) {
   if (nregions <= 0) return CXA_OK;
-  const int grid = nregions < 4096 ? nregions : 4096;
-  hipLaunchKernelGGL(k_pack_map, dim3(grid), dim3(kT), 0, s, regions, nregions, n, pmap,
+  hipLaunchKernelGGL(k_pack_map, pack_grid(nregions), dim3(kT), 0, s, regions, nregions, n, pmap,
                      conflict);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
@@ -659,13 +675,12 @@ int vec_pack_map(const PackRegion *regions, int nregions, int64_t n, int *pmap, 
 int vec_pack_b(const double *src, void *dst, bool dst_f32, const PackRegion *regions,
                int nregions, hipStream_t s) {
   if (nregions <= 0) return CXA_OK;
-  const int grid = nregions < 4096 ? nregions : 4096;
   if (dst_f32)
-    hipLaunchKernelGGL(k_pack_b<float>, dim3(grid), dim3(kT), 0, s, src, (float *)dst, regions,
-                       nregions);
-  else
-    hipLaunchKernelGGL(k_pack_b<double>, dim3(grid), dim3(kT), 0, s, src, (double *)dst,
+    hipLaunchKernelGGL(k_pack_b<float>, pack_grid(nregions), dim3(kT), 0, s, src, (float *)dst,
                        regions, nregions);
+  else
+    hipLaunchKernelGGL(k_pack_b<double>, pack_grid(nregions), dim3(kT), 0, s, src,
+                       (double *)dst, regions, nregions);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }
