@@ -56,6 +56,15 @@ int zero_now(void *dst, size_t bytes);
 // caller synchronises it before the results are used).
 int upload_stream(hipStream_t *s);
 
+// Device memory of objects that are made and dropped often (a DMRG run makes a chain per bond
+// update): pool_release keeps a block for reuse instead of freeing it (hipFree waits for the
+// device to drain), pool_alloc hands out a kept block of the same size class (sizes round up
+// to quarter octaves), after one device synchronisation if blocks were released since the
+// last, so that no launch queued before a release still uses the block.  At most a few GiB
+// per device are kept; beyond that, and for blocks the pool did not make, release frees.
+int pool_alloc(void **p, size_t bytes);
+void pool_release(void *p);
+
 }  // namespace cxa
 
 #define CXA_HIP(call)                                                                   \
@@ -64,6 +73,12 @@ int upload_stream(hipStream_t *s);
     if (e_ != hipSuccess)                                                               \
       return cxa::fail(CXA_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
                        __FILE__, __LINE__);                                             \
+  } while (0)
+
+#define CXA_RC(call)              \
+  do {                            \
+    const int rc_ = (call);       \
+    if (rc_) return rc_;          \
   } while (0)
 
 #define CXA_CHECK(cond, code, ...)                 \

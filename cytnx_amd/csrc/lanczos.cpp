@@ -251,10 +251,10 @@ struct cxa_chain {
     for (auto *v : {&mid64, &mid32, &fixed32, &packed_segs, &packed64, &packed32, &packed_outs,
                     &pack_regions_dev})
       for (void *p : *v)
-        if (p) (void)hipFree(p);
+        if (p) cxa::pool_release(p);
     for (void *p : {pk_segs, pk_reg_dev, (void *)pk_map, (void *)pk_psi32})
-      if (p) (void)hipFree(p);
-    if (ws) (void)hipFree(ws);
+      if (p) cxa::pool_release(p);
+    cxa::pool_release(ws);
   }
 };
 
@@ -269,7 +269,8 @@ int fill_packed(cxa_chain *c, int k, void *dst, bool f32) {
   hipStream_t s;
   if (const int rc = cxa::upload_stream(&s)) return rc;
   if (!c->pack_regions_dev[k]) {
-    CXA_HIP(hipMalloc(&c->pack_regions_dev[k], reg.size() * sizeof(cxa::PackRegion)));
+    CXA_RC(cxa::pool_alloc((void **)&c->pack_regions_dev[k],
+                           reg.size() * sizeof(cxa::PackRegion)));
     const int rc = cxa::upload_now(c->pack_regions_dev[k], reg.data(),
                                    reg.size() * sizeof(cxa::PackRegion));
     if (rc) return rc;
@@ -317,8 +318,8 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
   }
   c->mid64.assign(nsteps > 1 ? nsteps - 1 : 0, nullptr);
   for (int k = 0; k + 1 < nsteps; ++k)
-    CXA_HIP(hipMalloc(&c->mid64[k], std::max<int64_t>(1, out_elems[k]) * es));
-  if (wsmax > 0) CXA_HIP(hipMalloc(&c->ws, wsmax * 16));
+    CXA_RC(cxa::pool_alloc((void **)&c->mid64[k], std::max<int64_t>(1, out_elems[k]) * es));
+  if (wsmax > 0) CXA_RC(cxa::pool_alloc((void **)&c->ws, wsmax * 16));
   // k4-packed copies of the fixed B operands (read once here: the chain keeps using these
   // copies, so a fixed operand changed after creation needs a new chain)
   c->packed_segs.assign(nsteps, nullptr);
@@ -347,10 +348,11 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
       synced = true;
     }
     c->packed_elems[k] = elems;
-    CXA_HIP(hipMalloc(&c->packed_segs[k], std::max<size_t>(1, segs.size()) * sizeof(cxa::GSeg)));
+    CXA_RC(cxa::pool_alloc((void **)&c->packed_segs[k],
+                           std::max<size_t>(1, segs.size()) * sizeof(cxa::GSeg)));
     int rc = cxa::upload_now(c->packed_segs[k], segs.data(), segs.size() * sizeof(cxa::GSeg));
     if (rc) return rc;
-    CXA_HIP(hipMalloc(&c->packed64[k], std::max<int64_t>(1, elems) * sizeof(double)));
+    CXA_RC(cxa::pool_alloc((void **)&c->packed64[k], std::max<int64_t>(1, elems) * sizeof(double)));
     rc = fill_packed(c.get(), k, c->packed64[k], false);
     if (rc) return rc;
   }
@@ -369,14 +371,15 @@ extern "C" int cxa_chain_create(int nsteps, cxa_bc_plan *const *plans, const voi
     int rc = cxa::upload_now(c->packed_segs[k + 1], segs.data(),
                              segs.size() * sizeof(cxa::GSeg));
     if (rc) return rc;
-    CXA_HIP(hipMalloc(&c->packed_outs[k], std::max<size_t>(1, outs.size()) * sizeof(cxa::MixOut)));
+    CXA_RC(cxa::pool_alloc((void **)&c->packed_outs[k],
+                           std::max<size_t>(1, outs.size()) * sizeof(cxa::MixOut)));
     rc = cxa::upload_now(c->packed_outs[k], outs.data(), outs.size() * sizeof(cxa::MixOut));
     if (rc) return rc;
     // the packed intermediate: quads past K are never written and stay zero
     c->mid_elems[k] = std::max<int64_t>(1, elems);
-    CXA_HIP(hipFree(c->mid64[k]));
+    cxa::pool_release(c->mid64[k]);
     c->mid64[k] = nullptr;
-    CXA_HIP(hipMalloc(&c->mid64[k], c->mid_elems[k] * sizeof(double)));
+    CXA_RC(cxa::pool_alloc((void **)&c->mid64[k], c->mid_elems[k] * sizeof(double)));
     rc = cxa::zero_now(c->mid64[k], c->mid_elems[k] * sizeof(double));
     if (rc) return rc;
     c->a_packed[k + 1] = 1;
@@ -412,10 +415,11 @@ int build_packed_vector(cxa_chain *c) {
   int rc = upload_stream(&s);
   if (rc) return rc;
   const size_t nreg = c->pk_regions.size();
-  CXA_HIP(hipMalloc(&c->pk_reg_dev, nreg * sizeof(cxa::PackRegion)));
+  CXA_RC(cxa::pool_alloc((void **)&c->pk_reg_dev, nreg * sizeof(cxa::PackRegion)));
   CXA_HIP(hipMemcpyAsync(c->pk_reg_dev, c->pk_regions.data(), nreg * sizeof(cxa::PackRegion),
                          hipMemcpyHostToDevice, s));
-  CXA_HIP(hipMalloc(&c->pk_map, sizeof(int32_t) * (c->pk_n + 1)));  // + the conflict flag
+  // + the conflict flag
+  CXA_RC(cxa::pool_alloc((void **)&c->pk_map, sizeof(int32_t) * (c->pk_n + 1)));
   int *flag = c->pk_map + c->pk_n;
   CXA_HIP(hipMemsetAsync(c->pk_map, 0xff, sizeof(int32_t) * c->pk_n, s));
   CXA_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), s));
@@ -426,17 +430,19 @@ int build_packed_vector(cxa_chain *c) {
   CXA_HIP(hipMemcpyAsync(&conflict, flag, sizeof(int), hipMemcpyDeviceToHost, s));
   CXA_HIP(hipStreamSynchronize(s));
   if (conflict) {  // no packed vector for this chain
-    CXA_HIP(hipFree(c->pk_map));
+    cxa::pool_release(c->pk_map);
     c->pk_map = nullptr;
-    CXA_HIP(hipFree(c->pk_reg_dev));
+    cxa::pool_release(c->pk_reg_dev);
     c->pk_reg_dev = nullptr;
     return CXA_OK;
   }
-  CXA_HIP(hipMalloc(&c->pk_segs, std::max<size_t>(1, c->pk_segs0.size()) * sizeof(cxa::GSeg)));
+  CXA_RC(cxa::pool_alloc((void **)&c->pk_segs,
+                         std::max<size_t>(1, c->pk_segs0.size()) * sizeof(cxa::GSeg)));
   if (!c->pk_segs0.empty())
     CXA_HIP(hipMemcpyAsync(c->pk_segs, c->pk_segs0.data(),
                            c->pk_segs0.size() * sizeof(cxa::GSeg), hipMemcpyHostToDevice, s));
-  CXA_HIP(hipMalloc(&c->pk_psi32, sizeof(float) * std::max<int64_t>(1, c->pk_elems)));
+  CXA_RC(cxa::pool_alloc((void **)&c->pk_psi32,
+                         sizeof(float) * std::max<int64_t>(1, c->pk_elems)));
   CXA_HIP(hipMemsetAsync(c->pk_psi32, 0, sizeof(float) * std::max<int64_t>(1, c->pk_elems), s));
   CXA_HIP(hipStreamSynchronize(s));  // ready for runs on any stream
   return CXA_OK;
@@ -462,19 +468,20 @@ extern "C" int cxa_chain_prepare(cxa_chain *c, int prec, cxa_stream_t stream) {
   c->mid32.assign(c->mid64.size(), nullptr);
   for (int k = 0; k < c->nsteps; ++k) {
     if (c->packed_segs[k]) {  // packed steps keep only the packed FP32 copy
-      CXA_HIP(hipMalloc(&c->packed32[k],
-                        std::max<int64_t>(1, c->packed_elems[k]) * sizeof(float)));
+      CXA_RC(cxa::pool_alloc((void **)&c->packed32[k],
+                             std::max<int64_t>(1, c->packed_elems[k]) * sizeof(float)));
       int rc = fill_packed(c, k, c->packed32[k], true);
       if (rc) return rc;
       continue;
     }
-    CXA_HIP(hipMalloc(&c->fixed32[k], std::max<int64_t>(1, c->fixed_elems[k]) * sizeof(float)));
+    CXA_RC(cxa::pool_alloc((void **)&c->fixed32[k],
+                           std::max<int64_t>(1, c->fixed_elems[k]) * sizeof(float)));
     int rc = vec_cast_f64_f32((const double *)c->fixed[k], (float *)c->fixed32[k],
                               c->fixed_elems[k], s);
     if (rc) return rc;
   }
   for (size_t k = 0; k < c->mid32.size(); ++k) {
-    CXA_HIP(hipMalloc(&c->mid32[k], c->mid_elems[k] * sizeof(float)));
+    CXA_RC(cxa::pool_alloc((void **)&c->mid32[k], c->mid_elems[k] * sizeof(float)));
     if (c->c_packed[k]) {  // the padding of a packed intermediate must read as zero
       const int rc = zero_now(c->mid32[k], c->mid_elems[k] * sizeof(float));
       if (rc) return rc;
