@@ -88,7 +88,8 @@ def _context(n: int, maxiter: int, device: int):
             del _CTX_CACHE[k]
         _CTX_CACHE[key] = ctx
         return ctx
-    cap_n = max(n, int(n * CTX_HEADROOM))
+    # the headroom stays within the basis budget that sized maxiter
+    cap_n = max(n, min(int(n * CTX_HEADROOM), KRYLOV_BUDGET_BYTES // (8 * (maxiter + 1))))
     need = 8 * cap_n * (maxiter + 1)
     held = sum(8 * cn * (cm + 1) for cn, cm, _ in _CTX_CAP.values())
     if _CTX_CAP and held + need > CTX_CACHE_BYTES:

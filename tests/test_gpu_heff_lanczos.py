@@ -7,7 +7,9 @@
 * Lanczos: the reference's known 27x27 eigenvalue (-1628.9964650426593 within 1e-5), the
   lan_block_{A,B,C} U(1) operator (eigenvalue within 1e-12 of eigh, residual < 1e-6), Heff2 at
   small chi against numpy.linalg.eigh of the explicit matrix, FP32+refinement within 1e-10 of
-  FP64, and graph replay identical to plain launches."""
+  FP64, and graph replay identical to plain launches.
+* Set-up reuse: Lanczos contexts reshaped across vector lengths give bit-identical runs, and
+  chains built from pooled device blocks still match the oracle."""
 import os
 
 import numpy as np
