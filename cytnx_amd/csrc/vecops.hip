@@ -129,6 +129,7 @@ __global__ __launch_bounds__(kT) void k_residual(const double *__restrict__ y,
 // beta V[it - 1] with its |u|^2 partials into part_b.  This replaces k_dot_basis + k_residual:
 // one launch and one read of u fewer per iteration.  With p32 non-null the residual also goes,
 // rounded to FP32, to its packed position pmap[i] (skipped where pmap[i] < 0).
+template <bool PK>
 __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict__ y,
                                                        double *__restrict__ u,
                                                        double *__restrict__ V, int64_t n,
@@ -141,9 +142,21 @@ __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict_
   const int it = iter.get();
   double *cur = V + (int64_t)it * n;
   const double *prev = it > 0 ? V + (int64_t)(it - 1) * n : cur;
-  // the thread's first elements are loaded before the partial sums, so that both rounds of
-  // loads are in flight together (the vector does not depend on alpha and beta)
-  constexpr int R = 2;
+  // The first partials are loaded first, then the thread's first vector elements, so that
+  // both rounds of loads are in flight together (the vector does not depend on alpha and
+  // beta) and the partial sums wait only for their own loads (loads retire in issue order).
+  // The loads are unconditional, at clamped indices, with the values selected afterwards, so
+  // that no branch splits the sequence and the wait counts stay exact.
+  constexpr int PP = 4, R = 2;
+  double pa[PP], pb[PP];
+#pragma unroll
+  for (int q = 0; q < PP; ++q) {
+    const int i = threadIdx.x + q * kT;
+    const int ic = i < S ? i : S - 1;
+    const double va = dotp[ic], vb = dotp[S + ic];
+    pa[q] = i < S ? va : 0.0;
+    pb[q] = i < S ? vb : 0.0;
+  }
   const int64_t i0 = (int64_t)blockIdx.x * kT + threadIdx.x, step = (int64_t)gridDim.x * kT;
   double yr[R], ur[R], pr[R];
   int dr[R];
@@ -151,15 +164,27 @@ __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict_
   for (int q = 0; q < R; ++q) {
     const int64_t i = i0 + q * step;
     const bool in = i < n;
-    yr[q] = in ? y[i] : 0.0;
-    ur[q] = in ? u[i] : 0.0;
-    pr[q] = in && it > 0 ? prev[i] : 0.0;
-    dr[q] = in && p32 ? pmap[i] : -1;
+    const int64_t ic = in ? i : n - 1;
+    const double yv = y[ic], uv = u[ic], pv = prev[ic];  // prev is V[0] at it = 0
+    yr[q] = in ? yv : 0.0;
+    ur[q] = in ? uv : 0.0;
+    pr[q] = in && it > 0 ? pv : 0.0;
+    if constexpr (PK) {
+      const int dv = pmap[ic];
+      dr[q] = in ? dv : -1;
+    } else {
+      dr[q] = -1;
+    }
   }
-  // both sums in one pass over the partials (one round of loads, then two block reductions)
+  // both sums in one pass over the partials (then two block reductions)
   __shared__ double bc[2];
   double a0 = 0.0, a1 = 0.0;
-  for (int i = threadIdx.x; i < S; i += kT) {
+#pragma unroll
+  for (int q = 0; q < PP; ++q) {
+    a0 += pa[q];
+    a1 += pb[q];
+  }
+  for (int i = threadIdx.x + PP * kT; i < S; i += kT) {
     a0 += dotp[i];
     a1 += dotp[S + i];
   }
@@ -187,7 +212,7 @@ __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict_
       if (it > 0) cur[i] = vi;
       const double t = yr[q] / b - (a * vi + bp * pr[q]);
       u[i] = t;
-      if (dr[q] >= 0) p32[dr[q]] = (float)t;
+      if (PK && dr[q] >= 0) p32[dr[q]] = (float)t;
       s += t * t;
     }
   }
@@ -196,7 +221,7 @@ __global__ __launch_bounds__(kT) void k_residual_fused(const double *__restrict_
     if (it > 0) cur[i] = vi;
     const double t = y[i] / b - (a * vi + bp * prev[i]);
     u[i] = t;
-    if (p32) {
+    if (PK) {
       const int d = pmap[i];
       if (d >= 0) p32[d] = (float)t;
     }
@@ -505,8 +530,12 @@ int vec_residual(const double *y, double *u, const double *V, int64_t n, const d
 int vec_residual_fused(const double *y, double *u, double *V, int64_t n, const double *dotp,
                        int S, double *As, double *Bs, IterIdx iter, double *part_b,
                        const int *pmap, float *p32, hipStream_t s) {
-  hipLaunchKernelGGL(k_residual_fused, dim3(kPartials), dim3(kT), 0, s, y, u, V, n, dotp, S, As,
-                     Bs, iter, part_b, pmap, p32);
+  if (p32)
+    hipLaunchKernelGGL(k_residual_fused<true>, dim3(kPartials), dim3(kT), 0, s, y, u, V, n, dotp,
+                       S, As, Bs, iter, part_b, pmap, p32);
+  else
+    hipLaunchKernelGGL(k_residual_fused<false>, dim3(kPartials), dim3(kT), 0, s, y, u, V, n,
+                       dotp, S, As, Bs, iter, part_b, pmap, p32);
   CXA_HIP(hipGetLastError());
   return CXA_OK;
 }
