@@ -633,11 +633,30 @@ extern "C" int cxa_chain_tune(cxa_chain *c, const void *x, void *y, int reps, in
   int rc = chain_apply(c, x, y, s);  // every intermediate in place
   for (int k = 0; k < c->nsteps && rc == CXA_OK; ++k) {
     cxa_bc_plan *plan = const_cast<cxa_bc_plan *>(c->plans[k]);
-    // FP32 chains with a packed running vector: step 0 is timed in the form FP32 ground-state
-    // Lanczos runs launch it (reading the packed FP32 copy)
+    // FP32 chains with a packed running vector: step 0 runs in two forms, reading the packed
+    // FP32 copy (FP32 ground-state Lanczos runs) or the FP64 vector (every other matvec); a
+    // tiling is scored by the sum of both times and reported by the packed form's
     const bool pk = k == 0 && c->prec == CXA_PREC_FP32 && c->pk_psi32;
-    double best = 1e30;
+    double best = 1e30, best_ms = 0.0;
     int bk = -1, bs = -1;
+    // mean time of `reps` launches of step k in one form (after two untimed ones)
+    auto time_form = [&](bool packed, double *per) -> int {
+      for (int w = 0; w < 2; ++w) {
+        const int r2 = chain_step(c, k, x, y, s, nullptr, nullptr, packed);
+        if (r2) return r2;
+      }
+      CXA_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; ++r) {
+        const int r2 = chain_step(c, k, x, y, s, nullptr, nullptr, packed);
+        if (r2) return r2;
+      }
+      float t = 0;
+      CXA_HIP(hipEventRecord(e1, s));
+      CXA_HIP(hipEventSynchronize(e1));
+      CXA_HIP(hipEventElapsedTime(&t, e0, e1));
+      *per = (double)t / reps;
+      return CXA_OK;
+    };
     for (int ci = 0; ci < ncand && rc == CXA_OK; ++ci) {
       // mix plans: explicit 16x16, 32x16, 16x32 or 32x32 tiles; split 0 / 1 = eight / four
       // waves
@@ -647,25 +666,12 @@ extern "C" int cxa_chain_tune(cxa_chain *c, const void *x, void *y, int reps, in
       // the big-tile and row classes are real-only (complex plans fall back to the default)
       if (c->dtype != CXA_F64 && (cand[ci][0] == TK_4x2 || cand[ci][0] == TK_ROWS)) continue;
       if ((rc = cxa_block_contract_set_config(plan, c->prec, cand[ci][0], cand[ci][1]))) break;
-      for (int w = 0; w < 2 && rc == CXA_OK; ++w)
-        rc = chain_step(c, k, x, y, s, nullptr, nullptr, pk);
-      if (rc) break;
-      if (hipEventRecord(e0, s) != hipSuccess) {
-        rc = fail(CXA_ERR_HIP, "cxa_chain_tune: event record failed");
-        break;
-      }
-      for (int r = 0; r < reps && rc == CXA_OK; ++r)
-        rc = chain_step(c, k, x, y, s, nullptr, nullptr, pk);
-      if (rc) break;
-      float t = 0;
-      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
-        rc = fail(CXA_ERR_HIP, "cxa_chain_tune: event timing failed");
-        break;
-      }
-      const double per = (double)t / reps;
-      if (per < best) {
-        best = per;
+      double per = 0.0, per_plain = 0.0;
+      if ((rc = time_form(pk, &per))) break;
+      if (pk && (rc = time_form(false, &per_plain))) break;
+      if (per + per_plain < best) {
+        best = per + per_plain;
+        best_ms = per;
         bk = cand[ci][0];
         bs = cand[ci][1];
       }
@@ -673,7 +679,7 @@ extern "C" int cxa_chain_tune(cxa_chain *c, const void *x, void *y, int reps, in
     if (rc == CXA_OK) rc = cxa_block_contract_set_config(plan, c->prec, bk, bs);
     if (kinds) kinds[k] = bk;
     if (splits) splits[k] = bs;
-    if (ms) ms[k] = best;
+    if (ms) ms[k] = best_ms;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -348,8 +348,10 @@ int cxa_chain_step(void *chain, int step, const void *x, void *y, cxa_stream_t s
 /* Autotune: for each step at the chain's current precision, time every candidate tiling
  * (tile shape x workgroup split-K) `reps` times on x -> y and keep the fastest; per-step
  * choices and times (ms) are returned in kinds/splits/ms (arrays of nsteps, may be NULL).
- * In FP32 a step 0 with a packed running vector (cxa_chain_layout bit 3) is timed reading
- * that copy, as FP32 ground-state Lanczos runs launch it.  Synchronizes the stream. */
+ * In FP32 a step 0 with a packed running vector (cxa_chain_layout bit 3) runs in two forms,
+ * reading that copy (FP32 ground-state Lanczos runs) or the FP64 vector (other matvecs): the
+ * tiling with the smaller sum of both times is kept and ms reports the packed form's time.
+ * Synchronizes the stream. */
 int cxa_chain_tune(cxa_chain *chain, const void *x, void *y, int reps, int *kinds, int *splits,
                    double *ms, cxa_stream_t stream);
 int cxa_chain_flops(const cxa_chain *chain, double *flops);
