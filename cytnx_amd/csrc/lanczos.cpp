@@ -830,10 +830,11 @@ extern "C" int cxa_lanczos_reshape(cxa_lanczos *L, int64_t n, int maxiter) {
 namespace {
 
 // One iteration.  it >= 0: the iteration index is passed to the kernels by value; it < 0: they
-// read the device counter, which the iteration then advances (the captured form).
+// read the device counter plus `off` (the captured form; the capture advances the counter once
+// per captured batch).
 int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, bool cplx, int it,
-                 hipStream_t s) {
-  const IterIdx iter{it < 0 ? L->counter : nullptr, it < 0 ? 0 : it};
+                 hipStream_t s, int off = 0) {
+  const IterIdx iter{it < 0 ? L->counter : nullptr, it < 0 ? off : it};
   double *part_a = L->part, *part_b = L->part + kPartials;
   int rc;
   if (L->fused > 0) {  // the chain's last launch leaves the Hu . u and u . u partials
@@ -847,7 +848,7 @@ int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, bool c
       if ((rc = vec_reorth(L->cur, L->V, L->n, L->maxiter + 1, iter, L->part, L->h(), part_b,
                            cplx, s)))
         return rc;
-    return it < 0 ? vec_counter_add(L->counter, 1, s) : CXA_OK;
+    return CXA_OK;
   }
   // cur holds the previous residual u (V[0] at it = 0); w = H u
   if ((rc = fn(ctx, L->cur, L->w, (cxa_stream_t)s))) return rc;
@@ -860,7 +861,7 @@ int lanczos_body(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, bool c
     if ((rc = vec_reorth(L->cur, L->V, L->n, L->maxiter + 1, iter, L->part, L->h(), part_b, cplx,
                          s)))
       return rc;
-  return it < 0 ? vec_counter_add(L->counter, 1, s) : CXA_OK;
+  return CXA_OK;
 }
 
 // `iters` iterations (device-counter form) captured on stream ws as one executable graph
@@ -869,7 +870,8 @@ int capture_iterations(cxa_lanczos *L, cxa_matvec_fn fn, void *ctx, int reorth, 
   CXA_HIP(hipStreamBeginCapture(ws, hipStreamCaptureModeThreadLocal));
   int rc = CXA_OK;
   for (int i = 0; i < iters && rc == CXA_OK; ++i)
-    rc = lanczos_body(L, fn, ctx, reorth, cplx, -1, ws);
+    rc = lanczos_body(L, fn, ctx, reorth, cplx, -1, ws, i);
+  if (rc == CXA_OK) rc = vec_counter_add(L->counter, iters, ws);  // one launch per batch
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(ws, &g);
   if (rc == CXA_OK && e == hipSuccess) {

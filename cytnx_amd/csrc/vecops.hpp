@@ -12,12 +12,13 @@ constexpr int kPartials = 512;  // partial sums per reduction (fixed: determinis
 // out[(*counter) + offset] = x.y (mode 0) or sqrt(x.y) (mode 1); counter may be null (0)
 int vec_dot(const double *x, const double *y, int64_t n, double *part, double *out,
             const int *counter, int offset, int mode, hipStream_t s);
-// The Lanczos iteration index: read from a device counter (launches captured once and
-// replayed) or passed by value (launches enqueued per iteration).
+// The Lanczos iteration index: a device counter plus an offset fixed at capture (launches
+// captured once and replayed: the counter holds a captured batch's first iteration and
+// advances once per batch) or passed by value (launches enqueued per iteration).
 struct IterIdx {
   const int *p;
   int v;
-  __device__ __forceinline__ int get() const { return p ? *p : v; }
+  __device__ __forceinline__ int get() const { return p ? *p + v : v; }
 };
 
 // Lanczos iteration kernels.  Each reduction leaves kPartials partial sums that the next launch
